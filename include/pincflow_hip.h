@@ -1,0 +1,317 @@
+/*
+ * pincflow_hip.h -- C ABI of the MI355X-native semi-implicit dynamical-core
+ * step (libpincflow_hip.so).
+ *
+ * The library owns one HBM-resident model state per handle (`pcf_state`).
+ * Every entry point below mirrors a function of the PinCFlow.jl reference
+ * (paths relative to the reference repository root) and keeps its argument
+ * meaning and error behaviour:
+ *
+ *   return 0              success
+ *   return PCF_E_NOCONV   BiCGSTAB did not converge (reference errflag = true)
+ *   return < 0            the reference would have raised `error(...)`, or a
+ *                         HIP call failed; pcf_last_error() has the message.
+ *
+ * All arrays handed across the ABI are plain FP64 host or device pointers in
+ * the reference's Julia column-major layout (x fastest).  Haloed fields have
+ * (x_size + 2 nbx) x (y_size + 2 nby) x (z_size + 2 nbz) elements, Poisson
+ * vectors have x_size x y_size x z_size elements.
+ *
+ * One host thread per handle.  Each handle owns one HIP stream; no call
+ * returns while work it enqueued could still read a caller-owned host buffer.
+ */
+#ifndef PINCFLOW_HIP_H
+#define PINCFLOW_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCF_ABI_VERSION 1
+
+/* ---- return codes ------------------------------------------------------ */
+#define PCF_OK 0
+#define PCF_E_NOCONV 1     /* apply_bicgstab!.jl:146-149 (errflag = true)   */
+#define PCF_E_DT_ZERO (-2) /* solve_poisson!.jl:52-54                        */
+#define PCF_E_DIVNORM (-3) /* compute_lhs!.jl:149-157                        */
+#define PCF_E_DTMIN (-4)   /* compute_time_step.jl:142-150                   */
+#define PCF_E_OPTION (-5)  /* @dispatch.jl: invalid option                   */
+#define PCF_E_CONFIG (-6)  /* Domain.jl:146-174 / Grid.jl:221 / Atmosphere   */
+#define PCF_E_HIP (-7)     /* a HIP runtime call failed                      */
+#define PCF_E_ARG (-8)     /* bad argument (null pointer, unknown field ...)  */
+
+/* ---- namelist options (Types/NamelistTypes) ---------------------------- */
+enum pcf_model {
+  PCF_BOUSSINESQ = 0,
+  PCF_PSEUDO_INCOMPRESSIBLE = 1
+};
+
+enum pcf_background {
+  PCF_NEUTRAL_STRATIFICATION = 0, /* Atmosphere.jl (Boussinesq)            */
+  PCF_STABLE_STRATIFICATION = 1,  /* Atmosphere.jl (Boussinesq)            */
+  PCF_ISOTHERMAL = 2,             /* Atmosphere.jl (pseudo-incompressible) */
+  PCF_ISENTROPIC = 3,             /* Atmosphere.jl (pseudo-incompressible) */
+  PCF_REALISTIC = 4,              /* Atmosphere.jl (pseudo-incompressible) */
+  PCF_LAPSE_RATES = 5             /* Atmosphere.jl (pseudo-incompressible) */
+};
+
+/* The reference takes Julia closures for topography, stretching, initial
+ * fields and sponges (GridNamelist.jl, AtmosphereNamelist.jl,
+ * SpongeNamelist.jl).  Closures cannot run on the device, so the ABI takes a
+ * closed set of parametric forms that covers the reference examples. */
+enum pcf_topography {
+  PCF_TOPO_FLAT = 0,
+  PCF_TOPO_COSINE_HILL = 1, /* h0/2 (1 + cos(pi x / l0))  periodic_hill.jl:29 */
+  PCF_TOPO_AGNESI = 2,      /* h0 / (1 + (x^2+y^2)/l0^2)  mountain_wave.jl:34 */
+  PCF_TOPO_GAUSSIANS = 3    /* sum_n h_n exp(-((x-x_n)^2+(y-y_n)^2)/w_n^2)   */
+};
+
+enum pcf_stretching {
+  PCF_STRETCH_NONE = 0, /* zhat -> zhat (GridNamelist.jl default)            */
+  PCF_STRETCH_SINH = 1  /* zhat -> lz sinh(b zhat/lz) / sinh(b), b = param   */
+};
+
+enum pcf_buoyancy_initialization {
+  PCF_INIT_RHOP = 0,  /* Predictands.jl: buoyancy_initialization = :initial_rhop   */
+  PCF_INIT_THETAP = 1 /* Predictands.jl: buoyancy_initialization = :initial_thetap */
+};
+
+enum pcf_perturbation {
+  PCF_PERT_NONE = 0,
+  PCF_PERT_COSINE_BUBBLE = 1, /* A (1 + cos(pi r)) for r <= 1, cold_bubble.jl:21-28 */
+  PCF_PERT_VORTEX = 2         /* vortex.jl:23-39 winds, amplitude A             */
+};
+
+enum pcf_lhs_sponge {
+  PCF_LHS_SPONGE_NONE = 0,
+  PCF_LHS_SPONGE_SIN2 = 1 /* mountain_wave.jl:40-53 */
+};
+
+enum pcf_rhs_sponge {
+  PCF_RHS_SPONGE_NONE = 0,
+  PCF_RHS_SPONGE_SIN2_TOP = 1 /* periodic_hill.jl:33-36: A sin^2(...)/dt */
+};
+
+/* Arithmetic of the Poisson solve (an addition of this build; the reference
+ * is FP64 throughout). */
+enum pcf_precision {
+  PCF_PRECISION_FP64 = 0, /* reference arithmetic                            */
+  PCF_PRECISION_MIXED = 1 /* FP32 operator/preconditioner inside FP64
+                             iterative refinement                             */
+};
+
+#define PCF_MAX_HILLS 32
+
+typedef struct pcf_hill {
+  double height; /* m */
+  double x0, y0; /* m, centre */
+  double width;  /* m, e-folding half-width */
+} pcf_hill;
+
+/* Mirror of Namelists(...) (Types/NamelistTypes/Namelists.jl).  Field order
+ * is part of the ABI; pcf_config_default() fills the reference defaults. */
+typedef struct pcf_config {
+  int32_t abi_version;
+  /* DomainNamelist (DomainNamelist.jl:72-102) */
+  int32_t x_size, y_size, z_size;
+  int32_t nbx, nby, nbz;
+  double lx, ly, lz;
+  /* DiscretizationNamelist (DiscretizationNamelist.jl:44-60) */
+  double cfl_number, dtmin, dtmax;
+  int32_t adaptive_time_step;
+  /* PoissonNamelist (PoissonNamelist.jl:48-66) */
+  int32_t poisson_iterations;
+  double tolerance;
+  double dtau;
+  int32_t preconditioner;
+  int32_t preconditioner_iterations;
+  int32_t initial_cleaning;
+  int32_t tolerance_is_relative;
+  /* AtmosphereNamelist (AtmosphereNamelist.jl:108-156) */
+  int32_t model;
+  int32_t background;
+  int32_t specify_reynolds_number;
+  int32_t buoyancy_initialization;
+  double inverse_reynolds_number;
+  double kinematic_viscosity;
+  double buoyancy_frequency;
+  double potential_temperature;
+  double temperature;
+  double ground_pressure;
+  double coriolis_frequency;
+  double tropopause_height;
+  double troposphere_lapse_rate;
+  double stratosphere_lapse_rate;
+  double initial_u, initial_v, initial_w; /* m/s, uniform parts          */
+  int32_t perturbation;                   /* enum pcf_perturbation        */
+  int32_t pad0;
+  double pert_amplitude;                  /* kg/m^3 (rhop), K (thetap), m/s */
+  double pert_x0, pert_y0, pert_z0;       /* m */
+  double pert_rx, pert_ry, pert_rz;       /* m; <= 0 drops that direction */
+  /* GridNamelist (GridNamelist.jl:35-45) */
+  int32_t stretching;                     /* enum pcf_stretching          */
+  int32_t topography;                     /* enum pcf_topography          */
+  double stretching_parameter;
+  double topo_height, topo_width;         /* m (cosine hill, Agnesi)      */
+  int32_t n_hills;
+  int32_t pad1;
+  pcf_hill hills[PCF_MAX_HILLS];
+  /* SpongeNamelist (SpongeNamelist.jl:48-66) */
+  int32_t damp_horizontal_wind_on_rhs;
+  int32_t relax_to_mean;
+  int32_t lhs_sponge;                     /* enum pcf_lhs_sponge          */
+  int32_t rhs_sponge;                     /* enum pcf_rhs_sponge          */
+  double lhs_alpharmax;                   /* 1/s */
+  double lhs_dxr, lhs_dyr, lhs_dzr;       /* m; <= 0 disables a direction */
+  double lhs_divisor;
+  double rhs_zr;                          /* m */
+  double rhs_amplitude;
+  double relaxed_u, relaxed_v, relaxed_w; /* m/s */
+  /* OutputNamelist (OutputNamelist.jl) -- time-loop control only */
+  int32_t output_steps;
+  int32_t iterations;
+  int32_t nout;
+  int32_t precision;                      /* enum pcf_precision           */
+  double output_interval;                 /* s */
+  double tmax;                            /* s */
+} pcf_config;
+
+/* Fields reachable through pcf_get_field / pcf_set_field (haloed, FP64). */
+enum pcf_field {
+  PCF_F_RHO = 0, PCF_F_RHOP, PCF_F_U, PCF_F_V, PCF_F_W, PCF_F_PIP, /* Predictands */
+  PCF_F_PBAR, PCF_F_THETABAR, PCF_F_RHOBAR, PCF_F_N2,               /* Atmosphere  */
+  PCF_F_JAC, PCF_F_MET13, PCF_F_MET23, PCF_F_MET33,                 /* Grid        */
+  PCF_F_ZC, PCF_F_ZCTILDE,
+  PCF_F_ALPHAR, PCF_F_BETAR,                                        /* Sponge      */
+  PCF_F_DPIP, PCF_F_CORX, PCF_F_CORY,                               /* corrector   */
+  PCF_F_DRHO, PCF_F_DRHOP, PCF_F_DU, PCF_F_DV, PCF_F_DW,            /* Increments  */
+  PCF_F_RHOOLD, PCF_F_RHOPOLD, PCF_F_UOLD, PCF_F_VOLD, PCF_F_WOLD,  /* Backups     */
+  PCF_F_P0_RHO, PCF_F_P0_RHOP, PCF_F_P0_U, PCF_F_P0_V, PCF_F_P0_W,  /* p0 copy     */
+  PCF_F_PHIRHO_X, PCF_F_PHIRHO_Y, PCF_F_PHIRHO_Z,                   /* Fluxes      */
+  PCF_F_PHIRHOP_X, PCF_F_PHIRHOP_Y, PCF_F_PHIRHOP_Z,
+  PCF_F_PHIU_X, PCF_F_PHIU_Y, PCF_F_PHIU_Z,
+  PCF_F_PHIV_X, PCF_F_PHIV_Y, PCF_F_PHIV_Z,
+  PCF_F_PHIW_X, PCF_F_PHIW_Y, PCF_F_PHIW_Z,
+  PCF_F_COUNT
+};
+
+/* Poisson vectors (interior-sized, FP64) reachable through pcf_get_vector. */
+enum pcf_vector {
+  PCF_V_RHS = 0, PCF_V_SOLUTION,
+  PCF_V_AC, PCF_V_AL, PCF_V_AR, PCF_V_AB, PCF_V_AF, PCF_V_AD, PCF_V_AU,
+  PCF_V_ALD, PCF_V_ALU, PCF_V_ARD, PCF_V_ARU, PCF_V_ABD, PCF_V_ABU,
+  PCF_V_AFD, PCF_V_AFU, PCF_V_ADD, PCF_V_AUU, PCF_V_ALDD, PCF_V_ALUU,
+  PCF_V_ARDD, PCF_V_ARUU, PCF_V_ABDD, PCF_V_ABUU, PCF_V_AFDD, PCF_V_AFUU,
+  PCF_V_COUNT
+};
+
+/* Boundary sets of set_boundaries!(state, ::Boundary...) (set_boundaries!.jl:43-68). */
+enum pcf_boundaries { PCF_BOUNDARY_PREDICTANDS = 0, PCF_BOUNDARY_FLUXES = 2 };
+
+/* Variables of update! / apply_lhs_sponge! (Types.jl singletons). */
+enum pcf_variable { PCF_VAR_RHO = 0, PCF_VAR_RHOP, PCF_VAR_U, PCF_VAR_V, PCF_VAR_W, PCF_VAR_PIP };
+
+/* Operator kinds of apply_operator! (apply_operator!.jl:48 / :275). */
+enum pcf_operator { PCF_OP_TOTAL = 0, PCF_OP_HORIZONTAL = 1 };
+
+typedef struct pcf_state pcf_state;
+
+/* Per-step record of pcf_step (the values integrate prints). */
+typedef struct pcf_step_info {
+  double time;      /* s, after the step          integrate.jl:241        */
+  double dt;        /* s                          integrate.jl:230        */
+  int32_t niter[2]; /* BiCGSTAB iterations of the two implicit stages     */
+  int32_t output;   /* 1 if this step lands on an output time integrate.jl:233-239 */
+  int32_t pad;
+} pcf_step_info;
+
+/* ---- library ------------------------------------------------------------ */
+int pcf_abi_version(void);
+const char* pcf_last_error(void);
+/* Fills the reference namelist defaults (each Namelist constructor's keyword defaults). */
+void pcf_config_default(pcf_config* cfg);
+/* sizeof(pcf_config), for bindings that mirror the structure. */
+int pcf_config_size(void);
+
+/* ---- state (State.jl:106-131) ------------------------------------------- */
+/* Builds the grid, background, sponge arrays and initial predictands on the
+ * host in FP64 and uploads them; all later work stays on the device. */
+int pcf_create(const pcf_config* cfg, pcf_state** out);
+int pcf_destroy(pcf_state* st);
+/* Interior and haloed extents: dims[0..5] = nx, ny, nz, nxx, nyy, nzz. */
+int pcf_dims(const pcf_state* st, int32_t dims[6]);
+
+/* ---- time integration (Integration/) ------------------------------------ */
+/* Initial divergence cleaning, integrate.jl:136-156. */
+int pcf_initial_cleaning(pcf_state* st, int32_t* niter);
+/* One iteration of the time loop of integrate (integrate.jl:217-378):
+ * compute_time_step, output-time correction, compute_sponges!, the five
+ * semi-implicit stages.  Returns PCF_E_NOCONV / PCF_E_DTMIN like the reference. */
+int pcf_step(pcf_state* st, pcf_step_info* info);
+/* compute_time_step (compute_time_step.jl:46-153); dt is non-dimensional. */
+int pcf_compute_time_step(pcf_state* st, double* dt);
+/* compute_sponges! (compute_sponges!.jl:26-65), dt and time non-dimensional. */
+int pcf_compute_sponges(pcf_state* st, double dt, double time);
+/* explicit_integration!(state, p0, dt, time, LHS()) (explicit_integration!.jl:42-101)
+ * with p0 = the device copy made by pcf_backup_predictands. */
+int pcf_explicit_integration_lhs(pcf_state* st, double dt, double time);
+/* explicit_integration!(state, p0, dt, time, RHS()) (explicit_integration!.jl:103-131). */
+int pcf_explicit_integration_rhs(pcf_state* st, double dt, double time);
+/* implicit_integration! (implicit_integration!.jl:37-79); *niter gets the iterations. */
+int pcf_implicit_integration(pcf_state* st, double dt, double time, double rayleigh_factor,
+                             int32_t* niter);
+/* backup_predictands (backup_predictands.jl): p0 := predictands. */
+int pcf_backup_predictands(pcf_state* st);
+
+/* ---- boundaries (Boundaries/) ------------------------------------------- */
+int pcf_set_boundaries(pcf_state* st, int which);
+
+/* ---- advection (FluxCalculator/) ---------------------------------------- */
+/* reconstruct! followed by compute_fluxes!(state, p0) and
+ * set_boundaries!(state, BoundaryFluxes()) (explicit_integration!.jl:53-58);
+ * the MUSCL reconstruction is evaluated inside the flux kernels. */
+int pcf_compute_fluxes(pcf_state* st);
+
+/* ---- updates (Update/) --------------------------------------------------- */
+/* update!(state, dt, m, var[, LHS()]) (update!.jl:505-1192), m in 1..3. */
+int pcf_update_lhs(pcf_state* st, double dt, int32_t m, int var);
+/* update!(state, dt, var, RHS(), Explicit() | Implicit(), rf)
+ * (update!.jl:600-727, 809-881, 960-1032, 1195-1342). */
+int pcf_update_rhs(pcf_state* st, double dt, int var, int implicit, double rayleigh_factor);
+/* apply_lhs_sponge! (apply_lhs_sponge!.jl:255-636). */
+int pcf_apply_lhs_sponge(pcf_state* st, double dt, double time, int var);
+
+/* ---- pressure solve (PoissonSolver/) ------------------------------------ */
+/* apply_corrector! (apply_corrector!.jl:38-66). */
+int pcf_apply_corrector(pcf_state* st, double dt, double rayleigh_factor, int32_t* niter);
+/* compute_lhs! (compute_lhs!.jl:68-160): fills the RHS, returns tolref. */
+int pcf_compute_lhs(pcf_state* st, double* tolref);
+/* compute_operator! (compute_operator!.jl:40-1606). */
+int pcf_compute_operator(pcf_state* st, double dt, double rayleigh_factor);
+/* apply_operator!(sin, ls, Total()|Horizontal(), state) (apply_operator!.jl:48-479);
+ * d_in/d_out are device pointers to interior-sized FP64 vectors. */
+int pcf_apply_operator(pcf_state* st, const double* d_in, double* d_out, int kind);
+/* apply_preconditioner!(sin, sout, state) (apply_preconditioner!.jl:40-124);
+ * device pointers, interior-sized. */
+int pcf_apply_preconditioner(pcf_state* st, const double* d_in, double* d_out);
+/* apply_bicgstab!(state, tolref) (apply_bicgstab!.jl:24-150) on the current
+ * RHS and operator; solution left in the PCF_V_SOLUTION vector. */
+int pcf_apply_bicgstab(pcf_state* st, double tolref, int32_t* niter);
+/* compute_global_dot_product (compute_global_dot_product.jl:20-36), device pointers. */
+int pcf_dot(pcf_state* st, const double* d_a, const double* d_b, double* result);
+
+/* ---- host <-> device copies (test and output support) ------------------- */
+int pcf_get_field(pcf_state* st, int field, double* host_out);
+int pcf_set_field(pcf_state* st, int field, const double* host_in);
+int pcf_get_vector(pcf_state* st, int vec, double* host_out);
+int pcf_set_vector(pcf_state* st, int vec, const double* host_in);
+/* Residual record of the last BiCGSTAB solve: final ||r||/||b|| (RMS norms). */
+int pcf_last_residual(const pcf_state* st, double* rel_residual);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PINCFLOW_HIP_H */

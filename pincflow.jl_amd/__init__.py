@@ -1,0 +1,20 @@
+"""MI355X-native semi-implicit dynamical-core step of PinCFlow.jl.
+
+Host-side mirror of the reference interface: ``Namelists(...)`` and
+``integrate(namelists)`` (src/Integration/integrate.jl:68-415) over the C ABI
+of libpincflow_hip.so (include/pincflow_hip.h), whose HIP kernels keep every
+field resident in HBM.
+"""
+from . import _abi
+from .namelists import (AgnesiHill, AtmosphereNamelist, CosineBubble, CosineHill, DiscretizationNamelist,
+                        DomainNamelist, GaussianHills, GridNamelist, Namelists, OutputNamelist, PoissonNamelist,
+                        Sin2Sponge, Sin2TopSponge, SinhStretching, SpongeNamelist, TracerNamelist, Vortex,
+                        WKBNamelist, to_config)
+from . import examples, output
+
+__all__ = [
+    "AgnesiHill", "AtmosphereNamelist", "CosineBubble", "CosineHill", "DiscretizationNamelist", "DomainNamelist",
+    "GaussianHills", "GridNamelist", "Namelists", "OutputNamelist", "PoissonNamelist", "Sin2Sponge",
+    "Sin2TopSponge", "SinhStretching", "SpongeNamelist", "TracerNamelist", "Vortex", "WKBNamelist", "to_config",
+    "examples", "output",
+]
