@@ -159,14 +159,16 @@ double stress_tensor(State& st, int i, int j, int k, int mu, int nu) {
   double wl = 0.5 * (vertical_wind(st, i - 1, j, k) + vertical_wind(st, i - 1, j, k - 1));
   double wf = 0.5 * (vertical_wind(st, i, j + 1, k) + vertical_wind(st, i, j + 1, k - 1));
   double wb = 0.5 * (vertical_wind(st, i, j - 1, k) + vertical_wind(st, i, j - 1, k - 1));
+  // compute_stress_tensor.jl:112-118: 2/3 * (divergence) / jac, left to right
   auto div = [&]() {
-    return ((jacedger * u(i, j, k) - jacedgel * u(i - 1, j, k)) / dx +
+    return 2.0 / 3.0 *
+           ((jacedger * u(i, j, k) - jacedgel * u(i - 1, j, k)) / dx +
             (jacedgef * v(i, j, k) - jacedgeb * v(i, j - 1, k)) / dy +
             (jacedgeu * w(i, j, k) - jacedged * w(i, j, k - 1)) / dz) /
            jac(i, j, k);
   };
   if (mu == 1 && nu == 1)
-    return 2.0 * (u(i, j, k) - u(i - 1, j, k)) / dx + st.met13(i, j, k) * (uu - ud) / dz - 2.0 / 3.0 * div();
+    return 2.0 * (u(i, j, k) - u(i - 1, j, k)) / dx + st.met13(i, j, k) * (uu - ud) / dz - div();
   if ((mu == 1 && nu == 2) || (mu == 2 && nu == 1))
     return 0.5 * (uf - ub) / dy + 0.5 * st.met23(i, j, k) * (uu - ud) / dz + 0.5 * (vr - vl) / dx +
            0.5 * st.met13(i, j, k) * (vu - vd) / dz;
@@ -174,13 +176,13 @@ double stress_tensor(State& st, int i, int j, int k, int mu, int nu) {
     return 0.5 * (uu - ud) / dz / jac(i, j, k) + 0.5 * (wr - wl) / dx +
            st.met13(i, j, k) * (vertical_wind(st, i, j, k) - vertical_wind(st, i, j, k - 1)) / dz;
   if (mu == 2 && nu == 2)
-    return 2.0 * (v(i, j, k) - v(i, j - 1, k)) / dy + st.met23(i, j, k) * (vu - vd) / dz - 2.0 / 3.0 * div();
+    return 2.0 * (v(i, j, k) - v(i, j - 1, k)) / dy + st.met23(i, j, k) * (vu - vd) / dz - div();
   if ((mu == 2 && nu == 3) || (mu == 3 && nu == 2))
     return 0.5 * (vu - vd) / dz / jac(i, j, k) + 0.5 * (wf - wb) / dy +
            st.met23(i, j, k) * (vertical_wind(st, i, j, k) - vertical_wind(st, i, j, k - 1)) / dz;
   // mu == 3 && nu == 3
   return 2.0 * (vertical_wind(st, i, j, k) - vertical_wind(st, i, j, k - 1)) / dz / jac(i, j, k) -
-         2.0 / 3.0 * div();
+         div();
 }
 
 // Sponge closures of SpongeNamelist.jl:48-66 in the parametric forms of

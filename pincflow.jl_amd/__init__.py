@@ -11,8 +11,10 @@ from .namelists import (AgnesiHill, AtmosphereNamelist, CosineBubble, CosineHill
                         Sin2Sponge, Sin2TopSponge, SinhStretching, SpongeNamelist, TracerNamelist, Vortex,
                         WKBNamelist, to_config)
 from . import examples, output
+from .state import PinCFlowError, PoissonNotConverged, State, integrate
 
 __all__ = [
+    "State", "integrate", "PinCFlowError", "PoissonNotConverged",
     "AgnesiHill", "AtmosphereNamelist", "CosineBubble", "CosineHill", "DiscretizationNamelist", "DomainNamelist",
     "GaussianHills", "GridNamelist", "Namelists", "OutputNamelist", "PoissonNamelist", "Sin2Sponge",
     "Sin2TopSponge", "SinhStretching", "SpongeNamelist", "TracerNamelist", "Vortex", "WKBNamelist", "to_config",

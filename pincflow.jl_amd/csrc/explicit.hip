@@ -1,0 +1,932 @@
+// Explicit part of the step on the device: src/FluxCalculator (reconstruct!,
+// apply_3d_muscl!, compute_fluxes!) and src/Update (update!, apply_lhs_sponge!,
+// compute_sponges!) for the Boussinesq and pseudo-incompressible models.
+//
+// The MUSCL reconstruction is evaluated inside the flux kernels from the
+// reconstruction inputs phi (reconstruct!.jl:126-286), so the 30 haloed
+// reconstruction arrays of the reference are never stored.  Halo values of a
+// reconstruction are the periodic images of interior values in the
+// reference; the inputs phi keep that property, so evaluating the limiter at a
+// halo position gives the same bits.  Only fluxes the updates consume are
+// computed; the wall faces that set_boundaries!(state, BoundaryFluxes()) zeroes
+// are written as zeros.
+#include <cfloat>
+#include <cmath>
+
+#include "pcf_device.hpp"
+
+namespace pcf {
+
+namespace {
+
+enum { V_RHO = 0, V_RHOP, V_U, V_V, V_W };
+
+struct FluxArgs {
+  Fields F;
+  const double* phi[5];
+  const double* u0;
+  const double* v0;
+  const double* w0;
+  const double* st[6];  // stress 11, 12, 13, 22, 23, 33
+  double* flux[5][3];
+  Geo g;
+  double inv_re;        // 1 / re
+  int viscous;
+};
+
+__device__ __forceinline__ Fields fields_at(const FluxArgs& A) { return A.F; }
+
+// ---------------------------------------------------------------------------
+// reconstruction inputs (reconstruct!.jl:126-286) on layers k0-1..k1+1
+// ---------------------------------------------------------------------------
+__global__ void k_phi(Fields F, Geo g, double* phr, double* phrp, double* phu, double* phv, int pi) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y;
+  const int k = g.k0 - 1 + blockIdx.z;
+  if (i > g.nxx) return;
+  const size_t c = g.h(i, j, k);
+  const double pb = F.pbar[c];
+  if (pi) phr[c] = F.rho[c] / pb;
+  phrp[c] = F.rhop[c] / pb;  // reconstruct!.jl:154-176
+  if (i < g.nxx) {           // reconstruct!.jl:178-208
+    const size_t r = g.h(i + 1, j, k);
+    double rhoedge = 0.5 * (F.rho[c] + F.rho[r] + F.rhobar[c] + F.rhobar[r]);
+    double pedge = 0.5 * (pb + F.pbar[r]);
+    phu[c] = F.u[c] * rhoedge / pedge;
+  }
+  if (j < g.nyy) {  // reconstruct!.jl:210-240
+    const size_t f = g.h(i, j + 1, k);
+    double rhoedge = 0.5 * (F.rho[c] + F.rho[f] + F.rhobar[c] + F.rhobar[f]);
+    double pedge = 0.5 * (pb + F.pbar[f]);
+    phv[c] = F.v[c] * rhoedge / pedge;
+  }
+}
+
+// Cartesian vertical wind compute_vertical_wind(i, j, k) on i >= 2, j >= 2,
+// k <= nzz - 1 (compute_vertical_wind.jl:27-63).
+__global__ void k_wcart(Fields F, Geo g, double* wc) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.nxx) return;
+  wc[g.h(i, j, k)] = vertical_wind(F, g, i, j, k);
+}
+
+// phi for W (reconstruct!.jl:242-286): Cartesian w times the edge density
+// over the edge P, on layers k0-1..k1+1 and i, j >= 2.
+__global__ void k_phi_w(Fields F, Geo g, const double* wc, double* phw) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y;
+  const int k = g.k0 - 1 + blockIdx.z;
+  if (i > g.nxx) return;
+  const size_t c = g.h(i, j, k), up = g.h(i, j, k + 1);
+  const double jc = F.jac[c], ju = F.jac[up];
+  double rhoedgeu = (ju * (F.rho[c] + F.rhobar[c]) + jc * (F.rho[up] + F.rhobar[up])) / (jc + ju);
+  double pedgeu = (ju * F.pbar[c] + jc * F.pbar[up]) / (jc + ju);
+  phw[c] = wc[c] * (rhoedgeu / pedgeu);
+}
+
+// compute_stress_tensor.jl:55-164: the six distinct components at one cell.
+__global__ void k_stress(Fields F, Geo g, const double* wc, double* s11, double* s12, double* s13, double* s22,
+                         double* s23, double* s33, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const double *u = F.u, *v = F.v, *w = F.w, *jac = F.jac;
+  const double dx = g.dx, dy = g.dy, dz = g.dz;
+  const size_t c = g.h(i, j, k);
+  double jacedger = 0.5 * (jac[c] + jac[g.h(i + 1, j, k)]);
+  double jacedgel = 0.5 * (jac[c] + jac[g.h(i - 1, j, k)]);
+  double jacedgef = 0.5 * (jac[c] + jac[g.h(i, j + 1, k)]);
+  double jacedgeb = 0.5 * (jac[c] + jac[g.h(i, j - 1, k)]);
+  double jacedgeu = 2.0 * jac[c] * jac[g.h(i, j, k + 1)] / (jac[c] + jac[g.h(i, j, k + 1)]);
+  double jacedged = 2.0 * jac[c] * jac[g.h(i, j, k - 1)] / (jac[c] + jac[g.h(i, j, k - 1)]);
+  double uf = 0.5 * (u[g.h(i, j + 1, k)] + u[g.h(i - 1, j + 1, k)]);
+  double ub = 0.5 * (u[g.h(i, j - 1, k)] + u[g.h(i - 1, j - 1, k)]);
+  double uu = 0.5 * (u[g.h(i, j, k + 1)] + u[g.h(i - 1, j, k + 1)]);
+  double ud = 0.5 * (u[g.h(i, j, k - 1)] + u[g.h(i - 1, j, k - 1)]);
+  double vr = 0.5 * (v[g.h(i + 1, j, k)] + v[g.h(i + 1, j - 1, k)]);
+  double vl = 0.5 * (v[g.h(i - 1, j, k)] + v[g.h(i - 1, j - 1, k)]);
+  double vu = 0.5 * (v[g.h(i, j, k + 1)] + v[g.h(i, j - 1, k + 1)]);
+  double vd = 0.5 * (v[g.h(i, j, k - 1)] + v[g.h(i, j - 1, k - 1)]);
+  double wr = 0.5 * (wc[g.h(i + 1, j, k)] + wc[g.h(i + 1, j, k - 1)]);
+  double wl = 0.5 * (wc[g.h(i - 1, j, k)] + wc[g.h(i - 1, j, k - 1)]);
+  double wf = 0.5 * (wc[g.h(i, j + 1, k)] + wc[g.h(i, j + 1, k - 1)]);
+  double wb = 0.5 * (wc[g.h(i, j - 1, k)] + wc[g.h(i, j - 1, k - 1)]);
+  // compute_stress_tensor.jl:112-118: 2/3 * (divergence) / jac, left to right
+  double div = 2.0 / 3.0 *
+               ((jacedger * u[c] - jacedgel * u[g.h(i - 1, j, k)]) / dx +
+                (jacedgef * v[c] - jacedgeb * v[g.h(i, j - 1, k)]) / dy +
+                (jacedgeu * w[c] - jacedged * w[g.h(i, j, k - 1)]) / dz) /
+               jac[c];
+  double dwc = wc[c] - wc[g.h(i, j, k - 1)];
+  s11[c] = 2.0 * (u[c] - u[g.h(i - 1, j, k)]) / dx + F.m13[c] * (uu - ud) / dz - div;
+  s12[c] = 0.5 * (uf - ub) / dy + 0.5 * F.m23[c] * (uu - ud) / dz + 0.5 * (vr - vl) / dx +
+           0.5 * F.m13[c] * (vu - vd) / dz;
+  s13[c] = 0.5 * (uu - ud) / dz / jac[c] + 0.5 * (wr - wl) / dx + F.m13[c] * dwc / dz;
+  s22[c] = 2.0 * (v[c] - v[g.h(i, j - 1, k)]) / dy + F.m23[c] * (vu - vd) / dz - div;
+  s23[c] = 0.5 * (vu - vd) / dz / jac[c] + 0.5 * (wf - wb) / dy + F.m23[c] * dwc / dz;
+  s33[c] = 2.0 * dwc / dz / jac[c] - div;
+}
+
+// MUSCL face values from phi along a stride: right face of cell c and left
+// face of cell c + 1.
+__device__ __forceinline__ void faces(const double* phi, size_t c, size_t stride, double& right_of_c,
+                                      double& left_of_next) {
+  double pm = phi[c - stride], p0 = phi[c], p1 = phi[c + stride], p2 = phi[c + 2 * stride];
+  double t1, t2;
+  muscl(pm, p0, p1, t1, t2);
+  right_of_c = t2;
+  muscl(p0, p1, p2, t1, t2);
+  left_of_next = t1;
+}
+
+__device__ __forceinline__ double pe_x(const Fields& F, const Geo& g, int i, int j, int k) {
+  size_t c = g.h(i, j, k), r = g.h(i + 1, j, k);
+  return 0.5 * (F.jac[c] * F.pbar[c] + F.jac[r] * F.pbar[r]);
+}
+__device__ __forceinline__ double pe_y(const Fields& F, const Geo& g, int i, int j, int k) {
+  size_t c = g.h(i, j, k), f = g.h(i, j + 1, k);
+  return 0.5 * (F.jac[c] * F.pbar[c] + F.jac[f] * F.pbar[f]);
+}
+__device__ __forceinline__ double pe_z(const Fields& F, const Geo& g, int i, int j, int k) {
+  size_t c = g.h(i, j, k), u = g.h(i, j, k + 1);
+  return F.jac[c] * F.jac[u] * (F.pbar[c] + F.pbar[u]) / (F.jac[c] + F.jac[u]);
+}
+
+// One flux component of one variable at face (i, j, k)
+// (compute_fluxes!.jl:359-520, 587-1503).
+template <int VAR, int DIR>
+__global__ void k_flux(FluxArgs A, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const Geo& g = A.g;
+  const Fields& F = A.F;
+  const size_t c = g.h(i, j, k);
+  const size_t sx = 1, sy = (size_t)g.nxx, sz = (size_t)g.nxx * g.nyy;
+  const size_t stride = DIR == 0 ? sx : DIR == 1 ? sy : sz;
+  double right_of_c, left_of_next;
+  faces(A.phi[VAR], c, stride, right_of_c, left_of_next);
+  double val;
+  if (VAR == V_RHO || VAR == V_RHOP) {
+    double hi = left_of_next, lo = right_of_c;  // value at c + e, value at c
+    if (DIR == 0) {
+      if (VAR == V_RHO) {
+        const size_t r = g.h(i + 1, j, k);
+        double rhobaredger = 0.5 * (F.rhobar[c] + F.rhobar[r]);
+        double pe = 0.5 * (F.pbar[c] + F.pbar[r]);
+        hi = hi + rhobaredger / pe;
+        lo = lo + rhobaredger / pe;
+      }
+      val = upwind(pe_x(F, g, i, j, k) * A.u0[c], lo, hi);
+    } else if (DIR == 1) {
+      if (VAR == V_RHO) {
+        const size_t f = g.h(i, j + 1, k);
+        double rhobaredgef = 0.5 * (F.rhobar[c] + F.rhobar[f]);
+        double pe = 0.5 * (F.pbar[c] + F.pbar[f]);
+        hi = hi + rhobaredgef / pe;
+        lo = lo + rhobaredgef / pe;
+      }
+      val = upwind(pe_y(F, g, i, j, k) * A.v0[c], lo, hi);
+    } else {
+      const size_t u = g.h(i, j, k + 1);
+      const double jc = F.jac[c], ju = F.jac[u];
+      if (VAR == V_RHO) {
+        double rhobaredgeu = (ju * F.rhobar[c] + jc * F.rhobar[u]) / (jc + ju);
+        double pe = (ju * F.pbar[c] + jc * F.pbar[u]) / (jc + ju);
+        hi = hi + rhobaredgeu / pe;
+        lo = lo + rhobaredgeu / pe;
+      }
+      val = upwind(pe_z(F, g, i, j, k) * A.w0[c], lo, hi);
+    }
+  } else if (VAR == V_U) {
+    const double* S11 = A.st[0];
+    const double* S12 = A.st[1];
+    const double* S13 = A.st[2];
+    if (DIR == 0) {
+      double usurf = 0.5 * (pe_x(F, g, i, j, k) * A.u0[c] + pe_x(F, g, i + 1, j, k) * A.u0[g.h(i + 1, j, k)]);
+      val = upwind(usurf, right_of_c, left_of_next);
+      if (A.viscous) {
+        double coef_v = A.inv_re * F.rhobar[g.h(i + 1, j, g.k0)];
+        val -= coef_v * F.jac[g.h(i + 1, j, k)] * S11[g.h(i + 1, j, k)];
+      }
+    } else if (DIR == 1) {
+      double vsurf = 0.5 * (pe_y(F, g, i, j, k) * A.v0[c] + pe_y(F, g, i + 1, j, k) * A.v0[g.h(i + 1, j, k)]);
+      val = upwind(vsurf, right_of_c, left_of_next);
+      if (A.viscous) {
+        const size_t r = g.h(i + 1, j, k), f = g.h(i, j + 1, k), rf = g.h(i + 1, j + 1, k);
+        double coef_v = A.inv_re * 0.25 *
+                        (F.rhobar[g.h(i, j, g.k0)] + F.rhobar[g.h(i + 1, j, g.k0)] + F.rhobar[g.h(i, j + 1, g.k0)] +
+                         F.rhobar[g.h(i + 1, j + 1, g.k0)]);
+        val -= coef_v * 0.25 * (F.jac[c] * S12[c] + F.jac[r] * S12[r] + F.jac[f] * S12[f] + F.jac[rf] * S12[rf]);
+      }
+    } else {
+      double wsurf = 0.5 * (pe_z(F, g, i, j, k) * A.w0[c] + pe_z(F, g, i + 1, j, k) * A.w0[g.h(i + 1, j, k)]);
+      val = upwind(wsurf, right_of_c, left_of_next);
+      if (A.viscous) {
+        double coef_v = A.inv_re * 0.5 * (F.rhobar[g.h(i, j, g.k0)] + F.rhobar[g.h(i + 1, j, g.k0)]);
+        auto s13 = [&](size_t q) { return F.m13[q] * S11[q] + F.m23[q] * S12[q] + S13[q] / F.jac[q]; };
+        const size_t r = g.h(i + 1, j, k), u = g.h(i, j, k + 1), ru = g.h(i + 1, j, k + 1);
+        double t0 = s13(c), tr = s13(r), tu = s13(u), tru = s13(ru);
+        val -= coef_v * 0.5 *
+               (F.jac[c] * F.jac[u] * (t0 + tu) / (F.jac[c] + F.jac[u]) +
+                F.jac[r] * F.jac[ru] * (tr + tru) / (F.jac[r] + F.jac[ru]));
+      }
+    }
+  } else if (VAR == V_V) {
+    const double* S12 = A.st[1];
+    const double* S22 = A.st[3];
+    const double* S23 = A.st[4];
+    if (DIR == 0) {
+      double usurf = 0.5 * (pe_x(F, g, i, j, k) * A.u0[c] + pe_x(F, g, i, j + 1, k) * A.u0[g.h(i, j + 1, k)]);
+      val = upwind(usurf, right_of_c, left_of_next);
+      if (A.viscous) {
+        const size_t r = g.h(i + 1, j, k), f = g.h(i, j + 1, k), rf = g.h(i + 1, j + 1, k);
+        double coef_v = A.inv_re * 0.25 *
+                        (F.rhobar[g.h(i, j, g.k0)] + F.rhobar[g.h(i + 1, j, g.k0)] + F.rhobar[g.h(i, j + 1, g.k0)] +
+                         F.rhobar[g.h(i + 1, j + 1, g.k0)]);
+        val -= coef_v * 0.25 * (F.jac[c] * S12[c] + F.jac[r] * S12[r] + F.jac[f] * S12[f] + F.jac[rf] * S12[rf]);
+      }
+    } else if (DIR == 1) {
+      double vsurf = 0.5 * (pe_y(F, g, i, j, k) * A.v0[c] + pe_y(F, g, i, j + 1, k) * A.v0[g.h(i, j + 1, k)]);
+      val = upwind(vsurf, right_of_c, left_of_next);
+      if (A.viscous) {
+        double coef_v = A.inv_re * F.rhobar[g.h(i, j + 1, g.k0)];
+        val -= coef_v * F.jac[g.h(i, j + 1, k)] * S22[g.h(i, j + 1, k)];
+      }
+    } else {
+      double wsurf = 0.5 * (pe_z(F, g, i, j, k) * A.w0[c] + pe_z(F, g, i, j + 1, k) * A.w0[g.h(i, j + 1, k)]);
+      val = upwind(wsurf, right_of_c, left_of_next);
+      if (A.viscous) {
+        double coef_v = A.inv_re * 0.5 * (F.rhobar[g.h(i, j, g.k0)] + F.rhobar[g.h(i, j + 1, g.k0)]);
+        auto s23 = [&](size_t q) { return F.m13[q] * S12[q] + F.m23[q] * S22[q] + S23[q] / F.jac[q]; };
+        const size_t f = g.h(i, j + 1, k), u = g.h(i, j, k + 1), fu = g.h(i, j + 1, k + 1);
+        double t0 = s23(c), tf = s23(f), tu = s23(u), tfu = s23(fu);
+        val -= coef_v * 0.5 *
+               (F.jac[c] * F.jac[u] * (t0 + tu) / (F.jac[c] + F.jac[u]) +
+                F.jac[f] * F.jac[fu] * (tf + tfu) / (F.jac[f] + F.jac[fu]));
+      }
+    }
+  } else {  // W
+    const double* S13 = A.st[2];
+    const double* S23 = A.st[4];
+    const double* S33 = A.st[5];
+    if (DIR == 0) {
+      const size_t r = g.h(i + 1, j, k), u = g.h(i, j, k + 1), ru = g.h(i + 1, j, k + 1);
+      double pedger = 0.5 * (F.jac[c] * F.pbar[c] + F.jac[r] * F.pbar[r]);
+      double puedger = 0.5 * (F.jac[u] * F.pbar[u] + F.jac[ru] * F.pbar[ru]);
+      double usurf = ((F.jac[u] + F.jac[ru]) * pedger * A.u0[c] + (F.jac[c] + F.jac[r]) * puedger * A.u0[u]) /
+                     (F.jac[c] + F.jac[r] + F.jac[u] + F.jac[ru]);
+      val = upwind(usurf, right_of_c, left_of_next);
+      if (A.viscous) {
+        double coef_v = A.inv_re * 0.5 * (F.rhobar[g.h(i, j, g.k0)] + F.rhobar[g.h(i + 1, j, g.k0)]);
+        val -= coef_v * 0.5 *
+               (F.jac[c] * F.jac[u] * (S13[c] + S13[u]) / (F.jac[c] + F.jac[u]) +
+                F.jac[r] * F.jac[ru] * (S13[r] + S13[ru]) / (F.jac[r] + F.jac[ru]));
+      }
+    } else if (DIR == 1) {
+      const size_t f = g.h(i, j + 1, k), u = g.h(i, j, k + 1), fu = g.h(i, j + 1, k + 1);
+      double pedgef = 0.5 * (F.jac[c] * F.pbar[c] + F.jac[f] * F.pbar[f]);
+      double puedgef = 0.5 * (F.jac[u] * F.pbar[u] + F.jac[fu] * F.pbar[fu]);
+      double vsurf = ((F.jac[u] + F.jac[fu]) * pedgef * A.v0[c] + (F.jac[c] + F.jac[f]) * puedgef * A.v0[u]) /
+                     (F.jac[c] + F.jac[f] + F.jac[u] + F.jac[fu]);
+      val = upwind(vsurf, right_of_c, left_of_next);
+      if (A.viscous) {
+        // compute_fluxes!.jl:1351-1367 takes the (3, 1) component on this face
+        double coef_v = A.inv_re * 0.5 * (F.rhobar[g.h(i, j, g.k0)] + F.rhobar[g.h(i, j + 1, g.k0)]);
+        val -= coef_v * 0.5 *
+               (F.jac[c] * F.jac[u] * (S13[c] + S13[u]) / (F.jac[c] + F.jac[u]) +
+                F.jac[f] * F.jac[fu] * (S13[f] + S13[fu]) / (F.jac[f] + F.jac[fu]));
+      }
+    } else {
+      double wsurf = 0.5 * (pe_z(F, g, i, j, k) * A.w0[c] + pe_z(F, g, i, j, k + 1) * A.w0[g.h(i, j, k + 1)]);
+      val = upwind(wsurf, right_of_c, left_of_next);
+      if (A.viscous) {
+        const size_t u = g.h(i, j, k + 1);
+        double coef_v = A.inv_re * F.rhobar[g.h(i, j, g.k0)];
+        val -= coef_v * (F.jac[u] * F.m13[u] * S13[u] + F.jac[u] * F.m23[u] * S23[u] + S33[u]);
+      }
+    }
+  }
+  A.flux[VAR][DIR][c] = val;
+}
+
+// ---------------------------------------------------------------------------
+// RK3 updates of the left-hand side (update!.jl:505-1192)
+// ---------------------------------------------------------------------------
+struct UpdArgs {
+  Fields F;
+  Geo g;
+  Phys ph;
+  double* q;            // updated predictand
+  double* dq;           // increment
+  const double* f[3];   // flux of q
+  const double* fu[3];  // flux of u (W update)
+  const double* fv[3];  // flux of v (W update)
+  const double* rhoold;
+  const double* uold;
+  const double* vold;
+  double dt, am, bm;
+  int first;
+};
+
+__device__ __forceinline__ double divergence(const double* const* f, const Geo& g, int i, int j, int k) {
+  const size_t c = g.h(i, j, k);
+  return (f[0][c] - f[0][g.h(i - 1, j, k)]) / g.dx + (f[1][c] - f[1][g.h(i, j - 1, k)]) / g.dy +
+         (f[2][c] - f[2][g.h(i, j, k - 1)]) / g.dz;
+}
+
+__global__ void k_update_scalar(UpdArgs A, Range3 R) {  // update!.jl:521-597
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const size_t c = A.g.h(i, j, k);
+  double fluxdiff = divergence(A.f, A.g, i, j, k);
+  fluxdiff /= A.F.jac[c];
+  double fr = -fluxdiff;
+  double dqv = A.first ? 0.0 : A.dq[c];
+  dqv = A.dt * fr + A.am * dqv;
+  A.dq[c] = dqv;
+  A.q[c] += A.bm * dqv;
+}
+
+__global__ void k_update_u(UpdArgs A, Range3 R) {  // update!.jl:730-806
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const Geo& g = A.g;
+  const Fields& F = A.F;
+  const size_t c = g.h(i, j, k), r = g.h(i + 1, j, k);
+  double fluxdiff = divergence(A.f, g, i, j, k);
+  double jacedger = 0.5 * (F.jac[c] + F.jac[r]);
+  fluxdiff /= jacedger;
+  double vc = 0.5 * (F.v[c] + F.v[g.h(i, j - 1, k)]);
+  double vr = 0.5 * (F.v[r] + F.v[g.h(i + 1, j - 1, k)]);
+  double volforce = 0.5 * A.ph.fc * ((A.rhoold[c] + F.rhobar[c]) * vc + (A.rhoold[r] + F.rhobar[r]) * vr);
+  double force = -fluxdiff + volforce;
+  double rhom_1 = 0.5 * (A.rhoold[c] + A.rhoold[r]);
+  double rhom = 0.5 * (F.rho[c] + F.rho[r]);
+  double rhobaredger = 0.5 * (F.rhobar[c] + F.rhobar[r]);
+  rhom_1 += rhobaredger;
+  rhom += rhobaredger;
+  double um_1 = A.q[c];
+  double momm_1 = rhom_1 * um_1;
+  double d = A.first ? 0.0 : A.dq[c];
+  d = A.dt * force + A.am * d;
+  A.dq[c] = d;
+  double momm = momm_1 + A.bm * d;
+  A.q[c] = momm / rhom;
+}
+
+__global__ void k_update_v(UpdArgs A, Range3 R) {  // update!.jl:884-957
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const Geo& g = A.g;
+  const Fields& F = A.F;
+  const size_t c = g.h(i, j, k), f = g.h(i, j + 1, k);
+  double fluxdiff = divergence(A.f, g, i, j, k);
+  double jacedgef = 0.5 * (F.jac[c] + F.jac[f]);
+  fluxdiff /= jacedgef;
+  double uc = 0.5 * (A.uold[c] + A.uold[g.h(i - 1, j, k)]);
+  double uf = 0.5 * (A.uold[f] + A.uold[g.h(i - 1, j + 1, k)]);
+  double volforce = -0.5 * A.ph.fc * ((A.rhoold[c] + F.rhobar[c]) * uc + (A.rhoold[f] + F.rhobar[f]) * uf);
+  double force = -fluxdiff + volforce;
+  double rhom_1 = 0.5 * (A.rhoold[c] + A.rhoold[f]);
+  double rhom = 0.5 * (F.rho[c] + F.rho[f]);
+  double rhobaredgef = 0.5 * (F.rhobar[c] + F.rhobar[f]);
+  rhom_1 += rhobaredgef;
+  rhom += rhobaredgef;
+  double vm_1 = A.q[c];
+  double momm_1 = rhom_1 * vm_1;
+  double d = A.first ? 0.0 : A.dq[c];
+  d = A.dt * force + A.am * d;
+  A.dq[c] = d;
+  double momm = momm_1 + A.bm * d;
+  A.q[c] = momm / rhom;
+}
+
+__global__ void k_update_w(UpdArgs A, Range3 R) {  // update!.jl:1035-1192
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const Geo& g = A.g;
+  const Fields& F = A.F;
+  const size_t c = g.h(i, j, k), up = g.h(i, j, k + 1);
+  double fluxdiff = divergence(A.f, g, i, j, k);
+  const double jc = F.jac[c], ju = F.jac[up];
+  double jacedgeu = 2.0 * jc * ju / (jc + ju);
+  fluxdiff /= jacedgeu;
+  double fdu[2][2], fdv[2][2];
+  for (int ll = 0; ll <= 1; ++ll)
+    for (int mm = 0; mm <= 1; ++mm) {
+      fdu[ll][mm] = divergence(A.fu, g, i - ll, j, k + mm);
+      double jacedger = 0.5 * (F.jac[g.h(i - ll, j, k + mm)] + F.jac[g.h(i + 1 - ll, j, k + mm)]);
+      fdu[ll][mm] /= jacedger;
+    }
+  for (int ll = 0; ll <= 1; ++ll)
+    for (int mm = 0; mm <= 1; ++mm) {
+      fdv[ll][mm] = divergence(A.fv, g, i, j - ll, k + mm);
+      double jacedgef = 0.5 * (F.jac[g.h(i, j - ll, k + mm)] + F.jac[g.h(i, j + 1 - ll, k + mm)]);
+      fdv[ll][mm] /= jacedgef;
+    }
+  fluxdiff = transform_transformed(F, g, i, j, k, fdu[0][0], fdu[0][1], fdu[1][0], fdu[1][1], fdv[0][0], fdv[0][1],
+                                   fdv[1][0], fdv[1][1], fluxdiff);
+  const double fc = A.ph.fc;
+  double vc = 0.5 * (A.vold[c] + A.vold[g.h(i, j - 1, k)]);
+  double vu = 0.5 * (A.vold[up] + A.vold[g.h(i, j - 1, k + 1)]);
+  double uc = 0.5 * (A.uold[c] + A.uold[g.h(i - 1, j, k)]);
+  double uu = 0.5 * (A.uold[up] + A.uold[g.h(i - 1, j, k + 1)]);
+  double volforce = fc * (ju * F.m13[c] * (A.rhoold[c] + F.rhobar[c]) * vc +
+                          jc * F.m13[up] * (A.rhoold[up] + F.rhobar[up]) * vu) / (jc + ju) -
+                    fc * (ju * F.m23[c] * (A.rhoold[c] + F.rhobar[c]) * uc +
+                          jc * F.m23[up] * (A.rhoold[up] + F.rhobar[up]) * uu) / (jc + ju);
+  double force = -fluxdiff + volforce;
+  double rhom_1 = (ju * A.rhoold[c] + jc * A.rhoold[up]) / (jc + ju);
+  double rhom = (ju * F.rho[c] + jc * F.rho[up]) / (jc + ju);
+  double rhobaredgeu = (ju * F.rhobar[c] + jc * F.rhobar[up]) / (jc + ju);
+  rhom_1 += rhobaredgeu;
+  rhom += rhobaredgeu;
+  double wm_1 = A.q[c];
+  double momm_1 = rhom_1 * wm_1;
+  double d = A.first ? 0.0 : A.dq[c];
+  d = A.dt * force + A.am * d;
+  A.dq[c] = d;
+  double momm = momm_1 + A.bm * d;
+  A.q[c] = momm / rhom;
+}
+
+// ---------------------------------------------------------------------------
+// RHS updates (update!.jl:600-727, 809-881, 960-1032, 1195-1342)
+// ---------------------------------------------------------------------------
+struct RhsArgs {
+  Fields F;
+  Geo g;
+  Phys ph;
+  double *rhop, *u, *v, *w;
+  const double* pip;
+  const double* betar;
+  const double* rhopold;
+  const double* wold;
+  double dt, rf;
+  int implicit;
+};
+
+__global__ void k_rhs_u(RhsArgs A, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const Geo& g = A.g;
+  const Fields& F = A.F;
+  const size_t c = g.h(i, j, k), r = g.h(i + 1, j, k);
+  double rhoedger = 0.5 * (F.rho[c] + F.rho[r]);
+  double rhobaredger = 0.5 * (F.rhobar[c] + F.rhobar[r]);
+  rhoedger += rhobaredger;
+  double gradient = pressure_gradient_u(F, A.ph, g, A.pip, i, j, k);
+  double force = 0.0;
+  if (!A.implicit) {
+    A.u[c] += A.dt * (-gradient + force / rhoedger) * 1.0;
+  } else {
+    double factor = 1.0;
+    if (A.ph.damp) factor += A.dt * 0.5 * (A.betar[c] + A.betar[r]) * A.rf;
+    A.u[c] = 1.0 / factor * (A.u[c] + A.dt * (-gradient + force / rhoedger) * 1.0);
+  }
+}
+
+__global__ void k_rhs_v(RhsArgs A, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const Geo& g = A.g;
+  const Fields& F = A.F;
+  const size_t c = g.h(i, j, k), f = g.h(i, j + 1, k);
+  double rhoedgef = 0.5 * (F.rho[c] + F.rho[f]);
+  double rhobaredgef = 0.5 * (F.rhobar[c] + F.rhobar[f]);
+  rhoedgef += rhobaredgef;
+  double gradient = pressure_gradient_v(F, A.ph, g, A.pip, i, j, k);
+  double force = 0.0;
+  if (!A.implicit) {
+    A.v[c] += A.dt * (-gradient + force / rhoedgef) * 1.0;
+  } else {
+    double factor = 1.0;
+    if (A.ph.damp) factor += A.dt * 0.5 * (A.betar[c] + A.betar[f]) * A.rf;
+    A.v[c] = 1.0 / factor * (A.v[c] + A.dt * (-gradient + force / rhoedgef) * 1.0);
+  }
+}
+
+__global__ void k_rhs_w(RhsArgs A, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const Geo& g = A.g;
+  const Fields& F = A.F;
+  const size_t c = g.h(i, j, k), up = g.h(i, j, k + 1);
+  const double jc = F.jac[c], ju = F.jac[up], dt = A.dt, gnd = A.ph.g_ndim;
+  double rhoc = F.rho[c];
+  double rhou = F.rho[up];
+  double rhoedgeu = (ju * F.rho[c] + jc * F.rho[up]) / (jc + ju);
+  rhoc += F.rhobar[c];
+  rhou += F.rhobar[up];
+  rhoedgeu += (ju * F.rhobar[c] + jc * F.rhobar[up]) / (jc + ju);
+  double gradient = pressure_gradient_w(F, A.ph, g, A.pip, i, j, k);
+  double force = 0.0;
+  if (!A.implicit) {
+    double b = -gnd * (ju * A.rhopold[c] / rhoc / jc + jc * A.rhopold[up] / rhou / ju) / (jc + ju);
+    A.w[c] += dt * (b - gradient + force / rhoedgeu) * 1.0;
+  } else {
+    const double dt2 = dt * dt;
+    double n2edgeu = (ju * F.n2[c] + jc * F.n2[up]) / (jc + ju);
+    double factor = 1.0;
+    factor += dt * (ju * A.betar[c] + jc * A.betar[up]) / (jc + ju) * A.rf;
+    double b = -gnd * (ju * A.rhop[c] / rhoc / jc + jc * A.rhop[up] / rhou / ju) / (jc + ju);
+    double uc = 0.5 * (A.u[c] / 1.0 + A.u[g.h(i - 1, j, k)] / 1.0);
+    double uu = 0.5 * (A.u[up] / 1.0 + A.u[g.h(i - 1, j, k + 1)] / 1.0);
+    double vc = 0.5 * (A.v[c] / 1.0 + A.v[g.h(i, j - 1, k)] / 1.0);
+    double vu = 0.5 * (A.v[up] / 1.0 + A.v[g.h(i, j - 1, k + 1)] / 1.0);
+    double jpedgeu = 1.0;
+    double fw = buoyancy_factor_w(F, g, i, j, k);
+    A.w[c] = 1.0 / (factor + fw * n2edgeu * dt2) *
+             (A.w[c] - dt * gradient * jpedgeu + dt * b * jpedgeu + dt * force / rhoedgeu * jpedgeu +
+              jpedgeu * fw * n2edgeu * dt2 * (ju * (F.m13[c] * uc + F.m23[c] * vc) + jc * (F.m13[up] * uu + F.m23[up] * vu)) /
+                  (jc + ju));
+  }
+}
+
+// update!(state, dt, RhoP(), RHS(), Explicit()) (update!.jl:600-629); the
+// Cartesian vertical winds come from the current u, v, w.
+__global__ void k_rhs_rhop_explicit(RhsArgs A, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const Geo& g = A.g;
+  const Fields& F = A.F;
+  const size_t c = g.h(i, j, k);
+  const double gnd = A.ph.g_ndim;
+  double jpu = 1.0, jpd = 1.0;
+  double wvrt = 0.5 * (vertical_wind(F, g, i, j, k) / jpu + vertical_wind(F, g, i, j, k - 1) / jpd);
+  double buoy = -gnd * A.rhop[c] / (F.rho[c] + F.rhobar[c]);
+  double fb = buoyancy_factor_rhop(F, g, i, j, k);
+  buoy -= A.dt * fb * F.n2[c] * wvrt;
+  A.rhop[c] = -buoy * (F.rho[c] + F.rhobar[c]) / gnd;
+}
+
+// update!(state, dt, RhoP(), RHS(), Implicit(), rf) (update!.jl:632-727)
+__global__ void k_rhs_rhop_implicit(RhsArgs A, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const Geo& g = A.g;
+  const Fields& F = A.F;
+  const size_t c = g.h(i, j, k), up = g.h(i, j, k + 1), dn = g.h(i, j, k - 1);
+  const double dt = A.dt, dt2 = dt * dt, gnd = A.ph.g_ndim;
+  double rhoc = F.rho[c] + F.rhobar[c];
+  const double jc = F.jac[c], ju = F.jac[up], jd = F.jac[dn];
+  double rhoedgeu = (ju * F.rho[c] + jc * F.rho[up]) / (jc + ju);
+  rhoedgeu += (ju * F.rhobar[c] + jc * F.rhobar[up]) / (jc + ju);
+  double rhoedged = (jd * F.rho[c] + jc * F.rho[dn]) / (jc + jd);
+  rhoedged += (jd * F.rhobar[c] + jc * F.rhobar[dn]) / (jc + jd);
+  double jpedgeu = 1.0, jpedged = 1.0;
+  double wv = 0.5 * (A.wold[c] / jpedgeu + A.wold[dn] / jpedged);
+  double lower_gradient = pressure_gradient_w(F, A.ph, g, A.pip, i, j, k - 1);
+  double lower_force = 0.0;
+  double upper_gradient = pressure_gradient_w(F, A.ph, g, A.pip, i, j, k);
+  double upper_force = 0.0;
+  if (k == g.k0) {
+    lower_gradient = 0.0;
+    lower_force = 0.0;
+  } else if (k == g.z_size + g.nbz) {
+    upper_gradient = 0.0;
+    upper_force = 0.0;
+  }
+  double gradient = 0.5 * (lower_gradient + upper_gradient);
+  double force = 0.5 * (lower_force / rhoedged + upper_force / rhoedgeu) * rhoc;
+  double factor = 1.0;
+  factor += dt * A.betar[c] * A.rf;
+  double b = -gnd * A.rhop[c] / (F.rho[c] + F.rhobar[c]);
+  double fb = buoyancy_factor_rhop(F, g, i, j, k);
+  const double n2c = F.n2[c];
+  b = 1.0 / (factor + fb * n2c * dt2) *
+      (-fb * n2c * dt * jc * (wv + dt * (-gradient + force / rhoc)) + factor * b +
+       fb * n2c * dt * jc * factor * 0.5 *
+           (F.m13[c] * (A.u[c] / 1.0 + A.u[g.h(i - 1, j, k)] / 1.0) + F.m23[c] * (A.v[c] / 1.0 + A.v[g.h(i, j - 1, k)] / 1.0)));
+  A.rhop[c] = -b * (F.rho[c] + F.rhobar[c]) / gnd;
+}
+
+// ---------------------------------------------------------------------------
+// sponges
+// ---------------------------------------------------------------------------
+// compute_sponges!.jl:26-65: betar = rhs_sponge(...) tref with the 1/dt of
+// the closure applied here; alphar does not change.
+__global__ void k_betar(const double* num, double* betar, size_t n, double dtd, double tref) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    betar[q] = num[q] / dtd * tref;
+}
+
+// apply_lhs_sponge!.jl:255-530 (relax_to_mean = false)
+__global__ void k_lhs_sponge(double* q, const double* alphar, const double* jac, const double* m13, const double* m23,
+                             Geo g, Range3 R, double dt, int var, double bgu, double bgv, double bgw_u, double bgw_v,
+                             double bgw_w, double uref) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const size_t c = g.h(i, j, k);
+  double bg, alpha;
+  if (var == PCF_VAR_RHO || var == PCF_VAR_RHOP) {
+    bg = 0.0;
+    alpha = alphar[c];
+  } else if (var == PCF_VAR_U) {
+    bg = bgu;
+    alpha = 0.5 * (alphar[c] + alphar[g.h(i + 1, j, k)]);
+  } else if (var == PCF_VAR_V) {
+    bg = bgv;
+    alpha = 0.5 * (alphar[c] + alphar[g.h(i, j + 1, k)]);
+  } else {
+    const size_t up = g.h(i, j, k + 1);
+    const double jc = jac[c], ju = jac[up];
+    bg = (jc * (m13[up] * bgw_u + m23[up] * bgw_v + bgw_w / ju) + ju * (m13[c] * bgw_u + m23[c] * bgw_v + bgw_w / jc)) /
+         (jc + ju) / uref;
+    alpha = (ju * alphar[c] + jc * alphar[up]) / (jc + ju);
+  }
+  double old = q[c];
+  double beta = 1.0 / (1.0 + alpha * dt);
+  q[c] = (1.0 - beta) * bg + beta * old;
+}
+
+// horizontal mean of one layer for relax_to_mean (sequential sum over the layer)
+__global__ void k_layer_mean(const double* f, double* mean, Geo g, double nxy_x, double nxy_y) {
+  const int k = g.k0 + blockIdx.x;
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int j = g.j0; j <= g.j1; ++j)
+    for (int i = g.i0; i <= g.i1; ++i) s += f[g.h(i, j, k)] / nxy_x / nxy_y;
+  mean[k - g.k0] = s;
+}
+
+__global__ void k_lhs_sponge_mean(double* q, const double* alphar, const double* jac, const double* mean, Geo g,
+                                  Range3 R, double dt, int var) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const size_t c = g.h(i, j, k);
+  double alpha;
+  if (var == PCF_VAR_U) alpha = 0.5 * (alphar[c] + alphar[g.h(i + 1, j, k)]);
+  else if (var == PCF_VAR_V) alpha = 0.5 * (alphar[c] + alphar[g.h(i, j + 1, k)]);
+  else {
+    const size_t up = g.h(i, j, k + 1);
+    alpha = (jac[up] * alphar[c] + jac[c] * alphar[up]) / (jac[c] + jac[up]);
+  }
+  double bg = mean[k - g.k0];
+  double old = q[c];
+  double beta = 1.0 / (1.0 + alpha * dt);
+  q[c] = (1.0 - beta) * bg + beta * old;
+}
+
+// maxima of |u|, |v|, |w| over the interior (compute_time_step.jl:71-73)
+__global__ void k_cfl(const double* u, const double* v, const double* w, Geo g, double* partials) {
+  double m[3] = {0.0, 0.0, 0.0};
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    int i = g.i0 + (int)(t % g.nx);
+    int j = g.j0 + (int)((t / g.nx) % g.ny);
+    int k = g.k0 + (int)(t / ((long long)g.nx * g.ny));
+    size_t c = g.h(i, j, k);
+    m[0] = dmax(m[0], fabs(u[c]));
+    m[1] = dmax(m[1], fabs(v[c]));
+    m[2] = dmax(m[2], fabs(w[c]));
+  }
+  __shared__ double red[3][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int d = 0; d < 3; ++d) {
+    double x = m[d];
+    for (int off = 32; off > 0; off >>= 1) x = dmax(x, __shfl_down(x, off, 64));
+    if (lane == 0) red[d][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int d = 0; d < 3; ++d) {
+      double x = red[d][0];
+      for (int q = 1; q < 4; ++q) x = dmax(x, red[d][q]);
+      partials[d * gridDim.x + blockIdx.x] = x;
+    }
+}
+
+__global__ void k_cfl_final(const double* partials, int nparts, double* out) {
+  if (threadIdx.x != 0) return;
+  for (int d = 0; d < 3; ++d) {
+    double x = 0.0;
+    for (int q = 0; q < nparts; ++q) x = dmax(x, partials[d * nparts + q]);
+    out[d] = x;
+  }
+}
+
+__global__ void k_copy_field(double* dst, const double* src, size_t n) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    dst[q] = src[q];
+}
+
+Fields fields_of(pcf_state* st) { return make_fields(st); }
+
+template <int VAR, int DIR>
+void launch_flux(pcf_state* st, const FluxArgs& A, Range3 R) {
+  if (R.ni <= 0 || R.nj <= 0 || R.nk <= 0) return;
+  hipLaunchKernelGGL((k_flux<VAR, DIR>), dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+}
+
+}  // namespace
+
+// reconstruct! + set_boundaries!(Reconstructions) + compute_fluxes!(state, p0)
+// + set_boundaries!(Fluxes) (explicit_integration!.jl:53-58)
+int compute_fluxes(pcf_state* st) {
+  const Geo& g = st->g;
+  Fields F = fields_of(st);
+  const bool pi = st->cfg.model != PCF_BOUSSINESQ;
+  const int nlev = g.nz + 2;  // k0-1 .. k1+1
+  hipLaunchKernelGGL(k_phi, dim3(blocks_for(g.nxx, 64), g.nyy, nlev), dim3(64), 0, st->stream, F, g, st->phi[0],
+                     st->phi[1], st->phi[2], st->phi[3], pi ? 1 : 0);
+  hipLaunchKernelGGL(k_wcart, dim3(blocks_for(g.nxx - 1, 64), g.nyy - 1, g.nzz - 1), dim3(64), 0, st->stream, F, g,
+                     st->wcart);
+  hipLaunchKernelGGL(k_phi_w, dim3(blocks_for(g.nxx - 1, 64), g.nyy - 1, nlev), dim3(64), 0, st->stream, F, g,
+                     st->wcart, st->phi[4]);
+  if (st->ph.viscous) {
+    Range3 R{g.i0 - 1, g.nx + 2, g.j0 - 1, g.ny + 2, g.k0, g.nz};
+    hipLaunchKernelGGL(k_stress, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, F, g, st->wcart,
+                       st->stress[0], st->stress[1], st->stress[2], st->stress[3], st->stress[4], st->stress[5], R);
+  }
+  FluxArgs A;
+  A.F = F;
+  for (int q = 0; q < 5; ++q) A.phi[q] = st->phi[q];
+  A.u0 = st->p0[2];
+  A.v0 = st->p0[3];
+  A.w0 = st->p0[4];
+  for (int q = 0; q < 6; ++q) A.st[q] = st->stress[q];
+  for (int q = 0; q < 5; ++q)
+    for (int d = 0; d < 3; ++d) A.flux[q][d] = st->flux[q][d];
+  A.g = g;
+  A.inv_re = 1 / st->ph.re;
+  A.viscous = st->ph.viscous;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, i0 = g.i0, j0 = g.j0, k0 = g.k0;
+  // faces the updates consume (update!.jl loop ranges); vertical wall faces
+  // are zeroed below
+  if (pi) {
+    launch_flux<V_RHO, 0>(st, A, Range3{i0 - 1, nx + 1, j0, ny, k0, nz});
+    launch_flux<V_RHO, 1>(st, A, Range3{i0, nx, j0 - 1, ny + 1, k0, nz});
+    launch_flux<V_RHO, 2>(st, A, Range3{i0, nx, j0, ny, k0, nz - 1});
+  }
+  launch_flux<V_RHOP, 0>(st, A, Range3{i0 - 1, nx + 1, j0, ny, k0, nz});
+  launch_flux<V_RHOP, 1>(st, A, Range3{i0, nx, j0 - 1, ny + 1, k0, nz});
+  launch_flux<V_RHOP, 2>(st, A, Range3{i0, nx, j0, ny, k0, nz - 1});
+  launch_flux<V_U, 0>(st, A, Range3{i0 - 2, nx + 2, j0, ny, k0, nz});
+  launch_flux<V_U, 1>(st, A, Range3{i0 - 1, nx + 1, j0 - 1, ny + 1, k0, nz});
+  launch_flux<V_U, 2>(st, A, Range3{i0 - 1, nx + 1, j0, ny, k0, nz - 1});
+  launch_flux<V_V, 0>(st, A, Range3{i0 - 1, nx + 1, j0 - 1, ny + 1, k0, nz});
+  launch_flux<V_V, 1>(st, A, Range3{i0, nx, j0 - 2, ny + 2, k0, nz});
+  launch_flux<V_V, 2>(st, A, Range3{i0, nx, j0 - 1, ny + 1, k0, nz - 1});
+  launch_flux<V_W, 0>(st, A, Range3{i0 - 1, nx + 1, j0, ny, k0, nz - 1});
+  launch_flux<V_W, 1>(st, A, Range3{i0, nx, j0 - 1, ny + 1, k0, nz - 1});
+  launch_flux<V_W, 2>(st, A, Range3{i0, nx, j0, ny, k0 - 1, nz});
+  PCF_HIP(hipGetLastError());
+  return set_boundaries_fluxes(st);
+}
+
+int update_lhs(pcf_state* st, double dt, int m, int var) {
+  const Geo& g = st->g;
+  const bool pi = st->cfg.model != PCF_BOUSSINESQ;
+  UpdArgs A;
+  A.F = fields_of(st);
+  A.g = g;
+  A.ph = st->ph;
+  A.dt = dt;
+  A.am = st->alphark[m - 1];
+  A.bm = st->betark[m - 1];
+  A.first = m == 1;
+  A.rhoold = st->rhoold;
+  A.uold = st->uold;
+  A.vold = st->vold;
+  for (int d = 0; d < 3; ++d) {
+    A.fu[d] = st->flux[V_U][d];
+    A.fv[d] = st->flux[V_V][d];
+  }
+  Range3 R;
+  switch (var) {
+    case PCF_VAR_RHO:
+    case PCF_VAR_RHOP: {
+      if (var == PCF_VAR_RHO && !pi) return PCF_OK;
+      const int q = var == PCF_VAR_RHO ? V_RHO : V_RHOP;
+      A.q = var == PCF_VAR_RHO ? st->rho : st->rhop;
+      A.dq = var == PCF_VAR_RHO ? st->drho : st->drhop;
+      for (int d = 0; d < 3; ++d) A.f[d] = st->flux[q][d];
+      R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
+      hipLaunchKernelGGL(k_update_scalar, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      break;
+    }
+    case PCF_VAR_U:
+      A.q = st->u;
+      A.dq = st->du;
+      for (int d = 0; d < 3; ++d) A.f[d] = st->flux[V_U][d];
+      R = Range3{g.i0 - 1, g.nx + 1, g.j0, g.ny, g.k0, g.nz};
+      hipLaunchKernelGGL(k_update_u, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      break;
+    case PCF_VAR_V:
+      A.q = st->v;
+      A.dq = st->dv;
+      for (int d = 0; d < 3; ++d) A.f[d] = st->flux[V_V][d];
+      R = Range3{g.i0, g.nx, g.j0 - 1, g.ny + 1, g.k0, g.nz};
+      hipLaunchKernelGGL(k_update_v, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      break;
+    case PCF_VAR_W:
+      A.q = st->w;
+      A.dq = st->dw;
+      for (int d = 0; d < 3; ++d) A.f[d] = st->flux[V_W][d];
+      R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz - 1};
+      if (R.nk > 0)
+        hipLaunchKernelGGL(k_update_w, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      break;
+    default:
+      return PCF_E_ARG;
+  }
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+int update_rhs(pcf_state* st, double dt, int var, bool implicit, double rf) {
+  const Geo& g = st->g;
+  RhsArgs A;
+  A.F = fields_of(st);
+  A.g = g;
+  A.ph = st->ph;
+  A.rhop = st->rhop; A.u = st->u; A.v = st->v; A.w = st->w;
+  A.pip = st->pip;
+  A.betar = st->betar;
+  A.rhopold = st->rhopold;
+  A.wold = st->wold;
+  A.dt = dt;
+  A.rf = rf;
+  A.implicit = implicit ? 1 : 0;
+  Range3 R;
+  switch (var) {
+    case PCF_VAR_U:
+      R = Range3{g.i0 - 1, g.nx + 1, g.j0, g.ny, g.k0, g.nz};
+      hipLaunchKernelGGL(k_rhs_u, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      break;
+    case PCF_VAR_V:
+      R = Range3{g.i0, g.nx, g.j0 - 1, g.ny + 1, g.k0, g.nz};
+      hipLaunchKernelGGL(k_rhs_v, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      break;
+    case PCF_VAR_W:
+      R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz - 1};
+      if (R.nk > 0) hipLaunchKernelGGL(k_rhs_w, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      break;
+    case PCF_VAR_RHOP:
+      R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
+      if (implicit)
+        hipLaunchKernelGGL(k_rhs_rhop_implicit, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      else
+        hipLaunchKernelGGL(k_rhs_rhop_explicit, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      break;
+    default:
+      return PCF_E_ARG;
+  }
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+int apply_lhs_sponge(pcf_state* st, double dt, double time, int var) {
+  (void)time;
+  const Geo& g = st->g;
+  const pcf_config& n = st->cfg;
+  if (var == PCF_VAR_PIP) return PCF_OK;
+  if (var == PCF_VAR_RHO && n.model == PCF_BOUSSINESQ) return PCF_OK;
+  double* q = var == PCF_VAR_RHO ? st->rho : var == PCF_VAR_RHOP ? st->rhop : var == PCF_VAR_U ? st->u
+            : var == PCF_VAR_V ? st->v : st->w;
+  Range3 R{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
+  const unsigned blocks = blocks_for(R.count(), 256);
+  if (n.relax_to_mean && (var == PCF_VAR_U || var == PCF_VAR_V || var == PCF_VAR_W)) {
+    hipLaunchKernelGGL(k_layer_mean, dim3(g.nz), dim3(64), 0, st->stream, q, st->pc_q, g, (double)n.x_size,
+                       (double)n.y_size);
+    hipLaunchKernelGGL(k_lhs_sponge_mean, dim3(blocks), dim3(256), 0, st->stream, q, st->alphar, st->jac, st->pc_q, g,
+                       R, dt, var);
+  } else {
+    const double uref = st->uref;
+    hipLaunchKernelGGL(k_lhs_sponge, dim3(blocks), dim3(256), 0, st->stream, q, st->alphar, st->jac, st->met13,
+                       st->met23, g, R, dt, var, (n.relaxed_u + n.relaxed_u) / uref / 2,
+                       (n.relaxed_v + n.relaxed_v) / uref / 2, n.relaxed_u, n.relaxed_v, n.relaxed_w, uref);
+  }
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+int compute_sponges(pcf_state* st, double dt, double time) {
+  (void)time;
+  size_t n = st->g.nh();
+  hipLaunchKernelGGL(k_betar, dim3(2048), dim3(256), 0, st->stream, st->betar_s, st->betar, n, dt * st->tref,
+                     st->tref);
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+int cfl_maxima(pcf_state* st, double out[3]) {
+  const int nb = 512;
+  hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st->stream, st->u, st->v, st->w, st->g, st->partials);
+  hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(64), 0, st->stream, st->partials, nb, st->cfl_dev);
+  PCF_HIP(hipGetLastError());
+  PCF_HIP(hipMemcpyAsync(st->cfl_host, st->cfl_dev, 3 * sizeof(double), hipMemcpyDeviceToHost, st->stream));
+  PCF_HIP(hipStreamSynchronize(st->stream));
+  for (int d = 0; d < 3; ++d) out[d] = st->cfl_host[d];
+  return PCF_OK;
+}
+
+int copy_field(pcf_state* st, double* dst, const double* src) {
+  hipLaunchKernelGGL(k_copy_field, dim3(2048), dim3(256), 0, st->stream, dst, src, st->g.nh());
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+}  // namespace pcf

@@ -1,0 +1,194 @@
+// Internal declarations of libpincflow_hip.so.
+//
+// Device layout: every field is one FP64 array in the reference's Julia
+// column-major order, x fastest.  Haloed fields keep the reference's three
+// halo layers in x, y and z (nxx x nyy x nzz) so every kernel runs over the
+// reference's own index ranges (i0 - 1 : i1 and so on); Poisson vectors are
+// interior-sized (nx x ny x nz).  Indices in kernels are 1-based like the
+// reference loops they implement.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pincflow_hip.h"
+
+namespace pcf {
+
+// ---------------------------------------------------------------------------
+// grid geometry passed by value to kernels
+// ---------------------------------------------------------------------------
+struct Geo {
+  int nx, ny, nz;        // interior
+  int nxx, nyy, nzz;     // haloed
+  int nbx, nby, nbz;
+  int i0, i1, j0, j1, k0, k1;
+  int z_size;            // global z size (= nz on one GPU)
+  double dx, dy, dz;
+  __host__ __device__ size_t h(int i, int j, int k) const {  // haloed, 1-based
+    return (size_t)(i - 1) + (size_t)nxx * ((size_t)(j - 1) + (size_t)nyy * (size_t)(k - 1));
+  }
+  __host__ __device__ size_t v(int i, int j, int k) const {  // interior vector, 1-based
+    return (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
+  }
+  __host__ __device__ size_t nh() const { return (size_t)nxx * nyy * nzz; }
+  __host__ __device__ size_t nv() const { return (size_t)nx * ny * nz; }
+};
+
+// Physical constants used on the device (Constants.jl:108-182).
+struct Phys {
+  double kappa, kappainv, mainv2, ma2, g_ndim, re, fc;  // fc = f tref
+  double tref, lref, uref;
+  int model;        // enum pcf_model
+  int damp;         // damp_horizontal_wind_on_rhs
+  int viscous;      // 1 / re > eps
+};
+
+// Index range of a kernel launch (1-based, inclusive start, counts).
+struct Range3 {
+  int i0, ni, j0, nj, k0, nk;
+  __host__ __device__ long long count() const { return (long long)ni * nj * nk; }
+};
+
+// ---------------------------------------------------------------------------
+// the 25 operator coefficients (order of enum pcf_vector from PCF_V_AC)
+// ---------------------------------------------------------------------------
+enum {
+  T_AC = 0, T_AL, T_AR, T_AB, T_AF, T_AD, T_AU,
+  T_ALD, T_ALU, T_ARD, T_ARU, T_ABD, T_ABU, T_AFD, T_AFU,
+  T_ADD, T_AUU, T_ALDD, T_ALUU, T_ARDD, T_ARUU, T_ABDD, T_ABUU, T_AFDD, T_AFUU,
+  T_COUNT
+};
+
+struct Tensor {
+  double* a[T_COUNT];
+};
+
+// BiCGSTAB scalars kept on the device between kernels (apply_bicgstab!.jl:24-150).
+struct SolverScalars {
+  double rr0;      // <r, r0> of the current iterate
+  double vr0;      // <v, r0>
+  double alpha;
+  double ts, tt;   // <t, s>, <t, t>
+  double omega;
+  double rr;       // <r, r>
+  double rr0_new;  // <r_new, r0>
+  double beta;
+  double res;      // RMS residual of the last update
+  double b_norm;
+  double tol;
+  double tolref;
+  double divl2, divl2_norm;
+  int32_t done;    // 1 once res / b_norm <= tol
+  int32_t niter;
+  int32_t pad[2];
+};
+
+constexpr int kReduceBlock = 256;     // threads per reduction block
+constexpr int kMaxPartials = 4096;    // partial sums per reduction
+constexpr int kMaxDots = 4;
+
+// ---------------------------------------------------------------------------
+// host-side state
+// ---------------------------------------------------------------------------
+struct HostSetup;  // setup.cpp
+
+}  // namespace pcf
+
+struct pcf_state {
+  pcf_config cfg;
+  pcf::Geo g;
+  pcf::Phys ph;
+  double tref, lref, uref, rhoref, thetaref;
+  double alphark[3], betark[3], stepfrac[3];
+  double dzcmin;
+  double deta;            // preconditioner pseudo-time step
+  hipStream_t stream = nullptr;
+  int device = 0;
+
+  // predictands and copies (haloed)
+  double *rho, *rhop, *u, *v, *w, *pip;
+  double *p0[6], *p1[6];  // rho, rhop, u, v, w, pip
+  double *drho, *drhop, *du, *dv, *dw;
+  double *rhoold, *rhopold, *uold, *vold, *wold;
+  // background and grid (haloed, static)
+  double *pbar, *thetabar, *rhobar, *n2, *jac, *met13, *met23, *met33, *zc, *zctilde, *fcscal;
+  // sponges
+  double *alphar, *betar, *betar_s;  // betar_s: dt-independent part of betar
+  // fluxes (haloed): [var][dir]
+  double* flux[5][3];                 // rho, rhop, u, v, w
+  double* phi[5];                     // reconstruction inputs
+  double* wcart;                      // Cartesian vertical wind
+  double* stress[6];                  // 11, 12, 13, 22, 23, 33
+  // corrector
+  double *dpip, *corx, *cory;
+  // Poisson (interior)
+  pcf::Tensor T;
+  double *rhs, *solution;
+  double *bp, *br0, *br, *bs, *bt, *bv, *bvpc, *bmat;
+  double *pc_rhs, *pc_q, *pc_tmp;     // preconditioner work
+  double* partials;                   // reduction partials
+  pcf::SolverScalars* sc;             // device scalars
+  pcf::SolverScalars* sc_host;        // pinned copy
+  double* cfl_host;                   // pinned CFL maxima
+  double* cfl_dev;
+
+  // time loop
+  double time = 0.0, nextoutputtime = 0.0;
+  int itime = 0;
+  double last_residual = 0.0;
+
+  std::vector<void*> allocations;
+};
+
+namespace pcf {
+
+// ---- setup.cpp ----------------------------------------------------------
+int host_setup(pcf_state* st, std::string& err);
+
+// ---- errors -------------------------------------------------------------
+void set_error(const std::string& msg);
+#define PCF_HIP(call)                                                         \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::pcf::set_error(std::string(#call) + ": " + hipGetErrorString(e_));    \
+      return PCF_E_HIP;                                                       \
+    }                                                                         \
+  } while (0)
+
+// ---- boundaries.hip -------------------------------------------------------
+int bc_xy(pcf_state* st, double* const* fields, int n, int nbx = -1, int nby = -1, int nbz = -1);
+int bc_z(pcf_state* st, double* f, int mode, bool staggered);
+int set_boundaries_predictands(pcf_state* st);
+int set_boundaries_fluxes(pcf_state* st);
+
+// ---- poisson.hip ----------------------------------------------------------
+int compute_lhs(pcf_state* st, double* tolref_out);
+int compute_operator(pcf_state* st, double dt, double rf);
+int apply_operator(pcf_state* st, const double* sin, double* ls, bool total);
+int apply_preconditioner(pcf_state* st, const double* sin, double* sout);
+int apply_bicgstab(pcf_state* st, double tolref, int* niter);
+int apply_corrector(pcf_state* st, double dt, double rf, int* niter);
+int correct(pcf_state* st, double dt, double rf);
+int dot(pcf_state* st, const double* a, const double* b, double* out);
+
+// ---- explicit.hip --------------------------------------------------------
+int compute_fluxes(pcf_state* st);
+int update_lhs(pcf_state* st, double dt, int m, int var);
+int update_rhs(pcf_state* st, double dt, int var, bool implicit, double rf);
+int apply_lhs_sponge(pcf_state* st, double dt, double time, int var);
+int compute_sponges(pcf_state* st, double dt, double time);
+int cfl_maxima(pcf_state* st, double out[3]);
+int copy_field(pcf_state* st, double* dst, const double* src);
+
+// ---- launch helpers -------------------------------------------------------
+inline unsigned blocks_for(long long n, int threads) {
+  return (unsigned)((n + threads - 1) / threads);
+}
+
+}  // namespace pcf

@@ -1,0 +1,1074 @@
+// Pressure solve on the device: src/PoissonSolver (compute_lhs!,
+// compute_operator!, apply_operator!, apply_preconditioner!, apply_bicgstab!,
+// solve_poisson!, apply_corrector!, correct!) and the global dot products of
+// src/MPIOperations/compute_global_dot_product.jl.
+//
+// Reductions are two-stage and deterministic: each block reduces its values
+// with wavefront shuffles and LDS and writes one partial per block; a
+// one-block finalize kernel sums the partials in a fixed order and leaves the
+// BiCGSTAB scalars (alpha, omega, beta, residual, convergence flag) in device
+// memory, so an iteration needs no host round trip.
+#include <cmath>
+
+#include "pcf_device.hpp"
+
+namespace pcf {
+
+namespace {
+
+constexpr int kPartialStride = 1 << 17;  // partial slots per dot product
+
+// ---------------------------------------------------------------------------
+// block reduction of up to 3 values (blockDim.x * blockDim.y == 256)
+// ---------------------------------------------------------------------------
+template <int ND>
+__device__ __forceinline__ void block_partials(double (&val)[ND], double* partials, int block) {
+  __shared__ double red[ND][kReduceBlock / 64];
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    double x = val[d];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[d][wid] = x;
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      double s = red[d][0];
+      for (int q = 1; q < kReduceBlock / 64; ++q) s += red[d][q];
+      partials[(size_t)d * kPartialStride + block] = s;
+    }
+  }
+}
+
+// Sum nparts partials of dot slot d with one block of 1024 threads, in a
+// fixed order.
+__device__ double finalize_sum(const double* partials, int d, int nparts) {
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  double s = 0.0;
+  for (int q = tid; q < nparts; q += blockDim.x) s += partials[(size_t)d * kPartialStride + q];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = s;
+  __syncthreads();
+  double r = 0.0;
+  if (tid == 0)
+    for (int q = 0; q < (int)(blockDim.x / 64); ++q) r += red[q];
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+// ---------------------------------------------------------------------------
+// compute_lhs! (compute_lhs!.jl:68-160)
+// ---------------------------------------------------------------------------
+__global__ void k_compute_lhs(Fields F, Phys ph, Geo g, double* rhs, double* partials) {
+  int i = g.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = g.j0 + blockIdx.y * blockDim.y + threadIdx.y;
+  int k = g.k0 + blockIdx.z;
+  double val[2] = {0.0, 0.0};
+  if (i <= g.i1 && j <= g.j1) {
+    size_t c = g.h(i, j, k);
+    const double *jac = F.jac, *pbar = F.pbar;
+    double fcscal = sqrt(pbar[c] * pbar[c] / F.rhobar[c]);
+    double ur = F.u[c], ul = F.u[g.h(i - 1, j, k)], vf = F.v[c], vb = F.v[g.h(i, j - 1, k)];
+    double wu = F.w[c], wd = F.w[g.h(i, j, k - 1)];
+    size_t r = g.h(i + 1, j, k), l = g.h(i - 1, j, k), f = g.h(i, j + 1, k), b = g.h(i, j - 1, k);
+    size_t up = g.h(i, j, k + 1), dn = g.h(i, j, k - 1);
+    double pedger = 0.5 * (jac[c] * pbar[c] + jac[r] * pbar[r]);
+    double pedgel = 0.5 * (jac[c] * pbar[c] + jac[l] * pbar[l]);
+    double pedgef = 0.5 * (jac[c] * pbar[c] + jac[f] * pbar[f]);
+    double pedgeb = 0.5 * (jac[c] * pbar[c] + jac[b] * pbar[b]);
+    double pedgeu = jac[c] * jac[up] * (pbar[c] + pbar[up]) / (jac[c] + jac[up]);
+    double pedged = jac[c] * jac[dn] * (pbar[c] + pbar[dn]) / (jac[c] + jac[dn]);
+    double bu = (pedger * ur - pedgel * ul) / g.dx / jac[c] * ph.ma2 * ph.kappa;
+    double bv = (pedgef * vf - pedgeb * vb) / g.dy / jac[c] * ph.ma2 * ph.kappa;
+    double bw = (pedgeu * wu - pedged * wd) / g.dz / jac[c] * ph.ma2 * ph.kappa;
+    bu /= fcscal;
+    bv /= fcscal;
+    bw /= fcscal;
+    double bb = bu + bv + bw;
+    rhs[g.v(i - g.i0 + 1, j - g.j0 + 1, k - g.k0 + 1)] = bb;
+    val[0] = bb * bb;
+    val[1] = bu * bu + bv * bv + bw * bw;
+  }
+  int block = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  block_partials<2>(val, partials, block);
+}
+
+__global__ void k_finalize_lhs(const double* partials, int nparts, SolverScalars* sc, double nxyz) {
+  double a = finalize_sum(partials, 0, nparts);
+  double b = finalize_sum(partials, 1, nparts);
+  if (threadIdx.x == 0) {
+    sc->divl2 = sqrt(a / nxyz);
+    sc->divl2_norm = sqrt(b / nxyz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// compute_operator! (compute_operator!.jl:40-1606), Boussinesq / PI
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_compute_operator(Fields F, const double* betar, const double* fcs, Geo g,
+                                                          Tensor T, double dt, double rf, int damp) {
+  int i = g.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = g.j0 + blockIdx.y * blockDim.y + threadIdx.y;
+  int k = g.k0 + blockIdx.z;
+  if (i > g.i1 || j > g.j1) return;
+  const int nbz = g.nbz, z_size = g.z_size, k0 = g.k0;
+  const bool kbot = k == k0, kbot1 = k == k0 + 1;
+  const bool ktop = k == z_size + nbz, ktop1 = k == z_size + nbz - 1;
+  const double dx = g.dx, dy = g.dy, dz = g.dz, dt2 = dt * dt;
+  const double *jac = F.jac, *pbar = F.pbar, *rhobar = F.rhobar, *rho = F.rho, *n2 = F.n2;
+  const double *m13 = F.m13, *m23 = F.m23, *m33 = F.m33;
+#define H(a, b, c) g.h(i + (a), j + (b), k + (c))
+  const size_t C = H(0, 0, 0), R = H(1, 0, 0), L = H(-1, 0, 0), Fw = H(0, 1, 0), B = H(0, -1, 0);
+  const size_t U = H(0, 0, 1), D = H(0, 0, -1);
+  const size_t RU = H(1, 0, 1), RD = H(1, 0, -1), LU = H(-1, 0, 1), LD = H(-1, 0, -1);
+  const size_t FU = H(0, 1, 1), FD = H(0, 1, -1), BU = H(0, -1, 1), BD = H(0, -1, -1);
+  const double fcscal = fcs[C];
+  const double jacinv = 1.0 / jac[C];
+  const double jc = jac[C], ju = jac[U], jd = jac[D];
+  // P coefficients (divergence)
+  const double pedgerdiv = 0.5 * (jc * pbar[C] + jac[R] * pbar[R]);
+  const double pedgeldiv = 0.5 * (jc * pbar[C] + jac[L] * pbar[L]);
+  const double pedgefdiv = 0.5 * (jc * pbar[C] + jac[Fw] * pbar[Fw]);
+  const double pedgebdiv = 0.5 * (jc * pbar[C] + jac[B] * pbar[B]);
+  const double pedgeudiv = jc * ju * (pbar[C] + pbar[U]) / (jc + ju);
+  const double pedgeddiv = jc * jd * (pbar[C] + pbar[D]) / (jc + jd);
+  // P coefficients (pressure gradient)
+  const double pedgergra = 0.5 * (pbar[C] + pbar[R]);
+  const double pedgelgra = 0.5 * (pbar[C] + pbar[L]);
+  const double pedgefgra = 0.5 * (pbar[C] + pbar[Fw]);
+  const double pedgebgra = 0.5 * (pbar[C] + pbar[B]);
+  const double pedgeugra = (ju * pbar[C] + jc * pbar[U]) / (jc + ju);
+  const double pedgedgra = (jd * pbar[C] + jc * pbar[D]) / (jc + jd);
+  const double puedgergra = 0.5 * (pbar[U] + pbar[RU]);
+  const double puedgelgra = 0.5 * (pbar[U] + pbar[LU]);
+  const double puedgefgra = 0.5 * (pbar[U] + pbar[FU]);
+  const double puedgebgra = 0.5 * (pbar[U] + pbar[BU]);
+  const double pdedgergra = 0.5 * (pbar[D] + pbar[RD]);
+  const double pdedgelgra = 0.5 * (pbar[D] + pbar[LD]);
+  const double pdedgefgra = 0.5 * (pbar[D] + pbar[FD]);
+  const double pdedgebgra = 0.5 * (pbar[D] + pbar[BD]);
+  // density coefficients
+  const double rhoedger = 0.5 * (rho[C] + rho[R]) + 0.5 * (rhobar[C] + rhobar[R]);
+  const double rhoedgel = 0.5 * (rho[C] + rho[L]) + 0.5 * (rhobar[C] + rhobar[L]);
+  const double rhoedgef = 0.5 * (rho[C] + rho[Fw]) + 0.5 * (rhobar[C] + rhobar[Fw]);
+  const double rhoedgeb = 0.5 * (rho[C] + rho[B]) + 0.5 * (rhobar[C] + rhobar[B]);
+  const double rhoedgeu = (ju * rho[C] + jc * rho[U]) / (jc + ju) + (ju * rhobar[C] + jc * rhobar[U]) / (jc + ju);
+  const double rhoedged = (jd * rho[C] + jc * rho[D]) / (jc + jd) + (jd * rhobar[C] + jc * rhobar[D]) / (jc + jd);
+  const double fwu = buoyancy_factor_w(F, g, i, j, k);
+  const double fwd = buoyancy_factor_w(F, g, i, j, k - 1);
+  const double rhouedger = 0.5 * (rho[U] + rho[RU] + rhobar[U] + rhobar[RU]);
+  const double rhouedgel = 0.5 * (rho[U] + rho[LU] + rhobar[U] + rhobar[LU]);
+  const double rhouedgef = 0.5 * (rho[U] + rho[FU] + rhobar[U] + rhobar[FU]);
+  const double rhouedgeb = 0.5 * (rho[U] + rho[BU] + rhobar[U] + rhobar[BU]);
+  const double rhodedger = 0.5 * (rho[D] + rho[RD] + rhobar[D] + rhobar[RD]);
+  const double rhodedgel = 0.5 * (rho[D] + rho[LD] + rhobar[D] + rhobar[LD]);
+  const double rhodedgef = 0.5 * (rho[D] + rho[FD] + rhobar[D] + rhobar[FD]);
+  const double rhodedgeb = 0.5 * (rho[D] + rho[BD] + rhobar[D] + rhobar[BD]);
+  const double n2edgeu = (ju * n2[C] + jc * n2[U]) / (jc + ju);
+  const double n2edged = (jd * n2[C] + jc * n2[D]) / (jc + jd);
+  // metric-tensor interpolation
+  const double met13edger = 0.5 * (m13[C] + m13[R]);
+  const double met13edgel = 0.5 * (m13[C] + m13[L]);
+  const double met23edgef = 0.5 * (m23[C] + m23[Fw]);
+  const double met23edgeb = 0.5 * (m23[C] + m23[B]);
+  const double met13edgeu = (ju * m13[C] + jc * m13[U]) / (jc + ju);
+  const double met23edgeu = (ju * m23[C] + jc * m23[U]) / (jc + ju);
+  const double met33edgeu = (ju * m33[C] + jc * m33[U]) / (jc + ju);
+  const double met13edged = (jd * m13[C] + jc * m13[D]) / (jc + jd);
+  const double met23edged = (jd * m23[C] + jc * m23[D]) / (jc + jd);
+  const double met33edged = (jd * m33[C] + jc * m33[D]) / (jc + jd);
+  const double met13uedger = 0.5 * (m13[U] + m13[RU]);
+  const double met13uedgel = 0.5 * (m13[U] + m13[LU]);
+  const double met23uedgef = 0.5 * (m23[U] + m23[FU]);
+  const double met23uedgeb = 0.5 * (m23[U] + m23[BU]);
+  const double met13dedger = 0.5 * (m13[D] + m13[RD]);
+  const double met13dedgel = 0.5 * (m13[D] + m13[LD]);
+  const double met23dedgef = 0.5 * (m23[D] + m23[FD]);
+  const double met23dedgeb = 0.5 * (m23[D] + m23[BD]);
+  // Rayleigh damping (compute_operator!.jl:283-378)
+  double facedger = 1.0, facedgel = 1.0, facedgef = 1.0, facedgeb = 1.0;
+  double facuedger = 1.0, facuedgel = 1.0, facuedgef = 1.0, facuedgeb = 1.0;
+  double facdedger = 1.0, facdedgel = 1.0, facdedgef = 1.0, facdedgeb = 1.0;
+  double facedgeu = 1.0, facedged = 1.0;
+  if (damp) {
+    facedger += dt * 0.5 * (betar[C] + betar[R]) * rf;
+    facedgel += dt * 0.5 * (betar[C] + betar[L]) * rf;
+    facedgef += dt * 0.5 * (betar[C] + betar[Fw]) * rf;
+    facedgeb += dt * 0.5 * (betar[C] + betar[B]) * rf;
+    facuedger += dt * 0.5 * (betar[U] + betar[RU]) * rf;
+    facuedgel += dt * 0.5 * (betar[U] + betar[LU]) * rf;
+    facuedgef += dt * 0.5 * (betar[U] + betar[FU]) * rf;
+    facuedgeb += dt * 0.5 * (betar[U] + betar[BU]) * rf;
+    facdedger += dt * 0.5 * (betar[D] + betar[RD]) * rf;
+    facdedgel += dt * 0.5 * (betar[D] + betar[LD]) * rf;
+    facdedgef += dt * 0.5 * (betar[D] + betar[FD]) * rf;
+    facdedgeb += dt * 0.5 * (betar[D] + betar[BD]) * rf;
+  }
+  facedgeu += dt * (ju * betar[C] + jc * betar[U]) / (jc + ju) * rf;
+  facedged += dt * (jd * betar[C] + jc * betar[D]) / (jc + jd) * rf;
+  const double imphoredger = 1.0 / facedger, imphoredgel = 1.0 / facedgel;
+  const double imphoredgef = 1.0 / facedgef, imphoredgeb = 1.0 / facedgeb;
+  const double imphoruedger = 1.0 / facuedger, imphoruedgel = 1.0 / facuedgel;
+  const double imphoruedgef = 1.0 / facuedgef, imphoruedgeb = 1.0 / facuedgeb;
+  const double imphordedger = 1.0 / facdedger, imphordedgel = 1.0 / facdedgel;
+  const double imphordedgef = 1.0 / facdedgef, imphordedgeb = 1.0 / facdedgeb;
+  const double impveredgeu = 1.0 / (facedgeu + fwu * n2edgeu * dt2);
+  const double impveredged = 1.0 / (facedged + fwd * n2edged * dt2);
+  // N^2-coupled vertical terms of the horizontal gradients
+  auto Tu = [&](double met, double jfac, double imp, double rhoe) {
+    return jacinv / dz * pedgeudiv * impveredgeu * fwu * n2edgeu * dt2 * 0.5 * met * jfac / (jc + ju) * imp / rhoe;
+  };
+  auto Td = [&](double met, double jfac, double imp, double rhoe) {
+    return jacinv / dz * pedgeddiv * impveredged * fwd * n2edged * dt2 * 0.5 * met * jfac / (jc + jd) * imp / rhoe;
+  };
+  // gradient coefficients (compute_operator!.jl:380-719)
+  double gedger, gedgel, gedgef, gedgeb;
+  {
+    const double hr = jacinv / dx * pedgerdiv * imphoredger / rhoedger;
+    const double hl = -jacinv / dx * pedgeldiv * imphoredgel / rhoedgel;
+    const double hf = jacinv / dy * pedgefdiv * imphoredgef / rhoedgef;
+    const double hb = -jacinv / dy * pedgebdiv * imphoredgeb / rhoedgeb;
+    if (kbot) {
+      gedger = hr + Tu(m13[C], ju, imphoredger, rhoedger);
+      gedgel = hl + Tu(m13[C], ju, imphoredgel, rhoedgel);
+      gedgef = hf + Tu(m23[C], ju, imphoredgef, rhoedgef);
+      gedgeb = hb + Tu(m23[C], ju, imphoredgeb, rhoedgeb);
+    } else if (ktop) {
+      gedger = hr - Td(m13[C], jd, imphoredger, rhoedger);
+      gedgel = hl - Td(m13[C], jd, imphoredgel, rhoedgel);
+      gedgef = hf - Td(m23[C], jd, imphoredgef, rhoedgef);
+      gedgeb = hb - Td(m23[C], jd, imphoredgeb, rhoedgeb);
+    } else {
+      gedger = hr + Tu(m13[C], ju, imphoredger, rhoedger) - Td(m13[C], jd, imphoredger, rhoedger);
+      gedgel = hl + Tu(m13[C], ju, imphoredgel, rhoedgel) - Td(m13[C], jd, imphoredgel, rhoedgel);
+      gedgef = hf + Tu(m23[C], ju, imphoredgef, rhoedgef) - Td(m23[C], jd, imphoredgef, rhoedgef);
+      gedgeb = hb + Tu(m23[C], ju, imphoredgeb, rhoedgeb) - Td(m23[C], jd, imphoredgeb, rhoedgeb);
+    }
+  }
+  const double gedgeu = ktop ? 0.0 : jacinv / dz * pedgeudiv * impveredgeu / rhoedgeu;
+  const double gedged = kbot ? 0.0 : -jacinv / dz * pedgeddiv * impveredged / rhoedged;
+  const double guedger = ktop ? 0.0 : Tu(m13[U], jc, imphoruedger, rhouedger);
+  const double guedgel = ktop ? 0.0 : Tu(m13[U], jc, imphoruedgel, rhouedgel);
+  const double guedgef = ktop ? 0.0 : Tu(m23[U], jc, imphoruedgef, rhouedgef);
+  const double guedgeb = ktop ? 0.0 : Tu(m23[U], jc, imphoruedgeb, rhouedgeb);
+  const double gdedger = kbot ? 0.0 : -Td(m13[D], jc, imphordedger, rhodedger);
+  const double gdedgel = kbot ? 0.0 : -Td(m13[D], jc, imphordedgel, rhodedgel);
+  const double gdedgef = kbot ? 0.0 : -Td(m23[D], jc, imphordedgef, rhodedgef);
+  const double gdedgeb = kbot ? 0.0 : -Td(m23[D], jc, imphordedgeb, rhodedgeb);
+
+  // products G * P of compute_operator!.jl:721-1467
+  const double Rr = gedger * pedgergra, Ll = gedgel * pedgelgra;
+  const double Ff = gedgef * pedgefgra, Bb = gedgeb * pedgebgra;
+  const double Uu = gedgeu * pedgeugra, Dd = gedged * pedgedgra;
+  const double UR = guedger * puedgergra, UL = guedgel * puedgelgra;
+  const double UF = guedgef * puedgefgra, UB = guedgeb * puedgebgra;
+  const double DR = gdedger * pdedgergra, DL = gdedgel * pdedgelgra;
+  const double DF = gdedgef * pdedgefgra, DB = gdedgeb * pdedgebgra;
+  auto q4 = [&](double X, double m) { return X * 0.25 * m / dz; };
+  auto q1 = [&](double X, double m) { return X * m / dz; };
+  auto hx = [&](double X, double m, double d, double ja, double jb) { return X * 0.5 * m / d * ja / jb; };
+  const double jr = jac[R], jru = jac[RU], jrd = jac[RD];
+  const double jl = jac[L], jlu = jac[LU], jld = jac[LD];
+  const double jf = jac[Fw], jfu = jac[FU], jfd = jac[FD];
+  const double jb = jac[B], jbu = jac[BU], jbd = jac[BD];
+
+  double ac, ar, al, af, ab, au, ad, aru, ard, alu, ald, afu, afd, abu, abd;
+  double auu, add, aruu, ardd, aluu, aldd, afuu, afdd, abuu, abdd;
+  if (kbot) {
+    ac = -Rr * (1.0 / dx + 0.75 * met13edger / dz) + Ll * (1.0 / dx - 0.75 * met13edgel / dz) -
+         Ff * (1.0 / dy + 0.75 * met23edgef / dz) + Bb * (1.0 / dy - 0.75 * met23edgeb / dz) - Uu * met33edgeu / dz -
+         q4(UR, met13uedger) - q4(UL, met13uedgel) - q4(UF, met23uedgef) - q4(UB, met23uedgeb);
+  } else if (kbot1) {
+    ac = -Rr / dx + Ll / dx - Ff / dy + Bb / dy - Uu * met33edgeu / dz + Dd * met33edged / dz - q4(UR, met13uedger) -
+         q4(UL, met13uedgel) - q4(UF, met23uedgef) - q4(UB, met23uedgeb) + q1(DR, met13dedger) + q1(DL, met13dedgel) +
+         q1(DF, met23dedgef) + q1(DB, met23dedgeb);
+  } else if (ktop1) {
+    ac = -Rr / dx + Ll / dx - Ff / dy + Bb / dy - Uu * met33edgeu / dz + Dd * met33edged / dz - q1(UR, met13uedger) -
+         q1(UL, met13uedgel) - q1(UF, met23uedgef) - q1(UB, met23uedgeb) + q4(DR, met13dedger) + q4(DL, met13dedgel) +
+         q4(DF, met23dedgef) + q4(DB, met23dedgeb);
+  } else if (ktop) {
+    ac = -Rr * (1.0 / dx - 0.75 * met13edger / dz) + Ll * (1.0 / dx + 0.75 * met13edgel / dz) -
+         Ff * (1.0 / dy - 0.75 * met23edgef / dz) + Bb * (1.0 / dy + 0.75 * met23edgeb / dz) + Dd * met33edged / dz +
+         q4(DR, met13dedger) + q4(DL, met13dedgel) + q4(DF, met23dedgef) + q4(DB, met23dedgeb);
+  } else {
+    ac = -Rr / dx + Ll / dx - Ff / dy + Bb / dy - Uu * met33edgeu / dz + Dd * met33edged / dz - q4(UR, met13uedger) -
+         q4(UL, met13uedgel) - q4(UF, met23uedgef) - q4(UB, met23uedgeb) + q4(DR, met13dedger) + q4(DL, met13dedgel) +
+         q4(DF, met23dedgef) + q4(DB, met23dedgeb);
+  }
+  if (kbot) {
+    ar = Rr * (1.0 / dx - 0.75 * met13edger / dz) + hx(Uu, met13edgeu, dx, jru, jr + jru) - q4(UR, met13uedger);
+    al = -Ll * (1.0 / dx + 0.75 * met13edgel / dz) - hx(Uu, met13edgeu, dx, jlu, jl + jlu) - q4(UL, met13uedgel);
+    af = Ff * (1.0 / dy - 0.75 * met23edgef / dz) + hx(Uu, met23edgeu, dy, jfu, jf + jfu) - q4(UF, met23uedgef);
+    ab = -Bb * (1.0 / dy + 0.75 * met23edgeb / dz) - hx(Uu, met23edgeu, dy, jbu, jb + jbu) - q4(UB, met23uedgeb);
+  } else if (kbot1) {
+    ar = Rr / dx + hx(Uu, met13edgeu, dx, jru, jr + jru) + hx(Dd, met13edged, dx, jrd, jr + jrd) - q4(UR, met13uedger) +
+         q1(DR, met13dedger);
+    al = -Ll / dx - hx(Uu, met13edgeu, dx, jlu, jl + jlu) - hx(Dd, met13edged, dx, jld, jl + jld) - q4(UL, met13uedgel) +
+         q1(DL, met13dedgel);
+    af = Ff / dy + hx(Uu, met23edgeu, dy, jfu, jf + jfu) + hx(Dd, met23edged, dy, jfd, jf + jfd) - q4(UF, met23uedgef) +
+         q1(DF, met23dedgef);
+    ab = -Bb / dy - hx(Uu, met23edgeu, dy, jbu, jb + jbu) - hx(Dd, met23edged, dy, jbd, jb + jbd) - q4(UB, met23uedgeb) +
+         q1(DB, met23dedgeb);
+  } else if (ktop1) {
+    ar = Rr / dx + hx(Uu, met13edgeu, dx, jru, jr + jru) + hx(Dd, met13edged, dx, jrd, jr + jrd) - q1(UR, met13uedger) +
+         q4(DR, met13dedger);
+    al = -Ll / dx - hx(Uu, met13edgeu, dx, jlu, jl + jlu) - hx(Dd, met13edged, dx, jld, jl + jld) - q1(UL, met13uedgel) +
+         q4(DL, met13dedgel);
+    af = Ff / dy + hx(Uu, met23edgeu, dy, jfu, jf + jfu) + hx(Dd, met23edged, dy, jfd, jf + jfd) - q1(UF, met23uedgef) +
+         q4(DF, met23dedgef);
+    ab = -Bb / dy - hx(Uu, met23edgeu, dy, jbu, jb + jbu) - hx(Dd, met23edged, dy, jbd, jb + jbd) - q1(UB, met23uedgeb) +
+         q4(DB, met23dedgeb);
+  } else if (ktop) {
+    ar = Rr * (1.0 / dx + 0.75 * met13edger / dz) + hx(Dd, met13edged, dx, jrd, jr + jrd) + q4(DR, met13dedger);
+    al = -Ll * (1.0 / dx - 0.75 * met13edgel / dz) - hx(Dd, met13edged, dx, jld, jl + jld) + q4(DL, met13dedgel);
+    af = Ff * (1.0 / dy + 0.75 * met23edgef / dz) + hx(Dd, met23edged, dy, jfd, jf + jfd) + q4(DF, met23dedgef);
+    ab = -Bb * (1.0 / dy - 0.75 * met23edgeb / dz) - hx(Dd, met23edged, dy, jbd, jb + jbd) + q4(DB, met23dedgeb);
+  } else {
+    ar = Rr / dx + hx(Uu, met13edgeu, dx, jru, jr + jru) + hx(Dd, met13edged, dx, jrd, jr + jrd) - q4(UR, met13uedger) +
+         q4(DR, met13dedger);
+    al = -Ll / dx - hx(Uu, met13edgeu, dx, jlu, jl + jlu) - hx(Dd, met13edged, dx, jld, jl + jld) - q4(UL, met13uedgel) +
+         q4(DL, met13dedgel);
+    af = Ff / dy + hx(Uu, met23edgeu, dy, jfu, jf + jfu) + hx(Dd, met23edged, dy, jfd, jf + jfd) - q4(UF, met23uedgef) +
+         q4(DF, met23dedgef);
+    ab = -Bb / dy - hx(Uu, met23edgeu, dy, jbu, jb + jbu) - hx(Dd, met23edged, dy, jbd, jb + jbd) - q4(UB, met23uedgeb) +
+         q4(DB, met23dedgeb);
+  }
+  // A(i,j,k+1)
+  if (kbot) {
+    au = q1(Rr, met13edger) + q1(Ll, met13edgel) + q1(Ff, met23edgef) + q1(Bb, met23edgeb) + Uu * met33edgeu / dz -
+         UR / dx + UL / dx - UF / dy + UB / dy;
+  } else if (kbot1) {
+    au = q4(Rr, met13edger) + q4(Ll, met13edgel) + q4(Ff, met23edgef) + q4(Bb, met23edgeb) + Uu * met33edgeu / dz -
+         UR / dx + UL / dx - UF / dy + UB / dy - q4(DR, met13dedger) - q4(DL, met13dedgel) - q4(DF, met23dedgef) -
+         q4(DB, met23dedgeb);
+  } else if (ktop1) {
+    au = q4(Rr, met13edger) + q4(Ll, met13edgel) + q4(Ff, met23edgef) + q4(Bb, met23edgeb) + Uu * met33edgeu / dz -
+         UR * (1.0 / dx - 0.75 * met13uedger / dz) + UL * (1.0 / dx + 0.75 * met13uedgel / dz) -
+         UF * (1.0 / dy - 0.75 * met23uedgef / dz) + UB * (1.0 / dy + 0.75 * met23uedgeb / dz);
+  } else if (ktop) {
+    au = 0.0;
+  } else {
+    au = q4(Rr, met13edger) + q4(Ll, met13edgel) + q4(Ff, met23edgef) + q4(Bb, met23edgeb) + Uu * met33edgeu / dz -
+         UR / dx + UL / dx - UF / dy + UB / dy;
+  }
+  // A(i,j,k-1)
+  if (kbot) {
+    ad = 0.0;
+  } else if (kbot1) {
+    ad = -q4(Rr, met13edger) - q4(Ll, met13edgel) - q4(Ff, met23edgef) - q4(Bb, met23edgeb) - Dd * met33edged / dz -
+         DR * (1.0 / dx + 0.75 * met13dedger / dz) + DL * (1.0 / dx - 0.75 * met13dedgel / dz) -
+         DF * (1.0 / dy + 0.75 * met23dedgef / dz) + DB * (1.0 / dy - 0.75 * met23dedgeb / dz);
+  } else if (ktop1) {
+    ad = -q4(Rr, met13edger) - q4(Ll, met13edgel) - q4(Ff, met23edgef) - q4(Bb, met23edgeb) - Dd * met33edged / dz -
+         DR / dx + DL / dx - DF / dy + DB / dy + q4(UR, met13uedger) + q4(UL, met13uedgel) + q4(UF, met23uedgef) +
+         q4(UB, met23uedgeb);
+  } else if (ktop) {
+    ad = -q1(Rr, met13edger) - q1(Ll, met13edgel) - q1(Ff, met23edgef) - q1(Bb, met23edgeb) - Dd * met33edged / dz -
+         DR / dx + DL / dx - DF / dy + DB / dy;
+  } else {
+    ad = -q4(Rr, met13edger) - q4(Ll, met13edgel) - q4(Ff, met23edgef) - q4(Bb, met23edgeb) - Dd * met33edged / dz -
+         DR / dx + DL / dx - DF / dy + DB / dy;
+  }
+  // diagonal neighbours A(i+-1, j, k+-1), A(i, j+-1, k+-1)
+  if (kbot) {
+    aru = q1(Rr, met13edger) + hx(Uu, met13edgeu, dx, jr, jr + jru) + UR / dx;
+    ard = 0.0;
+    alu = q1(Ll, met13edgel) - hx(Uu, met13edgeu, dx, jl, jl + jlu) - UL / dx;
+    ald = 0.0;
+    afu = q1(Ff, met23edgef) + hx(Uu, met23edgeu, dy, jf, jf + jfu) + UF / dy;
+    afd = 0.0;
+    abu = q1(Bb, met23edgeb) - hx(Uu, met23edgeu, dy, jb, jb + jbu) - UB / dy;
+    abd = 0.0;
+  } else if (kbot1) {
+    aru = q4(Rr, met13edger) + hx(Uu, met13edgeu, dx, jr, jr + jru) + UR / dx - q4(DR, met13dedger);
+    ard = -q4(Rr, met13edger) + hx(Dd, met13edged, dx, jr, jr + jrd) + DR * (1.0 / dx - 0.75 * met13dedger / dz);
+    alu = q4(Ll, met13edgel) - hx(Uu, met13edgeu, dx, jl, jl + jlu) - UL / dx - q4(DL, met13dedgel);
+    ald = -q4(Ll, met13edgel) - hx(Dd, met13edged, dx, jl, jl + jld) - DL * (1.0 / dx + 0.75 * met13dedgel / dz);
+    afu = q4(Ff, met23edgef) + hx(Uu, met23edgeu, dy, jf, jf + jfu) + UF / dy - q4(DF, met23dedgef);
+    afd = -q4(Ff, met23edgef) + hx(Dd, met23edged, dy, jf, jf + jfd) + DF * (1.0 / dy - 0.75 * met23dedgef / dz);
+    abu = q4(Bb, met23edgeb) - hx(Uu, met23edgeu, dy, jb, jb + jbu) - UB / dy - q4(DB, met23dedgeb);
+    abd = -q4(Bb, met23edgeb) - hx(Dd, met23edged, dy, jb, jb + jbd) - DB * (1.0 / dy + 0.75 * met23dedgeb / dz);
+  } else if (ktop1) {
+    aru = q4(Rr, met13edger) + hx(Uu, met13edgeu, dx, jr, jr + jru) + UR * (1.0 / dx + 0.75 * met13uedger / dz);
+    ard = -q4(Rr, met13edger) + hx(Dd, met13edged, dx, jr, jr + jrd) + DR / dx + q4(UR, met13uedger);
+    alu = q4(Ll, met13edgel) - hx(Uu, met13edgeu, dx, jl, jl + jlu) - UL * (1.0 / dx - 0.75 * met13uedgel / dz);
+    ald = -q4(Ll, met13edgel) - hx(Dd, met13edged, dx, jl, jl + jld) - DL / dx + q4(UL, met13uedgel);
+    afu = q4(Ff, met23edgef) + hx(Uu, met23edgeu, dy, jf, jf + jfu) + UF * (1.0 / dy + 0.75 * met23uedgef / dz);
+    afd = -q4(Ff, met23edgef) + hx(Dd, met23edged, dy, jf, jf + jfd) + DF / dy + q4(UF, met23uedgef);
+    abu = q4(Bb, met23edgeb) - hx(Uu, met23edgeu, dy, jb, jb + jbu) - UB * (1.0 / dy - 0.75 * met23uedgeb / dz);
+    abd = -q4(Bb, met23edgeb) - hx(Dd, met23edged, dy, jb, jb + jbd) - DB / dy + q4(UB, met23uedgeb);
+  } else if (ktop) {
+    aru = 0.0;
+    ard = -q1(Rr, met13edger) + hx(Dd, met13edged, dx, jr, jr + jrd) + DR / dx;
+    alu = 0.0;
+    ald = -q1(Ll, met13edgel) - hx(Dd, met13edged, dx, jl, jl + jld) - DL / dx;
+    afu = 0.0;
+    afd = -q1(Ff, met23edgef) + hx(Dd, met23edged, dy, jf, jf + jfd) + DF / dy;
+    abu = 0.0;
+    abd = -q1(Bb, met23edgeb) - hx(Dd, met23edged, dy, jb, jb + jbd) - DB / dy;
+  } else {
+    aru = q4(Rr, met13edger) + hx(Uu, met13edgeu, dx, jr, jr + jru) + UR / dx;
+    ard = -q4(Rr, met13edger) + hx(Dd, met13edged, dx, jr, jr + jrd) + DR / dx;
+    alu = q4(Ll, met13edgel) - hx(Uu, met13edgeu, dx, jl, jl + jlu) - UL / dx;
+    ald = -q4(Ll, met13edgel) - hx(Dd, met13edged, dx, jl, jl + jld) - DL / dx;
+    afu = q4(Ff, met23edgef) + hx(Uu, met23edgeu, dy, jf, jf + jfu) + UF / dy;
+    afd = -q4(Ff, met23edgef) + hx(Dd, met23edged, dy, jf, jf + jfd) + DF / dy;
+    abu = q4(Bb, met23edgeb) - hx(Uu, met23edgeu, dy, jb, jb + jbu) - UB / dy;
+    abd = -q4(Bb, met23edgeb) - hx(Dd, met23edged, dy, jb, jb + jbd) - DB / dy;
+  }
+  // two levels up and down
+  const bool top2 = k >= z_size + nbz - 1, bot2 = k <= k0 + 1;
+  if (kbot) {
+    auu = -q4(Rr, met13edger) - q4(Ll, met13edgel) - q4(Ff, met23edgef) - q4(Bb, met23edgeb) + q4(UR, met13uedger) +
+          q4(UL, met13uedgel) + q4(UF, met23uedgef) + q4(UB, met23uedgeb);
+    aruu = -q4(Rr, met13edger) + q4(UR, met13uedger);
+    aluu = -q4(Ll, met13edgel) + q4(UL, met13uedgel);
+    afuu = -q4(Ff, met23edgef) + q4(UF, met23uedgef);
+    abuu = -q4(Bb, met23edgeb) + q4(UB, met23uedgeb);
+  } else if (top2) {
+    auu = 0.0; aruu = 0.0; aluu = 0.0; afuu = 0.0; abuu = 0.0;
+  } else {
+    auu = q4(UR, met13uedger) + q4(UL, met13uedgel) + q4(UF, met23uedgef) + q4(UB, met23uedgeb);
+    aruu = q4(UR, met13uedger);
+    aluu = q4(UL, met13uedgel);
+    afuu = q4(UF, met23uedgef);
+    abuu = q4(UB, met23uedgeb);
+  }
+  if (bot2) {
+    add = 0.0; ardd = 0.0; aldd = 0.0; afdd = 0.0; abdd = 0.0;
+  } else if (ktop) {
+    add = q4(Rr, met13edger) + q4(Ll, met13edgel) + q4(Ff, met23edgef) + q4(Bb, met23edgeb) - q4(DR, met13dedger) -
+          q4(DL, met13dedgel) - q4(DF, met23dedgef) - q4(DB, met23dedgeb);
+    ardd = q4(Rr, met13edger) - q4(DR, met13dedger);
+    aldd = q4(Ll, met13edgel) - q4(DL, met13dedgel);
+    afdd = q4(Ff, met23edgef) - q4(DF, met23dedgef);
+    abdd = q4(Bb, met23edgeb) - q4(DB, met23dedgeb);
+  } else {
+    add = -q4(DR, met13dedger) - q4(DL, met13dedgel) - q4(DF, met23dedgef) - q4(DB, met23dedgeb);
+    ardd = -q4(DR, met13dedger);
+    aldd = -q4(DL, met13dedgel);
+    afdd = -q4(DF, met23dedgef);
+    abdd = -q4(DB, met23dedgeb);
+  }
+  // scaling (compute_operator!.jl:1545-1570) and store
+  const size_t o = g.v(i - g.i0 + 1, j - g.j0 + 1, k - g.k0 + 1);
+  T.a[T_AC][o] = ac / (fcscal * fcscal);
+  T.a[T_AR][o] = ar / fcscal / fcs[R];
+  T.a[T_AL][o] = al / fcscal / fcs[L];
+  T.a[T_AF][o] = af / fcscal / fcs[Fw];
+  T.a[T_AB][o] = ab / fcscal / fcs[B];
+  T.a[T_AU][o] = au / fcscal / fcs[U];
+  T.a[T_AD][o] = ad / fcscal / fcs[D];
+  T.a[T_ARU][o] = aru / fcscal / fcs[RU];
+  T.a[T_ARD][o] = ard / fcscal / fcs[RD];
+  T.a[T_ALU][o] = alu / fcscal / fcs[LU];
+  T.a[T_ALD][o] = ald / fcscal / fcs[LD];
+  T.a[T_AFU][o] = afu / fcscal / fcs[FU];
+  T.a[T_AFD][o] = afd / fcscal / fcs[FD];
+  T.a[T_ABU][o] = abu / fcscal / fcs[BU];
+  T.a[T_ABD][o] = abd / fcscal / fcs[BD];
+  T.a[T_AUU][o] = auu / fcscal / fcs[H(0, 0, 2)];
+  T.a[T_ADD][o] = add / fcscal / fcs[H(0, 0, -2)];
+  T.a[T_ARUU][o] = aruu / fcscal / fcs[H(1, 0, 2)];
+  T.a[T_ARDD][o] = ardd / fcscal / fcs[H(1, 0, -2)];
+  T.a[T_ALUU][o] = aluu / fcscal / fcs[H(-1, 0, 2)];
+  T.a[T_ALDD][o] = aldd / fcscal / fcs[H(-1, 0, -2)];
+  T.a[T_AFUU][o] = afuu / fcscal / fcs[H(0, 1, 2)];
+  T.a[T_AFDD][o] = afdd / fcscal / fcs[H(0, 1, -2)];
+  T.a[T_ABUU][o] = abuu / fcscal / fcs[H(0, -1, 2)];
+  T.a[T_ABDD][o] = abdd / fcscal / fcs[H(0, -1, -2)];
+#undef H
+}
+
+// ---------------------------------------------------------------------------
+// apply_operator! (apply_operator!.jl:48-479) on interior vectors; the
+// neighbour of a boundary cell is the periodic image in x and y and zero
+// beyond the walls (the reference's halo of `s` holds exactly that).
+// ---------------------------------------------------------------------------
+struct ApplyArgs {
+  Tensor T;
+  const double* s;
+  double* ls;
+  int nx, ny, nz;
+};
+
+template <bool TOTAL, int NDOT>
+__global__ void __launch_bounds__(256) k_apply(ApplyArgs A, const double* d1, const double* d2, double* partials,
+                                               const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0-based interior
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  double val[NDOT > 0 ? NDOT : 1];
+#pragma unroll
+  for (int d = 0; d < (NDOT > 0 ? NDOT : 1); ++d) val[d] = 0.0;
+  if (i < nx && j < ny) {
+    const size_t sxy = (size_t)nx * ny;
+    const size_t o = (size_t)i + (size_t)nx * j + sxy * k;
+    const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
+    const int jp = j + 1 == ny ? 0 : j + 1, jm = j == 0 ? ny - 1 : j - 1;
+    const double* s = A.s;
+    auto S = [&](int ii, int jj, int kk) -> double {
+      return (kk < 0 || kk >= nz) ? 0.0 : s[(size_t)ii + (size_t)nx * jj + sxy * kk];
+    };
+    const double* const* T = A.T.a;
+    double v;
+    if (TOTAL) {
+      v = T[T_AL][o] * S(im, j, k) + T[T_AR][o] * S(ip, j, k) + T[T_AF][o] * S(i, jp, k) +
+          T[T_AB][o] * S(i, jm, k) + T[T_AU][o] * S(i, j, k + 1) + T[T_AD][o] * S(i, j, k - 1) +
+          T[T_AC][o] * s[o];
+    } else {
+      v = T[T_AL][o] * S(im, j, k) + T[T_AR][o] * S(ip, j, k) + T[T_AF][o] * S(i, jp, k) + T[T_AB][o] * S(i, jm, k);
+    }
+    v += T[T_ARU][o] * S(ip, j, k + 1) + T[T_ARD][o] * S(ip, j, k - 1) + T[T_ALU][o] * S(im, j, k + 1) +
+         T[T_ALD][o] * S(im, j, k - 1) + T[T_AFU][o] * S(i, jp, k + 1) + T[T_AFD][o] * S(i, jp, k - 1) +
+         T[T_ABU][o] * S(i, jm, k + 1) + T[T_ABD][o] * S(i, jm, k - 1) + T[T_AUU][o] * S(i, j, k + 2) +
+         T[T_ADD][o] * S(i, j, k - 2) + T[T_ARUU][o] * S(ip, j, k + 2) + T[T_ARDD][o] * S(ip, j, k - 2) +
+         T[T_ALUU][o] * S(im, j, k + 2) + T[T_ALDD][o] * S(im, j, k - 2) + T[T_AFUU][o] * S(i, jp, k + 2) +
+         T[T_AFDD][o] * S(i, jp, k - 2) + T[T_ABUU][o] * S(i, jm, k + 2) + T[T_ABDD][o] * S(i, jm, k - 2);
+    A.ls[o] = v;
+    if (NDOT == 1) {
+      val[0] = v * d1[o];  // <v, r0>
+    } else if (NDOT == 2) {
+      val[0] = v * d1[o];  // <t, s>
+      val[1] = v * v;      // <t, t>
+    }
+  }
+  if (NDOT > 0) {
+    int block = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    block_partials<(NDOT > 0 ? NDOT : 1)>(val, partials, block);
+  }
+  (void)d2;
+}
+
+// ---------------------------------------------------------------------------
+// apply_preconditioner! (apply_preconditioner!.jl:40-124)
+// ---------------------------------------------------------------------------
+// Line solve in z of (1 - deta L_v) s = rhs (apply_preconditioner!.jl:66-111)
+// for one column per thread.  FIRST: rhs = 0 + deta (0 - sin), the first
+// pseudo-time step from s = 0, whose horizontal apply is zero.
+template <bool FIRST>
+__global__ void k_pc_thomas(const double* ac, const double* ad, const double* au, const double* src, double* out,
+                            double* q, int nx, int ny, int nz, double deta, const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nx) return;
+  const size_t sxy = (size_t)nx * ny;
+  size_t o = (size_t)i + (size_t)nx * j;
+  double rhs = FIRST ? 0.0 + deta * (0.0 - src[o]) : src[o];
+  double denom = 1 - deta * ac[o];
+  double qk = deta * au[o] / denom;
+  double sk = rhs / denom;
+  q[o] = qk;
+  out[o] = sk;
+  for (int k = 1; k < nz; ++k) {
+    o += sxy;
+    rhs = FIRST ? 0.0 + deta * (0.0 - src[o]) : src[o];
+    const double adk = ad[o];
+    const double p = 1 / (1 - deta * ac[o] - deta * adk * qk);
+    qk = deta * au[o] * p;
+    sk = (rhs + deta * adk * sk) * p;
+    q[o] = qk;
+    out[o] = sk;
+  }
+  // downward sweep
+  for (int k = nz - 2; k >= 0; --k) {
+    const size_t oo = (size_t)i + (size_t)nx * j + sxy * k;
+    sk = out[oo] + q[oo] * sk;
+    out[oo] = sk;
+  }
+}
+
+// rhs of a later pseudo-time step: s + deta (L_h s - sin) (apply_preconditioner!.jl:62-63)
+__global__ void __launch_bounds__(256) k_pc_rhs(ApplyArgs A, const double* sin, double deta, const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  if (i >= nx || j >= ny) return;
+  const size_t sxy = (size_t)nx * ny;
+  const size_t o = (size_t)i + (size_t)nx * j + sxy * k;
+  const int ip = i + 1 == nx ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
+  const int jp = j + 1 == ny ? 0 : j + 1, jm = j == 0 ? ny - 1 : j - 1;
+  const double* s = A.s;
+  auto S = [&](int ii, int jj, int kk) -> double {
+    return (kk < 0 || kk >= nz) ? 0.0 : s[(size_t)ii + (size_t)nx * jj + sxy * kk];
+  };
+  const double* const* T = A.T.a;
+  double v = T[T_AL][o] * S(im, j, k) + T[T_AR][o] * S(ip, j, k) + T[T_AF][o] * S(i, jp, k) + T[T_AB][o] * S(i, jm, k);
+  v += T[T_ARU][o] * S(ip, j, k + 1) + T[T_ARD][o] * S(ip, j, k - 1) + T[T_ALU][o] * S(im, j, k + 1) +
+       T[T_ALD][o] * S(im, j, k - 1) + T[T_AFU][o] * S(i, jp, k + 1) + T[T_AFD][o] * S(i, jp, k - 1) +
+       T[T_ABU][o] * S(i, jm, k + 1) + T[T_ABD][o] * S(i, jm, k - 1) + T[T_AUU][o] * S(i, j, k + 2) +
+       T[T_ADD][o] * S(i, j, k - 2) + T[T_ARUU][o] * S(ip, j, k + 2) + T[T_ARDD][o] * S(ip, j, k - 2) +
+       T[T_ALUU][o] * S(im, j, k + 2) + T[T_ALDD][o] * S(im, j, k - 2) + T[T_AFUU][o] * S(i, jp, k + 2) +
+       T[T_AFDD][o] * S(i, jp, k - 2) + T[T_ABUU][o] * S(i, jm, k + 2) + T[T_ABDD][o] * S(i, jm, k - 2);
+  A.ls[o] = s[o] + deta * (v - sin[o]);
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB vector kernels (apply_bicgstab!.jl:24-150)
+// ---------------------------------------------------------------------------
+constexpr int kVecGrid = 2048;
+
+// r0 = b - A 0, p = r = r0, partial <r, r>
+__global__ void k_bicg_init(const double* b, double* x, double* r0, double* r, double* p, size_t n,
+                            double* partials) {
+  double val[1] = {0.0};
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    x[q] = 0.0;
+    double v = b[q] - 0.0;
+    r0[q] = v;
+    r[q] = v;
+    p[q] = v;
+    val[0] += v * v;
+  }
+  block_partials<1>(val, partials, blockIdx.x);
+}
+
+__global__ void k_bicg_start(const double* partials, int nparts, SolverScalars* sc, double nxyz, double tol,
+                             int have_rr0) {
+  double rr = finalize_sum(partials, 0, nparts);
+  if (threadIdx.x == 0) {
+    double res = sqrt(rr / nxyz);
+    sc->b_norm = res;
+    sc->res = res;
+    sc->tol = tol;
+    sc->niter = 0;
+    sc->rr0 = rr;  // <r0, r0> = <r, r0> for r = r0
+    sc->done = (res == 0.0 || res / res <= tol) ? 1 : 0;
+  }
+  (void)have_rr0;
+}
+
+// alpha = <r, r0> / <v, r0>, from the partials of the A p kernel
+__global__ void k_bicg_alpha(const double* partials, int nparts, SolverScalars* sc) {
+  if (sc->done) return;
+  double vr0 = finalize_sum(partials, 0, nparts);
+  if (threadIdx.x == 0) {
+    sc->vr0 = vr0;
+    sc->alpha = sc->rr0 / vr0;
+  }
+}
+
+// s = r - alpha v
+__global__ void k_bicg_s(const double* r, const double* v, double* s, size_t n, const SolverScalars* sc) {
+  if (sc->done) return;
+  const double alpha = sc->alpha;
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    s[q] = r[q] - alpha * v[q];
+}
+
+// omega = <t, s> / <t, t>
+__global__ void k_bicg_omega(const double* partials, int nparts, SolverScalars* sc) {
+  if (sc->done) return;
+  double ts = finalize_sum(partials, 0, nparts);
+  double tt = finalize_sum(partials, 1, nparts);
+  if (threadIdx.x == 0) {
+    sc->ts = ts;
+    sc->tt = tt;
+    sc->omega = ts / tt;
+  }
+}
+
+// x += alpha p + omega s; r = s - omega t; partials <r, r>, <r, r0>
+__global__ void k_bicg_xr(double* x, const double* p, const double* s, const double* t, double* r, const double* r0,
+                          size_t n, double* partials, const SolverScalars* sc) {
+  if (sc->done) return;
+  const double alpha = sc->alpha, omega = sc->omega;
+  double val[2] = {0.0, 0.0};
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    x[q] += alpha * p[q] + omega * s[q];
+    double rn = s[q] - omega * t[q];
+    r[q] = rn;
+    val[0] += rn * rn;
+    val[1] += rn * r0[q];
+  }
+  block_partials<2>(val, partials, blockIdx.x);
+}
+
+// residual, convergence and beta (apply_bicgstab!.jl:116-142)
+__global__ void k_bicg_check(const double* partials, int nparts, SolverScalars* sc, double nxyz) {
+  if (sc->done) return;
+  double rr = finalize_sum(partials, 0, nparts);
+  double rr0n = finalize_sum(partials, 1, nparts);
+  if (threadIdx.x == 0) {
+    double res = sqrt(rr / nxyz);
+    sc->rr = rr;
+    sc->res = res;
+    sc->niter = sc->niter + 1;
+    if (res / sc->b_norm <= sc->tol) {
+      sc->done = 1;
+    } else {
+      sc->beta = sc->alpha / sc->omega * rr0n / sc->rr0;
+      sc->rr0 = rr0n;
+    }
+    sc->rr0_new = rr0n;
+  }
+}
+
+// p = r + beta (p - omega v)
+__global__ void k_bicg_p(double* p, const double* r, const double* v, size_t n, const SolverScalars* sc) {
+  if (sc->done) return;
+  const double beta = sc->beta, omega = sc->omega;
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    p[q] = r[q] + beta * (p[q] - omega * v[q]);
+}
+
+__global__ void k_copy(double* dst, const double* src, size_t n) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    dst[q] = src[q];
+}
+
+// generic dot product partials
+__global__ void k_dot(const double* a, const double* b, size_t n, double* partials) {
+  double val[1] = {0.0};
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    val[0] += a[q] * b[q];
+  block_partials<1>(val, partials, blockIdx.x);
+}
+
+__global__ void k_finalize_one(const double* partials, int nparts, double* out) {
+  double s = finalize_sum(partials, 0, nparts);
+  if (threadIdx.x == 0) *out = s;
+}
+
+// solve_poisson!.jl:69-72: solution ./= sqrt(pbar^2 / rhobar); dpip = solution / dt
+__global__ void k_solution_to_dpip(double* sol, const double* fcs, double* dpip, Geo g, double dtinv) {
+  int i = g.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = g.j0 + blockIdx.y;
+  int k = g.k0 + blockIdx.z;
+  if (i > g.i1) return;
+  size_t o = g.v(i - g.i0 + 1, j - g.j0 + 1, k - g.k0 + 1), c = g.h(i, j, k);
+  double s = sol[o] / fcs[c];
+  sol[o] = s;
+  dpip[c] = dtinv * s;
+}
+
+// ---------------------------------------------------------------------------
+// correct! (correct!.jl:164-385)
+// ---------------------------------------------------------------------------
+__global__ void k_correct_u(Fields F, Phys ph, Geo g, const double* betar, const double* dpip, double* corx, double* u,
+                            Range3 R, double dt, double rf) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  double factor = 1.0;
+  if (ph.damp) factor += dt * 0.5 * (betar[g.h(i, j, k)] + betar[g.h(i + 1, j, k)]) * rf;
+  double gradient = pressure_gradient_u(F, ph, g, dpip, i, j, k);
+  size_t c = g.h(i, j, k);
+  corx[c] = dt / factor * gradient;
+  u[c] -= 1.0 * corx[c];
+}
+
+__global__ void k_correct_v(Fields F, Phys ph, Geo g, const double* betar, const double* dpip, double* cory, double* v,
+                            Range3 R, double dt, double rf) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  double factor = 1.0;
+  if (ph.damp) factor += dt * 0.5 * (betar[g.h(i, j, k)] + betar[g.h(i, j + 1, k)]) * rf;
+  double gradient = pressure_gradient_v(F, ph, g, dpip, i, j, k);
+  size_t c = g.h(i, j, k);
+  cory[c] = dt / factor * gradient;
+  v[c] -= 1.0 * cory[c];
+}
+
+__global__ void k_correct_w(Fields F, Phys ph, Geo g, const double* betar, const double* dpip, const double* corx,
+                            const double* cory, double* w, Range3 R, double dt, double rf) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const size_t c = g.h(i, j, k), up = g.h(i, j, k + 1);
+  const double jc = F.jac[c], ju = F.jac[up], dt2 = dt * dt;
+  double factor = 1.0;
+  factor += dt * (ju * betar[c] + jc * betar[up]) / (jc + ju) * rf;
+  double n2edgeu = (ju * F.n2[c] + jc * F.n2[up]) / (jc + ju);
+  double gradient = pressure_gradient_w(F, ph, g, dpip, i, j, k);
+  double jpedgeu = 1.0;
+  double fw = buoyancy_factor_w(F, g, i, j, k);
+  w[c] += -dt / (factor + fw * n2edgeu * dt2) * jpedgeu * gradient -
+          1.0 / (factor + fw * n2edgeu * dt2) * fw * n2edgeu * dt2 * jpedgeu * 0.5 *
+              (ju * (F.m13[c] * (corx[c] + corx[g.h(i - 1, j, k)]) + F.m23[c] * (cory[c] + cory[g.h(i, j - 1, k)])) +
+               jc * (F.m13[up] * (corx[up] + corx[g.h(i - 1, j, k + 1)]) +
+                     F.m23[up] * (cory[up] + cory[g.h(i, j - 1, k + 1)]))) /
+              (jc + ju);
+}
+
+__global__ void k_correct_rhop(Fields F, Phys ph, Geo g, const double* betar, const double* dpip, const double* corx,
+                               const double* cory, double* rhop, Range3 R, double dt, double rf) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const size_t c = g.h(i, j, k);
+  const double dt2 = dt * dt;
+  double factor = 1.0;
+  factor += dt * betar[c] * rf;
+  double lower_gradient = pressure_gradient_w(F, ph, g, dpip, i, j, k - 1);
+  double upper_gradient = pressure_gradient_w(F, ph, g, dpip, i, j, k);
+  if (k == g.k0) lower_gradient = 0.0;
+  else if (k == g.z_size + g.nbz) upper_gradient = 0.0;
+  double gradient = 0.5 * (lower_gradient + upper_gradient);
+  double fb = buoyancy_factor_rhop(F, g, i, j, k);
+  const double n2c = F.n2[c], jc = F.jac[c];
+  double db = -1.0 / (factor + fb * n2c * dt2) *
+              (-fb * n2c * dt2 * jc * gradient +
+               fb * n2c * dt * jc * factor * 0.5 *
+                   (F.m13[c] * (corx[c] + corx[g.h(i - 1, j, k)]) + F.m23[c] * (cory[c] + cory[g.h(i, j - 1, k)])));
+  rhop[c] -= (F.rho[c] + F.rhobar[c]) / ph.g_ndim * db;
+}
+
+__global__ void k_correct_pip(double* pip, const double* dpip, Range3 R, Geo g) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  size_t c = g.h(i, j, k);
+  pip[c] += dpip[c];
+}
+
+Fields fields_of(pcf_state* st) { return make_fields(st); }
+
+
+}  // namespace
+
+
+int compute_lhs(pcf_state* st, double* tolref_out) {
+  const Geo& g = st->g;
+  dim3 blk(64, 4), grd(blocks_for(g.nx, 64), blocks_for(g.ny, 4), g.nz);
+  int nparts = grd.x * grd.y * grd.z;
+  if (nparts > kPartialStride) return PCF_E_ARG;
+  hipLaunchKernelGGL(k_compute_lhs, grd, blk, 0, st->stream, fields_of(st), st->ph, g, st->rhs, st->partials);
+  hipLaunchKernelGGL(k_finalize_lhs, dim3(1), dim3(1024), 0, st->stream, st->partials, nparts, st->sc,
+                     (double)st->cfg.x_size * st->cfg.y_size * st->cfg.z_size);
+  PCF_HIP(hipGetLastError());
+  PCF_HIP(hipMemcpyAsync(st->sc_host, st->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, st->stream));
+  PCF_HIP(hipStreamSynchronize(st->stream));
+  // compute_lhs!.jl:143-157 (the two global sums are RMS over x_size y_size z_size)
+  const double divl2 = st->sc_host->divl2, divl2_norm = st->sc_host->divl2_norm;
+  double tolref;
+  if (divl2_norm != 0.0) {
+    tolref = divl2 / divl2_norm;
+  } else if (divl2 == 0.0) {
+    tolref = 1.0;
+  } else {
+    set_error("Error in compute_rhs: divl2_norm = 0 while divl2 != 0!");
+    return PCF_E_DIVNORM;
+  }
+  *tolref_out = tolref;
+  return PCF_OK;
+}
+
+int compute_operator(pcf_state* st, double dt, double rf) {
+  const Geo& g = st->g;
+  dim3 blk(64, 4), grd(blocks_for(g.nx, 64), blocks_for(g.ny, 4), g.nz);
+  hipLaunchKernelGGL(k_compute_operator, grd, blk, 0, st->stream, fields_of(st), st->betar, st->fcscal, g, st->T, dt,
+                     rf, st->ph.damp);
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+static ApplyArgs apply_args(pcf_state* st, const double* s, double* ls) {
+  ApplyArgs A;
+  A.T = st->T;
+  A.s = s;
+  A.ls = ls;
+  A.nx = st->g.nx;
+  A.ny = st->g.ny;
+  A.nz = st->g.nz;
+  return A;
+}
+
+static dim3 apply_grid(const Geo& g) { return dim3(blocks_for(g.nx, 64), blocks_for(g.ny, 4), g.nz); }
+
+int apply_operator(pcf_state* st, const double* sin, double* ls, bool total) {
+  dim3 grd = apply_grid(st->g);
+  if (total)
+    hipLaunchKernelGGL((k_apply<true, 0>), grd, dim3(64, 4), 0, st->stream, apply_args(st, sin, ls), nullptr, nullptr,
+                       nullptr, nullptr);
+  else
+    hipLaunchKernelGGL((k_apply<false, 0>), grd, dim3(64, 4), 0, st->stream, apply_args(st, sin, ls), nullptr, nullptr,
+                       nullptr, nullptr);
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+// M^-1 sin -> sout with preconditioner_iterations pseudo-time steps.  sout
+// must not alias sin.  `sc` (may be null) lets a converged solve skip it.
+static int precondition(pcf_state* st, const double* sin, double* sout, const SolverScalars* sc) {
+  const Geo& g = st->g;
+  dim3 tgrd(blocks_for(g.nx, 64), g.ny), tblk(64);
+  const double* ac = st->T.a[T_AC];
+  const double* ad = st->T.a[T_AD];
+  const double* au = st->T.a[T_AU];
+  const int iters = st->cfg.preconditioner_iterations;
+  // pseudo-time step 1 from s = 0
+  double* cur = iters == 1 ? sout : st->pc_tmp;
+  hipLaunchKernelGGL(k_pc_thomas<true>, tgrd, tblk, 0, st->stream, ac, ad, au, sin, cur, st->pc_q, g.nx, g.ny, g.nz,
+                     st->deta, sc);
+  for (int it = 2; it <= iters; ++it) {
+    hipLaunchKernelGGL(k_pc_rhs, apply_grid(g), dim3(64, 4), 0, st->stream, apply_args(st, cur, st->pc_rhs), sin,
+                       st->deta, sc);
+    double* next = it == iters ? sout : (cur == st->pc_tmp ? sout : st->pc_tmp);
+    hipLaunchKernelGGL(k_pc_thomas<false>, tgrd, tblk, 0, st->stream, ac, ad, au, st->pc_rhs, next, st->pc_q, g.nx,
+                       g.ny, g.nz, st->deta, sc);
+    cur = next;
+  }
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+int apply_preconditioner(pcf_state* st, const double* sin, double* sout) { return precondition(st, sin, sout, nullptr); }
+
+int dot(pcf_state* st, const double* a, const double* b, double* out) {
+  size_t n = st->g.nv();
+  hipLaunchKernelGGL(k_dot, dim3(kVecGrid), dim3(kReduceBlock), 0, st->stream, a, b, n, st->partials);
+  hipLaunchKernelGGL(k_finalize_one, dim3(1), dim3(1024), 0, st->stream, st->partials, kVecGrid, st->cfl_dev);
+  PCF_HIP(hipGetLastError());
+  PCF_HIP(hipMemcpyAsync(st->cfl_host, st->cfl_dev, sizeof(double), hipMemcpyDeviceToHost, st->stream));
+  PCF_HIP(hipStreamSynchronize(st->stream));
+  *out = st->cfl_host[0];
+  return PCF_OK;
+}
+
+// apply_bicgstab!.jl:24-150.  The host enqueues iterations and looks at the
+// convergence flag of iteration j - 1 while iteration j is queued; kernels of
+// iterations past convergence see the flag and return at once.
+int apply_bicgstab(pcf_state* st, double tolref, int* niter_out) {
+  const Geo& g = st->g;
+  const pcf_config& n = st->cfg;
+  const size_t N = g.nv();
+  const double nxyz = (double)n.x_size * n.y_size * n.z_size;
+  const double tol = n.tolerance_is_relative ? n.tolerance : n.tolerance / tolref;
+  SolverScalars* sc = st->sc;
+  hipLaunchKernelGGL(k_bicg_init, dim3(kVecGrid), dim3(kReduceBlock), 0, st->stream, st->rhs, st->solution, st->br0,
+                     st->br, st->bp, N, st->partials);
+  hipLaunchKernelGGL(k_bicg_start, dim3(1), dim3(1024), 0, st->stream, st->partials, kVecGrid, sc, nxyz, tol, 0);
+  PCF_HIP(hipGetLastError());
+  PCF_HIP(hipMemcpyAsync(st->sc_host, sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, st->stream));
+  PCF_HIP(hipStreamSynchronize(st->stream));
+  if (st->sc_host->done) {
+    st->last_residual = 0.0;
+    *niter_out = 0;
+    return PCF_OK;
+  }
+  const dim3 agrd = apply_grid(g);
+  const int aparts = agrd.x * agrd.y * agrd.z;
+  if (aparts > kPartialStride) return PCF_E_ARG;
+  const int maxit = n.poisson_iterations;
+  SolverScalars* hs = st->sc_host;  // pinned, two slots
+  hipEvent_t ev[2];
+  PCF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  PCF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int converged_at = -1, last_seen = 0;
+  int rc = PCF_OK;
+  for (int jb = 1; jb <= maxit; ++jb) {
+    // v = A M^-1 p, <v, r0>
+    if ((rc = n.preconditioner ? precondition(st, st->bp, st->bvpc, sc) : PCF_OK)) break;
+    const double* vin = n.preconditioner ? st->bvpc : st->bp;
+    hipLaunchKernelGGL((k_apply<true, 1>), agrd, dim3(64, 4), 0, st->stream, apply_args(st, vin, st->bv), st->br0,
+                       nullptr, st->partials, sc);
+    hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1024), 0, st->stream, st->partials, aparts, sc);
+    hipLaunchKernelGGL(k_bicg_s, dim3(kVecGrid), dim3(256), 0, st->stream, st->br, st->bv, st->bs, N, sc);
+    // t = A M^-1 s, <t, s>, <t, t>
+    if ((rc = n.preconditioner ? precondition(st, st->bs, st->bvpc, sc) : PCF_OK)) break;
+    const double* sin2 = n.preconditioner ? st->bvpc : st->bs;
+    hipLaunchKernelGGL((k_apply<true, 2>), agrd, dim3(64, 4), 0, st->stream, apply_args(st, sin2, st->bt), st->bs,
+                       nullptr, st->partials, sc);
+    hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1024), 0, st->stream, st->partials, aparts, sc);
+    hipLaunchKernelGGL(k_bicg_xr, dim3(kVecGrid), dim3(kReduceBlock), 0, st->stream, st->solution, st->bp, st->bs,
+                       st->bt, st->br, st->br0, N, st->partials, sc);
+    hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(1024), 0, st->stream, st->partials, kVecGrid, sc, nxyz);
+    hipLaunchKernelGGL(k_bicg_p, dim3(kVecGrid), dim3(256), 0, st->stream, st->bp, st->br, st->bv, N, sc);
+    PCF_HIP(hipGetLastError());
+    PCF_HIP(hipMemcpyAsync(hs + (jb & 1), sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, st->stream));
+    PCF_HIP(hipEventRecord(ev[jb & 1], st->stream));
+    if (jb >= 2) {
+      PCF_HIP(hipEventSynchronize(ev[(jb - 1) & 1]));
+      last_seen = jb - 1;
+      if (hs[(jb - 1) & 1].done) {
+        converged_at = hs[(jb - 1) & 1].niter;
+        break;
+      }
+    }
+  }
+  if (rc == PCF_OK && converged_at < 0) {
+    PCF_HIP(hipStreamSynchronize(st->stream));
+    int slot = (last_seen + 1) & 1;
+    // the newest copy holds the final state
+    SolverScalars fin = hs[slot];
+    for (int s = 0; s < 2; ++s)
+      if (hs[s].niter > fin.niter) fin = hs[s];
+    if (fin.done) converged_at = fin.niter;
+    else {
+      *niter_out = fin.niter;
+      st->last_residual = fin.res / fin.b_norm;
+      (void)hipEventDestroy(ev[0]);
+      (void)hipEventDestroy(ev[1]);
+      return PCF_E_NOCONV;
+    }
+  }
+  PCF_HIP(hipStreamSynchronize(st->stream));
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc) return rc;
+  SolverScalars fin = hs[0].niter >= hs[1].niter ? hs[0] : hs[1];
+  st->last_residual = fin.res / fin.b_norm;
+  *niter_out = converged_at;
+  // final x := M^-1 x (apply_bicgstab!.jl:132-135)
+  if (n.preconditioner) {
+    hipLaunchKernelGGL(k_copy, dim3(kVecGrid), dim3(256), 0, st->stream, st->bs, st->solution, N);
+    if ((rc = precondition(st, st->bs, st->solution, nullptr))) return rc;
+  }
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+// apply_corrector!.jl:38-66 with solve_poisson!.jl:39-75
+int apply_corrector(pcf_state* st, double dt, double rf, int* niter) {
+  double tolref = 0.0;
+  int rc = compute_lhs(st, &tolref);
+  if (rc) return rc;
+  if (dt == 0.0) {
+    set_error("Error in solve_poisson!: dt = 0.0!");
+    return PCF_E_DT_ZERO;
+  }
+  const double dtinv = 1.0 / dt;
+  if ((rc = compute_operator(st, dt, rf))) return rc;
+  rc = apply_bicgstab(st, tolref, niter);
+  if (rc) return rc;
+  const Geo& g = st->g;
+  hipLaunchKernelGGL(k_solution_to_dpip, dim3(blocks_for(g.nx, 64), g.ny, g.nz), dim3(64), 0, st->stream,
+                     st->solution, st->fcscal, st->dpip, g, dtinv);
+  PCF_HIP(hipGetLastError());
+  double* f[1] = {st->dpip};
+  if ((rc = bc_xy(st, f, 1))) return rc;
+  if ((rc = bc_z(st, st->dpip, +1, false))) return rc;
+  return correct(st, dt, rf);
+}
+
+int correct(pcf_state* st, double dt, double rf) {
+  const Geo& g = st->g;
+  Fields F = fields_of(st);
+  // U and V: k0:k1 (one process holds the top), i0-1:i1 resp. j0-1:j1
+  Range3 ru{g.i0 - 1, g.nx + 1, g.j0, g.ny, g.k0, g.nz};
+  Range3 rv{g.i0, g.nx, g.j0 - 1, g.ny + 1, g.k0, g.nz};
+  Range3 rw{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz - 1};
+  Range3 rr{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
+  Range3 rp{g.i0 - 1, g.nx + 2, g.j0 - 1, g.ny + 2, g.k0 - 1, g.nz + 2};
+  hipLaunchKernelGGL(k_correct_u, dim3(blocks_for(ru.count(), 256)), dim3(256), 0, st->stream, F, st->ph, g, st->betar,
+                     st->dpip, st->corx, st->u, ru, dt, rf);
+  hipLaunchKernelGGL(k_correct_v, dim3(blocks_for(rv.count(), 256)), dim3(256), 0, st->stream, F, st->ph, g, st->betar,
+                     st->dpip, st->cory, st->v, rv, dt, rf);
+  if (rw.nk > 0)
+    hipLaunchKernelGGL(k_correct_w, dim3(blocks_for(rw.count(), 256)), dim3(256), 0, st->stream, F, st->ph, g,
+                       st->betar, st->dpip, st->corx, st->cory, st->w, rw, dt, rf);
+  hipLaunchKernelGGL(k_correct_rhop, dim3(blocks_for(rr.count(), 256)), dim3(256), 0, st->stream, F, st->ph, g,
+                     st->betar, st->dpip, st->corx, st->cory, st->rhop, rr, dt, rf);
+  hipLaunchKernelGGL(k_correct_pip, dim3(blocks_for(rp.count(), 256)), dim3(256), 0, st->stream, st->pip, st->dpip,
+                     rp, g);
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+}  // namespace pcf

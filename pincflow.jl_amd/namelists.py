@@ -328,6 +328,7 @@ def to_config(nl: Namelists) -> _abi.PcfConfig:
     elif s.lhs_sponge is not None:
         raise ValueError("unsupported lhs_sponge form")
     c.rhs_sponge = _abi.RHS_SPONGE_NONE
+    c.rhs_amplitude = 1.0
     if isinstance(s.rhs_sponge, Sin2TopSponge):
         c.rhs_sponge = _abi.RHS_SPONGE_SIN2_TOP
         c.rhs_zr, c.rhs_amplitude = s.rhs_sponge.zr, s.rhs_sponge.amplitude

@@ -1,0 +1,204 @@
+"""Host handle of the device-resident model state and ``integrate``.
+
+``State(namelists)`` mirrors ``State(namelists)`` (src/Types/State.jl:106-131)
+and owns one ``pcf_state`` of libpincflow_hip.so; its methods are the module
+functions of the reference on that state.  ``integrate(namelists)`` mirrors
+src/Integration/integrate.jl:68-415 with the HDF5 output replaced by in-memory
+records of the output datasets.
+
+Errors follow the reference: a BiCGSTAB that does not converge raises
+``PoissonNotConverged`` (errflag = true), the ``error(...)`` cases raise
+``PinCFlowError`` with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import _abi
+from .namelists import Namelists, to_config
+
+
+class PinCFlowError(RuntimeError):
+    def __init__(self, rc: int, msg: str):
+        super().__init__(f"pincflow error {rc}: {msg}")
+        self.rc = rc
+
+
+class PoissonNotConverged(PinCFlowError):
+    pass
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class State:
+    """Device-resident model state (one pcf_state)."""
+
+    def __init__(self, namelists: Namelists):
+        self.namelists = namelists
+        self.lib = _abi.load_library()
+        cfg = to_config(namelists)
+        h = ctypes.c_void_p()
+        rc = self.lib.pcf_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc:
+            raise PinCFlowError(rc, self.lib.pcf_last_error().decode())
+        self.h = h
+        d = (ctypes.c_int32 * 6)()
+        self.lib.pcf_dims(self.h, d)
+        self.nx, self.ny, self.nz, self.nxx, self.nyy, self.nzz = list(d)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pcf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc: int, niter: Optional[int] = None):
+        if rc == _abi.E_NOCONV:
+            raise PoissonNotConverged(rc, f"BiCGSTAB did not converge in {niter} iterations")
+        if rc:
+            raise PinCFlowError(rc, self.lib.pcf_last_error().decode())
+
+    # ---- host copies ------------------------------------------------------
+    def get(self, name: str) -> np.ndarray:
+        a = np.empty((self.nxx, self.nyy, self.nzz), order="F")
+        self._check(self.lib.pcf_get_field(self.h, _abi.FIELD[name], _ptr(a)))
+        return a
+
+    def set(self, name: str, a: np.ndarray):
+        a = np.asfortranarray(a, dtype=np.float64)
+        self._check(self.lib.pcf_set_field(self.h, _abi.FIELD[name], _ptr(a)))
+
+    def get_vector(self, name: str) -> np.ndarray:
+        a = np.empty((self.nx, self.ny, self.nz), order="F")
+        self._check(self.lib.pcf_get_vector(self.h, _abi.VECTOR[name], _ptr(a)))
+        return a
+
+    def set_vector(self, name: str, a: np.ndarray):
+        a = np.asfortranarray(a, dtype=np.float64)
+        self._check(self.lib.pcf_set_vector(self.h, _abi.VECTOR[name], _ptr(a)))
+
+    # ---- Integration --------------------------------------------------------
+    def initial_cleaning(self) -> int:
+        n = ctypes.c_int32(0)
+        rc = self.lib.pcf_initial_cleaning(self.h, ctypes.byref(n))
+        self._check(rc, n.value)
+        return n.value
+
+    def step(self) -> _abi.PcfStepInfo:
+        info = _abi.PcfStepInfo()
+        rc = self.lib.pcf_step(self.h, ctypes.byref(info))
+        self._check(rc, max(info.niter))
+        return info
+
+    def compute_time_step(self) -> float:
+        dt = ctypes.c_double(0.0)
+        self._check(self.lib.pcf_compute_time_step(self.h, ctypes.byref(dt)))
+        return dt.value
+
+    def compute_sponges(self, dt: float, time: float):
+        self._check(self.lib.pcf_compute_sponges(self.h, dt, time))
+
+    def backup_predictands(self):
+        self._check(self.lib.pcf_backup_predictands(self.h))
+
+    def explicit_integration_lhs(self, dt: float, time: float):
+        self._check(self.lib.pcf_explicit_integration_lhs(self.h, dt, time))
+
+    def explicit_integration_rhs(self, dt: float, time: float):
+        self._check(self.lib.pcf_explicit_integration_rhs(self.h, dt, time))
+
+    def implicit_integration(self, dt: float, time: float, rayleigh_factor: float) -> int:
+        n = ctypes.c_int32(0)
+        rc = self.lib.pcf_implicit_integration(self.h, dt, time, rayleigh_factor, ctypes.byref(n))
+        self._check(rc, n.value)
+        return n.value
+
+    # ---- Boundaries / FluxCalculator / Update -------------------------------
+    def set_boundaries(self, which: int):
+        self._check(self.lib.pcf_set_boundaries(self.h, which))
+
+    def compute_fluxes(self):
+        self._check(self.lib.pcf_compute_fluxes(self.h))
+
+    def update_lhs(self, dt: float, m: int, var: int):
+        self._check(self.lib.pcf_update_lhs(self.h, dt, m, var))
+
+    def update_rhs(self, dt: float, var: int, implicit: bool, rayleigh_factor: float = 1.0):
+        self._check(self.lib.pcf_update_rhs(self.h, dt, var, int(implicit), rayleigh_factor))
+
+    def apply_lhs_sponge(self, dt: float, time: float, var: int):
+        self._check(self.lib.pcf_apply_lhs_sponge(self.h, dt, time, var))
+
+    # ---- PoissonSolver -------------------------------------------------------
+    def apply_corrector(self, dt: float, rayleigh_factor: float) -> int:
+        n = ctypes.c_int32(0)
+        rc = self.lib.pcf_apply_corrector(self.h, dt, rayleigh_factor, ctypes.byref(n))
+        self._check(rc, n.value)
+        return n.value
+
+    def compute_lhs(self) -> float:
+        t = ctypes.c_double(0.0)
+        self._check(self.lib.pcf_compute_lhs(self.h, ctypes.byref(t)))
+        return t.value
+
+    def compute_operator(self, dt: float, rayleigh_factor: float):
+        self._check(self.lib.pcf_compute_operator(self.h, dt, rayleigh_factor))
+
+    def apply_bicgstab(self, tolref: float) -> int:
+        n = ctypes.c_int32(0)
+        rc = self.lib.pcf_apply_bicgstab(self.h, tolref, ctypes.byref(n))
+        self._check(rc, n.value)
+        return n.value
+
+    def last_residual(self) -> float:
+        r = ctypes.c_double(0.0)
+        self._check(self.lib.pcf_last_residual(self.h, ctypes.byref(r)))
+        return r.value
+
+    def apply_operator_device(self, d_in: int, d_out: int, total: bool = True):
+        """apply_operator! on device buffers (interior-sized FP64, given as addresses)."""
+        self._check(self.lib.pcf_apply_operator(self.h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out),
+                                                _abi.OP_TOTAL if total else _abi.OP_HORIZONTAL))
+
+    def apply_preconditioner_device(self, d_in: int, d_out: int):
+        self._check(self.lib.pcf_apply_preconditioner(self.h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out)))
+
+    def dot_device(self, d_a: int, d_b: int) -> float:
+        r = ctypes.c_double(0.0)
+        self._check(self.lib.pcf_dot(self.h, ctypes.c_void_p(d_a), ctypes.c_void_p(d_b), ctypes.byref(r)))
+        return r.value
+
+
+def integrate(namelists: Namelists, records: bool = True, max_steps: int = 1 << 30
+              ) -> Tuple[State, List[dict], List[tuple]]:
+    """integrate(namelists) (integrate.jl:68-415) on the device.
+
+    Returns the state, the output records (datasets of write_output at t = 0
+    and at every output time) and per-step (time, dt, niter1, niter2)."""
+    from . import output
+
+    nl = namelists
+    st = State(nl)
+    recs: List[dict] = []
+    steps: List[tuple] = []
+    if nl.poisson.initial_cleaning:
+        st.initial_cleaning()
+    if records:
+        recs.append(output.record_datasets(nl, st.get, 0.0, True))
+    n = nl.output.iterations if nl.output.output_steps else max_steps
+    for itime in range(1, n + 1):
+        info = st.step()
+        steps.append((info.time, info.dt, info.niter[0], info.niter[1]))
+        if records and ((nl.output.output_steps and itime % nl.output.nout == 0) or
+                        (not nl.output.output_steps and info.output)):
+            recs.append(output.record_datasets(nl, st.get, info.time, False))
+        if not nl.output.output_steps and info.time >= nl.output.tmax:
+            break
+    return st, recs, steps

@@ -1,0 +1,259 @@
+"""Parity of the HIP library with the CPU oracle, kernel by kernel and step by step.
+
+The library evaluates the reference expressions in the reference's operation
+order with FP64 and without contraction, like the oracle; kernels without a
+reduction are expected to agree to the last bit.  Tolerances below:
+
+* per-kernel outputs (operator coefficients and applies, preconditioner,
+  fluxes, updates, corrector): relative 1e-13 of the field's max-norm;
+* anything downstream of a global reduction: the reduction order differs,
+  so BiCGSTAB iterates differ in the last bits and a solve may stop one
+  iteration earlier or later.  A solve must meet the reference criterion
+  ||b - A x|| / ||b|| <= tol, checked with the oracle's operator, and the two
+  solutions may differ only by what both criteria allow:
+  ||A (x_gpu - x_oracle)|| <= 2 tol ||b||.  Stages and whole steps are
+  compared with the Poisson tolerance tightened to 1e-12 (relative), where
+  the fields agree to 1e-9 relative.
+"""
+import hipmem
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KERNEL_RTOL = 1e-13
+SOLVE_RTOL = 1e-9
+STEP_RTOL = 1e-9
+
+
+def rel(a, b):
+    scale = max(float(np.max(np.abs(b))), 1e-300)
+    return float(np.max(np.abs(a - b))) / scale
+
+
+def sl(a, b):
+    """1-based inclusive index range -> 0-based slice."""
+    return slice(a - 1, b)
+
+
+def configs(pcf):
+    ph = pcf.examples.periodic_hill(x_size=24, z_size=20)
+    mw = pcf.examples.mountain_wave(x_size=12, y_size=10, z_size=9)
+    mw3 = pcf.examples.mountain_wave_3d(x_size=16, y_size=14, z_size=12, iterations=2)
+    cb = pcf.examples.cold_bubble(x_size=12, z_size=10)
+    return {"periodic_hill": ph, "mountain_wave": mw, "mountain_wave_3d": mw3, "cold_bubble": cb}
+
+
+CASES = ["periodic_hill", "mountain_wave", "mountain_wave_3d", "cold_bubble"]
+
+
+def pair(pcf, abi, oracle_mod, name, tight=False):
+    nl = configs(pcf)[name]
+    if tight:
+        nl.poisson.tolerance = 1e-12
+        nl.poisson.tolerance_is_relative = True
+    gpu = pcf.State(nl)
+    orc = oracle_mod.Oracle(pcf.to_config(nl), abi)
+    return nl, gpu, orc
+
+
+def perturb(gpu, orc, names, seed, amp=0.05):
+    """Identical random perturbations of haloed fields on both sides."""
+    rng = np.random.default_rng(seed)
+    for n in names:
+        a = orc.get(n)
+        scale = max(float(np.max(np.abs(a))), 1e-3)
+        a = a + amp * scale * rng.standard_normal(a.shape)
+        gpu.set(n, a)
+        orc.set(n, a)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_setup_fields(pcf, abi, oracle_mod, name):
+    _, gpu, orc = pair(pcf, abi, oracle_mod, name)
+    for f in ("PBAR", "THETABAR", "RHOBAR", "N2", "JAC", "MET13", "MET23", "MET33", "ZC", "ZCTILDE",
+              "RHO", "RHOP", "U", "V", "W", "PIP"):
+        a, b = gpu.get(f), orc.get(f)
+        assert np.array_equal(a, b), (f, float(np.max(np.abs(a - b))))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_sponges(pcf, abi, oracle_mod, name):
+    _, gpu, orc = pair(pcf, abi, oracle_mod, name)
+    for s in (gpu, orc):
+        s.compute_sponges(0.37, 5.0)
+    for f in ("ALPHAR", "BETAR"):
+        a, b = gpu.get(f), orc.get(f)
+        assert np.array_equal(a, b), (f, float(np.max(np.abs(a - b))))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_compute_lhs(pcf, abi, oracle_mod, name):
+    _, gpu, orc = pair(pcf, abi, oracle_mod, name)
+    perturb(gpu, orc, ["U", "V", "W"], seed=1)
+    for s in (gpu, orc):
+        s.set_boundaries(abi.BOUNDARY_PREDICTANDS)
+    t1, t2 = gpu.compute_lhs(), orc.compute_lhs()
+    err = rel(gpu.get_vector("RHS"), orc.get_vector("RHS"))
+    assert err <= KERNEL_RTOL, err
+    assert t1 == pytest.approx(t2, rel=1e-12)
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("dt,rf", [(1.0, 1.0), (0.37, 2.0)])
+def test_compute_operator(pcf, abi, oracle_mod, name, dt, rf):
+    _, gpu, orc = pair(pcf, abi, oracle_mod, name)
+    perturb(gpu, orc, ["RHO"], seed=2, amp=0.01)
+    for s in (gpu, orc):
+        s.compute_sponges(0.5, 3.0)
+        s.compute_operator(dt, rf)
+    for v in abi.VECTORS[2:]:
+        err = rel(gpu.get_vector(v), orc.get_vector(v))
+        assert err <= KERNEL_RTOL, (v, err)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_apply_operator_and_preconditioner(pcf, abi, oracle_mod, name):
+    _, gpu, orc = pair(pcf, abi, oracle_mod, name)
+    for s in (gpu, orc):
+        s.compute_sponges(0.5, 3.0)
+        s.compute_operator(0.8, 1.0)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((gpu.nx, gpu.ny, gpu.nz))
+    dx = hipmem.DeviceArray(x.shape).upload(x)
+    dy = hipmem.DeviceArray(x.shape)
+    for total in (True, False):
+        gpu.apply_operator_device(dx.ptr, dy.ptr, total)
+        err = rel(dy.download(), orc.apply_operator(x, total))
+        assert err <= KERNEL_RTOL, (total, err)
+    gpu.apply_preconditioner_device(dx.ptr, dy.ptr)
+    err = rel(dy.download(), orc.apply_preconditioner(x))
+    assert err <= KERNEL_RTOL, err
+    # compute_global_dot_product on the device
+    got = gpu.dot_device(dx.ptr, dx.ptr)
+    assert got == pytest.approx(float(np.sum(x * x)), rel=1e-13)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_bicgstab(pcf, abi, oracle_mod, name):
+    _, gpu, orc = pair(pcf, abi, oracle_mod, name)
+    perturb(gpu, orc, ["U", "V", "W"], seed=4)
+    for s in (gpu, orc):
+        s.set_boundaries(abi.BOUNDARY_PREDICTANDS)
+        s.compute_sponges(0.4, 1.0)
+    tolref = gpu.compute_lhs()
+    orc.compute_lhs()
+    for s in (gpu, orc):
+        s.compute_operator(0.4, 1.0)
+    n1 = gpu.apply_bicgstab(tolref)
+    rc, n2 = orc.apply_bicgstab(tolref)
+    assert rc == 0
+    assert abs(n1 - n2) <= 2 + n2 // 10, (n1, n2)
+    tol = 1e-8 / tolref
+    b = orc.get_vector("RHS")
+    xg, xo = gpu.get_vector("SOLUTION"), orc.get_vector("SOLUTION")
+    rms = lambda v: float(np.sqrt(np.mean(v * v)))
+    true_res = rms(b - orc.apply_operator(xg)) / rms(b)
+    assert true_res <= tol * (1 + 1e-6), (true_res, tol)
+    assert gpu.last_residual() == pytest.approx(true_res, rel=1e-6)
+    gap = rms(orc.apply_operator(xg - xo)) / rms(b)
+    assert gap <= 2 * tol, (gap, tol)
+    if n1 == n2:
+        err = rel(xg, xo)
+        assert err <= SOLVE_RTOL, err
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fluxes(pcf, abi, oracle_mod, name):
+    nl, gpu, orc = pair(pcf, abi, oracle_mod, name)
+    names = ["RHOP", "U", "V", "W"] + (["RHO"] if nl.atmosphere.model != "Boussinesq" else [])
+    perturb(gpu, orc, names, seed=5)
+    for s in (gpu, orc):
+        s.set_boundaries(abi.BOUNDARY_PREDICTANDS)
+        s.backup_predictands()
+    perturb(gpu, orc, ["U", "V", "W"], seed=6, amp=0.02)  # current winds differ from p0
+    for s in (gpu, orc):
+        s.set_boundaries(abi.BOUNDARY_PREDICTANDS)
+        s.compute_fluxes()
+    nb = 3
+    i0, j0, k0 = nb + 1, nb + 1, nb + 1
+    i1, j1, k1 = i0 + gpu.nx - 1, j0 + gpu.ny - 1, k0 + gpu.nz - 1
+    # faces the updates read (update!.jl loop ranges)
+    rng = {
+        "X": (sl(i0 - 1, i1), sl(j0, j1), sl(k0, k1)),
+        "Y": (sl(i0, i1), sl(j0 - 1, j1), sl(k0, k1)),
+        "Z": (sl(i0, i1), sl(j0, j1), sl(k0 - 1, k1)),
+    }
+    special = {
+        "PHIU_X": (sl(i0 - 2, i1), sl(j0, j1), sl(k0, k1)),
+        "PHIU_Y": (sl(i0 - 1, i1), sl(j0 - 1, j1), sl(k0, k1)),
+        "PHIU_Z": (sl(i0 - 1, i1), sl(j0, j1), sl(k0 - 1, k1)),
+        "PHIV_X": (sl(i0 - 1, i1), sl(j0 - 1, j1), sl(k0, k1)),
+        "PHIV_Y": (sl(i0, i1), sl(j0 - 2, j1), sl(k0, k1)),
+        "PHIV_Z": (sl(i0, i1), sl(j0 - 1, j1), sl(k0 - 1, k1)),
+        "PHIW_X": (sl(i0 - 1, i1), sl(j0, j1), sl(k0, k1 - 1)),
+        "PHIW_Y": (sl(i0, i1), sl(j0 - 1, j1), sl(k0, k1 - 1)),
+        "PHIW_Z": (sl(i0, i1), sl(j0, j1), sl(k0 - 1, k1)),
+    }
+    fl = ["PHIRHOP", "PHIU", "PHIV", "PHIW"] + (["PHIRHO"] if "RHO" in names else [])
+    for f in fl:
+        for d in "XYZ":
+            key = f"{f}_{d}"
+            r = special.get(key, rng[d])
+            a, b = gpu.get(key)[r], orc.get(key)[r]
+            err = rel(a, b)
+            assert err <= KERNEL_RTOL, (key, err)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_explicit_stages(pcf, abi, oracle_mod, name):
+    """explicit_integration! LHS (3 RK stages with sponges) and RHS on perturbed states."""
+    nl, gpu, orc = pair(pcf, abi, oracle_mod, name)
+    names = ["RHOP", "U", "V", "W", "PIP"] + (["RHO"] if nl.atmosphere.model != "Boussinesq" else [])
+    perturb(gpu, orc, names, seed=7, amp=0.03)
+    for s in (gpu, orc):
+        s.set_boundaries(abi.BOUNDARY_PREDICTANDS)
+        s.compute_sponges(0.3, 2.0)
+        s.backup_predictands()
+        s.explicit_integration_lhs(0.3, 2.0)
+    for f in names:
+        err = rel(gpu.get(f), orc.get(f))
+        assert err <= KERNEL_RTOL, ("lhs", f, err)
+    for s in (gpu, orc):
+        s.explicit_integration_rhs(0.3, 2.0)
+    for f in names:
+        err = rel(gpu.get(f), orc.get(f))
+        assert err <= KERNEL_RTOL, ("rhs", f, err)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_implicit_stage(pcf, abi, oracle_mod, name):
+    nl, gpu, orc = pair(pcf, abi, oracle_mod, name, tight=True)
+    names = ["RHOP", "U", "V", "W", "PIP"] + (["RHO"] if nl.atmosphere.model != "Boussinesq" else [])
+    perturb(gpu, orc, names, seed=8, amp=0.03)
+    for s in (gpu, orc):
+        s.set_boundaries(abi.BOUNDARY_PREDICTANDS)
+        s.compute_sponges(0.3, 2.0)
+    n1 = gpu.implicit_integration(0.3, 2.0, 2.0)
+    n2 = orc.implicit_integration(0.3, 2.0, 2.0)
+    assert abs(n1 - n2) <= 2 + n2 // 10, (n1, n2)
+    for f in names:
+        err = rel(gpu.get(f), orc.get(f))
+        assert err <= SOLVE_RTOL, (f, err)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_steps(pcf, abi, oracle_mod, name):
+    """initial cleaning + three full time steps of integrate."""
+    nl, gpu, orc = pair(pcf, abi, oracle_mod, name, tight=True)
+    m1, m2 = gpu.initial_cleaning(), orc.initial_cleaning()
+    assert abs(m1 - m2) <= 2 + m2 // 10
+    for _ in range(3):
+        a, b = gpu.step(), orc.step()
+        assert a.dt == pytest.approx(b.dt, rel=1e-10)
+        for q in range(2):
+            assert abs(a.niter[q] - b.niter[q]) <= 2 + b.niter[q] // 10
+    names = ["RHOP", "U", "V", "W", "PIP"] + (["RHO"] if nl.atmosphere.model != "Boussinesq" else [])
+    for f in names:
+        err = rel(gpu.get(f), orc.get(f))
+        assert err <= STEP_RTOL, (f, err)
