@@ -310,6 +310,24 @@ int pcf_set_vector(pcf_state* st, int vec, const double* host_in);
 /* Residual record of the last BiCGSTAB solve: final ||r||/||b|| (RMS norms). */
 int pcf_last_residual(const pcf_state* st, double* rel_residual);
 
+/* ---- kernel timing (instrumentation of this build; the reference has no
+ * counterpart: it prints wall-clock times only) ---------------------------- */
+enum pcf_kernel_group {
+  PCF_K_APPLY_TOTAL_DOT = 0, /* 25-point apply with a fused dot product     */
+  PCF_K_PC_THOMAS,           /* preconditioner line solves in z            */
+  PCF_K_PC_RHS,              /* preconditioner horizontal apply + update   */
+  PCF_K_COMPUTE_OPERATOR,    /* assembly of the 25 coefficients           */
+  PCF_K_BICG_VECTOR,         /* BiCGSTAB vector updates and reductions     */
+  PCF_K_FLUXES,              /* reconstruction, stress and flux kernels    */
+  PCF_K_UPDATES,             /* RK3 and RHS updates, sponges, boundaries   */
+  PCF_K_COUNT
+};
+/* Record HIP events around every launch of the groups above (on = 1) or stop
+ * (on = 0).  Turning it on clears the totals. */
+int pcf_profile_enable(pcf_state* st, int on);
+/* Launch count and summed device time of one group since it was enabled. */
+int pcf_profile_query(pcf_state* st, int group, int64_t* launches, double* total_ms);
+
 #ifdef __cplusplus
 }
 #endif

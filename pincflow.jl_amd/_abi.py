@@ -177,7 +177,11 @@ SIGNATURES = [
     ("pcf_get_vector", _I, [_P, _I, _PD]),
     ("pcf_set_vector", _I, [_P, _I, _PD]),
     ("pcf_last_residual", _I, [_P, _PD]),
+    ("pcf_profile_enable", _I, [_P, _I]),
+    ("pcf_profile_query", _I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _PD]),
 ]
+
+KERNEL_GROUPS = ["apply_total_dot", "pc_thomas", "pc_rhs", "compute_operator", "bicg_vector", "fluxes", "updates"]
 
 _lib = None
 

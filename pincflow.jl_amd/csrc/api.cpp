@@ -305,6 +305,7 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
 int pcf_destroy(pcf_state* st) {
   if (!st) return PCF_OK;
   if (st->stream) (void)hipStreamSynchronize(st->stream);
+  prof_release(st);
   for (void* p : st->allocations) (void)hipFree(p);
   if (st->sc_host) (void)hipHostFree(st->sc_host);
   if (st->cfl_host) (void)hipHostFree(st->cfl_host);
@@ -408,6 +409,7 @@ int pcf_step(pcf_state* st, pcf_step_info* info) {
   info->niter[1] = it2;
   if (rc) return rc;
   if (hipStreamSynchronize(st->stream) != hipSuccess) return PCF_E_HIP;
+  prof_flush(st);
   st->itime += 1;
   info->time = time * st->tref;
   info->dt = dt * st->tref;

@@ -115,6 +115,7 @@ int bc_z(pcf_state* st, double* f, int mode, bool staggered) {
 
 // set_boundaries!(state, BoundaryPredictands()) (set_boundaries!.jl:43-56)
 int set_boundaries_predictands(pcf_state* st) {
+  ProfScope prof(st, PCF_K_UPDATES);
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
   double* f[6];
   int n = 0;

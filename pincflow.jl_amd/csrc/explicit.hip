@@ -725,6 +725,7 @@ void launch_flux(pcf_state* st, const FluxArgs& A, Range3 R) {
 // reconstruct! + set_boundaries!(Reconstructions) + compute_fluxes!(state, p0)
 // + set_boundaries!(Fluxes) (explicit_integration!.jl:53-58)
 int compute_fluxes(pcf_state* st) {
+  ProfScope prof(st, PCF_K_FLUXES);
   const Geo& g = st->g;
   Fields F = fields_of(st);
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
@@ -777,6 +778,7 @@ int compute_fluxes(pcf_state* st) {
 }
 
 int update_lhs(pcf_state* st, double dt, int m, int var) {
+  ProfScope prof(st, PCF_K_UPDATES);
   const Geo& g = st->g;
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
   UpdArgs A;
@@ -837,6 +839,7 @@ int update_lhs(pcf_state* st, double dt, int m, int var) {
 }
 
 int update_rhs(pcf_state* st, double dt, int var, bool implicit, double rf) {
+  ProfScope prof(st, PCF_K_UPDATES);
   const Geo& g = st->g;
   RhsArgs A;
   A.F = fields_of(st);
@@ -879,6 +882,7 @@ int update_rhs(pcf_state* st, double dt, int var, bool implicit, double rf) {
 }
 
 int apply_lhs_sponge(pcf_state* st, double dt, double time, int var) {
+  ProfScope prof(st, PCF_K_UPDATES);
   (void)time;
   const Geo& g = st->g;
   const pcf_config& n = st->cfg;
@@ -924,6 +928,7 @@ int cfl_maxima(pcf_state* st, double out[3]) {
 }
 
 int copy_field(pcf_state* st, double* dst, const double* src) {
+  ProfScope prof(st, PCF_K_UPDATES);
   hipLaunchKernelGGL(k_copy_field, dim3(2048), dim3(256), 0, st->stream, dst, src, st->g.nh());
   PCF_HIP(hipGetLastError());
   return PCF_OK;

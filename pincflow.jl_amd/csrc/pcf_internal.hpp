@@ -142,8 +142,34 @@ struct pcf_state {
   int itime = 0;
   double last_residual = 0.0;
 
+  // kernel timing (profile.cpp)
+  struct ProfRec {
+    int group;
+    hipEvent_t a, b;
+  };
+  bool prof_on = false;
+  std::vector<ProfRec> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
+  double prof_ms[PCF_K_COUNT] = {};
+  long long prof_n[PCF_K_COUNT] = {};
+
   std::vector<void*> allocations;
 };
+
+namespace pcf {
+// Bracket a launch with events when profiling is on (profile.cpp).
+void prof_begin(pcf_state* st, int group, hipEvent_t* a);
+void prof_end(pcf_state* st, int group, hipEvent_t a);
+void prof_flush(pcf_state* st);
+void prof_release(pcf_state* st);
+struct ProfScope {
+  pcf_state* st;
+  int group;
+  hipEvent_t a = nullptr;
+  ProfScope(pcf_state* s, int g) : st(s), group(g) { prof_begin(st, group, &a); }
+  ~ProfScope() { prof_end(st, group, a); }
+};
+}  // namespace pcf
 
 namespace pcf {
 

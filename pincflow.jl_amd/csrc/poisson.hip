@@ -862,6 +862,7 @@ int compute_lhs(pcf_state* st, double* tolref_out) {
 int compute_operator(pcf_state* st, double dt, double rf) {
   const Geo& g = st->g;
   dim3 blk(64, 4), grd(blocks_for(g.nx, 64), blocks_for(g.ny, 4), g.nz);
+  ProfScope prof(st, PCF_K_COMPUTE_OPERATOR);
   hipLaunchKernelGGL(k_compute_operator, grd, blk, 0, st->stream, fields_of(st), st->betar, st->fcscal, g, st->T, dt,
                      rf, st->ph.damp);
   PCF_HIP(hipGetLastError());
@@ -904,14 +905,23 @@ static int precondition(pcf_state* st, const double* sin, double* sout, const So
   const int iters = st->cfg.preconditioner_iterations;
   // pseudo-time step 1 from s = 0
   double* cur = iters == 1 ? sout : st->pc_tmp;
-  hipLaunchKernelGGL(k_pc_thomas<true>, tgrd, tblk, 0, st->stream, ac, ad, au, sin, cur, st->pc_q, g.nx, g.ny, g.nz,
-                     st->deta, sc);
-  for (int it = 2; it <= iters; ++it) {
-    hipLaunchKernelGGL(k_pc_rhs, apply_grid(g), dim3(64, 4), 0, st->stream, apply_args(st, cur, st->pc_rhs), sin,
+  {
+    ProfScope prof(st, PCF_K_PC_THOMAS);
+    hipLaunchKernelGGL(k_pc_thomas<true>, tgrd, tblk, 0, st->stream, ac, ad, au, sin, cur, st->pc_q, g.nx, g.ny, g.nz,
                        st->deta, sc);
+  }
+  for (int it = 2; it <= iters; ++it) {
+    {
+      ProfScope prof(st, PCF_K_PC_RHS);
+      hipLaunchKernelGGL(k_pc_rhs, apply_grid(g), dim3(64, 4), 0, st->stream, apply_args(st, cur, st->pc_rhs), sin,
+                         st->deta, sc);
+    }
     double* next = it == iters ? sout : (cur == st->pc_tmp ? sout : st->pc_tmp);
-    hipLaunchKernelGGL(k_pc_thomas<false>, tgrd, tblk, 0, st->stream, ac, ad, au, st->pc_rhs, next, st->pc_q, g.nx,
-                       g.ny, g.nz, st->deta, sc);
+    {
+      ProfScope prof(st, PCF_K_PC_THOMAS);
+      hipLaunchKernelGGL(k_pc_thomas<false>, tgrd, tblk, 0, st->stream, ac, ad, au, st->pc_rhs, next, st->pc_q, g.nx,
+                         g.ny, g.nz, st->deta, sc);
+    }
     cur = next;
   }
   PCF_HIP(hipGetLastError());
@@ -966,20 +976,32 @@ int apply_bicgstab(pcf_state* st, double tolref, int* niter_out) {
     // v = A M^-1 p, <v, r0>
     if ((rc = n.preconditioner ? precondition(st, st->bp, st->bvpc, sc) : PCF_OK)) break;
     const double* vin = n.preconditioner ? st->bvpc : st->bp;
-    hipLaunchKernelGGL((k_apply<true, 1>), agrd, dim3(64, 4), 0, st->stream, apply_args(st, vin, st->bv), st->br0,
-                       nullptr, st->partials, sc);
-    hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1024), 0, st->stream, st->partials, aparts, sc);
-    hipLaunchKernelGGL(k_bicg_s, dim3(kVecGrid), dim3(256), 0, st->stream, st->br, st->bv, st->bs, N, sc);
+    {
+      ProfScope prof(st, PCF_K_APPLY_TOTAL_DOT);
+      hipLaunchKernelGGL((k_apply<true, 1>), agrd, dim3(64, 4), 0, st->stream, apply_args(st, vin, st->bv), st->br0,
+                         nullptr, st->partials, sc);
+    }
+    {
+      ProfScope prof(st, PCF_K_BICG_VECTOR);
+      hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1024), 0, st->stream, st->partials, aparts, sc);
+      hipLaunchKernelGGL(k_bicg_s, dim3(kVecGrid), dim3(256), 0, st->stream, st->br, st->bv, st->bs, N, sc);
+    }
     // t = A M^-1 s, <t, s>, <t, t>
     if ((rc = n.preconditioner ? precondition(st, st->bs, st->bvpc, sc) : PCF_OK)) break;
     const double* sin2 = n.preconditioner ? st->bvpc : st->bs;
-    hipLaunchKernelGGL((k_apply<true, 2>), agrd, dim3(64, 4), 0, st->stream, apply_args(st, sin2, st->bt), st->bs,
-                       nullptr, st->partials, sc);
-    hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1024), 0, st->stream, st->partials, aparts, sc);
-    hipLaunchKernelGGL(k_bicg_xr, dim3(kVecGrid), dim3(kReduceBlock), 0, st->stream, st->solution, st->bp, st->bs,
-                       st->bt, st->br, st->br0, N, st->partials, sc);
-    hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(1024), 0, st->stream, st->partials, kVecGrid, sc, nxyz);
-    hipLaunchKernelGGL(k_bicg_p, dim3(kVecGrid), dim3(256), 0, st->stream, st->bp, st->br, st->bv, N, sc);
+    {
+      ProfScope prof(st, PCF_K_APPLY_TOTAL_DOT);
+      hipLaunchKernelGGL((k_apply<true, 2>), agrd, dim3(64, 4), 0, st->stream, apply_args(st, sin2, st->bt), st->bs,
+                         nullptr, st->partials, sc);
+    }
+    {
+      ProfScope prof(st, PCF_K_BICG_VECTOR);
+      hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1024), 0, st->stream, st->partials, aparts, sc);
+      hipLaunchKernelGGL(k_bicg_xr, dim3(kVecGrid), dim3(kReduceBlock), 0, st->stream, st->solution, st->bp, st->bs,
+                         st->bt, st->br, st->br0, N, st->partials, sc);
+      hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(1024), 0, st->stream, st->partials, kVecGrid, sc, nxyz);
+      hipLaunchKernelGGL(k_bicg_p, dim3(kVecGrid), dim3(256), 0, st->stream, st->bp, st->br, st->bv, N, sc);
+    }
     PCF_HIP(hipGetLastError());
     PCF_HIP(hipMemcpyAsync(hs + (jb & 1), sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, st->stream));
     PCF_HIP(hipEventRecord(ev[jb & 1], st->stream));

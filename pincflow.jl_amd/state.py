@@ -170,6 +170,22 @@ class State:
     def apply_preconditioner_device(self, d_in: int, d_out: int):
         self._check(self.lib.pcf_apply_preconditioner(self.h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out)))
 
+    # ---- kernel timing -------------------------------------------------------
+    def profile(self, on: bool):
+        """Record HIP events around the kernel groups of the library (clears totals)."""
+        self._check(self.lib.pcf_profile_enable(self.h, int(on)))
+
+    def profile_report(self) -> dict:
+        """{group: (launches, total device ms)} since profile(True)."""
+        out = {}
+        for gid, name in enumerate(_abi.KERNEL_GROUPS):
+            n = ctypes.c_int64(0)
+            ms = ctypes.c_double(0.0)
+            self._check(self.lib.pcf_profile_query(self.h, gid, ctypes.byref(n), ctypes.byref(ms)))
+            if n.value:
+                out[name] = (int(n.value), float(ms.value))
+        return out
+
     def dot_device(self, d_a: int, d_b: int) -> float:
         r = ctypes.c_double(0.0)
         self._check(self.lib.pcf_dot(self.h, ctypes.c_void_p(d_a), ctypes.c_void_p(d_b), ctypes.byref(r)))
