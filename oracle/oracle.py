@@ -65,6 +65,8 @@ def lib() -> ctypes.CDLL:
             "orc_last_residual": (D, [P]),
             "orc_apply_corrector": (I, [P, D, D, PI32]),
             "orc_correct": (None, [P, D, D]),
+            "orc_solution_to_dpip": (None, [P, D]),
+            "orc_muscl_line": (None, [PD, I, PD, PD]),
             "orc_compute_sponges": (None, [P, D, D]),
             "orc_set_boundaries": (None, [P, I]),
             "orc_backup_predictands": (None, [P]),
@@ -199,6 +201,11 @@ class Oracle:
     def correct(self, dt: float, rf: float):
         lib().orc_correct(self.h, dt, rf)
 
+    def solution_to_dpip(self, dt: float):
+        """dpip = solution / (sqrt(pbar^2/rhobar) dt) plus its boundaries (the post-solve part of
+        apply_corrector!); the SOLUTION vector is divided in place, as in the reference."""
+        lib().orc_solution_to_dpip(self.h, dt)
+
     def compute_sponges(self, dt: float, time: float):
         lib().orc_compute_sponges(self.h, dt, time)
 
@@ -278,3 +285,14 @@ def integrate(nl, abi, output, records: bool = True, threads: int = 0, max_steps
 def abi_config(nl, abi):
     from importlib import import_module
     return import_module(abi.__name__.rsplit(".", 1)[0] + ".namelists").to_config(nl)
+
+
+def muscl_line(phi: np.ndarray):
+    """apply_1d_muscl! on one line: (left-face, right-face) limited states of every cell;
+    the two end cells keep the reference's placeholder 1000.0."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    left = np.empty_like(phi)
+    right = np.empty_like(phi)
+    lib().orc_muscl_line(_ptr(phi), len(phi), _ptr(left), _ptr(right))
+    return left, right
+

@@ -137,6 +137,7 @@ void apply_preconditioner(State& st, Arr3& sin, Arr3& sout);
 int apply_bicgstab(State& st, double tolref, int& niter);
 int apply_corrector(State& st, double dt, double rf, int& niter);
 void correct(State& st, double dt, double rf);
+void solution_to_dpip(State& st, double dtinv);
 double dot(const Arr3& a, const Arr3& b);
 double sumsq(const Arr3& a);
 

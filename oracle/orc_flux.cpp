@@ -378,3 +378,10 @@ void compute_fluxes(State& st, Predictands& p0) {
 }
 
 }  // namespace orc
+
+// Test hook: apply_1d_muscl! on one contiguous line of n values; face_left[q]
+// and face_right[q] are the limited states at the left and right faces of
+// cell q (the two components of the reference's phitilde view).
+extern "C" void orc_muscl_line(const double* phi, int n, double* face_left, double* face_right) {
+  orc::muscl_line(phi, face_left, face_right, 1, n);
+}

@@ -342,6 +342,7 @@ int orc_apply_corrector(State* st, double dt, double rf, int32_t* niter) {
 }
 
 void orc_correct(State* st, double dt, double rf) { correct(*st, dt, rf); }
+void orc_solution_to_dpip(State* st, double dt) { solution_to_dpip(*st, 1.0 / dt); }
 void orc_compute_sponges(State* st, double dt, double time) { compute_sponges(*st, dt, time); }
 void orc_set_boundaries(State* st, int which) {
   if (which == PCF_BOUNDARY_PREDICTANDS) set_boundaries_predictands(*st);

@@ -720,6 +720,14 @@ int apply_corrector(State& st, double dt, double rf, int& niter) {
   compute_operator(st, dt, rf);
   rc = apply_bicgstab(st, tolref, niter);
   if (rc) return rc;
+  solution_to_dpip(st, dtinv);
+  correct(st, dt, rf);
+  return PCF_OK;
+}
+
+// solve_poisson!.jl:69-72: dpip = x / (sqrt(pbar^2 / rhobar) dt), then the
+// boundaries of dpip (apply_corrector!.jl:58-60)
+void solution_to_dpip(State& st, double dtinv) {
   for (int k = st.k0; k <= st.k1; ++k)
     for (int j = st.j0; j <= st.j1; ++j)
       for (int i = st.i0; i <= st.i1; ++i) {
@@ -730,8 +738,6 @@ int apply_corrector(State& st, double dt, double rf, int& niter) {
   zonal_bc(st, st.dpip);
   meridional_bc(st, st.dpip);
   vertical_bc(st, st.dpip, +1);
-  correct(st, dt, rf);
-  return PCF_OK;
 }
 
 // correct!.jl:164-385
