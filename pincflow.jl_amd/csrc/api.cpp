@@ -270,7 +270,7 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
   for (int q = 0; q < T_COUNT; ++q)
     if ((rc = alloc(st, &st->T.a[q], NV))) return fail(rc);
   double** vecs[] = {&st->rhs, &st->solution, &st->bp, &st->br0, &st->br, &st->bs, &st->bt, &st->bv,
-                     &st->bvpc, &st->bmat, &st->pc_rhs, &st->pc_q, &st->pc_tmp};
+                     &st->bvpc, &st->bmat, &st->pc_rhs, &st->pc_q, &st->pc_tmp, &st->pc_P, &st->pc_E, &st->pc_Q};
   for (double** p : vecs)
     if ((rc = alloc(st, p, NV > (size_t)g.nz ? NV : (size_t)g.nz))) return fail(rc);
   if ((rc = alloc(st, &st->partials, (size_t)kMaxDots << 17))) return fail(rc);
@@ -526,6 +526,7 @@ int pcf_set_vector(pcf_state* st, int vec, const double* host_in) {
   if (!p) { set_error("unknown vector"); return PCF_E_ARG; }
   PCF_HIP(hipMemcpyAsync(p, host_in, st->g.nv() * sizeof(double), hipMemcpyHostToDevice, st->stream));
   PCF_HIP(hipStreamSynchronize(st->stream));
+  if (vec == PCF_V_AC || vec == PCF_V_AD || vec == PCF_V_AU) st->pc_factors_valid = false;
   return PCF_OK;
 }
 

@@ -130,7 +130,9 @@ struct pcf_state {
   pcf::Tensor T;
   double *rhs, *solution;
   double *bp, *br0, *br, *bs, *bt, *bv, *bvpc, *bmat;
-  double *pc_rhs, *pc_q, *pc_tmp;     // preconditioner work
+  double *pc_rhs, *pc_q, *pc_tmp;     // preconditioner work (pc_q: also scratch of the sponge)
+  double *pc_P, *pc_E, *pc_Q;         // line-solve factors of the current operator
+  bool pc_factors_valid = false;
   double* partials;                   // reduction partials
   pcf::SolverScalars* sc;             // device scalars
   pcf::SolverScalars* sc_host;        // pinned copy

@@ -48,8 +48,19 @@ __device__ __forceinline__ void block_partials(double (&val)[ND], double* partia
 __device__ double finalize_sum(const double* partials, int d, int nparts) {
   __shared__ double red[16];
   const int tid = threadIdx.x;
+  const int stride = blockDim.x;
+  const double* pp = partials + (size_t)d * kPartialStride;
   double s = 0.0;
-  for (int q = tid; q < nparts; q += blockDim.x) s += partials[(size_t)d * kPartialStride + q];
+  // the same order as one partial at a time, with 8 loads in flight
+  int q = tid;
+  for (; q + 7 * stride < nparts; q += 8 * stride) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = pp[q + u * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; q < nparts; q += stride) s += pp[q];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if ((tid & 63) == 0) red[tid >> 6] = s;
   __syncthreads();
@@ -549,39 +560,98 @@ __global__ void __launch_bounds__(256) k_apply(ApplyArgs A, const double* d1, co
 // ---------------------------------------------------------------------------
 // apply_preconditioner! (apply_preconditioner!.jl:40-124)
 // ---------------------------------------------------------------------------
-// Line solve in z of (1 - deta L_v) s = rhs (apply_preconditioner!.jl:66-111)
-// for one column per thread.  FIRST: rhs = 0 + deta (0 - sin), the first
-// pseudo-time step from s = 0, whose horizontal apply is zero.
-template <bool FIRST>
-__global__ void k_pc_thomas(const double* ac, const double* ad, const double* au, const double* src, double* out,
-                            double* q, int nx, int ny, int nz, double deta, const SolverScalars* sc) {
-  if (sc && sc->done) return;
+// Factors of the line solve in z of (1 - deta L_v) s = rhs
+// (apply_preconditioner!.jl:66-111), which depend only on the operator:
+// P = the divisor 1 - deta ac on the lowest level and the reciprocal pivot
+// 1 / (1 - deta ac - deta ad q) above it, E = deta ad, Q = deta au p.  A sweep
+// with them rounds exactly like the reference's loop, which evaluates
+// deta * ad * s as (deta * ad) * s.  One column per thread.
+__global__ void __launch_bounds__(64) k_pc_factors(const double* ac, const double* ad, const double* au, double* P,
+                                                   double* E, double* Q, int nx, int ny, int nz, double deta) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= nx) return;
   const size_t sxy = (size_t)nx * ny;
   size_t o = (size_t)i + (size_t)nx * j;
-  double rhs = FIRST ? 0.0 + deta * (0.0 - src[o]) : src[o];
-  double denom = 1 - deta * ac[o];
+  const double denom = 1 - deta * ac[o];
   double qk = deta * au[o] / denom;
-  double sk = rhs / denom;
-  q[o] = qk;
-  out[o] = sk;
+  P[o] = denom;
+  E[o] = deta * ad[o];
+  Q[o] = qk;
   for (int k = 1; k < nz; ++k) {
     o += sxy;
-    rhs = FIRST ? 0.0 + deta * (0.0 - src[o]) : src[o];
-    const double adk = ad[o];
-    const double p = 1 / (1 - deta * ac[o] - deta * adk * qk);
+    const double e = deta * ad[o];
+    const double p = 1 / (1 - deta * ac[o] - e * qk);
     qk = deta * au[o] * p;
-    sk = (rhs + deta * adk * sk) * p;
-    q[o] = qk;
+    P[o] = p;
+    E[o] = e;
+    Q[o] = qk;
+  }
+}
+
+// The line solve with precomputed factors, one column per thread and 64
+// adjacent columns per wavefront; the loads of 8 levels are issued before
+// their (sequential) recurrence steps.  The pointers are deliberately not
+// __restrict__: the possible aliasing of `out` keeps each batch of loads
+// ahead of the batch's stores, which is what keeps 24 loads in flight per
+// thread (with __restrict__ the compiler sinks them next to their uses).
+// FIRST: rhs = 0 + deta (0 - src), the first pseudo-time step from s = 0,
+// whose horizontal apply is zero.
+template <bool FIRST>
+__global__ void __launch_bounds__(64) k_pc_thomas(const double* P, const double* E, const double* Q, const double* src,
+                                                  double* out, int nx, int ny, int nz, double deta,
+                                                  const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  constexpr int U = 8;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nx) return;
+  const size_t sxy = (size_t)nx * ny;
+  const size_t c = (size_t)i + (size_t)nx * j;
+  auto rhs_of = [&](double v) -> double { return FIRST ? 0.0 + deta * (0.0 - v) : v; };
+  double sk = rhs_of(src[c]) / P[c];
+  out[c] = sk;
+  int k = 1;
+  for (; k + U <= nz; k += U) {
+    double r[U], e[U], p[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t o = c + sxy * (k + u);
+      r[u] = src[o];
+      e[u] = E[o];
+      p[u] = P[o];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      sk = (rhs_of(r[u]) + e[u] * sk) * p[u];
+      out[c + sxy * (k + u)] = sk;
+    }
+  }
+  for (; k < nz; ++k) {
+    const size_t o = c + sxy * k;
+    sk = (rhs_of(src[o]) + E[o] * sk) * P[o];
     out[o] = sk;
   }
   // downward sweep
-  for (int k = nz - 2; k >= 0; --k) {
-    const size_t oo = (size_t)i + (size_t)nx * j + sxy * k;
-    sk = out[oo] + q[oo] * sk;
-    out[oo] = sk;
+  k = nz - 2;
+  for (; k - (U - 1) >= 0; k -= U) {
+    double r[U], q[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const size_t o = c + sxy * (k - u);
+      r[u] = out[o];
+      q[u] = Q[o];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      sk = r[u] + q[u] * sk;
+      out[c + sxy * (k - u)] = sk;
+    }
+  }
+  for (; k >= 0; --k) {
+    const size_t o = c + sxy * k;
+    sk = out[o] + Q[o] * sk;
+    out[o] = sk;
   }
 }
 
@@ -610,6 +680,119 @@ __global__ void __launch_bounds__(256) k_pc_rhs(ApplyArgs A, const double* sin, 
        T[T_ALUU][o] * S(im, j, k + 2) + T[T_ALDD][o] * S(im, j, k - 2) + T[T_AFUU][o] * S(i, jp, k + 2) +
        T[T_AFDD][o] * S(i, jp, k - 2) + T[T_ABUU][o] * S(i, jm, k + 2) + T[T_ABDD][o] * S(i, jm, k - 2);
   A.ls[o] = s[o] + deta * (v - sin[o]);
+}
+
+// ---------------------------------------------------------------------------
+// The same stencils for an even nx, two x-adjacent cells per thread with
+// 16-byte loads and stores: one wavefront covers 128 consecutive cells of a
+// row, which halves the memory instructions per cell.  The coefficients are
+// streamed once per launch and are loaded non-temporally, so they do not
+// evict the neighbourhood of `s` from the caches.  Each cell's sum is
+// evaluated in the order of the scalar kernels above.
+//   MODE_TOTAL       ls = A s                          (apply_operator!, Total)
+//   MODE_HORIZONTAL  ls = H s                          (apply_operator!, Horizontal)
+//   MODE_PC_RHS      ls = s + deta (H s - sin)         (apply_preconditioner!.jl:62-63)
+// NDOT = 1: partials of <ls, d1>; NDOT = 2: of <ls, d1> and <ls, ls>.
+// Block 32 x 8 threads = 64 x 8 cells of one z-level.
+// ---------------------------------------------------------------------------
+enum { MODE_TOTAL = 0, MODE_HORIZONTAL = 1, MODE_PC_RHS = 2 };
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+constexpr int kPairBlockY = 8;
+
+template <int MODE, int NDOT>
+__global__ void __launch_bounds__(256) k_stencil_pair(ApplyArgs A, const double* __restrict__ d1,
+                                                      const double* __restrict__ sin, double deta, double* partials,
+                                                      const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  const int i0 = (blockIdx.x * 32 + threadIdx.x) * 2;
+  const int j = blockIdx.y * kPairBlockY + threadIdx.y;
+  const int k = blockIdx.z;
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  double val[NDOT > 0 ? NDOT : 1];
+#pragma unroll
+  for (int d = 0; d < (NDOT > 0 ? NDOT : 1); ++d) val[d] = 0.0;
+  if (i0 < nx && j < ny) {
+    const size_t sxy = (size_t)nx * ny;
+    const size_t o = (size_t)i0 + (size_t)nx * j + sxy * k;
+    const int im = i0 == 0 ? nx - 1 : i0 - 1, ip = i0 + 2 == nx ? 0 : i0 + 2;
+    const int jp = j + 1 == ny ? 0 : j + 1, jm = j == 0 ? ny - 1 : j - 1;
+    const double* s = A.s;
+    // s on levels k-2..k+2 (zero beyond the walls): own pair, the pairs of
+    // the y neighbours, and the x neighbours left of i0 and right of i0+1
+    dbl2 C[5], F[5], B[5];
+    double L[5], R[5];
+#pragma unroll
+    for (int w = 0; w < 5; ++w) {
+      const int kk = k - 2 + w;
+      if (kk < 0 || kk >= nz) {
+        C[w] = 0.0;
+        F[w] = 0.0;
+        B[w] = 0.0;
+        L[w] = 0.0;
+        R[w] = 0.0;
+        continue;
+      }
+      const size_t pl = sxy * kk;
+      C[w] = *(const dbl2*)(s + pl + (size_t)nx * j + i0);
+      F[w] = *(const dbl2*)(s + pl + (size_t)nx * jp + i0);
+      B[w] = *(const dbl2*)(s + pl + (size_t)nx * jm + i0);
+      L[w] = s[pl + (size_t)nx * j + im];
+      R[w] = s[pl + (size_t)nx * j + ip];
+    }
+    auto T = [&](int t) -> dbl2 { return __builtin_nontemporal_load((const dbl2*)(A.T.a[t] + o)); };
+    const dbl2 al = T(T_AL), ar = T(T_AR), af = T(T_AF), ab = T(T_AB);
+    dbl2 au = 0.0, ad = 0.0, ac = 0.0;
+    if (MODE == MODE_TOTAL) {
+      au = T(T_AU);
+      ad = T(T_AD);
+      ac = T(T_AC);
+    }
+    const dbl2 aru = T(T_ARU), ard = T(T_ARD), alu = T(T_ALU), ald = T(T_ALD), afu = T(T_AFU), afd = T(T_AFD);
+    const dbl2 abu = T(T_ABU), abd = T(T_ABD), auu = T(T_AUU), add = T(T_ADD), aruu = T(T_ARUU), ardd = T(T_ARDD);
+    const dbl2 aluu = T(T_ALUU), aldd = T(T_ALDD), afuu = T(T_AFUU), afdd = T(T_AFDD), abuu = T(T_ABUU),
+               abdd = T(T_ABDD);
+    dbl2 sn = 0.0;
+    if (MODE == MODE_PC_RHS) sn = *(const dbl2*)(sin + o);
+    double out[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      auto g = [&](const dbl2& a) -> double { return c == 0 ? a.x : a.y; };
+      auto XL = [&](int w) -> double { return c == 0 ? L[w] : C[w].x; };
+      auto XR = [&](int w) -> double { return c == 0 ? C[w].y : R[w]; };
+      auto X0 = [&](int w) -> double { return g(C[w]); };
+      auto YF = [&](int w) -> double { return g(F[w]); };
+      auto YB = [&](int w) -> double { return g(B[w]); };
+      double v;
+      if (MODE == MODE_TOTAL) {
+        v = g(al) * XL(2) + g(ar) * XR(2) + g(af) * YF(2) + g(ab) * YB(2) + g(au) * X0(3) + g(ad) * X0(1) +
+            g(ac) * X0(2);
+      } else {
+        v = g(al) * XL(2) + g(ar) * XR(2) + g(af) * YF(2) + g(ab) * YB(2);
+      }
+      v += g(aru) * XR(3) + g(ard) * XR(1) + g(alu) * XL(3) + g(ald) * XL(1) + g(afu) * YF(3) + g(afd) * YF(1) +
+           g(abu) * YB(3) + g(abd) * YB(1) + g(auu) * X0(4) + g(add) * X0(0) + g(aruu) * XR(4) + g(ardd) * XR(0) +
+           g(aluu) * XL(4) + g(aldd) * XL(0) + g(afuu) * YF(4) + g(afdd) * YF(0) + g(abuu) * YB(4) + g(abdd) * YB(0);
+      if (MODE == MODE_PC_RHS) v = X0(2) + deta * (v - g(sn));
+      out[c] = v;
+    }
+    dbl2 res;
+    res.x = out[0];
+    res.y = out[1];
+    __builtin_nontemporal_store(res, (dbl2*)(A.ls + o));
+    if (NDOT >= 1) {
+      const dbl2 dd = *(const dbl2*)(d1 + o);
+      val[0] = out[0] * dd.x;
+      val[0] += out[1] * dd.y;
+    }
+    if (NDOT == 2) {
+      val[1] = out[0] * out[0];
+      val[1] += out[1] * out[1];
+    }
+  }
+  if (NDOT > 0) {
+    const int block = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    block_partials<(NDOT > 0 ? NDOT : 1)>(val, partials, block);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -866,6 +1049,7 @@ int compute_operator(pcf_state* st, double dt, double rf) {
   hipLaunchKernelGGL(k_compute_operator, grd, blk, 0, st->stream, fields_of(st), st->betar, st->fcscal, g, st->T, dt,
                      rf, st->ph.damp);
   PCF_HIP(hipGetLastError());
+  st->pc_factors_valid = false;
   return PCF_OK;
 }
 
@@ -882,15 +1066,52 @@ static ApplyArgs apply_args(pcf_state* st, const double* s, double* ls) {
 
 static dim3 apply_grid(const Geo& g) { return dim3(blocks_for(g.nx, 64), blocks_for(g.ny, 4), g.nz); }
 
-int apply_operator(pcf_state* st, const double* sin, double* ls, bool total) {
-  dim3 grd = apply_grid(st->g);
-  if (total)
-    hipLaunchKernelGGL((k_apply<true, 0>), grd, dim3(64, 4), 0, st->stream, apply_args(st, sin, ls), nullptr, nullptr,
-                       nullptr, nullptr);
+// The two-cells-per-thread kernels need an even nx (16-byte aligned pairs).
+static bool use_pairs(const Geo& g) { return g.nx % 2 == 0; }
+static dim3 pair_grid(const Geo& g) { return dim3(blocks_for(g.nx, 64), blocks_for(g.ny, kPairBlockY), g.nz); }
+static dim3 pair_block() { return dim3(32, kPairBlockY); }
+
+// Number of partials the dot-fused apply writes.
+static int apply_parts(const Geo& g) {
+  const dim3 grd = use_pairs(g) ? pair_grid(g) : apply_grid(g);
+  return (int)(grd.x * grd.y * grd.z);
+}
+
+// ls = A s (TOTAL) with NDOT fused dot products: <ls, d1> (and <ls, ls>).
+template <int NDOT>
+static void launch_total(pcf_state* st, const double* s, double* ls, const double* d1, const SolverScalars* sc) {
+  const Geo& g = st->g;
+  if (use_pairs(g))
+    hipLaunchKernelGGL((k_stencil_pair<MODE_TOTAL, NDOT>), pair_grid(g), pair_block(), 0, st->stream,
+                       apply_args(st, s, ls), d1, nullptr, 0.0, st->partials, sc);
   else
-    hipLaunchKernelGGL((k_apply<false, 0>), grd, dim3(64, 4), 0, st->stream, apply_args(st, sin, ls), nullptr, nullptr,
-                       nullptr, nullptr);
+    hipLaunchKernelGGL((k_apply<true, NDOT>), apply_grid(g), dim3(64, 4), 0, st->stream, apply_args(st, s, ls), d1,
+                       nullptr, st->partials, sc);
+}
+
+int apply_operator(pcf_state* st, const double* sin, double* ls, bool total) {
+  const Geo& g = st->g;
+  if (total) {
+    launch_total<0>(st, sin, ls, nullptr, nullptr);
+  } else if (use_pairs(g)) {
+    hipLaunchKernelGGL((k_stencil_pair<MODE_HORIZONTAL, 0>), pair_grid(g), pair_block(), 0, st->stream,
+                       apply_args(st, sin, ls), nullptr, nullptr, 0.0, nullptr, nullptr);
+  } else {
+    hipLaunchKernelGGL((k_apply<false, 0>), apply_grid(g), dim3(64, 4), 0, st->stream, apply_args(st, sin, ls), nullptr,
+                       nullptr, nullptr, nullptr);
+  }
   PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+// Line-solve factors of the current operator (computed once per operator).
+static int ensure_pc_factors(pcf_state* st) {
+  if (st->pc_factors_valid) return PCF_OK;
+  const Geo& g = st->g;
+  hipLaunchKernelGGL(k_pc_factors, dim3(blocks_for(g.nx, 64), g.ny), dim3(64), 0, st->stream, st->T.a[T_AC],
+                     st->T.a[T_AD], st->T.a[T_AU], st->pc_P, st->pc_E, st->pc_Q, g.nx, g.ny, g.nz, st->deta);
+  PCF_HIP(hipGetLastError());
+  st->pc_factors_valid = true;
   return PCF_OK;
 }
 
@@ -898,29 +1119,32 @@ int apply_operator(pcf_state* st, const double* sin, double* ls, bool total) {
 // must not alias sin.  `sc` (may be null) lets a converged solve skip it.
 static int precondition(pcf_state* st, const double* sin, double* sout, const SolverScalars* sc) {
   const Geo& g = st->g;
+  int rc = ensure_pc_factors(st);
+  if (rc) return rc;
   dim3 tgrd(blocks_for(g.nx, 64), g.ny), tblk(64);
-  const double* ac = st->T.a[T_AC];
-  const double* ad = st->T.a[T_AD];
-  const double* au = st->T.a[T_AU];
   const int iters = st->cfg.preconditioner_iterations;
   // pseudo-time step 1 from s = 0
   double* cur = iters == 1 ? sout : st->pc_tmp;
   {
     ProfScope prof(st, PCF_K_PC_THOMAS);
-    hipLaunchKernelGGL(k_pc_thomas<true>, tgrd, tblk, 0, st->stream, ac, ad, au, sin, cur, st->pc_q, g.nx, g.ny, g.nz,
-                       st->deta, sc);
+    hipLaunchKernelGGL(k_pc_thomas<true>, tgrd, tblk, 0, st->stream, st->pc_P, st->pc_E, st->pc_Q, sin, cur, g.nx,
+                       g.ny, g.nz, st->deta, sc);
   }
   for (int it = 2; it <= iters; ++it) {
     {
       ProfScope prof(st, PCF_K_PC_RHS);
-      hipLaunchKernelGGL(k_pc_rhs, apply_grid(g), dim3(64, 4), 0, st->stream, apply_args(st, cur, st->pc_rhs), sin,
-                         st->deta, sc);
+      if (use_pairs(g))
+        hipLaunchKernelGGL((k_stencil_pair<MODE_PC_RHS, 0>), pair_grid(g), pair_block(), 0, st->stream,
+                           apply_args(st, cur, st->pc_rhs), nullptr, sin, st->deta, nullptr, sc);
+      else
+        hipLaunchKernelGGL(k_pc_rhs, apply_grid(g), dim3(64, 4), 0, st->stream, apply_args(st, cur, st->pc_rhs), sin,
+                           st->deta, sc);
     }
     double* next = it == iters ? sout : (cur == st->pc_tmp ? sout : st->pc_tmp);
     {
       ProfScope prof(st, PCF_K_PC_THOMAS);
-      hipLaunchKernelGGL(k_pc_thomas<false>, tgrd, tblk, 0, st->stream, ac, ad, au, st->pc_rhs, next, st->pc_q, g.nx,
-                         g.ny, g.nz, st->deta, sc);
+      hipLaunchKernelGGL(k_pc_thomas<false>, tgrd, tblk, 0, st->stream, st->pc_P, st->pc_E, st->pc_Q, st->pc_rhs, next,
+                         g.nx, g.ny, g.nz, st->deta, sc);
     }
     cur = next;
   }
@@ -962,8 +1186,7 @@ int apply_bicgstab(pcf_state* st, double tolref, int* niter_out) {
     *niter_out = 0;
     return PCF_OK;
   }
-  const dim3 agrd = apply_grid(g);
-  const int aparts = agrd.x * agrd.y * agrd.z;
+  const int aparts = apply_parts(g);
   if (aparts > kPartialStride) return PCF_E_ARG;
   const int maxit = n.poisson_iterations;
   SolverScalars* hs = st->sc_host;  // pinned, two slots
@@ -978,8 +1201,7 @@ int apply_bicgstab(pcf_state* st, double tolref, int* niter_out) {
     const double* vin = n.preconditioner ? st->bvpc : st->bp;
     {
       ProfScope prof(st, PCF_K_APPLY_TOTAL_DOT);
-      hipLaunchKernelGGL((k_apply<true, 1>), agrd, dim3(64, 4), 0, st->stream, apply_args(st, vin, st->bv), st->br0,
-                         nullptr, st->partials, sc);
+      launch_total<1>(st, vin, st->bv, st->br0, sc);
     }
     {
       ProfScope prof(st, PCF_K_BICG_VECTOR);
@@ -991,8 +1213,7 @@ int apply_bicgstab(pcf_state* st, double tolref, int* niter_out) {
     const double* sin2 = n.preconditioner ? st->bvpc : st->bs;
     {
       ProfScope prof(st, PCF_K_APPLY_TOTAL_DOT);
-      hipLaunchKernelGGL((k_apply<true, 2>), agrd, dim3(64, 4), 0, st->stream, apply_args(st, sin2, st->bt), st->bs,
-                         nullptr, st->partials, sc);
+      launch_total<2>(st, sin2, st->bt, st->bs, sc);
     }
     {
       ProfScope prof(st, PCF_K_BICG_VECTOR);

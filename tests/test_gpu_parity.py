@@ -41,10 +41,13 @@ def configs(pcf):
     mw = pcf.examples.mountain_wave(x_size=12, y_size=10, z_size=9)
     mw3 = pcf.examples.mountain_wave_3d(x_size=16, y_size=14, z_size=12, iterations=2)
     cb = pcf.examples.cold_bubble(x_size=12, z_size=10)
-    return {"periodic_hill": ph, "mountain_wave": mw, "mountain_wave_3d": mw3, "cold_bubble": cb}
+    # odd x_size: the one-cell-per-thread stencil kernels (even x_size uses two)
+    mwo = pcf.examples.mountain_wave(x_size=11, y_size=6, z_size=9)
+    return {"periodic_hill": ph, "mountain_wave": mw, "mountain_wave_3d": mw3, "cold_bubble": cb,
+            "mountain_wave_odd": mwo}
 
 
-CASES = ["periodic_hill", "mountain_wave", "mountain_wave_3d", "cold_bubble"]
+CASES = ["periodic_hill", "mountain_wave", "mountain_wave_3d", "cold_bubble", "mountain_wave_odd"]
 
 
 def pair(pcf, abi, oracle_mod, name, tight=False):
@@ -129,6 +132,14 @@ def test_apply_operator_and_preconditioner(pcf, abi, oracle_mod, name):
     gpu.apply_preconditioner_device(dx.ptr, dy.ptr)
     err = rel(dy.download(), orc.apply_preconditioner(x))
     assert err <= KERNEL_RTOL, err
+    # the line-solve factors follow coefficients changed through the ABI
+    for vec in ("AC", "AU"):
+        a = orc.get_vector(vec) * (1.0 + 0.1 * rng.standard_normal(x.shape))
+        gpu.set_vector(vec, a)
+        orc.set_vector(vec, a)
+        gpu.apply_preconditioner_device(dx.ptr, dy.ptr)
+        err = rel(dy.download(), orc.apply_preconditioner(x))
+        assert err <= KERNEL_RTOL, (vec, err)
     # compute_global_dot_product on the device
     got = gpu.dot_device(dx.ptr, dx.ptr)
     assert got == pytest.approx(float(np.sum(x * x)), rel=1e-13)
