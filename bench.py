@@ -27,11 +27,12 @@ from __graft_entry__ import load_package  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch of k_apply<true, ndot> measured by rocprofv3 FETCH_SIZE
-# and WRITE_SIZE passes (scaled by the counters' readings for a plain copy of
-# known size: x2.0 and x1.0), keyed by (nx, ny, nz, precision); summary and
-# commands in profiles/k_apply_mountain_wave_256x256x128_fp64_hbm_traffic.md.
-APPLY_TRAFFIC_BYTES = {(256, 256, 128, "fp64"): 2.113e9}
+# HBM bytes per launch of the dot-fused operator apply (k_stencil_pair<0, ndot>)
+# measured by rocprofv3 FETCH_SIZE and WRITE_SIZE passes (scaled by the
+# counters' readings for a plain copy of known size: x2.0 and x1.0), keyed by
+# (nx, ny, nz, precision); summary and commands in
+# profiles/k_stencil_pair_mountain_wave_256x256x128_fp64_hbm_traffic.md.
+APPLY_TRAFFIC_BYTES = {(256, 256, 128, "fp64"): 1.930e9}
 
 
 def parse():
@@ -89,7 +90,7 @@ def main():
         bytes_per_launch = 28 * 8 * n_cells
         avg_s = total_ms / launches * 1e-3
         achieved = bytes_per_launch / avg_s / 1e9
-        roofline = {"bound": "hbm", "kernel": "k_apply<true,ndot> (25-point operator apply + fused dot)",
+        roofline = {"bound": "hbm", "kernel": "k_stencil_pair<MODE_TOTAL, ndot> (25-point operator apply + fused dot products)",
                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": APPLY_TRAFFIC_BYTES.get((args.nx, args.nx, args.nz, args.precision)),
