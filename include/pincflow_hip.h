@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCF_ABI_VERSION 1
+#define PCF_ABI_VERSION 2
 
 /* ---- return codes ------------------------------------------------------ */
 #define PCF_OK 0
@@ -96,10 +96,18 @@ enum pcf_rhs_sponge {
 
 /* Arithmetic of the Poisson solve (an addition of this build; the reference
  * is FP64 throughout). */
+/* Initial guess of the mixed-precision solve (the FP64 solve starts from
+ * x = 0 as apply_bicgstab!.jl:44 does).  PREVIOUS: the solution of the same
+ * implicit stage in the previous time step, rescaled by the ratio of the
+ * time steps. */
+enum pcf_initial_guess {
+  PCF_GUESS_ZERO = 0,
+  PCF_GUESS_PREVIOUS = 1
+};
+
 enum pcf_precision {
   PCF_PRECISION_FP64 = 0, /* reference arithmetic                            */
-  PCF_PRECISION_MIXED = 1 /* FP32 operator/preconditioner inside FP64
-                             iterative refinement                             */
+  PCF_PRECISION_MIXED = 1 /* FP64 BiCGSTAB with the preconditioner in FP32   */
 };
 
 #define PCF_MAX_HILLS 32
@@ -176,6 +184,10 @@ typedef struct pcf_config {
   int32_t precision;                      /* enum pcf_precision           */
   double output_interval;                 /* s */
   double tmax;                            /* s */
+  /* Addition of this library (no reference counterpart): the initial guess
+   * of the mixed-precision solve. */
+  int32_t initial_guess;                  /* enum pcf_initial_guess; mixed only */
+  int32_t pad2;
 } pcf_config;
 
 /* Fields reachable through pcf_get_field / pcf_set_field (haloed, FP64). */
@@ -309,6 +321,7 @@ int pcf_get_vector(pcf_state* st, int vec, double* host_out);
 int pcf_set_vector(pcf_state* st, int vec, const double* host_in);
 /* Residual record of the last BiCGSTAB solve: final ||r||/||b|| (RMS norms). */
 int pcf_last_residual(const pcf_state* st, double* rel_residual);
+
 
 /* ---- kernel timing (instrumentation of this build; the reference has no
  * counterpart: it prints wall-clock times only) ---------------------------- */

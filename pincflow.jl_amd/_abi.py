@@ -13,7 +13,7 @@ from pathlib import Path
 PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "lib" / "libpincflow_hip.so"
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_HILLS = 32
 
 # return codes
@@ -37,6 +37,7 @@ PERT_NONE, PERT_COSINE_BUBBLE, PERT_VORTEX = 0, 1, 2
 LHS_SPONGE_NONE, LHS_SPONGE_SIN2 = 0, 1
 RHS_SPONGE_NONE, RHS_SPONGE_SIN2_TOP = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+GUESS_ZERO, GUESS_PREVIOUS = 0, 1
 
 FIELDS = [
     "RHO", "RHOP", "U", "V", "W", "PIP",
@@ -129,6 +130,8 @@ class PcfConfig(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("output_interval", ctypes.c_double),
         ("tmax", ctypes.c_double),
+        ("initial_guess", ctypes.c_int32),
+        ("pad2", ctypes.c_int32),
     ]
 
 

@@ -3,6 +3,7 @@
 // the state's stream; fields never leave HBM during a step.  Host and device
 // synchronise once per step for the CFL time step and a few times per
 // BiCGSTAB solve (see poisson.hip).
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -274,6 +275,7 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
   for (double** p : vecs)
     if ((rc = alloc(st, p, NV > (size_t)g.nz ? NV : (size_t)g.nz))) return fail(rc);
   if ((rc = alloc(st, &st->partials, (size_t)kMaxDots << 17))) return fail(rc);
+  if (cfg->precision == PCF_PRECISION_MIXED && (rc = alloc_mixed(st))) return fail(rc);
   if ((rc = alloc(st, &st->cfl_dev, 8))) return fail(rc);
   {
     void* p = nullptr;
@@ -392,7 +394,9 @@ int pcf_step(pcf_state* st, pcf_step_info* info) {
   TRY(copy_set(st, st->p0, cur, 6));  // backup_predictands
   TRY(explicit_lhs(st, 0.5 * dt, time));                      // (1)
   int it1 = 0, it2 = 0;
+  st->solve_slot = 0;
   int rc = implicit(st, 0.5 * dt, time, 1.0, &it1);          // (2)
+  st->solve_slot = -1;
   info->niter[0] = it1;
   if (rc) return rc;
   TRY(copy_set(st, st->p1, cur, 6));                          // p1 = predictands
@@ -405,7 +409,9 @@ int pcf_step(pcf_state* st, pcf_step_info* info) {
   TRY(explicit_rhs(st, 0.5 * dt, time));                      // (3)
   TRY(copy_set(st, st->p0, st->p1, 6));                       // p0 = p1
   TRY(explicit_lhs(st, dt, time));                            // (4)
+  st->solve_slot = 1;
   rc = implicit(st, 0.5 * dt, time, 2.0, &it2);               // (5)
+  st->solve_slot = -1;
   info->niter[1] = it2;
   if (rc) return rc;
   if (hipStreamSynchronize(st->stream) != hipSuccess) return PCF_E_HIP;
@@ -483,7 +489,7 @@ int pcf_apply_preconditioner(pcf_state* st, const double* d_in, double* d_out) {
 int pcf_apply_bicgstab(pcf_state* st, double tolref, int32_t* niter) {
   if (!st) return PCF_E_ARG;
   int it = 0;
-  int rc = apply_bicgstab(st, tolref, &it);
+  int rc = solve(st, tolref, 0.0, &it);
   if (niter) *niter = it;
   return rc;
 }
@@ -527,6 +533,7 @@ int pcf_set_vector(pcf_state* st, int vec, const double* host_in) {
   PCF_HIP(hipMemcpyAsync(p, host_in, st->g.nv() * sizeof(double), hipMemcpyHostToDevice, st->stream));
   PCF_HIP(hipStreamSynchronize(st->stream));
   if (vec == PCF_V_AC || vec == PCF_V_AD || vec == PCF_V_AU) st->pc_factors_valid = false;
+  if (vec >= PCF_V_AC) st->t32_valid = false;
   return PCF_OK;
 }
 

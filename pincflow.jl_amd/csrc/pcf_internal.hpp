@@ -133,6 +133,19 @@ struct pcf_state {
   double *pc_rhs, *pc_q, *pc_tmp;     // preconditioner work (pc_q: also scratch of the sponge)
   double *pc_P, *pc_E, *pc_Q;         // line-solve factors of the current operator
   bool pc_factors_valid = false;
+  // mixed precision (allocated when cfg.precision == PCF_PRECISION_MIXED):
+  // FP32 copies of the 22 horizontal operator coefficients and of the
+  // line-solve factors, FP32 preconditioner work, and z = M^-1 s
+  float* T32[pcf::T_COUNT] = {};
+  float *pc32_P = nullptr, *pc32_E = nullptr, *pc32_Q = nullptr;
+  float *m_pc_tmp = nullptr, *m_pc_cur = nullptr, *m_pc_rhs = nullptr;
+  double* bzpc = nullptr;
+  bool t32_valid = false;
+  // warm start of the mixed solve: solution / dt of each implicit stage of
+  // the previous time step (slot 0: first stage, 1: second stage)
+  double* m_prev[2] = {nullptr, nullptr};
+  bool m_prev_valid[2] = {false, false};
+  int solve_slot = -1;                // stage of the solve in progress, -1 outside pcf_step
   double* partials;                   // reduction partials
   pcf::SolverScalars* sc;             // device scalars
   pcf::SolverScalars* sc_host;        // pinned copy
@@ -201,9 +214,17 @@ int compute_operator(pcf_state* st, double dt, double rf);
 int apply_operator(pcf_state* st, const double* sin, double* ls, bool total);
 int apply_preconditioner(pcf_state* st, const double* sin, double* sout);
 int apply_bicgstab(pcf_state* st, double tolref, int* niter);
+int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess, double guess_scale, int* niter);
+int solve(pcf_state* st, double tolref, double dt, int* niter);  // apply_bicgstab, configured precision
+int pc_factors(pcf_state* st);
 int apply_corrector(pcf_state* st, double dt, double rf, int* niter);
 int correct(pcf_state* st, double dt, double rf);
 int dot(pcf_state* st, const double* a, const double* b, double* out);
+
+// ---- mixed.hip -------------------------------------------------------------
+int apply_bicgstab_mixed(pcf_state* st, double tolref, double dt, int* niter);
+int precondition_mixed(pcf_state* st, const double* sin, double* sout, const SolverScalars* sc);
+int alloc_mixed(pcf_state* st);
 
 // ---- explicit.hip --------------------------------------------------------
 int compute_fluxes(pcf_state* st);

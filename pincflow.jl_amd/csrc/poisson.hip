@@ -9,67 +9,14 @@
 // BiCGSTAB scalars (alpha, omega, beta, residual, convergence flag) in device
 // memory, so an iteration needs no host round trip.
 #include <cmath>
+#include <cstring>
 
 #include "pcf_device.hpp"
+#include "solver_common.hpp"
 
 namespace pcf {
 
 namespace {
-
-constexpr int kPartialStride = 1 << 17;  // partial slots per dot product
-
-// ---------------------------------------------------------------------------
-// block reduction of up to 3 values (blockDim.x * blockDim.y == 256)
-// ---------------------------------------------------------------------------
-template <int ND>
-__device__ __forceinline__ void block_partials(double (&val)[ND], double* partials, int block) {
-  __shared__ double red[ND][kReduceBlock / 64];
-  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-  const int lane = tid & 63, wid = tid >> 6;
-#pragma unroll
-  for (int d = 0; d < ND; ++d) {
-    double x = val[d];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-    if (lane == 0) red[d][wid] = x;
-  }
-  __syncthreads();
-  if (tid == 0) {
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      double s = red[d][0];
-      for (int q = 1; q < kReduceBlock / 64; ++q) s += red[d][q];
-      partials[(size_t)d * kPartialStride + block] = s;
-    }
-  }
-}
-
-// Sum nparts partials of dot slot d with one block of 1024 threads, in a
-// fixed order.
-__device__ double finalize_sum(const double* partials, int d, int nparts) {
-  __shared__ double red[16];
-  const int tid = threadIdx.x;
-  const int stride = blockDim.x;
-  const double* pp = partials + (size_t)d * kPartialStride;
-  double s = 0.0;
-  // the same order as one partial at a time, with 8 loads in flight
-  int q = tid;
-  for (; q + 7 * stride < nparts; q += 8 * stride) {
-    double v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = pp[q + u * stride];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) s += v[u];
-  }
-  for (; q < nparts; q += stride) s += pp[q];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if ((tid & 63) == 0) red[tid >> 6] = s;
-  __syncthreads();
-  double r = 0.0;
-  if (tid == 0)
-    for (int q = 0; q < (int)(blockDim.x / 64); ++q) r += red[q];
-  __syncthreads();
-  return r;  // valid in thread 0
-}
 
 // ---------------------------------------------------------------------------
 // compute_lhs! (compute_lhs!.jl:68-160)
@@ -779,12 +726,12 @@ __global__ void __launch_bounds__(256) k_stencil_pair(ApplyArgs A, const double*
     res.x = out[0];
     res.y = out[1];
     __builtin_nontemporal_store(res, (dbl2*)(A.ls + o));
-    if (NDOT >= 1) {
+    if constexpr (NDOT >= 1) {
       const dbl2 dd = *(const dbl2*)(d1 + o);
       val[0] = out[0] * dd.x;
       val[0] += out[1] * dd.y;
     }
-    if (NDOT == 2) {
+    if constexpr (NDOT == 2) {
       val[1] = out[0] * out[0];
       val[1] += out[1] * out[1];
     }
@@ -815,29 +762,26 @@ __global__ void k_bicg_init(const double* b, double* x, double* r0, double* r, d
   block_partials<1>(val, partials, blockIdx.x);
 }
 
-__global__ void k_bicg_start(const double* partials, int nparts, SolverScalars* sc, double nxyz, double tol,
-                             int have_rr0) {
-  double rr = finalize_sum(partials, 0, nparts);
-  if (threadIdx.x == 0) {
-    double res = sqrt(rr / nxyz);
-    sc->b_norm = res;
-    sc->res = res;
-    sc->tol = tol;
-    sc->niter = 0;
-    sc->rr0 = rr;  // <r0, r0> = <r, r0> for r = r0
-    sc->done = (res == 0.0 || res / res <= tol) ? 1 : 0;
-  }
-  (void)have_rr0;
+// x = c g (initial guess)
+__global__ void k_bicg_guess(const double* g, double* x, size_t n, double c) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    x[q] = c * g[q];
 }
 
-// alpha = <r, r0> / <v, r0>, from the partials of the A p kernel
-__global__ void k_bicg_alpha(const double* partials, int nparts, SolverScalars* sc) {
-  if (sc->done) return;
-  double vr0 = finalize_sum(partials, 0, nparts);
-  if (threadIdx.x == 0) {
-    sc->vr0 = vr0;
-    sc->alpha = sc->rr0 / vr0;
+// r0 = b - A x0 (ax = A x0), p = r = r0, partials <r, r> and <b, b>
+__global__ void k_bicg_init_guessed(const double* b, const double* ax, double* r0, double* r, double* p, size_t n,
+                                    double* partials) {
+  double val[2] = {0.0, 0.0};
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    const double bq = b[q];
+    const double v = bq - ax[q];
+    r0[q] = v;
+    r[q] = v;
+    p[q] = v;
+    val[0] += v * v;
+    val[1] += bq * bq;
   }
+  block_partials<2>(val, partials, blockIdx.x);
 }
 
 // s = r - alpha v
@@ -846,18 +790,6 @@ __global__ void k_bicg_s(const double* r, const double* v, double* s, size_t n, 
   const double alpha = sc->alpha;
   for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
     s[q] = r[q] - alpha * v[q];
-}
-
-// omega = <t, s> / <t, t>
-__global__ void k_bicg_omega(const double* partials, int nparts, SolverScalars* sc) {
-  if (sc->done) return;
-  double ts = finalize_sum(partials, 0, nparts);
-  double tt = finalize_sum(partials, 1, nparts);
-  if (threadIdx.x == 0) {
-    sc->ts = ts;
-    sc->tt = tt;
-    sc->omega = ts / tt;
-  }
 }
 
 // x += alpha p + omega s; r = s - omega t; partials <r, r>, <r, r0>
@@ -876,24 +808,24 @@ __global__ void k_bicg_xr(double* x, const double* p, const double* s, const dou
   block_partials<2>(val, partials, blockIdx.x);
 }
 
-// residual, convergence and beta (apply_bicgstab!.jl:116-142)
-__global__ void k_bicg_check(const double* partials, int nparts, SolverScalars* sc, double nxyz) {
+// The update of the textbook right-preconditioned form (van der Vorst 1992):
+// x += alpha y + omega z with y = M^-1 p, z = M^-1 s; r = s - omega t;
+// partials <r, r>, <r, r0>.  It equals the reference's form (x += alpha p +
+// omega s, then x := M^-1 x at the end) for a linear M, and stays consistent
+// with the residual recurrence when M is evaluated in FP32.
+__global__ void k_bicg_xr_yz(double* x, const double* y, const double* z, const double* s, const double* t, double* r,
+                             const double* r0, size_t n, double* partials, const SolverScalars* sc) {
   if (sc->done) return;
-  double rr = finalize_sum(partials, 0, nparts);
-  double rr0n = finalize_sum(partials, 1, nparts);
-  if (threadIdx.x == 0) {
-    double res = sqrt(rr / nxyz);
-    sc->rr = rr;
-    sc->res = res;
-    sc->niter = sc->niter + 1;
-    if (res / sc->b_norm <= sc->tol) {
-      sc->done = 1;
-    } else {
-      sc->beta = sc->alpha / sc->omega * rr0n / sc->rr0;
-      sc->rr0 = rr0n;
-    }
-    sc->rr0_new = rr0n;
+  const double alpha = sc->alpha, omega = sc->omega;
+  double val[2] = {0.0, 0.0};
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    x[q] += alpha * y[q] + omega * z[q];
+    double rn = s[q] - omega * t[q];
+    r[q] = rn;
+    val[0] += rn * rn;
+    val[1] += rn * r0[q];
   }
+  block_partials<2>(val, partials, blockIdx.x);
 }
 
 // p = r + beta (p - omega v)
@@ -1050,6 +982,7 @@ int compute_operator(pcf_state* st, double dt, double rf) {
                      rf, st->ph.damp);
   PCF_HIP(hipGetLastError());
   st->pc_factors_valid = false;
+  st->t32_valid = false;
   return PCF_OK;
 }
 
@@ -1105,7 +1038,7 @@ int apply_operator(pcf_state* st, const double* sin, double* ls, bool total) {
 }
 
 // Line-solve factors of the current operator (computed once per operator).
-static int ensure_pc_factors(pcf_state* st) {
+int pc_factors(pcf_state* st) {
   if (st->pc_factors_valid) return PCF_OK;
   const Geo& g = st->g;
   hipLaunchKernelGGL(k_pc_factors, dim3(blocks_for(g.nx, 64), g.ny), dim3(64), 0, st->stream, st->T.a[T_AC],
@@ -1119,7 +1052,7 @@ static int ensure_pc_factors(pcf_state* st) {
 // must not alias sin.  `sc` (may be null) lets a converged solve skip it.
 static int precondition(pcf_state* st, const double* sin, double* sout, const SolverScalars* sc) {
   const Geo& g = st->g;
-  int rc = ensure_pc_factors(st);
+  int rc = pc_factors(st);
   if (rc) return rc;
   dim3 tgrd(blocks_for(g.nx, 64), g.ny), tblk(64);
   const int iters = st->cfg.preconditioner_iterations;
@@ -1168,28 +1101,51 @@ int dot(pcf_state* st, const double* a, const double* b, double* out) {
 // apply_bicgstab!.jl:24-150.  The host enqueues iterations and looks at the
 // convergence flag of iteration j - 1 while iteration j is queued; kernels of
 // iterations past convergence see the flag and return at once.
-int apply_bicgstab(pcf_state* st, double tolref, int* niter_out) {
+//
+// mixed: the preconditioner runs in FP32 (mixed.hip) and the solution is
+// accumulated in the textbook form x += alpha M^-1 p + omega M^-1 s, which
+// keeps it consistent with the FP64 residual recurrence; guess (may be null):
+// the solve starts from x0 = guess_scale * guess instead of 0.
+int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess, double guess_scale,
+                  int* niter_out) {
   const Geo& g = st->g;
   const pcf_config& n = st->cfg;
   const size_t N = g.nv();
   const double nxyz = (double)n.x_size * n.y_size * n.z_size;
   const double tol = n.tolerance_is_relative ? n.tolerance : n.tolerance / tolref;
   SolverScalars* sc = st->sc;
-  hipLaunchKernelGGL(k_bicg_init, dim3(kVecGrid), dim3(kReduceBlock), 0, st->stream, st->rhs, st->solution, st->br0,
-                     st->br, st->bp, N, st->partials);
-  hipLaunchKernelGGL(k_bicg_start, dim3(1), dim3(1024), 0, st->stream, st->partials, kVecGrid, sc, nxyz, tol, 0);
+  const bool pc = n.preconditioner != 0;
+  auto prec = [&](const double* in, double* out, const SolverScalars* s) -> int {
+    return mixed ? precondition_mixed(st, in, out, s) : precondition(st, in, out, s);
+  };
+  if (guess) {
+    hipLaunchKernelGGL(k_bicg_guess, dim3(kVecGrid), dim3(256), 0, st->stream, guess, st->solution, N, guess_scale);
+    launch_total<0>(st, st->solution, st->bt, nullptr, nullptr);
+    hipLaunchKernelGGL(k_bicg_init_guessed, dim3(kVecGrid), dim3(kReduceBlock), 0, st->stream, st->rhs, st->bt,
+                       st->br0, st->br, st->bp, N, st->partials);
+  } else {
+    hipLaunchKernelGGL(k_bicg_init, dim3(kVecGrid), dim3(kReduceBlock), 0, st->stream, st->rhs, st->solution,
+                       st->br0, st->br, st->bp, N, st->partials);
+  }
+  hipLaunchKernelGGL(k_bicg_start, dim3(1), dim3(1024), 0, st->stream, st->partials, kVecGrid, sc, nxyz, tol,
+                     guess ? 1 : 0);
   PCF_HIP(hipGetLastError());
   PCF_HIP(hipMemcpyAsync(st->sc_host, sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, st->stream));
   PCF_HIP(hipStreamSynchronize(st->stream));
   if (st->sc_host->done) {
-    st->last_residual = 0.0;
+    st->last_residual = st->sc_host->b_norm > 0.0 ? st->sc_host->res / st->sc_host->b_norm : 0.0;
     *niter_out = 0;
     return PCF_OK;
   }
   const int aparts = apply_parts(g);
   if (aparts > kPartialStride) return PCF_E_ARG;
   const int maxit = n.poisson_iterations;
-  SolverScalars* hs = st->sc_host;  // pinned, two slots
+  // y = M^-1 p and z = M^-1 s are both kept for the textbook update
+  double* ybuf = st->bvpc;
+  double* zbuf = mixed ? st->bzpc : st->bvpc;
+  const bool yz = mixed && pc;
+  SolverScalars* hs = st->sc_host;  // pinned, two slots (no copy is pending here)
+  std::memset(hs, 0, 2 * sizeof(SolverScalars));
   hipEvent_t ev[2];
   PCF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   PCF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
@@ -1197,8 +1153,8 @@ int apply_bicgstab(pcf_state* st, double tolref, int* niter_out) {
   int rc = PCF_OK;
   for (int jb = 1; jb <= maxit; ++jb) {
     // v = A M^-1 p, <v, r0>
-    if ((rc = n.preconditioner ? precondition(st, st->bp, st->bvpc, sc) : PCF_OK)) break;
-    const double* vin = n.preconditioner ? st->bvpc : st->bp;
+    if ((rc = pc ? prec(st->bp, ybuf, sc) : PCF_OK)) break;
+    const double* vin = pc ? ybuf : st->bp;
     {
       ProfScope prof(st, PCF_K_APPLY_TOTAL_DOT);
       launch_total<1>(st, vin, st->bv, st->br0, sc);
@@ -1209,8 +1165,8 @@ int apply_bicgstab(pcf_state* st, double tolref, int* niter_out) {
       hipLaunchKernelGGL(k_bicg_s, dim3(kVecGrid), dim3(256), 0, st->stream, st->br, st->bv, st->bs, N, sc);
     }
     // t = A M^-1 s, <t, s>, <t, t>
-    if ((rc = n.preconditioner ? precondition(st, st->bs, st->bvpc, sc) : PCF_OK)) break;
-    const double* sin2 = n.preconditioner ? st->bvpc : st->bs;
+    if ((rc = pc ? prec(st->bs, zbuf, sc) : PCF_OK)) break;
+    const double* sin2 = pc ? zbuf : st->bs;
     {
       ProfScope prof(st, PCF_K_APPLY_TOTAL_DOT);
       launch_total<2>(st, sin2, st->bt, st->bs, sc);
@@ -1218,8 +1174,12 @@ int apply_bicgstab(pcf_state* st, double tolref, int* niter_out) {
     {
       ProfScope prof(st, PCF_K_BICG_VECTOR);
       hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1024), 0, st->stream, st->partials, aparts, sc);
-      hipLaunchKernelGGL(k_bicg_xr, dim3(kVecGrid), dim3(kReduceBlock), 0, st->stream, st->solution, st->bp, st->bs,
-                         st->bt, st->br, st->br0, N, st->partials, sc);
+      if (yz)
+        hipLaunchKernelGGL(k_bicg_xr_yz, dim3(kVecGrid), dim3(kReduceBlock), 0, st->stream, st->solution, ybuf, zbuf,
+                           st->bs, st->bt, st->br, st->br0, N, st->partials, sc);
+      else
+        hipLaunchKernelGGL(k_bicg_xr, dim3(kVecGrid), dim3(kReduceBlock), 0, st->stream, st->solution, st->bp,
+                           st->bs, st->bt, st->br, st->br0, N, st->partials, sc);
       hipLaunchKernelGGL(k_bicg_check, dim3(1), dim3(1024), 0, st->stream, st->partials, kVecGrid, sc, nxyz);
       hipLaunchKernelGGL(k_bicg_p, dim3(kVecGrid), dim3(256), 0, st->stream, st->bp, st->br, st->bv, N, sc);
     }
@@ -1258,13 +1218,25 @@ int apply_bicgstab(pcf_state* st, double tolref, int* niter_out) {
   SolverScalars fin = hs[0].niter >= hs[1].niter ? hs[0] : hs[1];
   st->last_residual = fin.res / fin.b_norm;
   *niter_out = converged_at;
-  // final x := M^-1 x (apply_bicgstab!.jl:132-135)
-  if (n.preconditioner) {
+  // final x := M^-1 x (apply_bicgstab!.jl:132-135); the textbook form has
+  // accumulated M^-1 already
+  if (pc && !yz) {
     hipLaunchKernelGGL(k_copy, dim3(kVecGrid), dim3(256), 0, st->stream, st->bs, st->solution, N);
     if ((rc = precondition(st, st->bs, st->solution, nullptr))) return rc;
   }
   PCF_HIP(hipGetLastError());
   return PCF_OK;
+}
+
+int apply_bicgstab(pcf_state* st, double tolref, int* niter_out) {
+  return bicgstab_core(st, tolref, false, nullptr, 0.0, niter_out);
+}
+
+// apply_bicgstab! in the configured precision; dt (0 if unknown) scales the
+// warm start of the mixed solve.
+int solve(pcf_state* st, double tolref, double dt, int* niter) {
+  if (st->cfg.precision == PCF_PRECISION_MIXED) return apply_bicgstab_mixed(st, tolref, dt, niter);
+  return apply_bicgstab(st, tolref, niter);
 }
 
 // apply_corrector!.jl:38-66 with solve_poisson!.jl:39-75
@@ -1278,7 +1250,7 @@ int apply_corrector(pcf_state* st, double dt, double rf, int* niter) {
   }
   const double dtinv = 1.0 / dt;
   if ((rc = compute_operator(st, dt, rf))) return rc;
-  rc = apply_bicgstab(st, tolref, niter);
+  rc = solve(st, tolref, dt, niter);
   if (rc) return rc;
   const Geo& g = st->g;
   hipLaunchKernelGGL(k_solution_to_dpip, dim3(blocks_for(g.nx, 64), g.ny, g.nz), dim3(64), 0, st->stream,

@@ -156,6 +156,13 @@ int host_setup(pcf_state* st, std::string& err) {
   }
   if (n.model != PCF_BOUSSINESQ && n.model != PCF_PSEUDO_INCOMPRESSIBLE) { err = "Invalid model"; return PCF_E_OPTION; }
   if (n.preconditioner_iterations < 1 && n.preconditioner) { err = "preconditioner_iterations < 1"; return PCF_E_CONFIG; }
+  if (n.precision != PCF_PRECISION_FP64 && n.precision != PCF_PRECISION_MIXED) { err = "Invalid precision"; return PCF_E_OPTION; }
+  if (n.initial_guess != PCF_GUESS_ZERO && n.initial_guess != PCF_GUESS_PREVIOUS) { err = "Invalid initial_guess"; return PCF_E_OPTION; }
+  if (n.initial_guess != PCF_GUESS_ZERO && n.precision != PCF_PRECISION_MIXED) {
+    err = "initial_guess other than zero needs precision = mixed (the FP64 solve starts from zero)";
+    return PCF_E_CONFIG;
+  }
+
 
   // Constants.jl:108-182
   const double gamma = 1.4, gammainv = 1.0 / gamma, kappa = (gamma - 1.0) / gamma, kappainv = 1.0 / kappa;

@@ -143,8 +143,12 @@ class PoissonNamelist:
     preconditioner_iterations: int = 2
     initial_cleaning: bool = True
     tolerance_is_relative: bool = False
-    # addition of this build: "fp64" (reference arithmetic) or "mixed"
+    # additions of this build: "fp64" (reference arithmetic) or "mixed" (the
+    # FP64 BiCGSTAB with the preconditioner evaluated in FP32)
     precision: str = "fp64"
+    # "zero" (as the reference) or "previous" (the same stage's solution of
+    # the previous time step); used by the mixed solve only
+    initial_guess: str = "zero"
 
 
 @dataclass
@@ -340,6 +344,8 @@ def to_config(nl: Namelists) -> _abi.PcfConfig:
     c.nout = o.nout
     c.output_interval = o.output_interval
     c.tmax = o.tmax
+    c.initial_guess = _option({"zero": _abi.GUESS_ZERO, "previous": _abi.GUESS_PREVIOUS}, po.initial_guess,
+                              "initial_guess")
     return c
 
 

@@ -1,0 +1,112 @@
+"""The mixed-precision pressure solve (precision = "mixed") on the device.
+
+The FP64 BiCGSTAB of the reference runs with its preconditioner evaluated in
+FP32, the solution accumulated in the textbook form x += alpha M^-1 p +
+omega M^-1 s. The acceptance test is the reference criterion itself,
+||b - A x|| / ||b|| <= tol (RMS norms, tol = tolerance / tolref unless
+relative; apply_bicgstab!.jl:49-53,123), with A applied by the FP64 oracle.
+Against the FP64 solution it may differ only by what both criteria allow:
+||A (x_mixed - x_fp64)|| <= 2 tol ||b||.
+
+Whole steps are compared with the FP64 library path, which is bit-matched to
+the oracle (test_gpu_parity.py).  The tolerance FIELD_RTOL below covers a
+solve that stops anywhere inside the 1e-8 criterion: both paths meet it, and
+the 1e-8 residual maps to about 1e-7 of the pressure correction.  The
+Exner-pressure fluctuation is compared up to a constant, the null space of
+the operator.
+"""
+import hipmem  # noqa: F401  (ensures the HIP runtime is loadable)
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELD_RTOL = 1e-5
+
+
+def rms(a):
+    return float(np.sqrt(np.mean(a * a)))
+
+
+def configs(pcf):
+    return {
+        "mountain_wave_3d": pcf.examples.mountain_wave_3d(x_size=16, y_size=12, z_size=12, iterations=3),
+        "periodic_hill": pcf.examples.periodic_hill(x_size=24, z_size=20, iterations=3, output_steps=True),
+        "mountain_wave_odd": pcf.examples.mountain_wave(x_size=11, y_size=6, z_size=9),
+        "cold_bubble": pcf.examples.cold_bubble(x_size=12, z_size=10),
+    }
+
+
+CASES = ["mountain_wave_3d", "periodic_hill", "mountain_wave_odd", "cold_bubble"]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_mixed_solve_meets_reference_criterion(pcf, abi, oracle_mod, name):
+    nl = configs(pcf)[name]
+    nl.poisson.precision = "mixed"
+    gpu = pcf.State(nl)
+    nl64 = configs(pcf)[name]
+    ref = pcf.State(nl64)
+    orc = oracle_mod.Oracle(pcf.to_config(nl64), abi)
+    dt, rf = 0.8, 1.0
+    rng = np.random.default_rng(7)
+    for f in ("U", "V", "W"):  # a divergent flow, so that b != 0 in every case
+        a = orc.get(f)
+        a = a + 0.1 * max(float(np.max(np.abs(a))), 1.0) * rng.standard_normal(a.shape)
+        for s in (gpu, ref, orc):
+            s.set(f, a)
+    for s in (gpu, ref, orc):
+        # periodic and wall halos keep b in the range of the singular operator
+        s.set_boundaries(abi.BOUNDARY_PREDICTANDS)
+        s.compute_sponges(dt, 3.0)
+    tolrefs = [s.compute_lhs() for s in (gpu, ref, orc)]
+    assert tolrefs[0] == pytest.approx(tolrefs[2], rel=1e-12)
+    for s in (gpu, ref, orc):
+        s.compute_operator(dt, rf)
+    n_mixed = gpu.apply_bicgstab(tolrefs[0])
+    n_fp64 = ref.apply_bicgstab(tolrefs[1])
+    print(name, "iterations mixed", n_mixed, "fp64", n_fp64)
+    # the FP32 preconditioner changes the iteration count only slightly
+    assert abs(n_mixed - n_fp64) <= max(3, n_fp64 // 5)
+    assert n_mixed > 0 and n_fp64 > 0
+    tol = nl.poisson.tolerance / tolrefs[0]
+    b = orc.get_vector("RHS")
+    x_m = gpu.get_vector("SOLUTION")
+    x_d = ref.get_vector("SOLUTION")
+    res = rms(b - orc.apply_operator(x_m)) / rms(b)
+    assert res <= tol, (res, tol)
+    assert gpu.last_residual() <= tol
+    assert rms(orc.apply_operator(x_m - x_d)) <= 2 * tol * rms(b)
+
+
+@pytest.mark.parametrize("guess", ["zero", "previous"])
+@pytest.mark.parametrize("name", ["mountain_wave_3d", "periodic_hill"])
+def test_mixed_steps_track_fp64(pcf, name, guess):
+    nl = configs(pcf)[name]
+    nl.poisson.precision = "mixed"
+    nl.poisson.initial_guess = guess
+    mixed = pcf.State(nl)
+    plain = pcf.State(configs(pcf)[name])
+    for s in (mixed, plain):
+        s.initial_cleaning()
+    for _ in range(3):
+        a = mixed.step()
+        b = plain.step()
+        assert a.dt == pytest.approx(b.dt, rel=1e-6)
+    for f in ("RHOP", "U", "V", "W", "PIP"):
+        x, y = mixed.get(f), plain.get(f)
+        d = x - y
+        if f == "PIP":
+            # a solve determines the pressure correction up to a constant (the
+            # operator's null space); different initial guesses pick different ones
+            d = d - float(np.mean(d[3:-3, 3:-3, 3:-3]))
+        scale = max(float(np.max(np.abs(y))), 1e-30)
+        err = float(np.max(np.abs(d))) / scale
+        assert err <= FIELD_RTOL, (f, err)
+
+
+def test_fp64_rejects_a_warm_start(pcf):
+    nl = configs(pcf)["mountain_wave_odd"]
+    nl.poisson.initial_guess = "previous"  # the FP64 solve starts from zero, as the reference
+    with pytest.raises(pcf.PinCFlowError):
+        pcf.State(nl)
