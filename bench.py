@@ -46,6 +46,11 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--cpu-sample-nx", type=int, default=128)
+    p.add_argument("--pc-iterations", type=int, default=2,
+                   help="pseudo-time steps of the preconditioner (the reference's default is 2)")
+    p.add_argument("--dtau", type=float, default=1.0, help="pseudo-time step of the preconditioner (default 1)")
+    p.add_argument("--preconditioner", default="line_relaxation", choices=["line_relaxation", "multigrid"],
+                   help="mixed precision: the reference's line relaxation or the multigrid V-cycle")
     p.add_argument("--initial-guess", default="zero", choices=["zero", "previous"],
                    help="mixed precision: start each solve from zero or from the same stage's previous solution")
     p.add_argument("--no-fp64-check", action="store_true",
@@ -63,6 +68,9 @@ def main():
     nl = pcf.examples.mountain_wave_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz,
                                        iterations=args.steps + args.warmup, precision=args.precision)
     nl.poisson.initial_guess = args.initial_guess
+    nl.poisson.preconditioner_type = args.preconditioner
+    nl.poisson.preconditioner_iterations = args.pc_iterations
+    nl.poisson.dtau = args.dtau
     t0 = time.perf_counter()
     st = pcf.State(nl)
     t_setup = time.perf_counter() - t0
@@ -124,9 +132,11 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f64 (preconditioner f32)",
         "data": "synthetic",
         "config": {"workload": (f"pseudo-incompressible mountain wave {args.nx}x{args.nx}x{args.nz}, " +
-                                ("FP64 BiCGSTAB with the reference preconditioner evaluated in FP32, initial "
-                                 f"guess {args.initial_guess}" if mixed else "plain preconditioned BiCGSTAB (FP64)")),
+                                (f"FP64 BiCGSTAB with the {args.preconditioner.replace('_', ' ')} preconditioner "
+                                 f"evaluated in FP32, initial guess {args.initial_guess}" if mixed
+                                 else "plain preconditioned BiCGSTAB (FP64)")),
                    "cells": n_cells, "poisson_tolerance": 1e-8,
+                   "preconditioner_iterations": args.pc_iterations, "dtau": args.dtau,
                    "bicgstab_iterations_per_solve": round(sum(sum(i) for i in iters) / (2 * len(iters)), 1),
                    "parallelism": "single GPU"},
         "roofline": roofline,

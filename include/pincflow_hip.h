@@ -105,6 +105,16 @@ enum pcf_initial_guess {
   PCF_GUESS_PREVIOUS = 1
 };
 
+/* Preconditioner of the mixed-precision solve.  LINE_RELAXATION: the
+ * reference's (apply_preconditioner!.jl:40-124), evaluated in FP32.
+ * MULTIGRID: V-cycles over Galerkin 2 x 2 horizontal aggregations of the
+ * operator with the reference's z-line pseudo-time step as the smoother on
+ * every level (FP32). */
+enum pcf_preconditioner_type {
+  PCF_PC_LINE_RELAXATION = 0,
+  PCF_PC_MULTIGRID = 1
+};
+
 enum pcf_precision {
   PCF_PRECISION_FP64 = 0, /* reference arithmetic                            */
   PCF_PRECISION_MIXED = 1 /* FP64 BiCGSTAB with the preconditioner in FP32   */
@@ -184,10 +194,10 @@ typedef struct pcf_config {
   int32_t precision;                      /* enum pcf_precision           */
   double output_interval;                 /* s */
   double tmax;                            /* s */
-  /* Addition of this library (no reference counterpart): the initial guess
-   * of the mixed-precision solve. */
-  int32_t initial_guess;                  /* enum pcf_initial_guess; mixed only */
-  int32_t pad2;
+  /* Additions of this library (no reference counterpart), mixed precision
+   * only: the initial guess and the preconditioner of the solve. */
+  int32_t initial_guess;                  /* enum pcf_initial_guess       */
+  int32_t preconditioner_type;            /* enum pcf_preconditioner_type */
 } pcf_config;
 
 /* Fields reachable through pcf_get_field / pcf_set_field (haloed, FP64). */

@@ -38,6 +38,7 @@ LHS_SPONGE_NONE, LHS_SPONGE_SIN2 = 0, 1
 RHS_SPONGE_NONE, RHS_SPONGE_SIN2_TOP = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 GUESS_ZERO, GUESS_PREVIOUS = 0, 1
+PC_LINE_RELAXATION, PC_MULTIGRID = 0, 1
 
 FIELDS = [
     "RHO", "RHOP", "U", "V", "W", "PIP",
@@ -131,7 +132,7 @@ class PcfConfig(ctypes.Structure):
         ("output_interval", ctypes.c_double),
         ("tmax", ctypes.c_double),
         ("initial_guess", ctypes.c_int32),
-        ("pad2", ctypes.c_int32),
+        ("preconditioner_type", ctypes.c_int32),
     ]
 
 

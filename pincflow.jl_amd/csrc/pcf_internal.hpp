@@ -68,6 +68,16 @@ struct Tensor {
   double* a[T_COUNT];
 };
 
+// One level of the multigrid preconditioner (FP32): operator, line-solve
+// factors and vectors; fx, fy: aggregation factors to the next level.
+struct MgLevel {
+  int nx = 0, ny = 0, nz = 0, fx = 1, fy = 1;
+  float deta = 0.0f;
+  float* a[T_COUNT] = {};
+  float *P = nullptr, *E = nullptr, *Q = nullptr;
+  float *b = nullptr, *s = nullptr, *rhs = nullptr, *col = nullptr;
+};
+
 // BiCGSTAB scalars kept on the device between kernels (apply_bicgstab!.jl:24-150).
 struct SolverScalars {
   double rr0;      // <r, r0> of the current iterate
@@ -141,6 +151,7 @@ struct pcf_state {
   float *m_pc_tmp = nullptr, *m_pc_cur = nullptr, *m_pc_rhs = nullptr;
   double* bzpc = nullptr;
   bool t32_valid = false;
+  std::vector<pcf::MgLevel> mg;       // multigrid levels (level 0 aliases the buffers above)
   // warm start of the mixed solve: solution / dt of each implicit stage of
   // the previous time step (slot 0: first stage, 1: second stage)
   double* m_prev[2] = {nullptr, nullptr};

@@ -1085,7 +1085,12 @@ static int precondition(pcf_state* st, const double* sin, double* sout, const So
   return PCF_OK;
 }
 
-int apply_preconditioner(pcf_state* st, const double* sin, double* sout) { return precondition(st, sin, sout, nullptr); }
+// apply_preconditioner! in the configured precision (and, for the mixed
+// solve, the configured preconditioner).
+int apply_preconditioner(pcf_state* st, const double* sin, double* sout) {
+  if (st->cfg.precision == PCF_PRECISION_MIXED) return precondition_mixed(st, sin, sout, nullptr);
+  return precondition(st, sin, sout, nullptr);
+}
 
 int dot(pcf_state* st, const double* a, const double* b, double* out) {
   size_t n = st->g.nv();

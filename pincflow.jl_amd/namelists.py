@@ -149,6 +149,8 @@ class PoissonNamelist:
     # "zero" (as the reference) or "previous" (the same stage's solution of
     # the previous time step); used by the mixed solve only
     initial_guess: str = "zero"
+    # "line_relaxation" (the reference's) or "multigrid"; mixed solve only
+    preconditioner_type: str = "line_relaxation"
 
 
 @dataclass
@@ -346,6 +348,8 @@ def to_config(nl: Namelists) -> _abi.PcfConfig:
     c.tmax = o.tmax
     c.initial_guess = _option({"zero": _abi.GUESS_ZERO, "previous": _abi.GUESS_PREVIOUS}, po.initial_guess,
                               "initial_guess")
+    c.preconditioner_type = _option({"line_relaxation": _abi.PC_LINE_RELAXATION, "multigrid": _abi.PC_MULTIGRID},
+                                    po.preconditioner_type, "preconditioner_type")
     return c
 
 

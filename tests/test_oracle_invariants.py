@@ -355,3 +355,51 @@ def test_viscous_fluxes_of_linear_shear(pcf, oracle_mod, flow, viscous):
         d = (out[1][n] - out[0][n])[core]
         want = expected if n in viscous else 0.0
         assert np.abs(d - want).max() <= 1e-10 * abs(expected), (n, np.abs(d - want).max())
+
+
+def test_multigrid_reference_accelerates_bicgstab(pcf, oracle_mod):
+    """The FP64 numpy restatement of the multigrid preconditioner (used to
+    check the GPU V-cycle) cuts BiCGSTAB's iterations against the
+    reference's line-relaxation preconditioner on the same operator."""
+    import mg_reference as mg
+
+    nl = pcf.examples.mountain_wave_3d(x_size=32, y_size=32, z_size=16, iterations=1)
+    o = make(pcf, oracle_mod, nl)
+    o.compute_sponges(0.05, 0.1)
+    o.compute_operator(0.05, 1.0)
+    C = {n: o.get_vector(n) for n in mg.NAMES}
+    c = o.constants()
+    b = o.apply_operator(np.random.default_rng(12).standard_normal((o.nx, o.ny, o.nz)))
+    s = np.random.default_rng(13).standard_normal(b.shape)
+    assert np.abs(mg.apply(C, s) - o.apply_operator(s)).max() <= 1e-13 * np.abs(o.apply_operator(s)).max()
+    lv = mg.levels(C, c["dx"], c["dy"])
+    assert [L["C"]["AC"].shape[:2] for L in lv] == [(32, 32), (16, 16), (8, 8), (4, 4)]
+    # one smoothing step from zero on the finest level is one pseudo-time
+    # step of the reference preconditioner
+    one = mg.smooth(lv[0], np.zeros_like(b), b, 2)
+    assert np.abs(one - o.apply_preconditioner(b)).max() <= 1e-12 * np.abs(one).max()
+
+    def bicgstab(M, tol=1e-8):
+        rms = lambda v: np.sqrt(np.mean(v * v))  # noqa: E731
+        x, r, p = np.zeros_like(b), b.copy(), b.copy()
+        r0, rho = b.copy(), float(np.vdot(b, b))
+        for k in range(1, 500):
+            y = M(p)
+            v = o.apply_operator(y)
+            alpha = rho / float(np.vdot(v, r0))
+            s_ = r - alpha * v
+            z = M(s_)
+            t = o.apply_operator(z)
+            omega = float(np.vdot(t, s_)) / float(np.vdot(t, t))
+            x, r = x + alpha * y + omega * z, s_ - omega * t
+            if rms(r) / rms(b) <= tol:
+                return x, k
+            rho_n = float(np.vdot(r, r0))
+            p = r + rho_n / rho * alpha / omega * (p - omega * v)
+            rho = rho_n
+        raise AssertionError("no convergence")
+
+    _, k_line = bicgstab(o.apply_preconditioner)
+    x, k_mg = bicgstab(lambda v: mg.vcycle(lv, v))
+    assert np.sqrt(np.mean((b - o.apply_operator(x)) ** 2)) <= 1.1e-8 * np.sqrt(np.mean(b ** 2))
+    assert k_mg * 2 < k_line, (k_mg, k_line)
