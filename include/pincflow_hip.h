@@ -264,6 +264,12 @@ int pcf_create(const pcf_config* cfg, pcf_state** out);
 int pcf_destroy(pcf_state* st);
 /* Interior and haloed extents: dims[0..5] = nx, ny, nz, nxx, nyy, nzz. */
 int pcf_dims(const pcf_state* st, int32_t dims[6]);
+/* Model time in seconds and the number of steps taken (integrate.jl:217-241). */
+int pcf_get_time(const pcf_state* st, double* time_s, int32_t* itime);
+/* Restart support (integrate.jl:168-196): set the model time to time_s
+ * seconds and the next output time to time_s + output_interval.  Fails with
+ * "Restart error: tmax < time!" (PCF_E_CONFIG) past tmax. */
+int pcf_set_time(pcf_state* st, double time_s);
 
 /* ---- time integration (Integration/) ------------------------------------ */
 /* Initial divergence cleaning, integrate.jl:136-156. */

@@ -156,6 +156,8 @@ SIGNATURES = [
     ("pcf_create", _I, [ctypes.POINTER(PcfConfig), ctypes.POINTER(_P)]),
     ("pcf_destroy", _I, [_P]),
     ("pcf_dims", _I, [_P, _PI32]),
+    ("pcf_get_time", _I, [_P, _PD, _PI32]),
+    ("pcf_set_time", _I, [_P, _D]),
     ("pcf_initial_cleaning", _I, [_P, _PI32]),
     ("pcf_step", _I, [_P, ctypes.POINTER(PcfStepInfo)]),
     ("pcf_compute_time_step", _I, [_P, _PD]),

@@ -323,6 +323,24 @@ int pcf_dims(const pcf_state* st, int32_t d[6]) {
   return PCF_OK;
 }
 
+int pcf_get_time(const pcf_state* st, double* time_s, int32_t* itime) {
+  if (!st || !time_s) return PCF_E_ARG;
+  *time_s = st->time * st->tref;
+  if (itime) *itime = st->itime;
+  return PCF_OK;
+}
+
+int pcf_set_time(pcf_state* st, double time_s) {  // integrate.jl:168-196
+  if (!st) return PCF_E_ARG;
+  if (st->cfg.tmax < time_s) {
+    set_error("Restart error: tmax < time!");
+    return PCF_E_CONFIG;
+  }
+  st->time = time_s / st->tref;
+  st->nextoutputtime = time_s + st->cfg.output_interval;
+  return PCF_OK;
+}
+
 int pcf_initial_cleaning(pcf_state* st, int32_t* niter) {  // integrate.jl:136-156
   if (!st) return PCF_E_ARG;
   TRY(set_boundaries_predictands(st));

@@ -109,3 +109,44 @@ def norms(records: List[Dict[str, np.ndarray]]):
         l2[key] = float(np.sqrt(np.sum(a * a)))
         linf[key] = float(np.max(np.abs(a))) if a.size else 0.0
     return l2, linf
+
+
+# datasets written once, with the first record (write_output.jl:73-165)
+STATIC = ("x", "y", "z", "ztilde", "rhobar", "thetabar", "n2", "p")
+
+
+def write_file(path: str, records: List[Dict[str, np.ndarray]]) -> None:
+    """Store output records the way create_output / write_output lay out the
+    reference's HDF5 file (create_output.jl, write_output.jl:73-379): the
+    grid and background datasets once, "t" as a vector over records and every
+    time-dependent field as an (x, y, z, t) array.  The container is a numpy
+    .npz archive (no HDF5 library in this environment); dataset names and
+    units are the reference's."""
+    out: Dict[str, np.ndarray] = {}
+    keys = []
+    for r in records:
+        keys += [k for k in r if k not in keys]
+    for k in keys:
+        parts = [r[k] for r in records if k in r]
+        if k in STATIC:
+            out[k] = parts[0]
+        elif k == "t":
+            out[k] = np.concatenate(parts).astype(np.float32)
+        else:
+            out[k] = np.stack(parts, axis=-1)
+    with open(path, "wb") as f:
+        np.savez(f, **out)
+
+
+def read_input(path: str, iin: int = -1):
+    """read_input! (read_input!.jl:13-105) for the datasets this library
+    integrates: returns (time in s, {rhop, us, vs, wts, pip}) of record iin
+    (1-based as in the reference; -1 = the last record)."""
+    with np.load(path, allow_pickle=False) as data:
+        idx = -1 if iin == -1 else iin - 1
+        time_s = float(data["t"][idx])
+        fields = {k: np.array(data[k][..., idx]) for k in ("rhop", "us", "vs", "wts", "pip") if k in data}
+    missing = {"rhop", "us", "vs", "wts", "pip"} - set(fields)
+    if missing:
+        raise ValueError(f"restart file {path} lacks {sorted(missing)} (write it with prepare_restart = true)")
+    return time_s, fields

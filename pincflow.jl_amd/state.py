@@ -186,18 +186,54 @@ class State:
                 out[name] = (int(n.value), float(ms.value))
         return out
 
+    # ---- time and restart ------------------------------------------------------
+    def time(self):
+        """(model time in s, steps taken)."""
+        t = ctypes.c_double(0.0)
+        n = ctypes.c_int32(0)
+        self._check(self.lib.pcf_get_time(self.h, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+    def restart(self, time_s: float, fields: dict):
+        """Continue from a stored record (integrate.jl:168-186, read_input!.jl):
+        fields are the dimensional datasets rhop, us, vs, wts and pip of
+        write_output; the interior of the predictands is replaced, rho follows
+        rhop in the pseudo-incompressible model, then the boundaries are set."""
+        from . import output
+
+        nl = self.namelists
+        c = output.constants(nl)
+        d = nl.domain
+        ii = slice(d.nbx, d.nbx + d.x_size)
+        jj = slice(d.nby, d.nby + d.y_size)
+        kk = slice(d.nbz, d.nbz + d.z_size)
+        for name, key, scale in (("RHOP", "rhop", c["rhoref"]), ("U", "us", c["uref"]), ("V", "vs", c["uref"]),
+                                 ("W", "wts", c["uref"]), ("PIP", "pip", 1.0)):
+            a = self.get(name)
+            a[ii, jj, kk] = np.asarray(fields[key], dtype=np.float64) / scale
+            self.set(name, a)
+            if name == "RHOP" and nl.atmosphere.model != "Boussinesq":
+                r = self.get("RHO")
+                r[ii, jj, kk] = a[ii, jj, kk]
+                self.set("RHO", r)
+        self.set_boundaries(_abi.BOUNDARY_PREDICTANDS)
+        self._check(self.lib.pcf_set_time(self.h, time_s))
+
     def dot_device(self, d_a: int, d_b: int) -> float:
         r = ctypes.c_double(0.0)
         self._check(self.lib.pcf_dot(self.h, ctypes.c_void_p(d_a), ctypes.c_void_p(d_b), ctypes.byref(r)))
         return r.value
 
 
-def integrate(namelists: Namelists, records: bool = True, max_steps: int = 1 << 30
+def integrate(namelists: Namelists, records: bool = True, max_steps: int = 1 << 30, write: bool = False
               ) -> Tuple[State, List[dict], List[tuple]]:
     """integrate(namelists) (integrate.jl:68-415) on the device.
 
-    Returns the state, the output records (datasets of write_output at t = 0
-    and at every output time) and per-step (time, dt, niter1, niter2)."""
+    Returns the state, the output records (datasets of write_output at the
+    start time and at every output time) and per-step (time, dt, niter1,
+    niter2).  With output.restart the run continues from record iin of
+    output.input_file; with write=True the records are stored in
+    output.output_file (output.write_file)."""
     from . import output
 
     nl = namelists
@@ -206,8 +242,11 @@ def integrate(namelists: Namelists, records: bool = True, max_steps: int = 1 << 
     steps: List[tuple] = []
     if nl.poisson.initial_cleaning:
         st.initial_cleaning()
+    if nl.output.restart:
+        time_s, fields = output.read_input(nl.output.input_file, nl.output.iin)
+        st.restart(time_s, fields)
     if records:
-        recs.append(output.record_datasets(nl, st.get, 0.0, True))
+        recs.append(output.record_datasets(nl, st.get, st.time()[0], True))
     n = nl.output.iterations if nl.output.output_steps else max_steps
     for itime in range(1, n + 1):
         info = st.step()
@@ -217,4 +256,6 @@ def integrate(namelists: Namelists, records: bool = True, max_steps: int = 1 << 
             recs.append(output.record_datasets(nl, st.get, info.time, False))
         if not nl.output.output_steps and info.time >= nl.output.tmax:
             break
+    if write and records:
+        output.write_file(nl.output.output_file, recs)
     return st, recs, steps
