@@ -102,7 +102,8 @@ enum pcf_rhs_sponge {
  * time steps. */
 enum pcf_initial_guess {
   PCF_GUESS_ZERO = 0,
-  PCF_GUESS_PREVIOUS = 1
+  PCF_GUESS_PREVIOUS = 1,
+  PCF_GUESS_LEARNED = 2 /* the attached network (pcf_set_guess_network) */
 };
 
 /* Preconditioner of the mixed-precision solve.  LINE_RELAXATION: the
@@ -356,6 +357,38 @@ enum pcf_kernel_group {
 int pcf_profile_enable(pcf_state* st, int on);
 /* Launch count and summed device time of one group since it was enabled. */
 int pcf_profile_query(pcf_state* st, int group, int64_t* launches, double* total_ms);
+
+/* ---- initial-guess network of the learned solve (no reference
+ * counterpart) -------------------------------------------------------------
+ * Six 3x3x3 convolutions, 3 -> 32 -> 32 -> 32 -> 32 -> 32 -> 1 channels,
+ * GELU after the first five; periodic in x and y, zero beyond the walls in
+ * z.  Parameters: one FP32 vector, layer by layer W[27][cout][cin] then
+ * b[cout] (tap t = (dx+1) + 3 (dy+1) + 9 (dz+1)).  Device vectors are
+ * channels-last FP32: input n x 3, output and target n, with voxel
+ * i + nx (j + ny k).  A handle serves one grid size (nx a multiple of 32);
+ * training = 1 keeps what the backward pass needs. */
+typedef struct pcf_net pcf_net;
+int pcf_net_param_count(void);
+int pcf_net_create(int32_t nx, int32_t ny, int32_t nz, int32_t training, pcf_net** out);
+int pcf_net_destroy(pcf_net* net);
+int pcf_net_set_params(pcf_net* net, const float* host_params);
+int pcf_net_get_params(pcf_net* net, float* host_params);
+/* output = network(input) */
+int pcf_net_forward(pcf_net* net, const float* d_input, float* d_output);
+/* mean-squared-error loss against d_target and its gradient (host FP32,
+ * pcf_net_param_count() values) */
+int pcf_net_gradients(pcf_net* net, const float* d_input, const float* d_target, float* host_grads, double* loss);
+/* one Adam step (learning rate lr) on the loss against d_target */
+int pcf_net_train_step(pcf_net* net, const float* d_input, const float* d_target, float lr, double* loss);
+/* Input channels of the learned initial guess for the right-hand side d_b
+ * and the previous solution d_prev (interior FP64 device vectors; d_prev may
+ * be null = zero): b / |b|, prev / |b| and jac - 1 (RMS norm |b|), written
+ * to d_input (n x 3 FP32); *b_norm gets |b|.  The guess is |b| times the
+ * network's output. */
+int pcf_guess_input(pcf_state* st, const double* d_b, const double* d_prev, float* d_input, double* b_norm);
+/* Attach the network used by initial_guess = PCF_GUESS_LEARNED (not owned;
+ * its grid must be the state's). */
+int pcf_set_guess_network(pcf_state* st, pcf_net* net);
 
 #ifdef __cplusplus
 }

@@ -10,7 +10,7 @@ from .namelists import (AgnesiHill, AtmosphereNamelist, CosineBubble, CosineHill
                         DomainNamelist, GaussianHills, GridNamelist, Namelists, OutputNamelist, PoissonNamelist,
                         Sin2Sponge, Sin2TopSponge, SinhStretching, SpongeNamelist, TracerNamelist, Vortex,
                         WKBNamelist, to_config)
-from . import examples, output
+from . import examples, learned, output
 from .state import PinCFlowError, PoissonNotConverged, State, integrate
 
 __all__ = [
@@ -18,5 +18,5 @@ __all__ = [
     "AgnesiHill", "AtmosphereNamelist", "CosineBubble", "CosineHill", "DiscretizationNamelist", "DomainNamelist",
     "GaussianHills", "GridNamelist", "Namelists", "OutputNamelist", "PoissonNamelist", "Sin2Sponge",
     "Sin2TopSponge", "SinhStretching", "SpongeNamelist", "TracerNamelist", "Vortex", "WKBNamelist", "to_config",
-    "examples", "output",
+    "examples", "learned", "output",
 ]

@@ -37,7 +37,7 @@ PERT_NONE, PERT_COSINE_BUBBLE, PERT_VORTEX = 0, 1, 2
 LHS_SPONGE_NONE, LHS_SPONGE_SIN2 = 0, 1
 RHS_SPONGE_NONE, RHS_SPONGE_SIN2_TOP = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-GUESS_ZERO, GUESS_PREVIOUS = 0, 1
+GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED = 0, 1, 2
 PC_LINE_RELAXATION, PC_MULTIGRID = 0, 1
 
 FIELDS = [
@@ -183,6 +183,16 @@ SIGNATURES = [
     ("pcf_get_vector", _I, [_P, _I, _PD]),
     ("pcf_set_vector", _I, [_P, _I, _PD]),
     ("pcf_last_residual", _I, [_P, _PD]),
+    ("pcf_net_param_count", _I, []),
+    ("pcf_net_create", _I, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_P)]),
+    ("pcf_net_destroy", _I, [_P]),
+    ("pcf_net_set_params", _I, [_P, ctypes.POINTER(ctypes.c_float)]),
+    ("pcf_net_get_params", _I, [_P, ctypes.POINTER(ctypes.c_float)]),
+    ("pcf_net_forward", _I, [_P, _P, _P]),
+    ("pcf_net_gradients", _I, [_P, _P, _P, ctypes.POINTER(ctypes.c_float), _PD]),
+    ("pcf_net_train_step", _I, [_P, _P, _P, ctypes.c_float, _PD]),
+    ("pcf_guess_input", _I, [_P, _P, _P, _P, _PD]),
+    ("pcf_set_guess_network", _I, [_P, _P]),
     ("pcf_profile_enable", _I, [_P, _I]),
     ("pcf_profile_query", _I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _PD]),
 ]

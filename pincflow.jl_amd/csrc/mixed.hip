@@ -525,9 +525,16 @@ int apply_bicgstab_mixed(pcf_state* st, double tolref, double dt, int* niter_out
   int rc = prepare_mixed(st);
   if (rc) return rc;
   const int slot = st->solve_slot;
-  const bool use_prev = n.initial_guess == PCF_GUESS_PREVIOUS && slot >= 0 && slot < 2 && dt > 0.0;
-  const double* guess = use_prev && st->m_prev_valid[slot] ? st->m_prev[slot] : nullptr;
-  rc = bicgstab_core(st, tolref, true, guess, dt, niter_out);
+  const bool stage = slot >= 0 && slot < 2 && dt > 0.0;
+  const bool use_prev = (n.initial_guess == PCF_GUESS_PREVIOUS || n.initial_guess == PCF_GUESS_LEARNED) && stage;
+  const double* prev = use_prev && st->m_prev_valid[slot] ? st->m_prev[slot] : nullptr;
+  const double* guess = n.initial_guess == PCF_GUESS_PREVIOUS ? prev : nullptr;
+  double scale = dt;
+  if (n.initial_guess == PCF_GUESS_LEARNED && stage) {
+    if ((rc = learned_guess(st, dt, prev, &guess))) return rc;
+    scale = 1.0;
+  }
+  rc = bicgstab_core(st, tolref, true, guess, scale, niter_out);
   if (rc) return rc;
   if (use_prev) {  // keep solution / dt for the same stage of the next step
     const size_t N = st->g.nv();

@@ -1,0 +1,659 @@
+// The initial-guess network of the learned solve (no reference counterpart).
+//
+// Six 3 x 3 x 3 convolutions, 3 -> 32 -> 32 -> 32 -> 32 -> 32 -> 1 channels,
+// GELU after the first five, bias on every layer; fields are periodic in x
+// and y and zero beyond the walls in z, like the pressure operator.  The
+// four 32 -> 32 layers run as implicit GEMMs on the bf16 matrix cores
+// (v_mfma_f32_32x32x16_bf16, FP32 accumulation): for each tap, a 32 (output
+// channels) x 32 (voxels) tile accumulates W[tap] (32 x 32) times the input
+// at the tap's neighbours (32 x 32).  The backward pass reuses that kernel
+// for the input gradient (transposed, tap-flipped weights) and has its own
+// MFMA kernel for the weight gradient (a GEMM over the voxels, split into
+// chunks whose partial sums are added in a fixed order).  The 3 -> 32 and
+// 32 -> 1 layers are small and run on the vector units.
+//
+// Layout: activations are channels-last [voxel][channel] with voxel =
+// i + nx (j + ny k); parameters are one FP32 vector, layer by layer W[27][cout][cin]
+// then b[cout], tap t = (dx + 1) + 3 (dy + 1) + 9 (dz + 1) weighing the input
+// at (i + dx, j + dy, k + dz).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pcf_internal.hpp"
+
+namespace pcf {
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+constexpr int kC = 32;        // hidden channels
+constexpr int kLayers = 6;
+constexpr int kChunk = 4096;  // voxels per partial sum of a weight gradient
+constexpr int kCin[kLayers] = {3, 32, 32, 32, 32, 32};
+constexpr int kCout[kLayers] = {32, 32, 32, 32, 32, 1};
+
+__host__ __device__ constexpr int layer_size(int l) { return 27 * kCout[l] * kCin[l] + kCout[l]; }
+__host__ __device__ constexpr int layer_offset(int l) {
+  int o = 0;
+  for (int q = 0; q < l; ++q) o += layer_size(q);
+  return o;
+}
+constexpr int kParams = layer_offset(kLayers);
+
+struct Dims {
+  int nx, ny, nz;
+  size_t n;
+};
+
+// Voxel index of the neighbour at tap t of voxel (i, j, k); -1 beyond the walls.
+__device__ __forceinline__ long long neighbour(const Dims& d, int i, int j, int k, int t) {
+  const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+  const int kk = k + dz;
+  if (kk < 0 || kk >= d.nz) return -1;
+  int ii = i + dx, jj = j + dy;
+  ii = ii < 0 ? ii + d.nx : (ii >= d.nx ? ii - d.nx : ii);
+  jj = jj < 0 ? jj + d.ny : (jj >= d.ny ? jj - d.ny : jj);
+  return (long long)ii + (long long)d.nx * (jj + (long long)d.ny * kk);
+}
+
+__device__ __forceinline__ void coords(const Dims& d, size_t p, int& i, int& j, int& k) {
+  i = (int)(p % d.nx);
+  j = (int)((p / d.nx) % d.ny);
+  k = (int)(p / ((size_t)d.nx * d.ny));
+}
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+  return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * expf(-0.5f * x * x) * 0.39894228040143268f;
+}
+
+// ---- layer 1: 3 -> 32 (vector units); z (pre-activation, may be null), a = gelu(z) in bf16
+__global__ void k_first(Dims d, const float* __restrict__ x, const float* __restrict__ W, float* z, bf16* a) {
+  const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (p >= d.n) return;
+  int i, j, k;
+  coords(d, p, i, j, k);
+  const float* b = W + 27 * kC * 3;
+  float acc[kC];
+#pragma unroll
+  for (int c = 0; c < kC; ++c) acc[c] = b[c];
+  for (int t = 0; t < 27; ++t) {
+    const long long q = neighbour(d, i, j, k, t);
+    if (q < 0) continue;
+    const float x0 = x[3 * q], x1 = x[3 * q + 1], x2 = x[3 * q + 2];
+    const float* w = W + t * kC * 3;
+#pragma unroll
+    for (int c = 0; c < kC; ++c) acc[c] += w[3 * c] * x0 + w[3 * c + 1] * x1 + w[3 * c + 2] * x2;
+  }
+#pragma unroll
+  for (int c = 0; c < kC; ++c) {
+    if (z) z[p * kC + c] = acc[c];
+    a[p * kC + c] = (bf16)gelu(acc[c]);
+  }
+}
+
+// ---- layers 2..5: 32 -> 32 implicit GEMM on the matrix cores.  One
+// wavefront computes 32 output channels x 32 voxels consecutive in x (nx is
+// a multiple of 32).  Operand layout of v_mfma_f32_32x32x16_bf16: lane l
+// holds A[l % 32][8 (l / 32) + e] and B[8 (l / 32) + e][l % 32], e = 0..7,
+// and D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32], q = 0..15.
+// FWD: out = W x + bias -> z (FP32, may be null) and gelu -> a (bf16).
+// DATA: out = W x (the input gradient, with transposed tap-flipped W) -> g (FP32).
+enum { MID_FWD = 0, MID_DATA = 1 };
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_mid(Dims d, const bf16* __restrict__ in, const bf16* __restrict__ W,
+                                             const float* __restrict__ bias, float* z, bf16* a, float* g) {
+  const int lane = threadIdx.x & 63;
+  const size_t tile = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;  // 32-voxel tile
+  const size_t p0 = tile * 32;
+  if (p0 >= d.n) return;
+  const int col = lane & 31, half = lane >> 5;
+  int i, j, k;
+  coords(d, p0 + col, i, j, k);
+  f16v acc = {};
+  for (int t = 0; t < 27; ++t) {
+    const long long q = neighbour(d, i, j, k, t);
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc) {
+      const int c0 = 16 * kc + 8 * half;
+      const bf8 av = *(const bf8*)(W + ((size_t)t * kC + col) * kC + c0);
+      bf8 bv;
+      if (q >= 0) {
+        bv = *(const bf8*)(in + (size_t)q * kC + c0);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bv[e] = (bf16)0.0f;
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+  }
+  const size_t p = p0 + col;
+#pragma unroll
+  for (int qi = 0; qi < 16; ++qi) {
+    const int co = 8 * (qi / 4) + 4 * half + (qi % 4);
+    float v = acc[qi];
+    if (MODE == MID_FWD) {
+      v += bias[co];
+      if (z) z[p * kC + co] = v;
+      a[p * kC + co] = (bf16)gelu(v);
+    } else {
+      g[p * kC + co] = v;
+    }
+  }
+}
+
+// ---- layer 6: 32 -> 1 (vector units)
+__global__ void k_last(Dims d, const bf16* __restrict__ in, const float* __restrict__ W, float* out) {
+  const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (p >= d.n) return;
+  int i, j, k;
+  coords(d, p, i, j, k);
+  float s = W[27 * kC];
+  for (int t = 0; t < 27; ++t) {
+    const long long q = neighbour(d, i, j, k, t);
+    if (q < 0) continue;
+    const bf8* v = (const bf8*)(in + (size_t)q * kC);
+    const float* w = W + t * kC;
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const bf8 x = v[h];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) s += w[8 * h + e] * (float)x[e];
+    }
+  }
+  out[p] = s;
+}
+
+// ---- loss: dout = 2 (out - target) / n; partial sums of (out - target)^2 per block
+__global__ void k_loss(Dims d, const float* out, const float* target, float* dout, double* partials) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < d.n; p += (size_t)gridDim.x * blockDim.x) {
+    const float e = out[p] - target[p];
+    dout[p] = 2.0f * e / (float)d.n;
+    s += (double)e * e;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// ---- layer 6 backward: input gradient g[p][c] = sum_t W[t][c] dout(p - tap_t)
+__global__ void k_last_data(Dims d, const float* dout, const float* W, float* g) {
+  const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (p >= d.n) return;
+  int i, j, k;
+  coords(d, p, i, j, k);
+  float acc[kC];
+#pragma unroll
+  for (int c = 0; c < kC; ++c) acc[c] = 0.0f;
+  for (int t = 0; t < 27; ++t) {
+    const long long q = neighbour(d, i, j, k, 26 - t);  // p - tap_t
+    if (q < 0) continue;
+    const float dv = dout[q];
+    const float* w = W + t * kC;
+#pragma unroll
+    for (int c = 0; c < kC; ++c) acc[c] += w[c] * dv;
+  }
+#pragma unroll
+  for (int c = 0; c < kC; ++c) g[p * kC + c] = acc[c];
+}
+
+// ---- layer 6 weight gradient: per chunk of voxels, W[t][c] += dout(p) in(p + tap_t)[c], b += dout(p)
+__global__ void k_last_weights(Dims d, const float* dout, const bf16* in, float* partials) {
+  const int slot = threadIdx.x;  // 0..863: (t, c); 864: the bias
+  const size_t p0 = (size_t)blockIdx.x * kChunk, p1 = p0 + kChunk < d.n ? p0 + kChunk : d.n;
+  if (slot > 27 * kC) return;
+  const int t = slot / kC, c = slot % kC;
+  float s = 0.0f;
+  for (size_t p = p0; p < p1; ++p) {
+    if (slot == 27 * kC) {
+      s += dout[p];
+      continue;
+    }
+    int i, j, k;
+    coords(d, p, i, j, k);
+    const long long q = neighbour(d, i, j, k, t);
+    if (q >= 0) s += dout[p] * (float)in[(size_t)q * kC + c];
+  }
+  partials[(size_t)blockIdx.x * (27 * kC + 1) + slot] = s;
+}
+
+// ---- GELU backward: dz = g * gelu'(z) (bf16 for the matrix cores), and per
+// block partial sums of dz per channel (the bias gradient, FP32)
+__global__ void k_gelu_back(Dims d, const float* g, const float* z, bf16* dz, float* bias_partials) {
+  __shared__ float red[256];
+  const size_t p0 = (size_t)blockIdx.x * kChunk, p1 = p0 + kChunk < d.n ? p0 + kChunk : d.n;
+  const int c = threadIdx.x % kC, lane_group = threadIdx.x / kC;  // 8 groups of 32
+  float s = 0.0f;
+  for (size_t p = p0 + lane_group; p < p1; p += 8) {
+    const float v = g[p * kC + c] * gelu_grad(z[p * kC + c]);
+    dz[p * kC + c] = (bf16)v;
+    s += v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < kC) {
+    float t = 0.0f;
+    for (int q = 0; q < 8; ++q) t += red[q * kC + threadIdx.x];
+    bias_partials[(size_t)blockIdx.x * kC + threadIdx.x] = t;
+  }
+}
+
+// ---- weight gradient of a 32 -> 32 layer on the matrix cores: for tap t and
+// a chunk of voxels, dW[t] (32 co x 32 ci) += sum_p dz(p)[co] in(p + tap_t)[ci],
+// with 16 voxels per MFMA step staged through LDS.  One wavefront per (chunk, tap).
+__global__ void __launch_bounds__(64) k_mid_weights(Dims d, const bf16* __restrict__ dz, const bf16* __restrict__ in,
+                                                    float* partials) {
+  __shared__ bf16 ld[16][kC];
+  __shared__ bf16 la[16][kC];
+  const int lane = threadIdx.x, t = blockIdx.y;
+  const int col = lane & 31, half = lane >> 5;
+  const size_t p0 = (size_t)blockIdx.x * kChunk, p1 = p0 + kChunk < d.n ? p0 + kChunk : d.n;
+  f16v acc = {};
+  for (size_t pb = p0; pb < p1; pb += 16) {
+    // each lane stages 8 channels of one voxel of both tiles
+    const int v = lane >> 2, c0 = 8 * (lane & 3);
+    const size_t p = pb + v;
+    bf8 dv, av;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) dv[e] = av[e] = (bf16)0.0f;
+    if (p < p1) {
+      dv = *(const bf8*)(dz + p * kC + c0);
+      int i, j, k;
+      coords(d, p, i, j, k);
+      const long long q = neighbour(d, i, j, k, t);
+      if (q >= 0) av = *(const bf8*)(in + (size_t)q * kC + c0);
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      ld[v][c0 + e] = dv[e];
+      la[v][c0 + e] = av[e];
+    }
+    __syncthreads();
+    bf8 A, B;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      A[e] = ld[8 * half + e][col];  // A[co][k = voxel]
+      B[e] = la[8 * half + e][col];  // B[k = voxel][ci]
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc, 0, 0, 0);
+    __syncthreads();
+  }
+  float* out = partials + ((size_t)blockIdx.x * 27 + t) * kC * kC;
+#pragma unroll
+  for (int qi = 0; qi < 16; ++qi) {
+    const int co = 8 * (qi / 4) + 4 * half + (qi % 4);
+    out[co * kC + col] = acc[qi];
+  }
+}
+
+// ---- layer 1 weight gradient: W[t][co][ci] += dz(p)[co] x(p + tap_t)[ci]
+__global__ void k_first_weights(Dims d, const bf16* dz, const float* x, float* partials) {
+  const int slot = blockIdx.y * blockDim.x + threadIdx.x;  // 0..2591
+  if (slot >= 27 * kC * 3) return;
+  const int t = slot / (kC * 3), co = (slot / 3) % kC, ci = slot % 3;
+  const size_t p0 = (size_t)blockIdx.x * kChunk, p1 = p0 + kChunk < d.n ? p0 + kChunk : d.n;
+  float s = 0.0f;
+  for (size_t p = p0; p < p1; ++p) {
+    int i, j, k;
+    coords(d, p, i, j, k);
+    const long long q = neighbour(d, i, j, k, t);
+    if (q >= 0) s += (float)dz[p * kC + co] * x[3 * q + ci];
+  }
+  partials[(size_t)blockIdx.x * (27 * kC * 3) + slot] = s;
+}
+
+// sum over chunks, in chunk order: out[s] = sum_c partials[c][s]
+__global__ void k_sum_chunks(const float* partials, int nchunks, int size, float* out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= size) return;
+  float acc = 0.0f;
+  for (int c = 0; c < nchunks; ++c) acc += partials[(size_t)c * size + s];
+  out[s] = acc;
+}
+
+// bf16 copies of the 32 -> 32 weights: W16[t][co][ci] and the transposed,
+// tap-flipped WT16[t][ci][co] = W[26 - t][co][ci] of the input gradient
+__global__ void k_pack(const float* params, bf16* w16, bf16* wt16) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;  // within one layer
+  const int l = 1 + blockIdx.y;
+  if (s >= 27 * kC * kC) return;
+  const float* W = params + layer_offset(l);
+  const int t = s / (kC * kC), co = (s / kC) % kC, ci = s % kC;
+  const size_t base = (size_t)(l - 1) * 27 * kC * kC;
+  w16[base + s] = (bf16)W[s];
+  wt16[base + ((size_t)(26 - t) * kC + ci) * kC + co] = (bf16)W[s];
+}
+
+// input channels of the learned guess: b / |b|, c_prev prev / |b| (prev may
+// be null: 0) and jac - 1, channels-last FP32
+__global__ void k_guess_input(pcf::Geo g, const double* b, const double* prev, const double* jac, float* in,
+                              double inv_bnorm, double prev_scale) {
+  const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (p >= g.nv()) return;
+  const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), k = (int)(p / ((size_t)g.nx * g.ny));
+  in[3 * p] = (float)(b[p] * inv_bnorm);
+  in[3 * p + 1] = prev ? (float)(prev[p] * prev_scale * inv_bnorm) : 0.0f;
+  in[3 * p + 2] = (float)(jac[g.h(g.i0 + i, g.j0 + j, g.k0 + k)] - 1.0);
+}
+
+// x = |b| out
+__global__ void k_guess_output(const float* out, double* x, size_t n, double bnorm) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    x[q] = bnorm * (double)out[q];
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 parameters
+__global__ void k_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const float gq = g[q];
+  const float mq = b1 * m[q] + (1.0f - b1) * gq;
+  const float vq = b2 * v[q] + (1.0f - b2) * gq * gq;
+  m[q] = mq;
+  v[q] = vq;
+  p[q] -= lr * (mq / c1) / (sqrtf(vq / c2) + eps);
+}
+
+}  // namespace
+}  // namespace pcf
+
+using pcf::bf16;
+struct pcf_net;
+
+struct pcf_net {
+  pcf::Dims d;
+  bool training = false;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+  bf16 *w16 = nullptr, *wt16 = nullptr;
+  float* z[5] = {};        // pre-activations of layers 1..5 (training)
+  bf16* a[5] = {};         // activations of layers 1..5 (inference: a[0], a[1] alternate)
+  float* out = nullptr;    // network output
+  float *dout = nullptr, *g = nullptr;
+  bf16* dzb = nullptr;
+  float* partials = nullptr;
+  double* loss_partials = nullptr;
+  double* loss_host = nullptr;
+  int step = 0;
+  std::vector<void*> allocations;
+};
+
+namespace pcf {
+namespace {
+
+int net_alloc(pcf_net* net, void** p, size_t bytes) {
+  PCF_HIP(hipMalloc(p, bytes));
+  net->allocations.push_back(*p);
+  return PCF_OK;
+}
+
+int nchunks(const pcf_net* net) { return (int)((net->d.n + kChunk - 1) / kChunk); }
+
+// forward pass over the FP32 input (n x 3); output in net->out
+int net_forward(pcf_net* net, const float* input, hipStream_t s) {
+  const Dims& d = net->d;
+  const unsigned vb = 256, vg = (unsigned)((d.n + vb - 1) / vb);
+  const unsigned tiles = (unsigned)((d.n + 31) / 32);
+  const unsigned mg = (tiles + 3) / 4;  // 4 wavefronts per block
+  const float* P = net->params;
+  bf16* cur = net->a[0];
+  hipLaunchKernelGGL(k_first, dim3(vg), dim3(vb), 0, s, d, input, P + layer_offset(0),
+                     net->training ? net->z[0] : nullptr, cur);
+  for (int l = 1; l <= 4; ++l) {
+    bf16* nxt = net->training ? net->a[l] : net->a[l & 1];
+    hipLaunchKernelGGL(k_mid<MID_FWD>, dim3(mg), dim3(256), 0, s, d, cur,
+                       net->w16 + (size_t)(l - 1) * 27 * kC * kC, P + layer_offset(l) + 27 * kC * kC,
+                       net->training ? net->z[l] : nullptr, nxt, nullptr);
+    cur = nxt;
+  }
+  hipLaunchKernelGGL(k_last, dim3(vg), dim3(vb), 0, s, d, cur, P + layer_offset(5), net->out);
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+// backward pass for the loss against target; gradients in net->grads, loss on the host
+int net_backward(pcf_net* net, const float* input, const float* target, double* loss) {
+  const Dims& d = net->d;
+  hipStream_t s = net->stream;
+  const unsigned vb = 256, vg = (unsigned)((d.n + vb - 1) / vb);
+  const unsigned tiles = (unsigned)((d.n + 31) / 32);
+  const unsigned mg = (tiles + 3) / 4;
+  const int nc = nchunks(net);
+  const float* P = net->params;
+  float* G = net->grads;
+  const unsigned lg = 1024;
+  hipLaunchKernelGGL(k_loss, dim3(lg), dim3(256), 0, s, d, net->out, target, net->dout, net->loss_partials);
+  // layer 6
+  hipLaunchKernelGGL(k_last_weights, dim3(nc), dim3(27 * kC + 1 + 31), 0, s, d, net->dout, net->a[4], net->partials);
+  hipLaunchKernelGGL(k_sum_chunks, dim3((27 * kC + 1 + 255) / 256), dim3(256), 0, s, net->partials, nc, 27 * kC + 1,
+                     G + layer_offset(5));
+  hipLaunchKernelGGL(k_last_data, dim3(vg), dim3(vb), 0, s, d, net->dout, P + layer_offset(5), net->g);
+  // layers 5..2
+  for (int l = 4; l >= 1; --l) {
+    hipLaunchKernelGGL(k_gelu_back, dim3(nc), dim3(256), 0, s, d, net->g, net->z[l], net->dzb, net->partials);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(kC), 0, s, net->partials, nc, kC,
+                       G + layer_offset(l) + 27 * kC * kC);
+    hipLaunchKernelGGL(k_mid_weights, dim3(nc, 27), dim3(64), 0, s, d, net->dzb, net->a[l - 1], net->partials);
+    hipLaunchKernelGGL(k_sum_chunks, dim3((27 * kC * kC + 255) / 256), dim3(256), 0, s, net->partials, nc,
+                       27 * kC * kC, G + layer_offset(l));
+    hipLaunchKernelGGL(k_mid<MID_DATA>, dim3(mg), dim3(256), 0, s, d, net->dzb,
+                       net->wt16 + (size_t)(l - 1) * 27 * kC * kC, nullptr, nullptr, nullptr, net->g);
+  }
+  // layer 1
+  hipLaunchKernelGGL(k_gelu_back, dim3(nc), dim3(256), 0, s, d, net->g, net->z[0], net->dzb, net->partials);
+  hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(kC), 0, s, net->partials, nc, kC, G + layer_offset(0) + 27 * kC * 3);
+  hipLaunchKernelGGL(k_first_weights, dim3(nc, (27 * kC * 3 + 255) / 256), dim3(256), 0, s, d, net->dzb, input,
+                     net->partials);
+  hipLaunchKernelGGL(k_sum_chunks, dim3((27 * kC * 3 + 255) / 256), dim3(256), 0, s, net->partials, nc, 27 * kC * 3,
+                     G + layer_offset(0));
+  PCF_HIP(hipGetLastError());
+  std::vector<double> lp(lg);
+  PCF_HIP(hipMemcpyAsync(lp.data(), net->loss_partials, lg * sizeof(double), hipMemcpyDeviceToHost, s));
+  PCF_HIP(hipStreamSynchronize(s));
+  double sum = 0.0;
+  for (double v : lp) sum += v;
+  if (loss) *loss = sum / (double)d.n;
+  return PCF_OK;
+}
+
+// input channels for b (and prev scaled by prev_scale) and the RMS norm of b
+int guess_input(pcf_state* st, const double* b, const double* prev, double prev_scale, float* input,
+                double* bnorm) {
+  double bb = 0.0;
+  int rc = dot(st, b, b, &bb);
+  if (rc) return rc;
+  const double nrm = std::sqrt(bb / (double)st->g.nv());
+  *bnorm = nrm;
+  const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+  hipLaunchKernelGGL(k_guess_input, dim3((unsigned)((st->g.nv() + 255) / 256)), dim3(256), 0, st->stream, st->g, b,
+                     prev, st->jac, input, inv, prev_scale);
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+int net_pack(pcf_net* net) {
+  hipLaunchKernelGGL(k_pack, dim3((27 * kC * kC + 255) / 256, 4), dim3(256), 0, net->stream, net->params, net->w16,
+                     net->wt16);
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+}  // namespace
+
+// The learned initial guess of the mixed solve (apply_bicgstab_mixed): x0 in
+// st->guess_x from the right-hand side, the stage's previous solution
+// (rescaled to this dt) and the Jacobian.
+int learned_guess(pcf_state* st, double dt, const double* prev, const double** x0) {
+  pcf_net* net = st->guess_net;
+  if (!net) {
+    set_error("initial_guess = learned needs a network (pcf_set_guess_network)");
+    return PCF_E_CONFIG;
+  }
+  double bnorm = 0.0;
+  int rc = guess_input(st, st->rhs, prev, dt, st->guess_in, &bnorm);
+  if (rc) return rc;
+  if ((rc = net_forward(net, st->guess_in, st->stream))) return rc;
+  hipLaunchKernelGGL(k_guess_output, dim3(2048), dim3(256), 0, st->stream, net->out, st->guess_x, st->g.nv(), bnorm);
+  PCF_HIP(hipGetLastError());
+  *x0 = st->guess_x;
+  return PCF_OK;
+}
+
+}  // namespace pcf
+
+extern "C" {
+
+int pcf_guess_input(pcf_state* st, const double* d_b, const double* d_prev, float* d_input, double* b_norm) {
+  if (!st || !d_b || !d_input) return PCF_E_ARG;
+  double nrm = 0.0;
+  int rc = pcf::guess_input(st, d_b, d_prev, 1.0, d_input, &nrm);
+  if (rc) return rc;
+  PCF_HIP(hipStreamSynchronize(st->stream));
+  if (b_norm) *b_norm = nrm;
+  return PCF_OK;
+}
+
+int pcf_set_guess_network(pcf_state* st, pcf_net* net) {
+  using namespace pcf;
+  if (!st) return PCF_E_ARG;
+  if (net && ((size_t)net->d.nx != (size_t)st->g.nx || net->d.ny != st->g.ny || net->d.nz != st->g.nz)) {
+    set_error("pcf_set_guess_network: the network's grid differs from the state's");
+    return PCF_E_ARG;
+  }
+  if (net && !st->guess_in) {
+    const size_t N = st->g.nv();
+    PCF_HIP(hipMalloc((void**)&st->guess_in, 3 * N * sizeof(float)));
+    st->allocations.push_back(st->guess_in);
+    PCF_HIP(hipMalloc((void**)&st->guess_x, N * sizeof(double)));
+    st->allocations.push_back(st->guess_x);
+  }
+  st->guess_net = net;
+  return PCF_OK;
+}
+
+
+int pcf_net_param_count(void) { return pcf::kParams; }
+
+int pcf_net_create(int32_t nx, int32_t ny, int32_t nz, int32_t training, pcf_net** out) {
+  using namespace pcf;
+  if (!out || nx < 32 || nx % 32 != 0 || ny < 1 || nz < 1) {
+    set_error("pcf_net_create: nx must be a positive multiple of 32");
+    return PCF_E_ARG;
+  }
+  pcf_net* net = new pcf_net();
+  net->d = Dims{nx, ny, nz, (size_t)nx * ny * nz};
+  net->training = training != 0;
+  auto fail = [&](int rc) {
+    for (void* p : net->allocations) (void)hipFree(p);
+    if (net->stream) (void)hipStreamDestroy(net->stream);
+    delete net;
+    return rc;
+  };
+  if (hipGetDevice(&net->device) != hipSuccess) return fail(PCF_E_HIP);
+  if (hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking) != hipSuccess) return fail(PCF_E_HIP);
+  const size_t N = net->d.n;
+  int rc;
+  if ((rc = net_alloc(net, (void**)&net->params, kParams * sizeof(float)))) return fail(rc);
+  if ((rc = net_alloc(net, (void**)&net->w16, 4 * 27 * kC * kC * sizeof(bf16)))) return fail(rc);
+  if ((rc = net_alloc(net, (void**)&net->wt16, 4 * 27 * kC * kC * sizeof(bf16)))) return fail(rc);
+  if ((rc = net_alloc(net, (void**)&net->out, N * sizeof(float)))) return fail(rc);
+  const int na = net->training ? 5 : 2;
+  for (int l = 0; l < na; ++l)
+    if ((rc = net_alloc(net, (void**)&net->a[l], N * kC * sizeof(bf16)))) return fail(rc);
+  if (net->training) {
+    for (int l = 0; l < 5; ++l)
+      if ((rc = net_alloc(net, (void**)&net->z[l], N * kC * sizeof(float)))) return fail(rc);
+    if ((rc = net_alloc(net, (void**)&net->grads, kParams * sizeof(float)))) return fail(rc);
+    if ((rc = net_alloc(net, (void**)&net->adam_m, kParams * sizeof(float)))) return fail(rc);
+    if ((rc = net_alloc(net, (void**)&net->adam_v, kParams * sizeof(float)))) return fail(rc);
+    if ((rc = net_alloc(net, (void**)&net->dout, N * sizeof(float)))) return fail(rc);
+    if ((rc = net_alloc(net, (void**)&net->g, N * kC * sizeof(float)))) return fail(rc);
+    if ((rc = net_alloc(net, (void**)&net->dzb, N * kC * sizeof(bf16)))) return fail(rc);
+    const size_t part = (size_t)nchunks(net) * 27 * kC * kC;
+    if ((rc = net_alloc(net, (void**)&net->partials, part * sizeof(float)))) return fail(rc);
+    if ((rc = net_alloc(net, (void**)&net->loss_partials, 1024 * sizeof(double)))) return fail(rc);
+    if (hipMemsetAsync(net->adam_m, 0, kParams * sizeof(float), net->stream) != hipSuccess) return fail(PCF_E_HIP);
+    if (hipMemsetAsync(net->adam_v, 0, kParams * sizeof(float), net->stream) != hipSuccess) return fail(PCF_E_HIP);
+  }
+  if (hipMemsetAsync(net->params, 0, kParams * sizeof(float), net->stream) != hipSuccess) return fail(PCF_E_HIP);
+  if (hipStreamSynchronize(net->stream) != hipSuccess) return fail(PCF_E_HIP);
+  *out = net;
+  return PCF_OK;
+}
+
+int pcf_net_destroy(pcf_net* net) {
+  if (!net) return PCF_E_ARG;
+  (void)hipStreamSynchronize(net->stream);
+  for (void* p : net->allocations) (void)hipFree(p);
+  (void)hipStreamDestroy(net->stream);
+  delete net;
+  return PCF_OK;
+}
+
+int pcf_net_set_params(pcf_net* net, const float* host) {
+  using namespace pcf;
+  if (!net || !host) return PCF_E_ARG;
+  PCF_HIP(hipMemcpyAsync(net->params, host, kParams * sizeof(float), hipMemcpyHostToDevice, net->stream));
+  int rc = net_pack(net);
+  if (rc) return rc;
+  PCF_HIP(hipStreamSynchronize(net->stream));
+  return PCF_OK;
+}
+
+int pcf_net_get_params(pcf_net* net, float* host) {
+  using namespace pcf;
+  if (!net || !host) return PCF_E_ARG;
+  PCF_HIP(hipMemcpyAsync(host, net->params, kParams * sizeof(float), hipMemcpyDeviceToHost, net->stream));
+  PCF_HIP(hipStreamSynchronize(net->stream));
+  return PCF_OK;
+}
+
+int pcf_net_forward(pcf_net* net, const float* d_input, float* d_output) {
+  using namespace pcf;
+  if (!net || !d_input || !d_output) return PCF_E_ARG;
+  int rc = net_forward(net, d_input, net->stream);
+  if (rc) return rc;
+  PCF_HIP(hipMemcpyAsync(d_output, net->out, net->d.n * sizeof(float), hipMemcpyDeviceToDevice, net->stream));
+  PCF_HIP(hipStreamSynchronize(net->stream));
+  return PCF_OK;
+}
+
+int pcf_net_gradients(pcf_net* net, const float* d_input, const float* d_target, float* host_grads, double* loss) {
+  using namespace pcf;
+  if (!net || !d_input || !d_target || !net->training) return PCF_E_ARG;
+  int rc = net_forward(net, d_input, net->stream);
+  if (rc) return rc;
+  if ((rc = net_backward(net, d_input, d_target, loss))) return rc;
+  if (host_grads) {
+    PCF_HIP(hipMemcpyAsync(host_grads, net->grads, kParams * sizeof(float), hipMemcpyDeviceToHost, net->stream));
+    PCF_HIP(hipStreamSynchronize(net->stream));
+  }
+  return PCF_OK;
+}
+
+int pcf_net_train_step(pcf_net* net, const float* d_input, const float* d_target, float lr, double* loss) {
+  using namespace pcf;
+  int rc = pcf_net_gradients(net, d_input, d_target, nullptr, loss);
+  if (rc) return rc;
+  net->step += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - std::pow(b1, (float)net->step), c2 = 1.0f - std::pow(b2, (float)net->step);
+  hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, net->stream, net->params, net->grads,
+                     net->adam_m, net->adam_v, kParams, lr, b1, b2, 1e-8f, c1, c2);
+  if ((rc = net_pack(net))) return rc;
+  PCF_HIP(hipStreamSynchronize(net->stream));
+  return PCF_OK;
+}
+
+}  // extern "C"

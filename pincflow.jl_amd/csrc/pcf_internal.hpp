@@ -109,6 +109,8 @@ struct HostSetup;  // setup.cpp
 
 }  // namespace pcf
 
+struct pcf_net;
+
 struct pcf_state {
   pcf_config cfg;
   pcf::Geo g;
@@ -157,6 +159,10 @@ struct pcf_state {
   double* m_prev[2] = {nullptr, nullptr};
   bool m_prev_valid[2] = {false, false};
   int solve_slot = -1;                // stage of the solve in progress, -1 outside pcf_step
+  // learned initial guess: the attached network and its FP32 input / output
+  pcf_net* guess_net = nullptr;
+  float *guess_in = nullptr, *guess_out = nullptr;
+  double* guess_x = nullptr;
   double* partials;                   // reduction partials
   pcf::SolverScalars* sc;             // device scalars
   pcf::SolverScalars* sc_host;        // pinned copy
@@ -236,6 +242,9 @@ int dot(pcf_state* st, const double* a, const double* b, double* out);
 int apply_bicgstab_mixed(pcf_state* st, double tolref, double dt, int* niter);
 int precondition_mixed(pcf_state* st, const double* sin, double* sout, const SolverScalars* sc);
 int alloc_mixed(pcf_state* st);
+
+// ---- net.hip -----------------------------------------------------------------
+int learned_guess(pcf_state* st, double dt, const double* prev, const double** x0);
 
 // ---- explicit.hip --------------------------------------------------------
 int compute_fluxes(pcf_state* st);

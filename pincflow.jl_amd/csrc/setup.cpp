@@ -157,7 +157,10 @@ int host_setup(pcf_state* st, std::string& err) {
   if (n.model != PCF_BOUSSINESQ && n.model != PCF_PSEUDO_INCOMPRESSIBLE) { err = "Invalid model"; return PCF_E_OPTION; }
   if (n.preconditioner_iterations < 1 && n.preconditioner) { err = "preconditioner_iterations < 1"; return PCF_E_CONFIG; }
   if (n.precision != PCF_PRECISION_FP64 && n.precision != PCF_PRECISION_MIXED) { err = "Invalid precision"; return PCF_E_OPTION; }
-  if (n.initial_guess != PCF_GUESS_ZERO && n.initial_guess != PCF_GUESS_PREVIOUS) { err = "Invalid initial_guess"; return PCF_E_OPTION; }
+  if (n.initial_guess != PCF_GUESS_ZERO && n.initial_guess != PCF_GUESS_PREVIOUS && n.initial_guess != PCF_GUESS_LEARNED) {
+    err = "Invalid initial_guess";
+    return PCF_E_OPTION;
+  }
   if (n.preconditioner_type != PCF_PC_LINE_RELAXATION && n.preconditioner_type != PCF_PC_MULTIGRID) {
     err = "Invalid preconditioner_type";
     return PCF_E_OPTION;

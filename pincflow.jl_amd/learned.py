@@ -1,0 +1,200 @@
+"""The learned initial guess of the mixed-precision solve.
+
+A small 3D convolutional network (six 3x3x3 layers, 32 channels, GELU; the
+bf16 matrix-core kernels of csrc/net.hip) maps the Poisson right-hand side
+b, the previous time step's solution of the same implicit stage and the
+terrain Jacobian to an initial guess x0 = |b| net(b/|b|, prev/|b|, jac-1)
+(|b| the RMS norm).  BiCGSTAB then starts from x0; its residual, measured
+against |b| as for a zero start, still decides convergence, so the guess can
+only change the iteration count, never the accuracy.
+
+Training data are the solver's own solves: after every time step the RHS
+vector and the solution of the step's second implicit stage are read back
+(the solution times sqrt(pbar^2/rhobar) undoes the scaling solve_poisson!
+applies in place, solve_poisson!.jl:69).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Tuple
+
+import numpy as np
+
+from . import _abi
+from .state import PinCFlowError, State
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class DeviceBuffer:
+    """A device allocation through the HIP runtime the library links."""
+
+    _hip = None
+
+    def __init__(self, nbytes: int):
+        hip = DeviceBuffer.hip()
+        p = ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise PinCFlowError(_abi.E_HIP, "hipMalloc failed")
+        self.ptr, self.nbytes = p.value, nbytes
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+                try:
+                    cls._hip = ctypes.CDLL(name)
+                    break
+                except OSError:
+                    continue
+            if cls._hip is None:
+                raise PinCFlowError(_abi.E_HIP, "libamdhip64.so not found")
+            cls._hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+            cls._hip.hipFree.argtypes = [ctypes.c_void_p]
+            cls._hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        return cls._hip
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes == self.nbytes
+        if DeviceBuffer.hip().hipMemcpy(ctypes.c_void_p(self.ptr), a.ctypes.data_as(ctypes.c_void_p), self.nbytes,
+                                        1) != 0:
+            raise PinCFlowError(_abi.E_HIP, "hipMemcpy failed")
+        return self
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes == self.nbytes
+        if DeviceBuffer.hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr), self.nbytes,
+                                        2) != 0:
+            raise PinCFlowError(_abi.E_HIP, "hipMemcpy failed")
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            DeviceBuffer.hip().hipFree(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+
+class GuessNetwork:
+    """One pcf_net handle (a grid size); parameters as one FP32 vector."""
+
+    def __init__(self, nx: int, ny: int, nz: int, training: bool = False, params: np.ndarray | None = None,
+                 seed: int = 0):
+        self.lib = _abi.load_library()
+        self.shape = (nx, ny, nz)
+        self.h = ctypes.c_void_p()
+        rc = self.lib.pcf_net_create(nx, ny, nz, int(training), ctypes.byref(self.h))
+        if rc:
+            raise PinCFlowError(rc, self.lib.pcf_last_error().decode())
+        self.n_params = self.lib.pcf_net_param_count()
+        self.set_params(params if params is not None else init_params(seed))
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters, got {p.size}")
+        rc = self.lib.pcf_net_set_params(self.h, _fptr(p))
+        if rc:
+            raise PinCFlowError(rc, self.lib.pcf_last_error().decode())
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        rc = self.lib.pcf_net_get_params(self.h, _fptr(out))
+        if rc:
+            raise PinCFlowError(rc, self.lib.pcf_last_error().decode())
+        return out
+
+    def train_step(self, d_input: DeviceBuffer, d_target: DeviceBuffer, lr: float) -> float:
+        loss = ctypes.c_double(0.0)
+        rc = self.lib.pcf_net_train_step(self.h, ctypes.c_void_p(d_input.ptr), ctypes.c_void_p(d_target.ptr),
+                                         ctypes.c_float(lr), ctypes.byref(loss))
+        if rc:
+            raise PinCFlowError(rc, self.lib.pcf_last_error().decode())
+        return loss.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pcf_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-scaled weights, zero biases, as one FP32 vector (layer by layer
+    W[27][cout][cin] then b[cout]); the last layer starts at zero, so an
+    untrained network proposes x0 = 0."""
+    rng = np.random.default_rng(seed)
+    chans = (3, 32, 32, 32, 32, 32, 1)
+    parts = []
+    for n, (cin, cout) in enumerate(zip(chans[:-1], chans[1:])):
+        w = rng.standard_normal(27 * cout * cin) * np.sqrt(2.0 / (27 * cin))
+        if n == len(chans) - 2:
+            w[:] = 0.0
+        parts += [w, np.zeros(cout)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def _interior_fcscal(st: State) -> np.ndarray:
+    d = st.namelists.domain
+    ii, jj, kk = (slice(d.nbx, d.nbx + d.x_size), slice(d.nby, d.nby + d.y_size), slice(d.nbz, d.nbz + d.z_size))
+    pbar, rhobar = st.get("PBAR")[ii, jj, kk], st.get("RHOBAR")[ii, jj, kk]
+    return np.sqrt(pbar * pbar / rhobar)
+
+
+def collect_samples(st: State, steps: int) -> List[Tuple[np.ndarray, np.ndarray]]:
+    """Run `steps` time steps and return, per step after the first, the
+    network input and target of the step's second implicit-stage solve:
+    (input n x 3 FP32 channels-last, target n FP32 = x / |b|)."""
+    fcs = _interior_fcscal(st)
+    n = st.nx * st.ny * st.nz
+    d_b, d_prev = DeviceBuffer(8 * n), DeviceBuffer(8 * n)
+    d_in = DeviceBuffer(12 * n)
+    out = []
+    prev, prev_dt = None, None
+    for _ in range(steps):
+        info = st.step()
+        b = st.get_vector("RHS")
+        x = st.get_vector("SOLUTION") * fcs
+        if prev is not None:
+            d_b.upload(np.asfortranarray(b).ravel(order="F"))
+            d_prev.upload(np.asfortranarray(prev * (info.dt / prev_dt)).ravel(order="F"))
+            bnorm = ctypes.c_double(0.0)
+            rc = st.lib.pcf_guess_input(st.h, ctypes.c_void_p(d_b.ptr), ctypes.c_void_p(d_prev.ptr),
+                                        ctypes.c_void_p(d_in.ptr), ctypes.byref(bnorm))
+            if rc:
+                raise PinCFlowError(rc, st.lib.pcf_last_error().decode())
+            inp = d_in.download(np.float32, (n * 3,))
+            target = (np.asfortranarray(x).ravel(order="F") / bnorm.value).astype(np.float32)
+            out.append((inp, target))
+        prev, prev_dt = x, info.dt
+    return out
+
+
+def train(net: GuessNetwork, samples, epochs: int, lr: float = 1e-3, log=None) -> List[float]:
+    """Adam on the mean squared error over the samples, one sample per step."""
+    n = samples[0][1].size
+    d_in, d_t = DeviceBuffer(12 * n), DeviceBuffer(4 * n)
+    losses = []
+    for e in range(epochs):
+        tot = 0.0
+        for inp, target in samples:
+            d_in.upload(inp)
+            d_t.upload(target)
+            tot += net.train_step(d_in, d_t, lr)
+        losses.append(tot / len(samples))
+        if log:
+            log(f"epoch {e + 1}: loss {losses[-1]:.4e}")
+    return losses
+
+
+def attach(st: State, net: GuessNetwork):
+    """Use net for initial_guess = "learned" of this state (same grid)."""
+    rc = st.lib.pcf_set_guess_network(st.h, net.h)
+    if rc:
+        raise PinCFlowError(rc, st.lib.pcf_last_error().decode())
+    st._guess_net = net  # keep the network alive with the state

@@ -26,11 +26,12 @@ def hip():
 
 
 class DeviceArray:
-    """FP64 device buffer holding a Fortran-ordered array."""
+    """Device buffer holding a Fortran-ordered array (FP64 unless dtype says otherwise)."""
 
-    def __init__(self, shape):
+    def __init__(self, shape, dtype=np.float64):
         self.shape = tuple(shape)
-        self.nbytes = int(np.prod(self.shape)) * 8
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
         p = ctypes.c_void_p()
         rc = hip().hipMalloc(ctypes.byref(p), self.nbytes)
         if rc != 0:
@@ -38,7 +39,7 @@ class DeviceArray:
         self.ptr = p.value
 
     def upload(self, a):
-        a = np.asfortranarray(a, dtype=np.float64)
+        a = np.asfortranarray(a, dtype=self.dtype)
         assert a.shape == self.shape
         rc = hip().hipMemcpy(ctypes.c_void_p(self.ptr), a.ctypes.data_as(ctypes.c_void_p), self.nbytes, 1)
         if rc != 0:
@@ -46,7 +47,7 @@ class DeviceArray:
         return self
 
     def download(self):
-        a = np.empty(self.shape, order="F")
+        a = np.empty(self.shape, order="F", dtype=self.dtype)
         rc = hip().hipMemcpy(a.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr), self.nbytes, 2)
         if rc != 0:
             raise RuntimeError(f"hipMemcpy D2H failed ({rc})")
