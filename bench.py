@@ -51,8 +51,12 @@ def parse():
     p.add_argument("--dtau", type=float, default=1.0, help="pseudo-time step of the preconditioner (default 1)")
     p.add_argument("--preconditioner", default="line_relaxation", choices=["line_relaxation", "multigrid"],
                    help="mixed precision: the reference's line relaxation or the multigrid V-cycle")
-    p.add_argument("--initial-guess", default="zero", choices=["zero", "previous"],
-                   help="mixed precision: start each solve from zero or from the same stage's previous solution")
+    p.add_argument("--initial-guess", default="zero", choices=["zero", "previous", "learned"],
+                   help="mixed precision: start each solve from zero, from the same stage's previous solution, or "
+                        "from the learned network's guess (trained first, on held-out terrain)")
+    p.add_argument("--train-runs", type=int, default=2, help="learned guess: training runs (random hills)")
+    p.add_argument("--train-steps", type=int, default=8, help="learned guess: time steps per training run")
+    p.add_argument("--train-epochs", type=int, default=10, help="learned guess: passes over the samples")
     p.add_argument("--no-fp64-check", action="store_true",
                    help="mixed precision: skip the FP64 rerun that measures the field differences")
     return p.parse_args()
@@ -69,10 +73,15 @@ def main():
                                        iterations=args.steps + args.warmup, precision=args.precision)
     nl.poisson.initial_guess = args.initial_guess
     nl.poisson.preconditioner_type = args.preconditioner
+    training = None
+    if args.initial_guess == "learned":
+        training = train_guess(pcf, args)
     nl.poisson.preconditioner_iterations = args.pc_iterations
     nl.poisson.dtau = args.dtau
     t0 = time.perf_counter()
     st = pcf.State(nl)
+    if training is not None:
+        pcf.learned.attach(st, training.pop("net"))
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     n_clean = st.initial_cleaning()
@@ -142,9 +151,38 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "error_vs_fp64": error_vs_fp64,
+        "learned_training": training,
         "kernel_ms": {k: [v[0], round(v[1], 3)] for k, v in sorted(prof.items())},
     }
     print(json.dumps(out), flush=True)
+
+
+def train_guess(pcf, args):
+    """Train the initial-guess network on the solver's own solves of
+    held-out configurations (the same physics with seeded random hills, not
+    the benchmark's single hill), on the device; returns the inference
+    network and a summary (not part of the timed region)."""
+    t0 = time.perf_counter()
+    samples = []
+    for r in range(args.train_runs):
+        nl = pcf.examples.gaussian_hills_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz,
+                                            iterations=args.train_steps, seed=100 + r, n_hills=8, precision="mixed")
+        nl.poisson.preconditioner_type = args.preconditioner
+        st = pcf.State(nl)
+        st.initial_cleaning()
+        samples += pcf.learned.collect_samples(st, args.train_steps)
+        st.close()
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    net = pcf.learned.GuessNetwork(args.nx, args.nx, args.nz, training=True, seed=0)
+    losses = pcf.learned.train(net, samples, epochs=args.train_epochs, lr=2e-3,
+                               log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+    params = net.params()
+    net.close()
+    t_train = time.perf_counter() - t0
+    infer = pcf.learned.GuessNetwork(args.nx, args.nx, args.nz, training=False, params=params)
+    return {"net": infer, "samples": len(samples), "epochs": args.train_epochs, "loss_first": losses[0],
+            "loss_last": losses[-1], "data_s": round(t_data, 1), "train_s": round(t_train, 1)}
 
 
 def fp64_difference(pcf, args, st_mixed):
