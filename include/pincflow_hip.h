@@ -389,6 +389,9 @@ int pcf_guess_input(pcf_state* st, const double* d_b, const double* d_prev, floa
 /* Attach the network used by initial_guess = PCF_GUESS_LEARNED (not owned;
  * its grid must be the state's). */
 int pcf_set_guess_network(pcf_state* st, pcf_net* net);
+/* Number of solves whose initial guess was replaced by the zero start
+ * because it left a residual no smaller than |b|. */
+int pcf_guess_fallbacks(const pcf_state* st, int32_t* count);
 
 #ifdef __cplusplus
 }

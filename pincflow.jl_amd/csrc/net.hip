@@ -524,6 +524,12 @@ int pcf_guess_input(pcf_state* st, const double* d_b, const double* d_prev, floa
   return PCF_OK;
 }
 
+int pcf_guess_fallbacks(const pcf_state* st, int32_t* count) {
+  if (!st || !count) return PCF_E_ARG;
+  *count = st->guess_fallbacks;
+  return PCF_OK;
+}
+
 int pcf_set_guess_network(pcf_state* st, pcf_net* net) {
   using namespace pcf;
   if (!st) return PCF_E_ARG;

@@ -163,6 +163,7 @@ struct pcf_state {
   pcf_net* guess_net = nullptr;
   float *guess_in = nullptr, *guess_out = nullptr;
   double* guess_x = nullptr;
+  int guess_fallbacks = 0;            // solves whose guess was dropped for a zero start
   double* partials;                   // reduction partials
   pcf::SolverScalars* sc;             // device scalars
   pcf::SolverScalars* sc_host;        // pinned copy

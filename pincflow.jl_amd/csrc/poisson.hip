@@ -1110,7 +1110,8 @@ int dot(pcf_state* st, const double* a, const double* b, double* out) {
 // mixed: the preconditioner runs in FP32 (mixed.hip) and the solution is
 // accumulated in the textbook form x += alpha M^-1 p + omega M^-1 s, which
 // keeps it consistent with the FP64 residual recurrence; guess (may be null):
-// the solve starts from x0 = guess_scale * guess instead of 0.
+// the solve starts from x0 = guess_scale * guess instead of 0, unless x0
+// leaves a residual no smaller than |b|, in which case it starts from 0.
 int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess, double guess_scale,
                   int* niter_out) {
   const Geo& g = st->g;
@@ -1137,6 +1138,16 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
   PCF_HIP(hipGetLastError());
   PCF_HIP(hipMemcpyAsync(st->sc_host, sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, st->stream));
   PCF_HIP(hipStreamSynchronize(st->stream));
+  if (guess && !(st->sc_host->res < st->sc_host->b_norm)) {
+    // the guess does not reduce the residual: start from zero, as the plain solve
+    hipLaunchKernelGGL(k_bicg_init, dim3(kVecGrid), dim3(kReduceBlock), 0, st->stream, st->rhs, st->solution,
+                       st->br0, st->br, st->bp, N, st->partials);
+    hipLaunchKernelGGL(k_bicg_start, dim3(1), dim3(1024), 0, st->stream, st->partials, kVecGrid, sc, nxyz, tol, 0);
+    PCF_HIP(hipGetLastError());
+    PCF_HIP(hipMemcpyAsync(st->sc_host, sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, st->stream));
+    PCF_HIP(hipStreamSynchronize(st->stream));
+    st->guess_fallbacks += 1;
+  }
   if (st->sc_host->done) {
     st->last_residual = st->sc_host->b_norm > 0.0 ? st->sc_host->res / st->sc_host->b_norm : 0.0;
     *niter_out = 0;

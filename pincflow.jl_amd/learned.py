@@ -175,21 +175,48 @@ def collect_samples(st: State, steps: int) -> List[Tuple[np.ndarray, np.ndarray]
     return out
 
 
-def train(net: GuessNetwork, samples, epochs: int, lr: float = 1e-3, log=None) -> List[float]:
-    """Adam on the mean squared error over the samples, one sample per step."""
+def train(net: GuessNetwork, samples, epochs: int, lr: float = 3e-4, log=None) -> List[float]:
+    """Adam on the mean squared error over the samples, one sample per step.
+
+    The targets are divided by their RMS over all samples, so the loss starts
+    near 1 for the zero output of an untrained network; the scale is folded
+    into the (linear) last layer afterwards, so the network's output is the
+    unscaled x / |b|.  Returns the normalised loss per epoch."""
     n = samples[0][1].size
+    scale = float(np.sqrt(np.mean([np.mean(t.astype(np.float64) ** 2) for _, t in samples]))) or 1.0
     d_in, d_t = DeviceBuffer(12 * n), DeviceBuffer(4 * n)
     losses = []
     for e in range(epochs):
         tot = 0.0
         for inp, target in samples:
             d_in.upload(inp)
-            d_t.upload(target)
+            d_t.upload((target / scale).astype(np.float32))
             tot += net.train_step(d_in, d_t, lr)
         losses.append(tot / len(samples))
         if log:
             log(f"epoch {e + 1}: loss {losses[-1]:.4e}")
+    p = net.params()
+    p[-(27 * 32 + 1):] *= scale  # last layer: W[27][1][32] and b[1]
+    net.set_params(p)
     return losses
+
+
+def evaluate(net: GuessNetwork, samples) -> float:
+    """Mean squared error of the network's output against the targets,
+    relative to the zero output's (1 = no better than a zero start)."""
+    n = samples[0][1].size
+    d_in, d_out = DeviceBuffer(12 * n), DeviceBuffer(4 * n)
+    num = den = 0.0
+    for inp, target in samples:
+        d_in.upload(inp)
+        rc = net.lib.pcf_net_forward(net.h, ctypes.c_void_p(d_in.ptr), ctypes.c_void_p(d_out.ptr))
+        if rc:
+            raise PinCFlowError(rc, net.lib.pcf_last_error().decode())
+        out = d_out.download(np.float32, (n,)).astype(np.float64)
+        t = target.astype(np.float64)
+        num += float(np.sum((out - t) ** 2))
+        den += float(np.sum(t * t))
+    return num / den
 
 
 def attach(st: State, net: GuessNetwork):

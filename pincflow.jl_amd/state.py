@@ -186,6 +186,13 @@ class State:
                 out[name] = (int(n.value), float(ms.value))
         return out
 
+    def guess_fallbacks(self) -> int:
+        """Solves whose initial guess (previous or learned) left a residual no
+        smaller than |b| and was replaced by the zero start."""
+        n = ctypes.c_int32(0)
+        self._check(self.lib.pcf_guess_fallbacks(self.h, ctypes.byref(n)))
+        return n.value
+
     # ---- time and restart ------------------------------------------------------
     def time(self):
         """(model time in s, steps taken)."""

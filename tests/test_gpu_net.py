@@ -4,7 +4,8 @@
 The device keeps the 32 -> 32 weights, the activations and the backward
 signal of the matrix-core layers in bf16 and accumulates in FP32.  The
 forward check emulates the bf16 rounding of weights and activations in
-numpy, so the two differ only by FP32 summation order (FORWARD_RTOL).  The
+numpy; FP32 summation order then still flips the odd activation by one bf16
+step (0.4 %), which FORWARD_RTOL allows for.  The
 gradient check compares against the exact FP64 gradient: bf16 operands in
 the backward GEMMs leave a relative error of a few 1e-3 (GRAD_RTOL, in the
 2-norm per layer).
@@ -19,7 +20,7 @@ import cnn_reference as cnn
 
 pytestmark = pytest.mark.gpu
 
-FORWARD_RTOL = 2e-3
+FORWARD_RTOL = 1e-2
 GRAD_RTOL = 3e-2
 
 
