@@ -65,10 +65,13 @@ def parse():
 
 
 def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False):
-    """Setup, initial cleaning, warmup, then the timed steps of one method;
-    kernel groups are timed with HIP events over the timed steps."""
+    """Setup, initial cleaning, warmup, then the timed steps of one method,
+    with HIP events around the launches of the operator apply only (its
+    average duration gives the roofline figure; events around every launch
+    would delay the launch-heavy preconditioners), then one more step with
+    events around every kernel group for the breakdown."""
     nl = pcf.examples.mountain_wave_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz,
-                                       iterations=args.steps + args.warmup, precision=precision)
+                                       iterations=args.steps + args.warmup + 1, precision=precision)
     nl.poisson.initial_guess = guess
     nl.poisson.preconditioner_type = preconditioner
     nl.poisson.preconditioner_iterations = args.pc_iterations
@@ -88,7 +91,7 @@ def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False
         info = st.step()
         print(f"# [{tag}] warmup step {w + 1}: dt {info.dt:.3f}s iterations {list(info.niter)}", file=sys.stderr,
               flush=True)
-    st.profile(True)
+    st.profile(True, groups=["apply_total_dot"])
     iters = []
     t0 = time.perf_counter()
     for s in range(args.steps):
@@ -98,8 +101,12 @@ def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False
               flush=True)
     elapsed = time.perf_counter() - t0
     prof = st.profile_report()
+    st.profile(True)
+    info = st.step()
+    breakdown = st.profile_report()
     st.profile(False)
-    res = {"nl": nl, "iters": iters, "prof": prof, "elapsed": elapsed,
+    print(f"# [{tag}] breakdown step: iterations {list(info.niter)}", file=sys.stderr, flush=True)
+    res = {"nl": nl, "iters": iters, "prof": prof, "breakdown": breakdown, "elapsed": elapsed,
            "value": args.steps / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
            "iterations_per_solve": sum(sum(i) for i in iters) / (2 * len(iters))}
     if keep:
@@ -171,7 +178,8 @@ def main():
                      "value": round(r["value"], 4), "ms_per_step": round(r["ms_per_step"], 2),
                      "speedup_vs_plain": round(r["value"] / main_run["value"], 3),
                      "bicgstab_iterations_per_solve": round(r["iterations_per_solve"], 1),
-                     "error_vs_fp64": field_difference(r["state"], st)}
+                     "error_vs_fp64": field_difference(r["state"], st),
+                     "kernel_ms_one_step": {k: [v[0], round(v[1], 3)] for k, v in sorted(r["breakdown"].items())}}
             if guess == "learned":
                 entry["learned_training"] = vtrain
                 entry["guess_fallbacks"] = r["state"].guess_fallbacks()
@@ -209,7 +217,7 @@ def main():
         "variants": variants,
         "error_vs_fp64": error_vs_fp64,
         "learned_training": training,
-        "kernel_ms": {k: [v[0], round(v[1], 3)] for k, v in sorted(prof.items())},
+        "kernel_ms_one_step": {k: [v[0], round(v[1], 3)] for k, v in sorted(main_run["breakdown"].items())},
     }
     print(json.dumps(out), flush=True)
 
@@ -232,7 +240,7 @@ def train_guess(pcf, args, preconditioner=None):
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
     net = pcf.learned.GuessNetwork(args.nx, args.nx, args.nz, training=True, seed=0)
-    losses = pcf.learned.train(net, samples, epochs=args.train_epochs, lr=2e-3,
+    losses = pcf.learned.train(net, samples, epochs=args.train_epochs,
                                log=lambda m: print("# " + m, file=sys.stderr, flush=True))
     params = net.params()
     net.close()
@@ -248,10 +256,10 @@ def fp64_difference(pcf, args, st_mixed):
     steps (the FP64 library path agrees with the FP64 oracle to 1e-9); the
     Exner pressure is compared up to a constant."""
     nl = pcf.examples.mountain_wave_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz,
-                                       iterations=args.steps + args.warmup, precision="fp64")
+                                       iterations=args.steps + args.warmup + 1, precision="fp64")
     ref = pcf.State(nl)
     ref.initial_cleaning()
-    for _ in range(args.steps + args.warmup):
+    for _ in range(args.steps + args.warmup + 1):
         ref.step()
     out = {}
     for f in ("RHO", "RHOP", "U", "V", "W", "PIP"):

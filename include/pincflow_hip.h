@@ -355,6 +355,10 @@ enum pcf_kernel_group {
 /* Record HIP events around every launch of the groups above (on = 1) or stop
  * (on = 0).  Turning it on clears the totals. */
 int pcf_profile_enable(pcf_state* st, int on);
+/* The same for the groups whose bit (1 << group) is set in mask only, so
+ * that the events of the other groups do not delay their launches; mask 0
+ * stops. */
+int pcf_profile_groups(pcf_state* st, uint32_t mask);
 /* Launch count and summed device time of one group since it was enabled. */
 int pcf_profile_query(pcf_state* st, int group, int64_t* launches, double* total_ms);
 

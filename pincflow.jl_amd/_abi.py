@@ -194,6 +194,7 @@ SIGNATURES = [
     ("pcf_guess_input", _I, [_P, _P, _P, _P, _PD]),
     ("pcf_set_guess_network", _I, [_P, _P]),
     ("pcf_guess_fallbacks", _I, [_P, _PI32]),
+    ("pcf_profile_groups", _I, [_P, ctypes.c_uint32]),
     ("pcf_profile_enable", _I, [_P, _I]),
     ("pcf_profile_query", _I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _PD]),
 ]

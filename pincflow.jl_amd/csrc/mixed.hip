@@ -14,7 +14,9 @@
 // from the same stage's solution of the previous time step, rescaled by the
 // ratio of the time steps (the solution is proportional to dt times the
 // pressure correction); the criterion stays |r| / |b|.
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "pcf_device.hpp"
 #include "solver_common.hpp"
@@ -293,38 +295,117 @@ __global__ void __launch_bounds__(64) k_mg_factors(const float* ac, const float*
   }
 }
 
-// b_coarse = R (b - A s): one thread per coarse cell sums the residuals of
-// its fx x fy fine cells.
-template <int FX, int FY, typename BT>
-__global__ void k_mg_restrict(MgCoefs A, const BT* __restrict__ b, const float* __restrict__ s, float* bc, int nx,
-                              int ny, int nz, int ncx, int ncy, const SolverScalars* sc) {
+// r = b - A s on one level, one thread per cell (block 64 x 4).
+template <typename BT>
+__global__ void __launch_bounds__(256) k_mg_resid(MgCoefs A, const BT* __restrict__ b, const float* __restrict__ s,
+                                                  float* __restrict__ r, int nx, int ny, int nz,
+                                                  const SolverScalars* sc) {
   if (sc && sc->done) return;
-  const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y, k = blockIdx.z;
-  if (I >= ncx) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= nx || j >= ny) return;
   constexpr int oi[T_COUNT] = {0, -1, 1, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0};
   constexpr int oj[T_COUNT] = {0, 0, 0, -1, 1, 0, 0, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0, 0, 0, -1, -1, 1, 1};
   constexpr int ok[T_COUNT] = {0, 0, 0, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -2, 2, -2, 2, -2, 2, -2, 2, -2, 2};
+  const size_t sxy = (size_t)nx * ny;
+  const size_t o = (size_t)i + (size_t)nx * j + sxy * k;
+  float v = 0.0f;
+#pragma unroll
+  for (int t = 0; t < T_COUNT; ++t) {
+    const int kk = k + ok[t];
+    if (kk < 0 || kk >= nz) continue;
+    int ii = i + oi[t], jj = j + oj[t];
+    ii = ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii);
+    jj = jj < 0 ? jj + ny : (jj >= ny ? jj - ny : jj);
+    v += A.a[t][o] * s[(size_t)ii + (size_t)nx * jj + sxy * kk];
+  }
+  r[o] = (float)b[o] - v;
+}
+
+// b_coarse = R r: one thread per coarse cell sums its fx x fy fine cells.
+template <int FX, int FY>
+__global__ void k_mg_sum(const float* __restrict__ r, float* __restrict__ bc, int nx, int ny, int ncx, int ncy,
+                         const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y, k = blockIdx.z;
+  if (I >= ncx) return;
   const size_t sxy = (size_t)nx * ny;
   float sum = 0.0f;
 #pragma unroll
   for (int bb = 0; bb < FY; ++bb)
 #pragma unroll
-    for (int a = 0; a < FX; ++a) {
-      const int i = FX * I + a, j = FY * J + bb;
-      const size_t o = (size_t)i + (size_t)nx * j + sxy * k;
-      float v = 0.0f;
-#pragma unroll
-      for (int t = 0; t < T_COUNT; ++t) {
-        const int kk = k + ok[t];
-        if (kk < 0 || kk >= nz) continue;
-        int ii = i + oi[t], jj = j + oj[t];
-        ii = ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii);
-        jj = jj < 0 ? jj + ny : (jj >= ny ? jj - ny : jj);
-        v += A.a[t][o] * s[(size_t)ii + (size_t)nx * jj + sxy * kk];
-      }
-      sum += (float)b[o] - v;
-    }
+    for (int a = 0; a < FX; ++a) sum += r[(size_t)(FX * I + a) + (size_t)nx * (FY * J + bb) + sxy * k];
   bc[(size_t)I + (size_t)ncx * J + (size_t)ncx * ncy * k] = sum;
+}
+
+// b_coarse = R (b - A s) for 2 x 2 aggregation and nx % 4 == 0: block
+// 16 x 16 threads, 4 x-adjacent fine cells per thread (the loads of
+// k_pc_rhs32); the x pairs are summed in the thread, the y pairs across the
+// lanes l and l + 16 of a wavefront (rows j and j + 1).
+template <typename BT>
+__global__ void __launch_bounds__(256) k_resid_restrict4(MgCoefs A, const BT* __restrict__ b,
+                                                         const float* __restrict__ s, float* __restrict__ bc, int nx,
+                                                         int ny, int nz, const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  constexpr int W = 4;
+  const int i0 = (blockIdx.x * 16 + threadIdx.x) * W;
+  const int j = blockIdx.y * 16 + threadIdx.y;
+  const int k = blockIdx.z;
+  const bool act = i0 < nx && j < ny;
+  float c0 = 0.0f, c1 = 0.0f;
+  if (act) {
+    const size_t sxy = (size_t)nx * ny;
+    const size_t o = (size_t)i0 + (size_t)nx * j + sxy * k;
+    const int im = i0 == 0 ? nx - 1 : i0 - 1, ip = i0 + W == nx ? 0 : i0 + W;
+    const int jp = j + 1 == ny ? 0 : j + 1, jm = j == 0 ? ny - 1 : j - 1;
+    float C[5][W], F[5][W], B[5][W], L[5], R[5];
+#pragma unroll
+    for (int w = 0; w < 5; ++w) {
+      const int kk = k - 2 + w;
+      if (kk < 0 || kk >= nz) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) C[w][c] = F[w][c] = B[w][c] = 0.0f;
+        L[w] = R[w] = 0.0f;
+        continue;
+      }
+      const size_t pl = sxy * kk;
+      ld<W>(s + pl + (size_t)nx * j + i0, C[w]);
+      ld<W>(s + pl + (size_t)nx * jp + i0, F[w]);
+      ld<W>(s + pl + (size_t)nx * jm + i0, B[w]);
+      L[w] = s[pl + (size_t)nx * j + im];
+      R[w] = s[pl + (size_t)nx * j + ip];
+    }
+    float a[T_COUNT][W];
+#pragma unroll
+    for (int t = 0; t < T_COUNT; ++t) ld_stream<W>(A.a[t] + o, a[t]);
+    float r[W];
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      auto XL = [&](int w) -> float { return c == 0 ? L[w] : C[w][c - 1]; };
+      auto XR = [&](int w) -> float { return c == W - 1 ? R[w] : C[w][c + 1]; };
+      auto X0 = [&](int w) -> float { return C[w][c]; };
+      auto YF = [&](int w) -> float { return F[w][c]; };
+      auto YB = [&](int w) -> float { return B[w][c]; };
+      float v = a[T_AL][c] * XL(2) + a[T_AR][c] * XR(2) + a[T_AF][c] * YF(2) + a[T_AB][c] * YB(2) +
+                a[T_AU][c] * X0(3) + a[T_AD][c] * X0(1) + a[T_AC][c] * X0(2);
+      v += a[T_ARU][c] * XR(3) + a[T_ARD][c] * XR(1) + a[T_ALU][c] * XL(3) + a[T_ALD][c] * XL(1) +
+           a[T_AFU][c] * YF(3) + a[T_AFD][c] * YF(1) + a[T_ABU][c] * YB(3) + a[T_ABD][c] * YB(1) +
+           a[T_AUU][c] * X0(4) + a[T_ADD][c] * X0(0) + a[T_ARUU][c] * XR(4) + a[T_ARDD][c] * XR(0) +
+           a[T_ALUU][c] * XL(4) + a[T_ALDD][c] * XL(0) + a[T_AFUU][c] * YF(4) + a[T_AFDD][c] * YF(0) +
+           a[T_ABUU][c] * YB(4) + a[T_ABDD][c] * YB(0);
+      r[c] = (float)b[o + c] - v;
+    }
+    c0 = r[0] + r[1];
+    c1 = r[2] + r[3];
+  }
+  // every lane takes part in the exchange; lane + 16 is row j + 1 of the same x
+  const float u0 = __shfl_down(c0, 16), u1 = __shfl_down(c1, 16);
+  if (act && (j & 1) == 0) {
+    const int ncx = nx / 2, ncy = ny / 2;
+    float2 v;
+    v.x = c0 + u0;
+    v.y = c1 + u1;
+    *(float2*)(bc + (size_t)(i0 / 2) + (size_t)ncx * (j / 2) + (size_t)ncx * ncy * k) = v;
+  }
 }
 
 // s += omega P e
@@ -335,6 +416,225 @@ __global__ void k_mg_prolong(float* s, const float* e, int nx, int ny, int nz, i
   if (i >= nx) return;
   const size_t o = (size_t)i + (size_t)nx * j + (size_t)nx * ny * k;
   s[o] += omega * e[(size_t)(i / fx) + (size_t)ncx * (j / fy) + (size_t)ncx * ncy * k];
+}
+
+// ---------------------------------------------------------------------------
+// Line solves with one wavefront per column.  The z recurrences of the line
+// solve are affine maps, s_k = fa_k + fb_k s_(k-1) upwards and
+// y_k = col_k + q_k y_(k+1) downwards; lane l holds NPL consecutive levels,
+// composes its maps and an inclusive scan over the 64 lanes (shuffles)
+// composes the rest, so a column costs one round trip to memory instead of
+// nz dependent steps.  Used where there are too few columns to hide the
+// latency of the thread-per-column sweep.
+// ---------------------------------------------------------------------------
+template <int NPL>
+__device__ __forceinline__ void scan_line(const float (&fa)[NPL], const float (&fb)[NPL], const float (&q)[NPL],
+                                          int nz, float (&y)[NPL]) {
+  const int lane = threadIdx.x & 63;
+  float A = 0.0f, B = 1.0f;
+#pragma unroll
+  for (int u = 0; u < NPL; ++u) {
+    A = fa[u] + fb[u] * A;
+    B = fb[u] * B;
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const float Ap = __shfl_up(A, off), Bp = __shfl_up(B, off);
+    if (lane >= off) {
+      A = A + B * Ap;
+      B = B * Bp;
+    }
+  }
+  float sv = __shfl_up(A, 1);
+  if (lane == 0) sv = 0.0f;  // the map of level 0 does not depend on it
+  float col[NPL];
+#pragma unroll
+  for (int u = 0; u < NPL; ++u) {
+    sv = fa[u] + fb[u] * sv;
+    col[u] = NPL * lane + u < nz ? sv : 0.0f;
+  }
+  float Cc = 0.0f, D = 1.0f;
+#pragma unroll
+  for (int u = NPL - 1; u >= 0; --u) {
+    Cc = col[u] + q[u] * Cc;
+    D = q[u] * D;
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const float Cn = __shfl_down(Cc, off), Dn = __shfl_down(D, off);
+    if (lane + off < 64) {
+      Cc = Cc + D * Cn;
+      D = D * Dn;
+    }
+  }
+  float t = __shfl_down(Cc, 1);
+  if (lane == 63) t = 0.0f;  // above the top level: q = 0 there
+#pragma unroll
+  for (int u = NPL - 1; u >= 0; --u) {
+    t = col[u] + q[u] * t;
+    y[u] = t;
+  }
+}
+
+// The forward map of level k of a column from its line-solve factors and
+// right-hand side r (k = 0: P holds the pivot itself); identity maps above nz.
+template <typename IdxT>
+__device__ __forceinline__ void line_maps(const float* P, const float* E, const float* Q, IdxT o, int k, int nz,
+                                          float r, float& fa, float& fb, float& q) {
+  if (k >= nz) {
+    fa = 0.0f;
+    fb = 1.0f;
+    q = 1.0f;
+    return;
+  }
+  const float p = P[o];
+  if (k == 0) {
+    fa = r / p;
+    fb = 0.0f;
+  } else {
+    fa = r * p;
+    fb = E[o] * p;
+  }
+  q = k == nz - 1 ? 0.0f : Q[o];
+}
+
+// Line solve of ncols columns, block 256 = 4 columns.  FIRST: rhs = deta (0 - src).
+template <int NPL, bool FIRST, typename SrcT, typename OutT>
+__global__ void __launch_bounds__(256) k_line_scan(const float* __restrict__ P, const float* __restrict__ E,
+                                                   const float* __restrict__ Q, const SrcT* __restrict__ src,
+                                                   OutT* __restrict__ out, int ncols, int nz, float deta,
+                                                   const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= ncols) return;  // uniform over the wavefront
+  const int lane = threadIdx.x & 63;
+  float fa[NPL], fb[NPL], q[NPL], y[NPL];
+#pragma unroll
+  for (int u = 0; u < NPL; ++u) {
+    const int k = NPL * lane + u;
+    const size_t o = (size_t)c + (size_t)ncols * k;
+    float r = 0.0f;
+    if (k < nz) r = FIRST ? deta * (0.0f - (float)src[o]) : (float)src[o];
+    line_maps(P, E, Q, o, k, nz, r, fa[u], fb[u], q[u]);
+  }
+  scan_line<NPL>(fa, fb, q, nz, y);
+#pragma unroll
+  for (int u = 0; u < NPL; ++u) {
+    const int k = NPL * lane + u;
+    if (k < nz) out[(size_t)c + (size_t)ncols * k] = (OutT)y[u];
+  }
+}
+
+// One pseudo-time step of column (i, j): the right-hand side
+// s + deta (H s - b) at the lane's levels (FIRST: deta (0 - b), from s = 0),
+// then the line solve.  rd(i, j, k) reads the current iterate.
+// 32-bit indices: column-path levels hold at most 1024 x 512 cells.
+template <int NPL, bool FIRST, typename BT, class Rd>
+__device__ __forceinline__ void column_step(const MgCoefs& A, const float* P, const float* E, const float* Q,
+                                            const BT* b, Rd rd, int i, int j, int nx, int ny, int nz, float deta,
+                                            float (&y)[NPL]) {
+  const int lane = threadIdx.x & 63;
+  const int sxy = nx * ny;
+  const int c = i + nx * j;
+  const int im = i == 0 ? nx - 1 : i - 1, ip = i + 1 == nx ? 0 : i + 1;
+  const int jm = j == 0 ? ny - 1 : j - 1, jp = j + 1 == ny ? 0 : j + 1;
+  float fa[NPL], fb[NPL], q[NPL];
+#pragma unroll
+  for (int u = 0; u < NPL; ++u) {
+    const int k = NPL * lane + u;
+    const int o = c + sxy * k;
+    float r = 0.0f;
+    if (k < nz) {
+      if (FIRST) {
+        r = deta * (0.0f - (float)b[o]);
+      } else {
+        auto S = [&](int ii, int jj, int kk) -> float { return kk < 0 || kk >= nz ? 0.0f : rd(ii, jj, kk); };
+        auto T = [&](int t) -> float { return A.a[t][o]; };
+        float v = T(T_AL) * S(im, j, k) + T(T_AR) * S(ip, j, k) + T(T_AF) * S(i, jp, k) + T(T_AB) * S(i, jm, k);
+        v += T(T_ARU) * S(ip, j, k + 1) + T(T_ARD) * S(ip, j, k - 1) + T(T_ALU) * S(im, j, k + 1) +
+             T(T_ALD) * S(im, j, k - 1) + T(T_AFU) * S(i, jp, k + 1) + T(T_AFD) * S(i, jp, k - 1) +
+             T(T_ABU) * S(i, jm, k + 1) + T(T_ABD) * S(i, jm, k - 1) + T(T_AUU) * S(i, j, k + 2) +
+             T(T_ADD) * S(i, j, k - 2) + T(T_ARUU) * S(ip, j, k + 2) + T(T_ARDD) * S(ip, j, k - 2) +
+             T(T_ALUU) * S(im, j, k + 2) + T(T_ALDD) * S(im, j, k - 2) + T(T_AFUU) * S(i, jp, k + 2) +
+             T(T_AFDD) * S(i, jp, k - 2) + T(T_ABUU) * S(i, jm, k + 2) + T(T_ABDD) * S(i, jm, k - 2);
+        r = S(i, j, k) + deta * (v - (float)b[o]);
+      }
+    }
+    line_maps(P, E, Q, o, k, nz, r, fa[u], fb[u], q[u]);
+  }
+  scan_line<NPL>(fa, fb, q, nz, y);
+}
+
+// One smoothing step of a level with few columns, one wavefront per column
+// (block 256 = 4 columns); s_old and out must differ.  PRO: the iterate is
+// s_old + omega P e (the coarse-level correction, fx = 1 << sx, fy = 1 << sy).
+template <int NPL, bool FIRST, bool PRO, typename BT, typename OutT>
+__global__ void __launch_bounds__(256) k_col_smooth(MgCoefs A, const float* __restrict__ P,
+                                                    const float* __restrict__ E, const float* __restrict__ Q,
+                                                    const BT* __restrict__ b, const float* __restrict__ s_old,
+                                                    const float* __restrict__ e, float omega, int sx, int sy, int ncx,
+                                                    int ncy, OutT* __restrict__ out, int nx, int ny, int nz,
+                                                    float deta, const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= nx * ny) return;  // uniform over the wavefront
+  const int i = c % nx, j = c / nx;
+  const int sxy = nx * ny, scxy = ncx * ncy;
+  auto rd = [&](int ii, int jj, int kk) -> float {
+    float v = s_old[ii + nx * jj + sxy * kk];
+    if (PRO) v += omega * e[(ii >> sx) + ncx * (jj >> sy) + scxy * kk];
+    return v;
+  };
+  float y[NPL];
+  column_step<NPL, FIRST, BT>(A, P, E, Q, b, rd, i, j, nx, ny, nz, deta, y);
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int u = 0; u < NPL; ++u) {
+    const int k = NPL * lane + u;
+    if (k < nz) out[c + sxy * k] = (OutT)y[u];
+  }
+}
+
+// All smoothing steps of the coarsest level in one workgroup of up to 16
+// wavefronts, wavefront w smoothing the columns w, w + 16, ... (nx ny <= 16),
+// the iterate ping-ponging between two LDS buffers (2 nx ny nz floats of
+// dynamic shared memory), so a column's new values are stored as soon as
+// they are computed.
+template <int NPL, typename BT, typename OutT>
+__global__ void __launch_bounds__(1024) k_coarse_solve(MgCoefs A, const float* __restrict__ P,
+                                                      const float* __restrict__ E, const float* __restrict__ Q,
+                                                      const BT* __restrict__ b, OutT* __restrict__ out, int nx,
+                                                      int ny, int nz, float deta, int steps,
+                                                      const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  extern __shared__ float lds[];
+  const int ncols = nx * ny;
+  const int sxy = ncols, n = ncols * nz;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int step = 0; step < steps; ++step) {
+    const float* cur = lds + (step & 1) * n;
+    float* nxt = lds + ((step + 1) & 1) * n;
+    auto rd = [&](int ii, int jj, int kk) -> float { return cur[ii + nx * jj + sxy * kk]; };
+#pragma unroll 1
+    for (int c = w; c < ncols; c += nw) {  // uniform over the wavefront
+      float y[NPL];
+      if (step == 0)
+        column_step<NPL, true, BT>(A, P, E, Q, b, rd, c % nx, c / nx, nx, ny, nz, deta, y);
+      else
+        column_step<NPL, false, BT>(A, P, E, Q, b, rd, c % nx, c / nx, nx, ny, nz, deta, y);
+#pragma unroll
+      for (int u = 0; u < NPL; ++u) {
+        const int k = NPL * lane + u;
+        if (k < nz) {
+          if (step + 1 < steps)
+            nxt[c + sxy * k] = y[u];
+          else
+            out[c + sxy * k] = (OutT)y[u];
+        }
+      }
+    }
+    __syncthreads();  // the step's writes before the next step's reads
+  }
 }
 
 PcArgs32 pc_args32(pcf_state* st, const float* s, float* out) {
@@ -384,85 +684,210 @@ int prepare_mixed(pcf_state* st) {
   return PCF_OK;
 }
 
-// Smoothing steps of one level: the first from s = 0 when from_zero; the
-// last writes to out64 when given (FP64 result of the finest level).
+// Level paths.  Levels with at most kColumnCols columns smooth with one
+// fused wavefront-per-column kernel per step; larger ones with the
+// row-parallel horizontal kernel and a line solve, the latter one wavefront
+// per column up to kScanCols columns and one thread per column beyond (where
+// the columns alone fill the chip).  The coarsest level runs all its steps
+// in one workgroup when it has at most 16 columns.
+constexpr size_t kColumnCols = 1024;
+constexpr size_t kScanCols = 16384;
+constexpr int kScanMaxNz = 512;
+constexpr size_t kCoarseLdsFloats = 16384;
+
+size_t columns(const MgLevel& L) { return (size_t)L.nx * L.ny; }
+bool column_path(const MgLevel& L) { return columns(L) <= kColumnCols && L.nz <= kScanMaxNz; }
+
+template <class F>
+void with_npl(int nz, F&& f) {
+  if (nz <= 64)
+    f(std::integral_constant<int, 1>{});
+  else if (nz <= 128)
+    f(std::integral_constant<int, 2>{});
+  else if (nz <= 256)
+    f(std::integral_constant<int, 4>{});
+  else
+    f(std::integral_constant<int, 8>{});
+}
+
+MgCoefs coefs_of(const MgLevel& L) {
+  MgCoefs A;
+  for (int t = 0; t < T_COUNT; ++t) A.a[t] = L.a[t];
+  return A;
+}
+
+// Line solve of a row-path level (FIRST: rhs = deta (0 - src)).
+template <bool FIRST, typename SrcT, typename OutT>
+void line_solve(pcf_state* st, const MgLevel& L, const SrcT* src, OutT* out, const SolverScalars* sc) {
+  ProfScope prof(st, PCF_K_PC_THOMAS);
+  const size_t nc = columns(L);
+  if (nc <= kScanCols && L.nz <= kScanMaxNz) {
+    with_npl(L.nz, [&](auto npl) {
+      hipLaunchKernelGGL((k_line_scan<decltype(npl)::value, FIRST, SrcT, OutT>), dim3(blocks_for(nc, 4)), dim3(256),
+                         0, st->stream, L.P, L.E, L.Q, src, out, (int)nc, L.nz, L.deta, sc);
+    });
+  } else {
+    hipLaunchKernelGGL((k_pc_thomas32<FIRST, SrcT, OutT>), dim3(blocks_for(L.nx, 64), L.ny), dim3(64), 0,
+                       st->stream, L.P, L.E, L.Q, src, out, L.col, L.nx, L.ny, L.nz, L.deta, sc);
+  }
+}
+
+// steps smoothing steps on level L.  s: the buffer holding the current
+// iterate, null to start from zero; e (with the coarser level C): the coarse
+// correction, s += omega P e before the first step.  The last step writes to
+// out64 when given (the FP64 result of the finest level); otherwise returns
+// the buffer holding the result.
 template <typename BT>
-void mg_smooth(pcf_state* st, const MgLevel& L, const BT* b, bool from_zero, int steps, double* out64,
-               const SolverScalars* sc) {
-  const dim3 tgrd(blocks_for(L.nx, 64), L.ny), tblk(64);
+float* mg_smooth(pcf_state* st, const MgLevel& L, const BT* b, float* s, const float* e, const MgLevel* C, int steps,
+                 double* out64, const SolverScalars* sc) {
+  const bool col = column_path(L);
+  const size_t nc = columns(L);
+  int step = 0;
+  if (!s) {  // first step from zero
+    const bool last = steps == 1;
+    if (col) {
+      ProfScope prof(st, PCF_K_PC_THOMAS);
+      with_npl(L.nz, [&](auto npl) {
+        constexpr int NPL = decltype(npl)::value;
+        if (last && out64)
+          hipLaunchKernelGGL((k_col_smooth<NPL, true, false, BT, double>), dim3(blocks_for(nc, 4)), dim3(256), 0,
+                             st->stream, coefs_of(L), L.P, L.E, L.Q, b, nullptr, nullptr, 0.0f, 0, 0, 1, 1, out64,
+                             L.nx, L.ny, L.nz, L.deta, sc);
+        else
+          hipLaunchKernelGGL((k_col_smooth<NPL, true, false, BT, float>), dim3(blocks_for(nc, 4)), dim3(256), 0,
+                             st->stream, coefs_of(L), L.P, L.E, L.Q, b, nullptr, nullptr, 0.0f, 0, 0, 1, 1, L.s, L.nx,
+                             L.ny, L.nz, L.deta, sc);
+      });
+    } else {
+      if (last && out64)
+        line_solve<true>(st, L, b, out64, sc);
+      else
+        line_solve<true>(st, L, b, L.s, sc);
+    }
+    s = L.s;
+    step = 1;
+  }
+  if (e && !col) {
+    ProfScope prof(st, PCF_K_PC_RHS);
+    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks_for(L.nx, 64), L.ny, L.nz), dim3(64), 0, st->stream, s, e, L.nx,
+                       L.ny, L.nz, L.fx, L.fy, C->nx, C->ny, kMgOmega, sc);
+  }
   const bool quads = L.nx % 4 == 0;
   const dim3 sgrd = quads ? dim3(blocks_for(L.nx, 64), blocks_for(L.ny, 16), L.nz)
                           : dim3(blocks_for(L.nx, 64), blocks_for(L.ny, 4), L.nz);
   const dim3 sblk = quads ? dim3(16, 16) : dim3(64, 4);
-  PcArgs32 A;
-  for (int t = 0; t < T_COUNT; ++t) A.a[t] = L.a[t];
-  A.s = L.s;
-  A.out = L.rhs;
-  A.nx = L.nx;
-  A.ny = L.ny;
-  A.nz = L.nz;
-  for (int step = 0; step < steps; ++step) {
+  for (; step < steps; ++step) {
     const bool last = step == steps - 1;
-    if (step == 0 && from_zero) {
+    if (col) {
+      const bool pro = e && step == 0;
+      float* dst = s == L.s ? L.rhs : L.s;
       ProfScope prof(st, PCF_K_PC_THOMAS);
-      if (last && out64)
-        hipLaunchKernelGGL((k_pc_thomas32<true, BT, double>), tgrd, tblk, 0, st->stream, L.P, L.E, L.Q, b, out64,
-                           L.col, L.nx, L.ny, L.nz, L.deta, sc);
-      else
-        hipLaunchKernelGGL((k_pc_thomas32<true, BT, float>), tgrd, tblk, 0, st->stream, L.P, L.E, L.Q, b, L.s, L.col,
-                           L.nx, L.ny, L.nz, L.deta, sc);
+      with_npl(L.nz, [&](auto npl) {
+        constexpr int NPL = decltype(npl)::value;
+        const int sx = L.fx == 2, sy = L.fy == 2;
+        const int ncx = C ? C->nx : 1, ncy = C ? C->ny : 1;
+        const dim3 g(blocks_for(nc, 4)), blk(256);
+        if (last && out64) {
+          if (pro)
+            hipLaunchKernelGGL((k_col_smooth<NPL, false, true, BT, double>), g, blk, 0, st->stream, coefs_of(L), L.P,
+                               L.E, L.Q, b, s, e, kMgOmega, sx, sy, ncx, ncy, out64, L.nx, L.ny, L.nz, L.deta, sc);
+          else
+            hipLaunchKernelGGL((k_col_smooth<NPL, false, false, BT, double>), g, blk, 0, st->stream, coefs_of(L), L.P,
+                               L.E, L.Q, b, s, e, kMgOmega, sx, sy, ncx, ncy, out64, L.nx, L.ny, L.nz, L.deta, sc);
+        } else {
+          if (pro)
+            hipLaunchKernelGGL((k_col_smooth<NPL, false, true, BT, float>), g, blk, 0, st->stream, coefs_of(L), L.P,
+                               L.E, L.Q, b, s, e, kMgOmega, sx, sy, ncx, ncy, dst, L.nx, L.ny, L.nz, L.deta, sc);
+          else
+            hipLaunchKernelGGL((k_col_smooth<NPL, false, false, BT, float>), g, blk, 0, st->stream, coefs_of(L), L.P,
+                               L.E, L.Q, b, s, e, kMgOmega, sx, sy, ncx, ncy, dst, L.nx, L.ny, L.nz, L.deta, sc);
+        }
+      });
+      s = dst;
       continue;
     }
     {
       ProfScope prof(st, PCF_K_PC_RHS);
+      PcArgs32 A;
+      for (int t = 0; t < T_COUNT; ++t) A.a[t] = L.a[t];
+      A.s = s;
+      A.out = L.rhs;
+      A.nx = L.nx;
+      A.ny = L.ny;
+      A.nz = L.nz;
       if (quads)
         hipLaunchKernelGGL((k_pc_rhs32<4, BT>), sgrd, sblk, 0, st->stream, A, b, L.deta, sc);
       else
         hipLaunchKernelGGL((k_pc_rhs32<1, BT>), sgrd, sblk, 0, st->stream, A, b, L.deta, sc);
     }
-    ProfScope prof(st, PCF_K_PC_THOMAS);
     if (last && out64)
-      hipLaunchKernelGGL((k_pc_thomas32<false, float, double>), tgrd, tblk, 0, st->stream, L.P, L.E, L.Q, L.rhs,
-                         out64, L.col, L.nx, L.ny, L.nz, L.deta, sc);
+      line_solve<false>(st, L, (const float*)L.rhs, out64, sc);
     else
-      hipLaunchKernelGGL((k_pc_thomas32<false, float, float>), tgrd, tblk, 0, st->stream, L.P, L.E, L.Q, L.rhs, L.s,
-                         L.col, L.nx, L.ny, L.nz, L.deta, sc);
+      line_solve<false>(st, L, (const float*)L.rhs, L.s, sc);
+    s = L.s;
   }
+  return s;
+}
+
+// b_coarse = R (b - A s)
+template <typename BT>
+void mg_restrict(pcf_state* st, const MgLevel& L, const MgLevel& C, const BT* b, const float* s,
+                 const SolverScalars* sc) {
+  ProfScope prof(st, PCF_K_PC_RHS);
+  const MgCoefs A = coefs_of(L);
+  if (!column_path(L) && L.fx == 2 && L.fy == 2 && L.nx % 4 == 0) {
+    hipLaunchKernelGGL((k_resid_restrict4<BT>), dim3(blocks_for(L.nx, 64), blocks_for(L.ny, 16), L.nz),
+                       dim3(16, 16), 0, st->stream, A, b, s, C.b, L.nx, L.ny, L.nz, sc);
+    return;
+  }
+  float* r = L.col;  // the line-solve scratch is free here
+  hipLaunchKernelGGL((k_mg_resid<BT>), dim3(blocks_for(L.nx, 64), blocks_for(L.ny, 4), L.nz), dim3(64, 4), 0,
+                     st->stream, A, b, s, r, L.nx, L.ny, L.nz, sc);
+  const dim3 cgrd(blocks_for(C.nx, 64), C.ny, C.nz), blk(64);
+  if (L.fx == 2 && L.fy == 2)
+    hipLaunchKernelGGL((k_mg_sum<2, 2>), cgrd, blk, 0, st->stream, r, C.b, L.nx, L.ny, C.nx, C.ny, sc);
+  else if (L.fx == 2)
+    hipLaunchKernelGGL((k_mg_sum<2, 1>), cgrd, blk, 0, st->stream, r, C.b, L.nx, L.ny, C.nx, C.ny, sc);
+  else
+    hipLaunchKernelGGL((k_mg_sum<1, 2>), cgrd, blk, 0, st->stream, r, C.b, L.nx, L.ny, C.nx, C.ny, sc);
+}
+
+// The coarsest level: kMgCoarseSteps smoothing steps from zero.
+template <typename BT>
+float* mg_coarsest(pcf_state* st, const MgLevel& L, const BT* b, double* out64, const SolverScalars* sc) {
+  const size_t nc = columns(L);
+  const size_t n = nc * (size_t)L.nz;
+  if (nc <= 16 && L.nz <= 256 && 2 * n <= kCoarseLdsFloats) {
+    ProfScope prof(st, PCF_K_PC_THOMAS);
+    const unsigned lds = (unsigned)(2 * n * sizeof(float));
+    const dim3 blk(64 * (unsigned)nc);
+    with_npl(L.nz, [&](auto npl) {
+      constexpr int NPL = decltype(npl)::value;
+      if constexpr (NPL <= 4) {
+        if (out64)
+          hipLaunchKernelGGL((k_coarse_solve<NPL, BT, double>), dim3(1), blk, lds, st->stream, coefs_of(L), L.P, L.E,
+                             L.Q, b, out64, L.nx, L.ny, L.nz, L.deta, kMgCoarseSteps, sc);
+        else
+          hipLaunchKernelGGL((k_coarse_solve<NPL, BT, float>), dim3(1), blk, lds, st->stream, coefs_of(L), L.P, L.E,
+                             L.Q, b, L.s, L.nx, L.ny, L.nz, L.deta, kMgCoarseSteps, sc);
+      }
+    });
+    return L.s;
+  }
+  return mg_smooth<BT>(st, L, b, nullptr, nullptr, nullptr, kMgCoarseSteps, out64, sc);
 }
 
 // One V-cycle on level l for the right-hand side b; the finest level writes
-// its result to out64.
+// its result to out64; returns the buffer holding the level's result.
 template <typename BT>
-void mg_vcycle(pcf_state* st, size_t l, const BT* b, double* out64, const SolverScalars* sc) {
+float* mg_vcycle(pcf_state* st, size_t l, const BT* b, double* out64, const SolverScalars* sc) {
   const MgLevel& L = st->mg[l];
-  if (l + 1 == st->mg.size()) {
-    mg_smooth(st, L, b, true, kMgCoarseSteps, out64, sc);
-    return;
-  }
+  if (l + 1 == st->mg.size()) return mg_coarsest<BT>(st, L, b, out64, sc);
   const MgLevel& C = st->mg[l + 1];
-  mg_smooth(st, L, b, true, kMgNu, nullptr, sc);
-  MgCoefs A;
-  for (int t = 0; t < T_COUNT; ++t) A.a[t] = L.a[t];
-  const dim3 cgrd(blocks_for(C.nx, 64), C.ny, C.nz), blk(64);
-  {
-    ProfScope prof(st, PCF_K_PC_RHS);
-    if (L.fx == 2 && L.fy == 2)
-      hipLaunchKernelGGL((k_mg_restrict<2, 2, BT>), cgrd, blk, 0, st->stream, A, b, L.s, C.b, L.nx, L.ny, L.nz, C.nx,
-                         C.ny, sc);
-    else if (L.fx == 2)
-      hipLaunchKernelGGL((k_mg_restrict<2, 1, BT>), cgrd, blk, 0, st->stream, A, b, L.s, C.b, L.nx, L.ny, L.nz, C.nx,
-                         C.ny, sc);
-    else
-      hipLaunchKernelGGL((k_mg_restrict<1, 2, BT>), cgrd, blk, 0, st->stream, A, b, L.s, C.b, L.nx, L.ny, L.nz, C.nx,
-                         C.ny, sc);
-  }
-  mg_vcycle<float>(st, l + 1, C.b, nullptr, sc);
-  {
-    ProfScope prof(st, PCF_K_PC_RHS);
-    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks_for(L.nx, 64), L.ny, L.nz), blk, 0, st->stream, L.s, C.s, L.nx,
-                       L.ny, L.nz, L.fx, L.fy, C.nx, C.ny, kMgOmega, sc);
-  }
-  mg_smooth(st, L, b, false, kMgNu, out64, sc);
+  float* s = mg_smooth<BT>(st, L, b, nullptr, nullptr, nullptr, kMgNu, nullptr, sc);
+  mg_restrict<BT>(st, L, C, b, s, sc);
+  const float* e = mg_vcycle<float>(st, l + 1, C.b, nullptr, sc);
+  return mg_smooth<BT>(st, L, b, s, e, &C, kMgNu, out64, sc);
 }
 
 }  // namespace

@@ -181,6 +181,7 @@ struct pcf_state {
     hipEvent_t a, b;
   };
   bool prof_on = false;
+  uint32_t prof_mask = 0;             // groups recorded while prof_on
   std::vector<ProfRec> prof_pending;
   std::vector<hipEvent_t> prof_pool;
   double prof_ms[PCF_K_COUNT] = {};

@@ -19,9 +19,8 @@ hipEvent_t take(pcf_state* st) {
 }  // namespace
 
 void prof_begin(pcf_state* st, int group, hipEvent_t* a) {
-  (void)group;
   *a = nullptr;
-  if (!st->prof_on) return;
+  if (!st->prof_on || !((st->prof_mask >> group) & 1u)) return;
   *a = take(st);
   if (*a) (void)hipEventRecord(*a, st->stream);
 }
@@ -59,18 +58,22 @@ void prof_release(pcf_state* st) {
 
 extern "C" {
 
-int pcf_profile_enable(pcf_state* st, int on) {
+int pcf_profile_groups(pcf_state* st, uint32_t mask) {
   if (!st) return PCF_E_ARG;
   pcf::prof_flush(st);
-  if (on) {
+  mask &= (1u << PCF_K_COUNT) - 1u;
+  if (mask) {
     for (int g = 0; g < PCF_K_COUNT; ++g) {
       st->prof_ms[g] = 0.0;
       st->prof_n[g] = 0;
     }
   }
-  st->prof_on = on != 0;
+  st->prof_on = mask != 0;
+  st->prof_mask = mask;
   return PCF_OK;
 }
+
+int pcf_profile_enable(pcf_state* st, int on) { return pcf_profile_groups(st, on ? ~0u : 0u); }
 
 int pcf_profile_query(pcf_state* st, int group, int64_t* launches, double* total_ms) {
   if (!st || group < 0 || group >= PCF_K_COUNT || !launches || !total_ms) return PCF_E_ARG;

@@ -171,9 +171,16 @@ class State:
         self._check(self.lib.pcf_apply_preconditioner(self.h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out)))
 
     # ---- kernel timing -------------------------------------------------------
-    def profile(self, on: bool):
-        """Record HIP events around the kernel groups of the library (clears totals)."""
-        self._check(self.lib.pcf_profile_enable(self.h, int(on)))
+    def profile(self, on: bool, groups=None):
+        """Record HIP events around the kernel groups of the library (clears
+        totals); groups: names from _abi.KERNEL_GROUPS to record (all if None)."""
+        if groups is None:
+            self._check(self.lib.pcf_profile_enable(self.h, int(on)))
+            return
+        mask = 0
+        for name in groups:
+            mask |= 1 << _abi.KERNEL_GROUPS.index(name)
+        self._check(self.lib.pcf_profile_groups(self.h, mask if on else 0))
 
     def profile_report(self) -> dict:
         """{group: (launches, total device ms)} since profile(True)."""

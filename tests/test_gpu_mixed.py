@@ -34,10 +34,17 @@ def configs(pcf):
         "periodic_hill": pcf.examples.periodic_hill(x_size=24, z_size=20, iterations=3, output_steps=True),
         "mountain_wave_odd": pcf.examples.mountain_wave(x_size=11, y_size=6, z_size=9),
         "cold_bubble": pcf.examples.cold_bubble(x_size=12, z_size=10),
+        # multigrid level paths: 64 x 64 row-path finest level with the fused
+        # restriction and wavefront-per-column line solves, 72 levels (two
+        # per lane); 256 x 72 with thread-per-column line solves and 2 x 1
+        # aggregation once ny is odd
+        "mg_deep": pcf.examples.mountain_wave_3d(x_size=64, y_size=64, z_size=72, iterations=3),
+        "mg_wide": pcf.examples.mountain_wave_3d(x_size=256, y_size=72, z_size=8, iterations=3),
     }
 
 
 CASES = ["mountain_wave_3d", "periodic_hill", "mountain_wave_odd", "cold_bubble"]
+MG_CASES = CASES + ["mg_deep", "mg_wide"]
 
 
 @pytest.mark.parametrize("pc", ["line_relaxation", "multigrid"])
@@ -119,15 +126,19 @@ def test_fp64_rejects_a_warm_start(pcf):
         pcf.State(nl)
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", MG_CASES)
 def test_multigrid_vcycle_matches_fp64_restatement(pcf, name):
     """The FP32 V-cycle on the device against the FP64 numpy restatement of the
     same algorithm (tests/mg_reference.py); FP32 rounding through the levels
-    stays below MG_RTOL of the result's max norm."""
+    stays below MG_RTOL of the result's max norm.  The coarse levels are
+    nearly singular (the operator's null space), which amplifies the rounding
+    of their right-hand sides: 2e-4 on the 64 x 64 x 72 grid, the same for a
+    row-sequential and a scan-based evaluation of the line solves, while a
+    wrong index, sign or coefficient gives errors of order one."""
     import hipmem
     import mg_reference as mg
 
-    MG_RTOL = 1e-4
+    MG_RTOL = 1e-3
     nl = configs(pcf)[name]
     nl.poisson.precision = "mixed"
     nl.poisson.preconditioner_type = "multigrid"
