@@ -244,48 +244,79 @@ struct MgCoefs {
   float* a[T_COUNT];
 };
 
-// R A P: one thread per coarse cell (I, J, k).
+// Memory layout of a level's arrays: x fastest (the finest level and the
+// levels smoothed with row-parallel kernels) or z fastest (the coarse levels
+// smoothed one column per wavefront, whose accesses are then contiguous
+// across the lanes of a wavefront).
+struct Lay {
+  int nx, ny, nz;
+  bool zf;
+  __host__ __device__ size_t at(int i, int j, int k) const {
+    return zf ? (size_t)k + (size_t)nz * ((size_t)i + (size_t)nx * j)
+              : (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+  }
+  // 32-bit form for the levels of at most kColumnCols x kScanMaxNz cells
+  __device__ int at32(int i, int j, int k) const { return zf ? k + nz * (i + nx * j) : i + nx * (j + ny * k); }
+  __host__ __device__ size_t count() const { return (size_t)nx * ny * nz; }
+  // the cell of the t-th element in memory order
+  __device__ void cell(size_t t, int& i, int& j, int& k) const {
+    if (zf) {
+      k = (int)(t % nz);
+      const size_t c = t / nz;
+      i = (int)(c % nx);
+      j = (int)(c / nx);
+    } else {
+      i = (int)(t % nx);
+      const size_t r = t / nx;
+      j = (int)(r % ny);
+      k = (int)(r / ny);
+    }
+  }
+};
+
+// R A P: one thread per coarse cell, in the coarse level's memory order.
 template <int FX, int FY>
-__global__ void k_mg_coarsen(MgCoefs fine, MgCoefs coarse, int nx, int ny, int nz, int ncx, int ncy) {
-  const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y, k = blockIdx.z;
-  if (I >= ncx) return;
+__global__ void k_mg_coarsen(MgCoefs fine, MgCoefs coarse, Lay fl, Lay cl) {
+  const size_t t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t0 >= cl.count()) return;
+  int I, J, k;
+  cl.cell(t0, I, J, k);
   constexpr int oi[T_COUNT] = {0, -1, 1, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0};
   constexpr int oj[T_COUNT] = {0, 0, 0, -1, 1, 0, 0, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0, 0, 0, -1, -1, 1, 1};
   constexpr int ok[T_COUNT] = {0, 0, 0, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -2, 2, -2, 2, -2, 2, -2, 2, -2, 2};
   float acc[T_COUNT];
 #pragma unroll
   for (int t = 0; t < T_COUNT; ++t) acc[t] = 0.0f;
-  const size_t sxy = (size_t)nx * ny;
 #pragma unroll
   for (int b = 0; b < FY; ++b)
 #pragma unroll
     for (int a = 0; a < FX; ++a) {
-      const size_t o = (size_t)(FX * I + a) + (size_t)nx * (FY * J + b) + sxy * k;
+      const size_t o = fl.at(FX * I + a, FY * J + b, k);
 #pragma unroll
       for (int t = 0; t < T_COUNT; ++t) {
         const int target = coef_of(floor_div(a + oi[t], FX), floor_div(b + oj[t], FY), ok[t]);
         acc[target] += fine.a[t][o];
       }
     }
-  const size_t oc = (size_t)I + (size_t)ncx * J + (size_t)ncx * ncy * k;
 #pragma unroll
-  for (int t = 0; t < T_COUNT; ++t) coarse.a[t][oc] = acc[t];
+  for (int t = 0; t < T_COUNT; ++t) coarse.a[t][t0] = acc[t];
 }
 
-// Line-solve factors (see k_pc_factors) of a coarse operator, in FP32.
+// Line-solve factors (see k_pc_factors) of a coarse operator, in FP32: one
+// thread per column.
 __global__ void __launch_bounds__(64) k_mg_factors(const float* ac, const float* ad, const float* au, float* P, float* E,
-                                                   float* Q, int nx, int ny, int nz, float deta) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= nx) return;
-  const size_t sxy = (size_t)nx * ny;
-  size_t o = (size_t)i + (size_t)nx * j;
+                                                   float* Q, Lay l, float deta) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= l.nx * l.ny) return;
+  const int i = c % l.nx, j = c / l.nx;
+  size_t o = l.at(i, j, 0);
   const float denom = 1 - deta * ac[o];
   float qk = deta * au[o] / denom;
   P[o] = denom;
   E[o] = deta * ad[o];
   Q[o] = qk;
-  for (int k = 1; k < nz; ++k) {
-    o += sxy;
+  for (int k = 1; k < l.nz; ++k) {
+    o = l.at(i, j, k);
     const float e = deta * ad[o];
     const float p = 1 / (1 - deta * ac[o] - e * qk);
     qk = deta * au[o] * p;
@@ -295,46 +326,47 @@ __global__ void __launch_bounds__(64) k_mg_factors(const float* ac, const float*
   }
 }
 
-// r = b - A s on one level, one thread per cell (block 64 x 4).
+// r = b - A s on one level, one thread per cell in memory order.
 template <typename BT>
 __global__ void __launch_bounds__(256) k_mg_resid(MgCoefs A, const BT* __restrict__ b, const float* __restrict__ s,
-                                                  float* __restrict__ r, int nx, int ny, int nz,
-                                                  const SolverScalars* sc) {
+                                                  float* __restrict__ r, Lay l, const SolverScalars* sc) {
   if (sc && sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
-  if (i >= nx || j >= ny) return;
+  const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (o >= l.count()) return;
+  int i, j, k;
+  l.cell(o, i, j, k);
   constexpr int oi[T_COUNT] = {0, -1, 1, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0};
   constexpr int oj[T_COUNT] = {0, 0, 0, -1, 1, 0, 0, 0, 0, 0, 0, -1, -1, 1, 1, 0, 0, 0, 0, 0, 0, -1, -1, 1, 1};
   constexpr int ok[T_COUNT] = {0, 0, 0, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -2, 2, -2, 2, -2, 2, -2, 2, -2, 2};
-  const size_t sxy = (size_t)nx * ny;
-  const size_t o = (size_t)i + (size_t)nx * j + sxy * k;
   float v = 0.0f;
 #pragma unroll
   for (int t = 0; t < T_COUNT; ++t) {
     const int kk = k + ok[t];
-    if (kk < 0 || kk >= nz) continue;
+    if (kk < 0 || kk >= l.nz) continue;
     int ii = i + oi[t], jj = j + oj[t];
-    ii = ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii);
-    jj = jj < 0 ? jj + ny : (jj >= ny ? jj - ny : jj);
-    v += A.a[t][o] * s[(size_t)ii + (size_t)nx * jj + sxy * kk];
+    ii = ii < 0 ? ii + l.nx : (ii >= l.nx ? ii - l.nx : ii);
+    jj = jj < 0 ? jj + l.ny : (jj >= l.ny ? jj - l.ny : jj);
+    v += A.a[t][o] * s[l.at(ii, jj, kk)];
   }
   r[o] = (float)b[o] - v;
 }
 
-// b_coarse = R r: one thread per coarse cell sums its fx x fy fine cells.
+// b_coarse = R r: one thread per coarse cell (coarse memory order) sums its
+// fx x fy fine cells.
 template <int FX, int FY>
-__global__ void k_mg_sum(const float* __restrict__ r, float* __restrict__ bc, int nx, int ny, int ncx, int ncy,
+__global__ void k_mg_sum(const float* __restrict__ r, float* __restrict__ bc, Lay fl, Lay cl,
                          const SolverScalars* sc) {
   if (sc && sc->done) return;
-  const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y, k = blockIdx.z;
-  if (I >= ncx) return;
-  const size_t sxy = (size_t)nx * ny;
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= cl.count()) return;
+  int I, J, k;
+  cl.cell(t, I, J, k);
   float sum = 0.0f;
 #pragma unroll
   for (int bb = 0; bb < FY; ++bb)
 #pragma unroll
-    for (int a = 0; a < FX; ++a) sum += r[(size_t)(FX * I + a) + (size_t)nx * (FY * J + bb) + sxy * k];
-  bc[(size_t)I + (size_t)ncx * J + (size_t)ncx * ncy * k] = sum;
+    for (int a = 0; a < FX; ++a) sum += r[fl.at(FX * I + a, FY * J + bb, k)];
+  bc[t] = sum;
 }
 
 // b_coarse = R (b - A s) for 2 x 2 aggregation and nx % 4 == 0: block
@@ -344,7 +376,7 @@ __global__ void k_mg_sum(const float* __restrict__ r, float* __restrict__ bc, in
 template <typename BT>
 __global__ void __launch_bounds__(256) k_resid_restrict4(MgCoefs A, const BT* __restrict__ b,
                                                          const float* __restrict__ s, float* __restrict__ bc, int nx,
-                                                         int ny, int nz, const SolverScalars* sc) {
+                                                         int ny, int nz, Lay cl, const SolverScalars* sc) {
   if (sc && sc->done) return;
   constexpr int W = 4;
   const int i0 = (blockIdx.x * 16 + threadIdx.x) * W;
@@ -400,22 +432,26 @@ __global__ void __launch_bounds__(256) k_resid_restrict4(MgCoefs A, const BT* __
   // every lane takes part in the exchange; lane + 16 is row j + 1 of the same x
   const float u0 = __shfl_down(c0, 16), u1 = __shfl_down(c1, 16);
   if (act && (j & 1) == 0) {
-    const int ncx = nx / 2, ncy = ny / 2;
-    float2 v;
-    v.x = c0 + u0;
-    v.y = c1 + u1;
-    *(float2*)(bc + (size_t)(i0 / 2) + (size_t)ncx * (j / 2) + (size_t)ncx * ncy * k) = v;
+    if (cl.zf) {
+      bc[cl.at(i0 / 2, j / 2, k)] = c0 + u0;
+      bc[cl.at(i0 / 2 + 1, j / 2, k)] = c1 + u1;
+    } else {
+      float2 v;
+      v.x = c0 + u0;
+      v.y = c1 + u1;
+      *(float2*)(bc + cl.at(i0 / 2, j / 2, k)) = v;
+    }
   }
 }
 
-// s += omega P e
-__global__ void k_mg_prolong(float* s, const float* e, int nx, int ny, int nz, int fx, int fy, int ncx, int ncy,
-                             float omega, const SolverScalars* sc) {
+// s += omega P e on an x-fastest level; e in the coarse level's layout
+__global__ void k_mg_prolong(float* s, const float* e, int nx, int ny, int nz, int fx, int fy, Lay cl, float omega,
+                             const SolverScalars* sc) {
   if (sc && sc->done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
   if (i >= nx) return;
   const size_t o = (size_t)i + (size_t)nx * j + (size_t)nx * ny * k;
-  s[o] += omega * e[(size_t)(i / fx) + (size_t)ncx * (j / fy) + (size_t)ncx * ncy * k];
+  s[o] += omega * e[cl.at(i / fx, j / fy, k)];
 }
 
 // ---------------------------------------------------------------------------
@@ -527,22 +563,21 @@ __global__ void __launch_bounds__(256) k_line_scan(const float* __restrict__ P, 
 
 // One pseudo-time step of column (i, j): the right-hand side
 // s + deta (H s - b) at the lane's levels (FIRST: deta (0 - b), from s = 0),
-// then the line solve.  rd(i, j, k) reads the current iterate.
-// 32-bit indices: column-path levels hold at most 1024 x 512 cells.
+// then the line solve.  rd(i, j, k) reads the current iterate; the level's
+// arrays are in layout l.
 template <int NPL, bool FIRST, typename BT, class Rd>
 __device__ __forceinline__ void column_step(const MgCoefs& A, const float* P, const float* E, const float* Q,
-                                            const BT* b, Rd rd, int i, int j, int nx, int ny, int nz, float deta,
+                                            const BT* b, Rd rd, const Lay& l, int i, int j, float deta,
                                             float (&y)[NPL]) {
   const int lane = threadIdx.x & 63;
-  const int sxy = nx * ny;
-  const int c = i + nx * j;
+  const int nx = l.nx, ny = l.ny, nz = l.nz;
   const int im = i == 0 ? nx - 1 : i - 1, ip = i + 1 == nx ? 0 : i + 1;
   const int jm = j == 0 ? ny - 1 : j - 1, jp = j + 1 == ny ? 0 : j + 1;
   float fa[NPL], fb[NPL], q[NPL];
 #pragma unroll
   for (int u = 0; u < NPL; ++u) {
     const int k = NPL * lane + u;
-    const int o = c + sxy * k;
+    const int o = l.at32(i, j, k < nz ? k : nz - 1);
     float r = 0.0f;
     if (k < nz) {
       if (FIRST) {
@@ -567,75 +602,76 @@ __device__ __forceinline__ void column_step(const MgCoefs& A, const float* P, co
 
 // One smoothing step of a level with few columns, one wavefront per column
 // (block 256 = 4 columns); s_old and out must differ.  PRO: the iterate is
-// s_old + omega P e (the coarse-level correction, fx = 1 << sx, fy = 1 << sy).
+// s_old + omega P e (e: the coarse-level correction in layout cl,
+// fx = 1 << sx, fy = 1 << sy).
 template <int NPL, bool FIRST, bool PRO, typename BT, typename OutT>
 __global__ void __launch_bounds__(256) k_col_smooth(MgCoefs A, const float* __restrict__ P,
                                                     const float* __restrict__ E, const float* __restrict__ Q,
                                                     const BT* __restrict__ b, const float* __restrict__ s_old,
-                                                    const float* __restrict__ e, float omega, int sx, int sy, int ncx,
-                                                    int ncy, OutT* __restrict__ out, int nx, int ny, int nz,
-                                                    float deta, const SolverScalars* sc) {
+                                                    const float* __restrict__ e, float omega, int sx, int sy, Lay l,
+                                                    Lay cl, OutT* __restrict__ out, float deta,
+                                                    const SolverScalars* sc) {
   if (sc && sc->done) return;
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= nx * ny) return;  // uniform over the wavefront
-  const int i = c % nx, j = c / nx;
-  const int sxy = nx * ny, scxy = ncx * ncy;
+  if (c >= l.nx * l.ny) return;  // uniform over the wavefront
+  const int i = c % l.nx, j = c / l.nx;
   auto rd = [&](int ii, int jj, int kk) -> float {
-    float v = s_old[ii + nx * jj + sxy * kk];
-    if (PRO) v += omega * e[(ii >> sx) + ncx * (jj >> sy) + scxy * kk];
+    float v = s_old[l.at32(ii, jj, kk)];
+    if (PRO) v += omega * e[cl.at32(ii >> sx, jj >> sy, kk)];
     return v;
   };
   float y[NPL];
-  column_step<NPL, FIRST, BT>(A, P, E, Q, b, rd, i, j, nx, ny, nz, deta, y);
+  column_step<NPL, FIRST, BT>(A, P, E, Q, b, rd, l, i, j, deta, y);
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int u = 0; u < NPL; ++u) {
     const int k = NPL * lane + u;
-    if (k < nz) out[c + sxy * k] = (OutT)y[u];
+    if (k < l.nz) out[l.at32(i, j, k)] = (OutT)y[u];
   }
 }
 
 // All smoothing steps of the coarsest level in one workgroup of up to 16
 // wavefronts, wavefront w smoothing the columns w, w + 16, ... (nx ny <= 16),
-// the iterate ping-ponging between two LDS buffers (2 nx ny nz floats of
-// dynamic shared memory), so a column's new values are stored as soon as
-// they are computed.
+// the iterate ping-ponging between two z-fastest LDS buffers (2 nx ny nz
+// floats of dynamic shared memory), so a column's new values are stored as
+// soon as they are computed.
 template <int NPL, typename BT, typename OutT>
 __global__ void __launch_bounds__(1024) k_coarse_solve(MgCoefs A, const float* __restrict__ P,
-                                                      const float* __restrict__ E, const float* __restrict__ Q,
-                                                      const BT* __restrict__ b, OutT* __restrict__ out, int nx,
-                                                      int ny, int nz, float deta, int steps,
-                                                      const SolverScalars* sc) {
+                                                       const float* __restrict__ E, const float* __restrict__ Q,
+                                                       const BT* __restrict__ b, OutT* __restrict__ out, Lay l,
+                                                       float deta, int steps, const SolverScalars* sc) {
   if (sc && sc->done) return;
   extern __shared__ float lds[];
-  const int ncols = nx * ny;
-  const int sxy = ncols, n = ncols * nz;
+  const int nx = l.nx, nz = l.nz;
+  const int ncols = l.nx * l.ny, n = ncols * nz;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   for (int step = 0; step < steps; ++step) {
     const float* cur = lds + (step & 1) * n;
     float* nxt = lds + ((step + 1) & 1) * n;
-    auto rd = [&](int ii, int jj, int kk) -> float { return cur[ii + nx * jj + sxy * kk]; };
+    auto rd = [&](int ii, int jj, int kk) -> float { return cur[kk + nz * (ii + nx * jj)]; };
 #pragma unroll 1
     for (int c = w; c < ncols; c += nw) {  // uniform over the wavefront
       float y[NPL];
       if (step == 0)
-        column_step<NPL, true, BT>(A, P, E, Q, b, rd, c % nx, c / nx, nx, ny, nz, deta, y);
+        column_step<NPL, true, BT>(A, P, E, Q, b, rd, l, c % nx, c / nx, deta, y);
       else
-        column_step<NPL, false, BT>(A, P, E, Q, b, rd, c % nx, c / nx, nx, ny, nz, deta, y);
+        column_step<NPL, false, BT>(A, P, E, Q, b, rd, l, c % nx, c / nx, deta, y);
 #pragma unroll
       for (int u = 0; u < NPL; ++u) {
         const int k = NPL * lane + u;
         if (k < nz) {
           if (step + 1 < steps)
-            nxt[c + sxy * k] = y[u];
+            nxt[k + nz * c] = y[u];
           else
-            out[c + sxy * k] = (OutT)y[u];
+            out[l.at32(c % nx, c / nx, k)] = (OutT)y[u];
         }
       }
     }
     __syncthreads();  // the step's writes before the next step's reads
   }
 }
+
+Lay lay_of(const MgLevel& L) { return Lay{L.nx, L.ny, L.nz, L.zf}; }
 
 PcArgs32 pc_args32(pcf_state* st, const float* s, float* out) {
   PcArgs32 A;
@@ -669,15 +705,16 @@ int prepare_mixed(pcf_state* st) {
       fa.a[t] = F.a[t];
       ca.a[t] = C.a[t];
     }
-    const dim3 grd(blocks_for(C.nx, 64), C.ny, C.nz), blk(64);
+    const Lay fl = lay_of(F), cl = lay_of(C);
+    const dim3 grd(blocks_for(cl.count(), 256)), blk(256);
     if (F.fx == 2 && F.fy == 2)
-      hipLaunchKernelGGL((k_mg_coarsen<2, 2>), grd, blk, 0, st->stream, fa, ca, F.nx, F.ny, F.nz, C.nx, C.ny);
+      hipLaunchKernelGGL((k_mg_coarsen<2, 2>), grd, blk, 0, st->stream, fa, ca, fl, cl);
     else if (F.fx == 2)
-      hipLaunchKernelGGL((k_mg_coarsen<2, 1>), grd, blk, 0, st->stream, fa, ca, F.nx, F.ny, F.nz, C.nx, C.ny);
+      hipLaunchKernelGGL((k_mg_coarsen<2, 1>), grd, blk, 0, st->stream, fa, ca, fl, cl);
     else
-      hipLaunchKernelGGL((k_mg_coarsen<1, 2>), grd, blk, 0, st->stream, fa, ca, F.nx, F.ny, F.nz, C.nx, C.ny);
-    hipLaunchKernelGGL(k_mg_factors, dim3(blocks_for(C.nx, 64), C.ny), dim3(64), 0, st->stream, C.a[T_AC],
-                       C.a[T_AD], C.a[T_AU], C.P, C.E, C.Q, C.nx, C.ny, C.nz, C.deta);
+      hipLaunchKernelGGL((k_mg_coarsen<1, 2>), grd, blk, 0, st->stream, fa, ca, fl, cl);
+    hipLaunchKernelGGL(k_mg_factors, dim3(blocks_for((long long)C.nx * C.ny, 64)), dim3(64), 0, st->stream,
+                       C.a[T_AC], C.a[T_AD], C.a[T_AU], C.P, C.E, C.Q, cl, C.deta);
   }
   PCF_HIP(hipGetLastError());
   st->t32_valid = true;
@@ -691,7 +728,7 @@ int prepare_mixed(pcf_state* st) {
 // the columns alone fill the chip).  The coarsest level runs all its steps
 // in one workgroup when it has at most 16 columns.
 constexpr size_t kColumnCols = 1024;
-constexpr size_t kScanCols = 16384;
+constexpr size_t kScanCols = 4096;
 constexpr int kScanMaxNz = 512;
 constexpr size_t kCoarseLdsFloats = 16384;
 
@@ -749,14 +786,15 @@ float* mg_smooth(pcf_state* st, const MgLevel& L, const BT* b, float* s, const f
       ProfScope prof(st, PCF_K_PC_THOMAS);
       with_npl(L.nz, [&](auto npl) {
         constexpr int NPL = decltype(npl)::value;
+        const Lay l = lay_of(L);
         if (last && out64)
           hipLaunchKernelGGL((k_col_smooth<NPL, true, false, BT, double>), dim3(blocks_for(nc, 4)), dim3(256), 0,
-                             st->stream, coefs_of(L), L.P, L.E, L.Q, b, nullptr, nullptr, 0.0f, 0, 0, 1, 1, out64,
-                             L.nx, L.ny, L.nz, L.deta, sc);
+                             st->stream, coefs_of(L), L.P, L.E, L.Q, b, nullptr, nullptr, 0.0f, 0, 0, l, l, out64,
+                             L.deta, sc);
         else
           hipLaunchKernelGGL((k_col_smooth<NPL, true, false, BT, float>), dim3(blocks_for(nc, 4)), dim3(256), 0,
-                             st->stream, coefs_of(L), L.P, L.E, L.Q, b, nullptr, nullptr, 0.0f, 0, 0, 1, 1, L.s, L.nx,
-                             L.ny, L.nz, L.deta, sc);
+                             st->stream, coefs_of(L), L.P, L.E, L.Q, b, nullptr, nullptr, 0.0f, 0, 0, l, l, L.s,
+                             L.deta, sc);
       });
     } else {
       if (last && out64)
@@ -770,7 +808,7 @@ float* mg_smooth(pcf_state* st, const MgLevel& L, const BT* b, float* s, const f
   if (e && !col) {
     ProfScope prof(st, PCF_K_PC_RHS);
     hipLaunchKernelGGL(k_mg_prolong, dim3(blocks_for(L.nx, 64), L.ny, L.nz), dim3(64), 0, st->stream, s, e, L.nx,
-                       L.ny, L.nz, L.fx, L.fy, C->nx, C->ny, kMgOmega, sc);
+                       L.ny, L.nz, L.fx, L.fy, lay_of(*C), kMgOmega, sc);
   }
   const bool quads = L.nx % 4 == 0;
   const dim3 sgrd = quads ? dim3(blocks_for(L.nx, 64), blocks_for(L.ny, 16), L.nz)
@@ -785,22 +823,22 @@ float* mg_smooth(pcf_state* st, const MgLevel& L, const BT* b, float* s, const f
       with_npl(L.nz, [&](auto npl) {
         constexpr int NPL = decltype(npl)::value;
         const int sx = L.fx == 2, sy = L.fy == 2;
-        const int ncx = C ? C->nx : 1, ncy = C ? C->ny : 1;
+        const Lay l = lay_of(L), cl = C ? lay_of(*C) : l;
         const dim3 g(blocks_for(nc, 4)), blk(256);
         if (last && out64) {
           if (pro)
             hipLaunchKernelGGL((k_col_smooth<NPL, false, true, BT, double>), g, blk, 0, st->stream, coefs_of(L), L.P,
-                               L.E, L.Q, b, s, e, kMgOmega, sx, sy, ncx, ncy, out64, L.nx, L.ny, L.nz, L.deta, sc);
+                               L.E, L.Q, b, s, e, kMgOmega, sx, sy, l, cl, out64, L.deta, sc);
           else
             hipLaunchKernelGGL((k_col_smooth<NPL, false, false, BT, double>), g, blk, 0, st->stream, coefs_of(L), L.P,
-                               L.E, L.Q, b, s, e, kMgOmega, sx, sy, ncx, ncy, out64, L.nx, L.ny, L.nz, L.deta, sc);
+                               L.E, L.Q, b, s, e, kMgOmega, sx, sy, l, cl, out64, L.deta, sc);
         } else {
           if (pro)
             hipLaunchKernelGGL((k_col_smooth<NPL, false, true, BT, float>), g, blk, 0, st->stream, coefs_of(L), L.P,
-                               L.E, L.Q, b, s, e, kMgOmega, sx, sy, ncx, ncy, dst, L.nx, L.ny, L.nz, L.deta, sc);
+                               L.E, L.Q, b, s, e, kMgOmega, sx, sy, l, cl, dst, L.deta, sc);
           else
             hipLaunchKernelGGL((k_col_smooth<NPL, false, false, BT, float>), g, blk, 0, st->stream, coefs_of(L), L.P,
-                               L.E, L.Q, b, s, e, kMgOmega, sx, sy, ncx, ncy, dst, L.nx, L.ny, L.nz, L.deta, sc);
+                               L.E, L.Q, b, s, e, kMgOmega, sx, sy, l, cl, dst, L.deta, sc);
         }
       });
       s = dst;
@@ -837,19 +875,20 @@ void mg_restrict(pcf_state* st, const MgLevel& L, const MgLevel& C, const BT* b,
   const MgCoefs A = coefs_of(L);
   if (!column_path(L) && L.fx == 2 && L.fy == 2 && L.nx % 4 == 0) {
     hipLaunchKernelGGL((k_resid_restrict4<BT>), dim3(blocks_for(L.nx, 64), blocks_for(L.ny, 16), L.nz),
-                       dim3(16, 16), 0, st->stream, A, b, s, C.b, L.nx, L.ny, L.nz, sc);
+                       dim3(16, 16), 0, st->stream, A, b, s, C.b, L.nx, L.ny, L.nz, lay_of(C), sc);
     return;
   }
   float* r = L.col;  // the line-solve scratch is free here
-  hipLaunchKernelGGL((k_mg_resid<BT>), dim3(blocks_for(L.nx, 64), blocks_for(L.ny, 4), L.nz), dim3(64, 4), 0,
-                     st->stream, A, b, s, r, L.nx, L.ny, L.nz, sc);
-  const dim3 cgrd(blocks_for(C.nx, 64), C.ny, C.nz), blk(64);
+  const Lay fl = lay_of(L), cl = lay_of(C);
+  hipLaunchKernelGGL((k_mg_resid<BT>), dim3(blocks_for(fl.count(), 256)), dim3(256), 0, st->stream, A, b, s, r, fl,
+                     sc);
+  const dim3 cgrd(blocks_for(cl.count(), 256)), blk(256);
   if (L.fx == 2 && L.fy == 2)
-    hipLaunchKernelGGL((k_mg_sum<2, 2>), cgrd, blk, 0, st->stream, r, C.b, L.nx, L.ny, C.nx, C.ny, sc);
+    hipLaunchKernelGGL((k_mg_sum<2, 2>), cgrd, blk, 0, st->stream, r, C.b, fl, cl, sc);
   else if (L.fx == 2)
-    hipLaunchKernelGGL((k_mg_sum<2, 1>), cgrd, blk, 0, st->stream, r, C.b, L.nx, L.ny, C.nx, C.ny, sc);
+    hipLaunchKernelGGL((k_mg_sum<2, 1>), cgrd, blk, 0, st->stream, r, C.b, fl, cl, sc);
   else
-    hipLaunchKernelGGL((k_mg_sum<1, 2>), cgrd, blk, 0, st->stream, r, C.b, L.nx, L.ny, C.nx, C.ny, sc);
+    hipLaunchKernelGGL((k_mg_sum<1, 2>), cgrd, blk, 0, st->stream, r, C.b, fl, cl, sc);
 }
 
 // The coarsest level: kMgCoarseSteps smoothing steps from zero.
@@ -866,10 +905,10 @@ float* mg_coarsest(pcf_state* st, const MgLevel& L, const BT* b, double* out64, 
       if constexpr (NPL <= 4) {
         if (out64)
           hipLaunchKernelGGL((k_coarse_solve<NPL, BT, double>), dim3(1), blk, lds, st->stream, coefs_of(L), L.P, L.E,
-                             L.Q, b, out64, L.nx, L.ny, L.nz, L.deta, kMgCoarseSteps, sc);
+                             L.Q, b, out64, lay_of(L), L.deta, kMgCoarseSteps, sc);
         else
           hipLaunchKernelGGL((k_coarse_solve<NPL, BT, float>), dim3(1), blk, lds, st->stream, coefs_of(L), L.P, L.E,
-                             L.Q, b, L.s, L.nx, L.ny, L.nz, L.deta, kMgCoarseSteps, sc);
+                             L.Q, b, L.s, lay_of(L), L.deta, kMgCoarseSteps, sc);
       }
     });
     return L.s;
@@ -1037,6 +1076,7 @@ int alloc_mixed(pcf_state* st) {
     float** v[] = {&C.P, &C.E, &C.Q, &C.b, &C.s, &C.rhs, &C.col};
     for (float** p : v)
       if ((rc = ac(p))) return rc;
+    C.zf = column_path(C);  // coarse levels smoothed column by column: z fastest
     st->mg.push_back(C);
   }
   return PCF_OK;

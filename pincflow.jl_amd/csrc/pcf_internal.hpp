@@ -72,6 +72,7 @@ struct Tensor {
 // factors and vectors; fx, fy: aggregation factors to the next level.
 struct MgLevel {
   int nx = 0, ny = 0, nz = 0, fx = 1, fy = 1;
+  bool zf = false;  // arrays stored z fastest (else x fastest)
   float deta = 0.0f;
   float* a[T_COUNT] = {};
   float *P = nullptr, *E = nullptr, *Q = nullptr;
