@@ -54,6 +54,10 @@ def parse():
     p.add_argument("--initial-guess", default="zero", choices=["zero", "previous", "learned"],
                    help="mixed precision: start each solve from zero, from the same stage's previous solution, or "
                         "from the learned network's guess (trained first, on held-out terrain)")
+    p.add_argument("--mg-smoothing-steps", type=int, default=2, help="multigrid pre- and post-smoothing steps")
+    p.add_argument("--mg-coarse-steps", type=int, default=20, help="multigrid steps on the coarsest level")
+    p.add_argument("--storage", default="fp32", choices=["fp32", "bf16"],
+                   help="mixed precision: storage of the finest level's coefficients in the preconditioner")
     p.add_argument("--train-runs", type=int, default=2, help="learned guess: training runs (random hills)")
     p.add_argument("--train-steps", type=int, default=8, help="learned guess: time steps per training run")
     p.add_argument("--train-epochs", type=int, default=10, help="learned guess: passes over the samples")
@@ -76,6 +80,10 @@ def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False
     nl.poisson.preconditioner_type = preconditioner
     nl.poisson.preconditioner_iterations = args.pc_iterations
     nl.poisson.dtau = args.dtau
+    if precision == "mixed":
+        nl.poisson.mg_smoothing_steps = args.mg_smoothing_steps
+        nl.poisson.mg_coarse_steps = args.mg_coarse_steps
+        nl.poisson.preconditioner_storage = args.storage
     tag = f"{precision}/{preconditioner}/{guess}"
     t0 = time.perf_counter()
     st = pcf.State(nl)

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCF_ABI_VERSION 2
+#define PCF_ABI_VERSION 3
 
 /* ---- return codes ------------------------------------------------------ */
 #define PCF_OK 0
@@ -116,6 +116,15 @@ enum pcf_preconditioner_type {
   PCF_PC_MULTIGRID = 1
 };
 
+/* Storage of the finest level's operator coefficients read by the
+ * mixed-precision preconditioner (its horizontal steps and the multigrid
+ * residual): FP32, or BF16 with FP32 arithmetic.  The line-solve factors stay
+ * FP32. */
+enum pcf_preconditioner_storage {
+  PCF_STORAGE_FP32 = 0,
+  PCF_STORAGE_BF16 = 1
+};
+
 enum pcf_precision {
   PCF_PRECISION_FP64 = 0, /* reference arithmetic                            */
   PCF_PRECISION_MIXED = 1 /* FP64 BiCGSTAB with the preconditioner in FP32   */
@@ -199,6 +208,9 @@ typedef struct pcf_config {
    * only: the initial guess and the preconditioner of the solve. */
   int32_t initial_guess;                  /* enum pcf_initial_guess       */
   int32_t preconditioner_type;            /* enum pcf_preconditioner_type */
+  int32_t mg_smoothing_steps;             /* multigrid pre- and post-smoothing steps (2) */
+  int32_t mg_coarse_steps;                /* multigrid steps on the coarsest level (20)  */
+  int32_t preconditioner_storage;         /* enum pcf_preconditioner_storage            */
 } pcf_config;
 
 /* Fields reachable through pcf_get_field / pcf_set_field (haloed, FP64). */

@@ -13,7 +13,7 @@ from pathlib import Path
 PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "lib" / "libpincflow_hip.so"
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_HILLS = 32
 
 # return codes
@@ -39,6 +39,7 @@ RHS_SPONGE_NONE, RHS_SPONGE_SIN2_TOP = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED = 0, 1, 2
 PC_LINE_RELAXATION, PC_MULTIGRID = 0, 1
+STORAGE_FP32, STORAGE_BF16 = 0, 1
 
 FIELDS = [
     "RHO", "RHOP", "U", "V", "W", "PIP",
@@ -133,6 +134,9 @@ class PcfConfig(ctypes.Structure):
         ("tmax", ctypes.c_double),
         ("initial_guess", ctypes.c_int32),
         ("preconditioner_type", ctypes.c_int32),
+        ("mg_smoothing_steps", ctypes.c_int32),
+        ("mg_coarse_steps", ctypes.c_int32),
+        ("preconditioner_storage", ctypes.c_int32),
     ]
 
 

@@ -227,6 +227,8 @@ void pcf_config_default(pcf_config* c) {
   c->iterations = 1;
   c->output_interval = 3.6e3;
   c->tmax = 3.6e3;
+  c->mg_smoothing_steps = 2;
+  c->mg_coarse_steps = 20;
 }
 
 int pcf_create(const pcf_config* cfg, pcf_state** out) {

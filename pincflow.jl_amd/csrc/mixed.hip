@@ -27,11 +27,19 @@ namespace {
 
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
-struct PcArgs32 {
-  const float* a[T_COUNT];  // the 22 horizontal coefficients are used here
+// CT: float, or uint16_t holding bf16 bits (the upper half of an FP32)
+template <typename CT>
+struct PcArgs {
+  const CT* a[T_COUNT];  // the 22 horizontal coefficients are used here
   const float* s;
   float* out;
   int nx, ny, nz;
+};
+using PcArgs32 = PcArgs<float>;
+
+template <typename CT>
+struct Coefs {
+  const CT* a[T_COUNT];
 };
 
 // W consecutive floats: W = 4 as one 16-byte access, W = 1 as a scalar.
@@ -73,13 +81,31 @@ __device__ __forceinline__ void st_stream(float* p, const float (&v)[W]) {
   }
 }
 
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+// W consecutive coefficients, streamed (not kept in the caches), as FP32.
+template <int W, typename CT>
+__device__ __forceinline__ void ld_coef(const CT* p, float (&o)[W]) {
+  if constexpr (sizeof(CT) == 4) {
+    ld_stream<W>((const float*)p, o);
+  } else if constexpr (W == 4) {
+    const u32x2 v = __builtin_nontemporal_load((const u32x2*)p);
+    o[0] = __uint_as_float(v.x << 16);
+    o[1] = __uint_as_float(v.x & 0xffff0000u);
+    o[2] = __uint_as_float(v.y << 16);
+    o[3] = __uint_as_float(v.y & 0xffff0000u);
+  } else {
+    o[0] = __uint_as_float((unsigned)__builtin_nontemporal_load(p) << 16);
+  }
+}
+
 // The later pseudo-time steps of the preconditioner in FP32:
 // out = s + deta (H s - sin) with H the horizontal 22-point part of the
 // operator (apply_preconditioner!.jl:62-63); sin: the FP64 or FP32 input.
 // W x-adjacent cells per thread, block = (64 / W) x (4 W) threads covering
-// 64 x (4 W) cells of one z-level.
-template <int W, typename SinT>
-__global__ void __launch_bounds__(256) k_pc_rhs32(PcArgs32 A, const SinT* __restrict__ sin, float deta,
+// 64 x (4 W) cells of one z-level; CT: the coefficients' storage.
+template <int W, typename SinT, typename CT = float>
+__global__ void __launch_bounds__(256) k_pc_rhs32(PcArgs<CT> A, const SinT* __restrict__ sin, float deta,
                                                   const SolverScalars* sc) {
   if (sc && sc->done) return;
   constexpr int BX = 64 / W;
@@ -114,7 +140,7 @@ __global__ void __launch_bounds__(256) k_pc_rhs32(PcArgs32 A, const SinT* __rest
 #pragma unroll
   for (int t = 0; t < T_COUNT; ++t) {
     if (t == T_AC || t == T_AD || t == T_AU) continue;
-    ld_stream<W>(A.a[t] + o, a[t]);
+    ld_coef<W, CT>(A.a[t] + o, a[t]);
   }
   float out[W];
 #pragma unroll
@@ -218,9 +244,7 @@ __global__ void k_scale_copy(const double* x, double* g, size_t n, double c) {
 // the fine stencil couples a cell with x or y neighbours but never with
 // diagonal ones, R A P has the same 25-point shape.
 // ---------------------------------------------------------------------------
-constexpr int kMgNu = 2;             // pre- and post-smoothing steps
 constexpr float kMgOmega = 1.5f;     // over-correction of the coarse-level correction
-constexpr int kMgCoarseSteps = 20;   // smoothing steps on the coarsest level
 constexpr int kMgMinSize = 8;        // a dimension is coarsened while even and >= this
 
 // (di, dj, dk) of the 25 coefficients in the order of T_AC ... T_AFUU
@@ -373,8 +397,8 @@ __global__ void k_mg_sum(const float* __restrict__ r, float* __restrict__ bc, La
 // 16 x 16 threads, 4 x-adjacent fine cells per thread (the loads of
 // k_pc_rhs32); the x pairs are summed in the thread, the y pairs across the
 // lanes l and l + 16 of a wavefront (rows j and j + 1).
-template <typename BT>
-__global__ void __launch_bounds__(256) k_resid_restrict4(MgCoefs A, const BT* __restrict__ b,
+template <typename BT, typename CT>
+__global__ void __launch_bounds__(256) k_resid_restrict4(Coefs<CT> A, const BT* __restrict__ b,
                                                          const float* __restrict__ s, float* __restrict__ bc, int nx,
                                                          int ny, int nz, Lay cl, const SolverScalars* sc) {
   if (sc && sc->done) return;
@@ -408,7 +432,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict4(MgCoefs A, const BT* __
     }
     float a[T_COUNT][W];
 #pragma unroll
-    for (int t = 0; t < T_COUNT; ++t) ld_stream<W>(A.a[t] + o, a[t]);
+    for (int t = 0; t < T_COUNT; ++t) ld_coef<W, CT>(A.a[t] + o, a[t]);
     float r[W];
 #pragma unroll
     for (int c = 0; c < W; ++c) {
@@ -673,15 +697,46 @@ __global__ void __launch_bounds__(1024) k_coarse_solve(MgCoefs A, const float* _
 
 Lay lay_of(const MgLevel& L) { return Lay{L.nx, L.ny, L.nz, L.zf}; }
 
-PcArgs32 pc_args32(pcf_state* st, const float* s, float* out) {
-  PcArgs32 A;
-  for (int t = 0; t < T_COUNT; ++t) A.a[t] = st->T32[t];
-  A.s = s;
-  A.out = out;
-  A.nx = st->g.nx;
-  A.ny = st->g.ny;
-  A.nz = st->g.nz;
-  return A;
+bool bf16_storage(const pcf_state* st) { return st->cfg.preconditioner_storage == PCF_STORAGE_BF16; }
+
+// out = s + deta (H s - sin) on an x-fastest level with FP32 coefficients
+// a32; the finest level reads its coefficients in the configured storage.
+template <typename SinT>
+void pc_rhs(pcf_state* st, float* const* a32, bool finest, const float* s, float* out, const SinT* sin, int nx,
+            int ny, int nz, float deta, const SolverScalars* sc) {
+  ProfScope prof(st, PCF_K_PC_RHS);
+  const bool quads = nx % 4 == 0;
+  const dim3 grd = quads ? dim3(blocks_for(nx, 64), blocks_for(ny, 16), nz)
+                         : dim3(blocks_for(nx, 64), blocks_for(ny, 4), nz);
+  const dim3 blk = quads ? dim3(16, 16) : dim3(64, 4);
+  auto go = [&](auto A) {
+    using CT = std::remove_const_t<std::remove_pointer_t<std::remove_reference_t<decltype(A.a[0])>>>;
+    A.s = s;
+    A.out = out;
+    A.nx = nx;
+    A.ny = ny;
+    A.nz = nz;
+    if (quads)
+      hipLaunchKernelGGL((k_pc_rhs32<4, SinT, CT>), grd, blk, 0, st->stream, A, sin, deta, sc);
+    else
+      hipLaunchKernelGGL((k_pc_rhs32<1, SinT, CT>), grd, blk, 0, st->stream, A, sin, deta, sc);
+  };
+  if (finest && bf16_storage(st)) {
+    PcArgs<uint16_t> A;
+    for (int t = 0; t < T_COUNT; ++t) A.a[t] = st->T16[t];
+    go(A);
+  } else {
+    PcArgs<float> A;
+    for (int t = 0; t < T_COUNT; ++t) A.a[t] = a32[t];
+    go(A);
+  }
+}
+
+__global__ void k_to_bf16(const float* a, uint16_t* b, size_t n) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    const unsigned u = __float_as_uint(a[q]);
+    b[q] = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);  // round to nearest even
+  }
 }
 
 // FP32 copies of the horizontal operator coefficients and of the line-solve
@@ -693,6 +748,9 @@ int prepare_mixed(pcf_state* st) {
   const size_t N = st->g.nv();
   for (int t = 0; t < T_COUNT; ++t)
     hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, st->stream, st->T.a[t], st->T32[t], N);
+  if (bf16_storage(st))
+    for (int t = 0; t < T_COUNT; ++t)
+      hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, st->stream, st->T32[t], st->T16[t], N);
   hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, st->stream, st->pc_P, st->pc32_P, N);
   hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, st->stream, st->pc_E, st->pc32_E, N);
   hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, st->stream, st->pc_Q, st->pc32_Q, N);
@@ -810,10 +868,6 @@ float* mg_smooth(pcf_state* st, const MgLevel& L, const BT* b, float* s, const f
     hipLaunchKernelGGL(k_mg_prolong, dim3(blocks_for(L.nx, 64), L.ny, L.nz), dim3(64), 0, st->stream, s, e, L.nx,
                        L.ny, L.nz, L.fx, L.fy, lay_of(*C), kMgOmega, sc);
   }
-  const bool quads = L.nx % 4 == 0;
-  const dim3 sgrd = quads ? dim3(blocks_for(L.nx, 64), blocks_for(L.ny, 16), L.nz)
-                          : dim3(blocks_for(L.nx, 64), blocks_for(L.ny, 4), L.nz);
-  const dim3 sblk = quads ? dim3(16, 16) : dim3(64, 4);
   for (; step < steps; ++step) {
     const bool last = step == steps - 1;
     if (col) {
@@ -844,20 +898,7 @@ float* mg_smooth(pcf_state* st, const MgLevel& L, const BT* b, float* s, const f
       s = dst;
       continue;
     }
-    {
-      ProfScope prof(st, PCF_K_PC_RHS);
-      PcArgs32 A;
-      for (int t = 0; t < T_COUNT; ++t) A.a[t] = L.a[t];
-      A.s = s;
-      A.out = L.rhs;
-      A.nx = L.nx;
-      A.ny = L.ny;
-      A.nz = L.nz;
-      if (quads)
-        hipLaunchKernelGGL((k_pc_rhs32<4, BT>), sgrd, sblk, 0, st->stream, A, b, L.deta, sc);
-      else
-        hipLaunchKernelGGL((k_pc_rhs32<1, BT>), sgrd, sblk, 0, st->stream, A, b, L.deta, sc);
-    }
+    pc_rhs<BT>(st, L.a, &L == &st->mg[0], s, L.rhs, b, L.nx, L.ny, L.nz, L.deta, sc);
     if (last && out64)
       line_solve<false>(st, L, (const float*)L.rhs, out64, sc);
     else
@@ -874,8 +915,18 @@ void mg_restrict(pcf_state* st, const MgLevel& L, const MgLevel& C, const BT* b,
   ProfScope prof(st, PCF_K_PC_RHS);
   const MgCoefs A = coefs_of(L);
   if (!column_path(L) && L.fx == 2 && L.fy == 2 && L.nx % 4 == 0) {
-    hipLaunchKernelGGL((k_resid_restrict4<BT>), dim3(blocks_for(L.nx, 64), blocks_for(L.ny, 16), L.nz),
-                       dim3(16, 16), 0, st->stream, A, b, s, C.b, L.nx, L.ny, L.nz, lay_of(C), sc);
+    const dim3 grd(blocks_for(L.nx, 64), blocks_for(L.ny, 16), L.nz), blk(16, 16);
+    if (&L == &st->mg[0] && bf16_storage(st)) {
+      Coefs<uint16_t> A16;
+      for (int t = 0; t < T_COUNT; ++t) A16.a[t] = st->T16[t];
+      hipLaunchKernelGGL((k_resid_restrict4<BT, uint16_t>), grd, blk, 0, st->stream, A16, b, s, C.b, L.nx, L.ny,
+                         L.nz, lay_of(C), sc);
+    } else {
+      Coefs<float> A32;
+      for (int t = 0; t < T_COUNT; ++t) A32.a[t] = L.a[t];
+      hipLaunchKernelGGL((k_resid_restrict4<BT, float>), grd, blk, 0, st->stream, A32, b, s, C.b, L.nx, L.ny, L.nz,
+                         lay_of(C), sc);
+    }
     return;
   }
   float* r = L.col;  // the line-solve scratch is free here
@@ -891,7 +942,7 @@ void mg_restrict(pcf_state* st, const MgLevel& L, const MgLevel& C, const BT* b,
     hipLaunchKernelGGL((k_mg_sum<1, 2>), cgrd, blk, 0, st->stream, r, C.b, fl, cl, sc);
 }
 
-// The coarsest level: kMgCoarseSteps smoothing steps from zero.
+// The coarsest level: mg_coarse_steps smoothing steps from zero.
 template <typename BT>
 float* mg_coarsest(pcf_state* st, const MgLevel& L, const BT* b, double* out64, const SolverScalars* sc) {
   const size_t nc = columns(L);
@@ -905,15 +956,15 @@ float* mg_coarsest(pcf_state* st, const MgLevel& L, const BT* b, double* out64, 
       if constexpr (NPL <= 4) {
         if (out64)
           hipLaunchKernelGGL((k_coarse_solve<NPL, BT, double>), dim3(1), blk, lds, st->stream, coefs_of(L), L.P, L.E,
-                             L.Q, b, out64, lay_of(L), L.deta, kMgCoarseSteps, sc);
+                             L.Q, b, out64, lay_of(L), L.deta, st->cfg.mg_coarse_steps, sc);
         else
           hipLaunchKernelGGL((k_coarse_solve<NPL, BT, float>), dim3(1), blk, lds, st->stream, coefs_of(L), L.P, L.E,
-                             L.Q, b, L.s, lay_of(L), L.deta, kMgCoarseSteps, sc);
+                             L.Q, b, L.s, lay_of(L), L.deta, st->cfg.mg_coarse_steps, sc);
       }
     });
     return L.s;
   }
-  return mg_smooth<BT>(st, L, b, nullptr, nullptr, nullptr, kMgCoarseSteps, out64, sc);
+  return mg_smooth<BT>(st, L, b, nullptr, nullptr, nullptr, st->cfg.mg_coarse_steps, out64, sc);
 }
 
 // One V-cycle on level l for the right-hand side b; the finest level writes
@@ -923,10 +974,10 @@ float* mg_vcycle(pcf_state* st, size_t l, const BT* b, double* out64, const Solv
   const MgLevel& L = st->mg[l];
   if (l + 1 == st->mg.size()) return mg_coarsest<BT>(st, L, b, out64, sc);
   const MgLevel& C = st->mg[l + 1];
-  float* s = mg_smooth<BT>(st, L, b, nullptr, nullptr, nullptr, kMgNu, nullptr, sc);
+  float* s = mg_smooth<BT>(st, L, b, nullptr, nullptr, nullptr, st->cfg.mg_smoothing_steps, nullptr, sc);
   mg_restrict<BT>(st, L, C, b, s, sc);
   const float* e = mg_vcycle<float>(st, l + 1, C.b, nullptr, sc);
-  return mg_smooth<BT>(st, L, b, s, e, &C, kMgNu, out64, sc);
+  return mg_smooth<BT>(st, L, b, s, e, &C, st->cfg.mg_smoothing_steps, out64, sc);
 }
 
 }  // namespace
@@ -942,10 +993,6 @@ int precondition_mixed(pcf_state* st, const double* sin, double* sout, const Sol
   }
   const Geo& g = st->g;
   const dim3 tgrd(blocks_for(g.nx, 64), g.ny), tblk(64);
-  const bool quads = g.nx % 4 == 0;
-  const dim3 sgrd = quads ? dim3(blocks_for(g.nx, 64), blocks_for(g.ny, 16), g.nz)
-                          : dim3(blocks_for(g.nx, 64), blocks_for(g.ny, 4), g.nz);
-  const dim3 sblk = quads ? dim3(16, 16) : dim3(64, 4);
   const float deta = (float)st->deta;
   const int iters = st->cfg.preconditioner_iterations;
   float* col = st->m_pc_tmp;  // upward-sweep scratch of the line solves
@@ -963,12 +1010,7 @@ int precondition_mixed(pcf_state* st, const double* sin, double* sout, const Sol
   for (int it = 2; it <= iters; ++it) {
     {
       ProfScope prof(st, PCF_K_PC_RHS);
-      if (quads)
-        hipLaunchKernelGGL((k_pc_rhs32<4, double>), sgrd, sblk, 0, st->stream, pc_args32(st, cur, rhs), sin, deta,
-                           sc);
-      else
-        hipLaunchKernelGGL((k_pc_rhs32<1, double>), sgrd, sblk, 0, st->stream, pc_args32(st, cur, rhs), sin, deta,
-                           sc);
+      pc_rhs<double>(st, st->T32, true, cur, rhs, sin, g.nx, g.ny, g.nz, deta, sc);
     }
     {
       ProfScope prof(st, PCF_K_PC_THOMAS);
@@ -1021,6 +1063,11 @@ int alloc_mixed(pcf_state* st) {
   int rc;
   for (int t = 0; t < T_COUNT; ++t)
     if ((rc = af(&st->T32[t]))) return rc;
+  if (st->cfg.preconditioner_storage == PCF_STORAGE_BF16)
+    for (int t = 0; t < T_COUNT; ++t) {
+      PCF_HIP(hipMalloc((void**)&st->T16[t], N * sizeof(uint16_t)));
+      st->allocations.push_back(st->T16[t]);
+    }
   float** fl[] = {&st->pc32_P, &st->pc32_E, &st->pc32_Q, &st->m_pc_tmp, &st->m_pc_cur, &st->m_pc_rhs};
   for (float** p : fl)
     if ((rc = af(p))) return rc;

@@ -150,6 +150,7 @@ struct pcf_state {
   // FP32 copies of the 22 horizontal operator coefficients and of the
   // line-solve factors, FP32 preconditioner work, and z = M^-1 s
   float* T32[pcf::T_COUNT] = {};
+  uint16_t* T16[pcf::T_COUNT] = {};   // bf16 bits of T32 (preconditioner_storage BF16)
   float *pc32_P = nullptr, *pc32_E = nullptr, *pc32_Q = nullptr;
   float *m_pc_tmp = nullptr, *m_pc_cur = nullptr, *m_pc_rhs = nullptr;
   double* bzpc = nullptr;

@@ -169,6 +169,18 @@ int host_setup(pcf_state* st, std::string& err) {
     err = "the multigrid preconditioner needs precision = mixed and preconditioner = true";
     return PCF_E_CONFIG;
   }
+  if (n.preconditioner_type == PCF_PC_MULTIGRID && (n.mg_smoothing_steps < 1 || n.mg_coarse_steps < 1)) {
+    err = "mg_smoothing_steps and mg_coarse_steps must be at least 1";
+    return PCF_E_CONFIG;
+  }
+  if (n.preconditioner_storage != PCF_STORAGE_FP32 && n.preconditioner_storage != PCF_STORAGE_BF16) {
+    err = "Invalid preconditioner_storage";
+    return PCF_E_OPTION;
+  }
+  if (n.preconditioner_storage != PCF_STORAGE_FP32 && (n.precision != PCF_PRECISION_MIXED || !n.preconditioner)) {
+    err = "preconditioner_storage = bf16 needs precision = mixed and preconditioner = true";
+    return PCF_E_CONFIG;
+  }
   if (n.initial_guess != PCF_GUESS_ZERO && n.precision != PCF_PRECISION_MIXED) {
     err = "initial_guess other than zero needs precision = mixed (the FP64 solve starts from zero)";
     return PCF_E_CONFIG;

@@ -151,6 +151,12 @@ class PoissonNamelist:
     initial_guess: str = "zero"
     # "line_relaxation" (the reference's) or "multigrid"; mixed solve only
     preconditioner_type: str = "line_relaxation"
+    # multigrid: pre- and post-smoothing steps per level, steps on the coarsest level
+    mg_smoothing_steps: int = 2
+    mg_coarse_steps: int = 20
+    # "fp32" or "bf16": storage of the finest level's coefficients read by the
+    # mixed-precision preconditioner
+    preconditioner_storage: str = "fp32"
 
 
 @dataclass
@@ -350,6 +356,10 @@ def to_config(nl: Namelists) -> _abi.PcfConfig:
                                "learned": _abi.GUESS_LEARNED}, po.initial_guess, "initial_guess")
     c.preconditioner_type = _option({"line_relaxation": _abi.PC_LINE_RELAXATION, "multigrid": _abi.PC_MULTIGRID},
                                     po.preconditioner_type, "preconditioner_type")
+    c.mg_smoothing_steps = int(po.mg_smoothing_steps)
+    c.mg_coarse_steps = int(po.mg_coarse_steps)
+    c.preconditioner_storage = _option({"fp32": _abi.STORAGE_FP32, "bf16": _abi.STORAGE_BF16},
+                                       po.preconditioner_storage, "preconditioner_storage")
     return c
 
 

@@ -10,7 +10,11 @@ The smoother is the pseudo-time step of apply_preconditioner!
 (apply_preconditioner!.jl:58-111): (1 - deta L_v) s' = s + deta (L_h s - b),
 with deta scaled per level by the aggregation of the couplings.  V-cycle:
 2 pre-smoothing steps from 0, residual restriction, coarse correction with
-weight 1.5, 2 post-smoothing steps; 20 steps from 0 on the coarsest level.
+weight 1.5, 2 post-smoothing steps; 20 steps from 0 on the coarsest level
+(the defaults of mg_smoothing_steps and mg_coarse_steps).  round_apply
+emulates a reduced storage of the finest level's coefficients where they are
+applied (the horizontal part of the smoother and the residual), the line
+solve keeping the unrounded ones.
 """
 import numpy as np
 
@@ -75,9 +79,11 @@ def coarsen(C, fx, fy):
     return out
 
 
-def levels(C, dx, dy, dtau=1.0):
+def levels(C, dx, dy, dtau=1.0, round_apply=None):
     cx, cy = 1.0 / dx ** 2, 1.0 / dy ** 2
     lv = [{"C": C, "deta": dtau / (2 * (cx + cy))}]
+    if round_apply is not None:
+        lv[0]["A"] = {n: round_apply(c) for n, c in C.items()}
     while True:
         nx, ny, _ = lv[-1]["C"]["AC"].shape
         fx = 2 if nx % 2 == 0 and nx >= MIN_SIZE else 1
@@ -90,19 +96,27 @@ def levels(C, dx, dy, dtau=1.0):
 
 
 def smooth(L, s, b, steps):
+    A = L.get("A", L["C"])
     for _ in range(steps):
-        s = line_solve(L["C"], L["deta"], s + L["deta"] * (apply(L["C"], s, True) - b))
+        s = line_solve(L["C"], L["deta"], s + L["deta"] * (apply(A, s, True) - b))
     return s
 
 
-def vcycle(lv, b, l=0):
+def vcycle(lv, b, l=0, nu=NU, coarse_steps=COARSE_STEPS):
     L = lv[l]
     if l == len(lv) - 1:
-        return smooth(L, np.zeros_like(b), b, COARSE_STEPS)
+        return smooth(L, np.zeros_like(b), b, coarse_steps)
     fx, fy = L["f"]
-    s = smooth(L, np.zeros_like(b), b, NU)
-    r = b - apply(L["C"], s)
+    s = smooth(L, np.zeros_like(b), b, nu)
+    r = b - apply(L.get("A", L["C"]), s)
     rc = sum(r[a::fx, c::fy, :] for a in range(fx) for c in range(fy))
-    e = vcycle(lv, rc, l + 1)
+    e = vcycle(lv, rc, l + 1, nu, coarse_steps)
     s = s + OMEGA * np.repeat(np.repeat(e, fx, axis=0), fy, axis=1)
-    return smooth(L, s, b, NU)
+    return smooth(L, s, b, nu)
+
+
+def bf16(c):
+    """FP64 -> FP32 -> bf16 (round to nearest even), returned as FP64."""
+    u = np.asarray(c, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)

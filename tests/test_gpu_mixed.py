@@ -50,9 +50,22 @@ MG_CASES = CASES + ["mg_deep", "mg_wide"]
 @pytest.mark.parametrize("pc", ["line_relaxation", "multigrid"])
 @pytest.mark.parametrize("name", CASES)
 def test_mixed_solve_meets_reference_criterion(pcf, abi, oracle_mod, name, pc):
+    solve_case(pcf, abi, oracle_mod, name, pc, "fp32")
+
+
+@pytest.mark.parametrize("pc", ["line_relaxation", "multigrid"])
+def test_bf16_storage_solve_meets_reference_criterion(pcf, abi, oracle_mod, pc):
+    """bf16 coefficients in the preconditioner's row kernels (a finest level
+    wide enough for them): the accuracy is the reference criterion's, as for
+    any preconditioner."""
+    solve_case(pcf, abi, oracle_mod, "mg_deep", pc, "bf16")
+
+
+def solve_case(pcf, abi, oracle_mod, name, pc, storage):
     nl = configs(pcf)[name]
     nl.poisson.precision = "mixed"
     nl.poisson.preconditioner_type = pc
+    nl.poisson.preconditioner_storage = storage
     gpu = pcf.State(nl)
     nl64 = configs(pcf)[name]
     ref = pcf.State(nl64)
@@ -74,8 +87,10 @@ def test_mixed_solve_meets_reference_criterion(pcf, abi, oracle_mod, name, pc):
         s.compute_operator(dt, rf)
     n_mixed = gpu.apply_bicgstab(tolrefs[0])
     n_fp64 = ref.apply_bicgstab(tolrefs[1])
-    print(name, pc, "iterations mixed", n_mixed, "fp64", n_fp64)
-    if pc == "line_relaxation":
+    print(name, pc, storage, "iterations mixed", n_mixed, "fp64", n_fp64)
+    if storage != "fp32":
+        pass  # the criterion below is the test
+    elif pc == "line_relaxation":
         # the FP32 preconditioner changes the iteration count only slightly
         assert abs(n_mixed - n_fp64) <= max(3, n_fp64 // 5)
     else:
@@ -126,15 +141,20 @@ def test_fp64_rejects_a_warm_start(pcf):
         pcf.State(nl)
 
 
-@pytest.mark.parametrize("name", MG_CASES)
-def test_multigrid_vcycle_matches_fp64_restatement(pcf, name):
+@pytest.mark.parametrize("name,storage,nu,coarse", [(n, "fp32", 2, 20) for n in MG_CASES] +
+                         [("mg_deep", "bf16", 2, 20), ("mg_wide", "bf16", 2, 20), ("mg_deep", "fp32", 1, 7),
+                          ("mountain_wave_3d", "fp32", 3, 5)])
+def test_multigrid_vcycle_matches_fp64_restatement(pcf, name, storage, nu, coarse):
     """The FP32 V-cycle on the device against the FP64 numpy restatement of the
     same algorithm (tests/mg_reference.py); FP32 rounding through the levels
     stays below MG_RTOL of the result's max norm.  The coarse levels are
     nearly singular (the operator's null space), which amplifies the rounding
     of their right-hand sides: 2e-4 on the 64 x 64 x 72 grid, the same for a
     row-sequential and a scan-based evaluation of the line solves, while a
-    wrong index, sign or coefficient gives errors of order one."""
+    wrong index, sign or coefficient gives errors of order one.  With bf16
+    storage the restatement rounds the finest level's applied coefficients
+    the same way (FP64 -> FP32 -> bf16, nearest even); the cases with bf16 have
+    a finest level wide enough for the row kernels, which read bf16."""
     import hipmem
     import mg_reference as mg
 
@@ -142,6 +162,9 @@ def test_multigrid_vcycle_matches_fp64_restatement(pcf, name):
     nl = configs(pcf)[name]
     nl.poisson.precision = "mixed"
     nl.poisson.preconditioner_type = "multigrid"
+    nl.poisson.preconditioner_storage = storage
+    nl.poisson.mg_smoothing_steps = nu
+    nl.poisson.mg_coarse_steps = coarse
     gpu = pcf.State(nl)
     gpu.compute_sponges(0.8, 3.0)
     gpu.compute_operator(0.8, 1.0)
@@ -149,13 +172,13 @@ def test_multigrid_vcycle_matches_fp64_restatement(pcf, name):
     c = pcf.output.constants(nl)
     dx = nl.domain.lx / c["lref"] / nl.domain.x_size
     dy = nl.domain.ly / c["lref"] / nl.domain.y_size
-    lv = mg.levels(C, dx, dy, nl.poisson.dtau)
+    lv = mg.levels(C, dx, dy, nl.poisson.dtau, round_apply=mg.bf16 if storage == "bf16" else None)
     x = np.random.default_rng(5).standard_normal((gpu.nx, gpu.ny, gpu.nz))
-    want = mg.vcycle(lv, x)
+    want = mg.vcycle(lv, x, nu=nu, coarse_steps=coarse)
     dx_ = hipmem.DeviceArray(x.shape).upload(x)
     dy_ = hipmem.DeviceArray(x.shape)
     gpu.apply_preconditioner_device(dx_.ptr, dy_.ptr)
     got = dy_.download()
     err = float(np.max(np.abs(got - want))) / float(np.max(np.abs(want)))
-    print(name, "levels", len(lv), "rel err", err)
+    print(name, storage, nu, coarse, "levels", len(lv), "rel err", err)
     assert err <= MG_RTOL, err
