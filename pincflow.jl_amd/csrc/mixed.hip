@@ -28,12 +28,48 @@ namespace {
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
 // CT: float, or uint16_t holding bf16 bits (the upper half of an FP32)
+// Memory layout of a level's arrays: x fastest (the finest level and the
+// levels smoothed with row-parallel kernels) or z fastest (the coarse levels
+// smoothed one column per wavefront, whose accesses are then contiguous
+// across the lanes of a wavefront).
+struct Lay {
+  int nx, ny, nz;
+  bool zf;
+  __host__ __device__ size_t at(int i, int j, int k) const {
+    return zf ? (size_t)k + (size_t)nz * ((size_t)i + (size_t)nx * j)
+              : (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+  }
+  // 32-bit form for the levels of at most kColumnCols x kScanMaxNz cells
+  __device__ int at32(int i, int j, int k) const { return zf ? k + nz * (i + nx * j) : i + nx * (j + ny * k); }
+  __host__ __device__ size_t count() const { return (size_t)nx * ny * nz; }
+  // the cell of the t-th element in memory order
+  __device__ void cell(size_t t, int& i, int& j, int& k) const {
+    if (zf) {
+      k = (int)(t % nz);
+      const size_t c = t / nz;
+      i = (int)(c % nx);
+      j = (int)(c / nx);
+    } else {
+      i = (int)(t % nx);
+      const size_t r = t / nx;
+      j = (int)(r % ny);
+      k = (int)(r / ny);
+    }
+  }
+};
+
 template <typename CT>
 struct PcArgs {
   const CT* a[T_COUNT];  // the 22 horizontal coefficients are used here
   const float* s;
   float* out;
   int nx, ny, nz;
+  // coarse-level correction added to s first (k_pc_rhs32<..., PRO = true>):
+  // s + omega e at the coarse cell (i >> sx, j >> sy, k) in layout cl
+  const float* e;
+  float omega;
+  int sx, sy;
+  Lay cl;
 };
 using PcArgs32 = PcArgs<float>;
 
@@ -103,8 +139,10 @@ __device__ __forceinline__ void ld_coef(const CT* p, float (&o)[W]) {
 // out = s + deta (H s - sin) with H the horizontal 22-point part of the
 // operator (apply_preconditioner!.jl:62-63); sin: the FP64 or FP32 input.
 // W x-adjacent cells per thread, block = (64 / W) x (4 W) threads covering
-// 64 x (4 W) cells of one z-level; CT: the coefficients' storage.
-template <int W, typename SinT, typename CT = float>
+// 64 x (4 W) cells of one z-level; CT: the coefficients' storage.  PRO: the
+// iterate is s + omega P e, the multigrid prolongation of the coarse-level
+// correction fused into the first post-smoothing step.
+template <int W, typename SinT, typename CT = float, bool PRO = false>
 __global__ void __launch_bounds__(256) k_pc_rhs32(PcArgs<CT> A, const SinT* __restrict__ sin, float deta,
                                                   const SolverScalars* sc) {
   if (sc && sc->done) return;
@@ -135,6 +173,17 @@ __global__ void __launch_bounds__(256) k_pc_rhs32(PcArgs<CT> A, const SinT* __re
     ld<W>(s + pl + (size_t)nx * jm + i0, B[w]);
     L[w] = s[pl + (size_t)nx * j + im];
     R[w] = s[pl + (size_t)nx * j + ip];
+    if constexpr (PRO) {
+      auto E = [&](int ii, int jj) -> float { return A.omega * A.e[A.cl.at(ii >> A.sx, jj >> A.sy, kk)]; };
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        C[w][c] += E(i0 + c, j);
+        F[w][c] += E(i0 + c, jp);
+        B[w][c] += E(i0 + c, jm);
+      }
+      L[w] += E(im, j);
+      R[w] += E(ip, j);
+    }
   }
   float a[T_COUNT][W];
 #pragma unroll
@@ -266,36 +315,6 @@ __host__ __device__ constexpr int floor_div(int a, int b) { return a >= 0 ? a / 
 
 struct MgCoefs {
   float* a[T_COUNT];
-};
-
-// Memory layout of a level's arrays: x fastest (the finest level and the
-// levels smoothed with row-parallel kernels) or z fastest (the coarse levels
-// smoothed one column per wavefront, whose accesses are then contiguous
-// across the lanes of a wavefront).
-struct Lay {
-  int nx, ny, nz;
-  bool zf;
-  __host__ __device__ size_t at(int i, int j, int k) const {
-    return zf ? (size_t)k + (size_t)nz * ((size_t)i + (size_t)nx * j)
-              : (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
-  }
-  // 32-bit form for the levels of at most kColumnCols x kScanMaxNz cells
-  __device__ int at32(int i, int j, int k) const { return zf ? k + nz * (i + nx * j) : i + nx * (j + ny * k); }
-  __host__ __device__ size_t count() const { return (size_t)nx * ny * nz; }
-  // the cell of the t-th element in memory order
-  __device__ void cell(size_t t, int& i, int& j, int& k) const {
-    if (zf) {
-      k = (int)(t % nz);
-      const size_t c = t / nz;
-      i = (int)(c % nx);
-      j = (int)(c / nx);
-    } else {
-      i = (int)(t % nx);
-      const size_t r = t / nx;
-      j = (int)(r % ny);
-      k = (int)(r / ny);
-    }
-  }
 };
 
 // R A P: one thread per coarse cell, in the coarse level's memory order.
@@ -466,16 +485,6 @@ __global__ void __launch_bounds__(256) k_resid_restrict4(Coefs<CT> A, const BT* 
       *(float2*)(bc + cl.at(i0 / 2, j / 2, k)) = v;
     }
   }
-}
-
-// s += omega P e on an x-fastest level; e in the coarse level's layout
-__global__ void k_mg_prolong(float* s, const float* e, int nx, int ny, int nz, int fx, int fy, Lay cl, float omega,
-                             const SolverScalars* sc) {
-  if (sc && sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-  if (i >= nx) return;
-  const size_t o = (size_t)i + (size_t)nx * j + (size_t)nx * ny * k;
-  s[o] += omega * e[cl.at(i / fx, j / fy, k)];
 }
 
 // ---------------------------------------------------------------------------
@@ -702,8 +711,9 @@ bool bf16_storage(const pcf_state* st) { return st->cfg.preconditioner_storage =
 // out = s + deta (H s - sin) on an x-fastest level with FP32 coefficients
 // a32; the finest level reads its coefficients in the configured storage.
 template <typename SinT>
-void pc_rhs(pcf_state* st, float* const* a32, bool finest, const float* s, float* out, const SinT* sin, int nx,
-            int ny, int nz, float deta, const SolverScalars* sc) {
+void pc_rhs(pcf_state* st, float* const* a32, const uint16_t* const* a16, const float* s, float* out,
+            const SinT* sin, int nx, int ny, int nz, float deta, const SolverScalars* sc, const float* e = nullptr,
+            const MgLevel* F = nullptr, const MgLevel* C = nullptr) {
   ProfScope prof(st, PCF_K_PC_RHS);
   const bool quads = nx % 4 == 0;
   const dim3 grd = quads ? dim3(blocks_for(nx, 64), blocks_for(ny, 16), nz)
@@ -716,14 +726,26 @@ void pc_rhs(pcf_state* st, float* const* a32, bool finest, const float* s, float
     A.nx = nx;
     A.ny = ny;
     A.nz = nz;
-    if (quads)
-      hipLaunchKernelGGL((k_pc_rhs32<4, SinT, CT>), grd, blk, 0, st->stream, A, sin, deta, sc);
-    else
-      hipLaunchKernelGGL((k_pc_rhs32<1, SinT, CT>), grd, blk, 0, st->stream, A, sin, deta, sc);
+    A.e = e;
+    A.omega = kMgOmega;
+    A.sx = F && F->fx == 2;
+    A.sy = F && F->fy == 2;
+    A.cl = C ? lay_of(*C) : Lay{1, 1, 1, false};
+    if (e) {
+      if (quads)
+        hipLaunchKernelGGL((k_pc_rhs32<4, SinT, CT, true>), grd, blk, 0, st->stream, A, sin, deta, sc);
+      else
+        hipLaunchKernelGGL((k_pc_rhs32<1, SinT, CT, true>), grd, blk, 0, st->stream, A, sin, deta, sc);
+    } else {
+      if (quads)
+        hipLaunchKernelGGL((k_pc_rhs32<4, SinT, CT>), grd, blk, 0, st->stream, A, sin, deta, sc);
+      else
+        hipLaunchKernelGGL((k_pc_rhs32<1, SinT, CT>), grd, blk, 0, st->stream, A, sin, deta, sc);
+    }
   };
-  if (finest && bf16_storage(st)) {
+  if (a16 && bf16_storage(st)) {
     PcArgs<uint16_t> A;
-    for (int t = 0; t < T_COUNT; ++t) A.a[t] = st->T16[t];
+    for (int t = 0; t < T_COUNT; ++t) A.a[t] = a16[t];
     go(A);
   } else {
     PcArgs<float> A;
@@ -773,6 +795,9 @@ int prepare_mixed(pcf_state* st) {
       hipLaunchKernelGGL((k_mg_coarsen<1, 2>), grd, blk, 0, st->stream, fa, ca, fl, cl);
     hipLaunchKernelGGL(k_mg_factors, dim3(blocks_for((long long)C.nx * C.ny, 64)), dim3(64), 0, st->stream,
                        C.a[T_AC], C.a[T_AD], C.a[T_AU], C.P, C.E, C.Q, cl, C.deta);
+    if (C.a16[0])
+      for (int t = 0; t < T_COUNT; ++t)
+        hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, st->stream, C.a[t], C.a16[t], cl.count());
   }
   PCF_HIP(hipGetLastError());
   st->t32_valid = true;
@@ -863,11 +888,8 @@ float* mg_smooth(pcf_state* st, const MgLevel& L, const BT* b, float* s, const f
     s = L.s;
     step = 1;
   }
-  if (e && !col) {
-    ProfScope prof(st, PCF_K_PC_RHS);
-    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks_for(L.nx, 64), L.ny, L.nz), dim3(64), 0, st->stream, s, e, L.nx,
-                       L.ny, L.nz, L.fx, L.fy, lay_of(*C), kMgOmega, sc);
-  }
+  // the row path adds the coarse correction in its first horizontal kernel
+  const int first_post = step;
   for (; step < steps; ++step) {
     const bool last = step == steps - 1;
     if (col) {
@@ -898,7 +920,8 @@ float* mg_smooth(pcf_state* st, const MgLevel& L, const BT* b, float* s, const f
       s = dst;
       continue;
     }
-    pc_rhs<BT>(st, L.a, &L == &st->mg[0], s, L.rhs, b, L.nx, L.ny, L.nz, L.deta, sc);
+    pc_rhs<BT>(st, L.a, L.a16[0] ? L.a16 : nullptr, s, L.rhs, b, L.nx, L.ny, L.nz, L.deta, sc,
+               step == first_post ? e : nullptr, &L, C);
     if (last && out64)
       line_solve<false>(st, L, (const float*)L.rhs, out64, sc);
     else
@@ -916,9 +939,9 @@ void mg_restrict(pcf_state* st, const MgLevel& L, const MgLevel& C, const BT* b,
   const MgCoefs A = coefs_of(L);
   if (!column_path(L) && L.fx == 2 && L.fy == 2 && L.nx % 4 == 0) {
     const dim3 grd(blocks_for(L.nx, 64), blocks_for(L.ny, 16), L.nz), blk(16, 16);
-    if (&L == &st->mg[0] && bf16_storage(st)) {
+    if (L.a16[0] && bf16_storage(st)) {
       Coefs<uint16_t> A16;
-      for (int t = 0; t < T_COUNT; ++t) A16.a[t] = st->T16[t];
+      for (int t = 0; t < T_COUNT; ++t) A16.a[t] = L.a16[t];
       hipLaunchKernelGGL((k_resid_restrict4<BT, uint16_t>), grd, blk, 0, st->stream, A16, b, s, C.b, L.nx, L.ny,
                          L.nz, lay_of(C), sc);
     } else {
@@ -1010,7 +1033,7 @@ int precondition_mixed(pcf_state* st, const double* sin, double* sout, const Sol
   for (int it = 2; it <= iters; ++it) {
     {
       ProfScope prof(st, PCF_K_PC_RHS);
-      pc_rhs<double>(st, st->T32, true, cur, rhs, sin, g.nx, g.ny, g.nz, deta, sc);
+      pc_rhs<double>(st, st->T32, st->T16[0] ? st->T16 : nullptr, cur, rhs, sin, g.nx, g.ny, g.nz, deta, sc);
     }
     {
       ProfScope prof(st, PCF_K_PC_THOMAS);
@@ -1084,7 +1107,10 @@ int alloc_mixed(pcf_state* st) {
   L0.ny = g.ny;
   L0.nz = g.nz;
   L0.deta = (float)st->deta;
-  for (int t = 0; t < T_COUNT; ++t) L0.a[t] = st->T32[t];
+  for (int t = 0; t < T_COUNT; ++t) {
+    L0.a[t] = st->T32[t];
+    L0.a16[t] = st->T16[t];
+  }
   L0.P = st->pc32_P;
   L0.E = st->pc32_E;
   L0.Q = st->pc32_Q;
@@ -1124,6 +1150,11 @@ int alloc_mixed(pcf_state* st) {
     for (float** p : v)
       if ((rc = ac(p))) return rc;
     C.zf = column_path(C);  // coarse levels smoothed column by column: z fastest
+    if (!C.zf && st->cfg.preconditioner_storage == PCF_STORAGE_BF16)
+      for (int t = 0; t < T_COUNT; ++t) {
+        PCF_HIP(hipMalloc((void**)&C.a16[t], Nc * sizeof(uint16_t)));
+        st->allocations.push_back(C.a16[t]);
+      }
     st->mg.push_back(C);
   }
   return PCF_OK;

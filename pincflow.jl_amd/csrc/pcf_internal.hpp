@@ -75,6 +75,7 @@ struct MgLevel {
   bool zf = false;  // arrays stored z fastest (else x fastest)
   float deta = 0.0f;
   float* a[T_COUNT] = {};
+  uint16_t* a16[T_COUNT] = {};  // bf16 copies read by the row kernels (storage BF16)
   float *P = nullptr, *E = nullptr, *Q = nullptr;
   float *b = nullptr, *s = nullptr, *rhs = nullptr, *col = nullptr;
 };

@@ -12,9 +12,11 @@ with deta scaled per level by the aggregation of the couplings.  V-cycle:
 2 pre-smoothing steps from 0, residual restriction, coarse correction with
 weight 1.5, 2 post-smoothing steps; 20 steps from 0 on the coarsest level
 (the defaults of mg_smoothing_steps and mg_coarse_steps).  round_apply
-emulates a reduced storage of the finest level's coefficients where they are
-applied (the horizontal part of the smoother and the residual), the line
-solve keeping the unrounded ones.
+emulates a reduced storage of the coefficients where the device's row
+kernels apply them: the horizontal part of the smoother on the levels with
+more than row_columns columns, and their residual where it is restricted by
+the fused 2 x 2 kernel (nx % 4 == 0); the line solve keeps the unrounded
+coefficients.
 """
 import numpy as np
 
@@ -79,20 +81,26 @@ def coarsen(C, fx, fy):
     return out
 
 
-def levels(C, dx, dy, dtau=1.0, round_apply=None):
+def levels(C, dx, dy, dtau=1.0, round_apply=None, row_columns=1024):
     cx, cy = 1.0 / dx ** 2, 1.0 / dy ** 2
     lv = [{"C": C, "deta": dtau / (2 * (cx + cy))}]
-    if round_apply is not None:
-        lv[0]["A"] = {n: round_apply(c) for n, c in C.items()}
     while True:
         nx, ny, _ = lv[-1]["C"]["AC"].shape
         fx = 2 if nx % 2 == 0 and nx >= MIN_SIZE else 1
         fy = 2 if ny % 2 == 0 and ny >= MIN_SIZE else 1
         if fx == 1 and fy == 1:
-            return lv
+            break
         lv[-1]["f"] = (fx, fy)
         cx, cy = cx * fy, cy * fx
         lv.append({"C": coarsen(lv[-1]["C"], fx, fy), "deta": dtau / (2 * (cx + cy))})
+    if round_apply is not None:
+        for L in lv:
+            nx, ny, _ = L["C"]["AC"].shape
+            if nx * ny > row_columns:
+                L["A"] = {n: round_apply(c) for n, c in L["C"].items()}
+                if L.get("f") == (2, 2) and nx % 4 == 0:
+                    L["R"] = L["A"]
+    return lv
 
 
 def smooth(L, s, b, steps):
@@ -108,7 +116,7 @@ def vcycle(lv, b, l=0, nu=NU, coarse_steps=COARSE_STEPS):
         return smooth(L, np.zeros_like(b), b, coarse_steps)
     fx, fy = L["f"]
     s = smooth(L, np.zeros_like(b), b, nu)
-    r = b - apply(L.get("A", L["C"]), s)
+    r = b - apply(L.get("R", L["C"]), s)
     rc = sum(r[a::fx, c::fy, :] for a in range(fx) for c in range(fy))
     e = vcycle(lv, rc, l + 1, nu, coarse_steps)
     s = s + OMEGA * np.repeat(np.repeat(e, fx, axis=0), fy, axis=1)
