@@ -345,10 +345,11 @@ __global__ void k_guess_input(pcf::Geo g, const double* b, const double* prev, c
   in[3 * p + 2] = (float)(jac[g.h(g.i0 + i, g.j0 + j, g.k0 + k)] - 1.0);
 }
 
-// x = |b| out
-__global__ void k_guess_output(const float* out, double* x, size_t n, double bnorm) {
+// x = c prev + |b| out: the network corrects the rescaled previous solution
+// (prev may be null: the first step of a run)
+__global__ void k_guess_output(const float* out, double* x, size_t n, double bnorm, const double* prev, double c) {
   for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
-    x[q] = bnorm * (double)out[q];
+    x[q] = (prev ? c * prev[q] : 0.0) + bnorm * (double)out[q];
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 parameters
@@ -492,8 +493,9 @@ int net_pack(pcf_net* net) {
 }  // namespace
 
 // The learned initial guess of the mixed solve (apply_bicgstab_mixed): x0 in
-// st->guess_x from the right-hand side, the stage's previous solution
-// (rescaled to this dt) and the Jacobian.
+// st->guess_x, the stage's previous solution rescaled to this dt plus the
+// network's correction from the right-hand side, that rescaled solution and
+// the Jacobian.
 int learned_guess(pcf_state* st, double dt, const double* prev, const double** x0) {
   pcf_net* net = st->guess_net;
   if (!net) {
@@ -504,7 +506,8 @@ int learned_guess(pcf_state* st, double dt, const double* prev, const double** x
   int rc = guess_input(st, st->rhs, prev, dt, st->guess_in, &bnorm);
   if (rc) return rc;
   if ((rc = net_forward(net, st->guess_in, st->stream))) return rc;
-  hipLaunchKernelGGL(k_guess_output, dim3(2048), dim3(256), 0, st->stream, net->out, st->guess_x, st->g.nv(), bnorm);
+  hipLaunchKernelGGL(k_guess_output, dim3(2048), dim3(256), 0, st->stream, net->out, st->guess_x, st->g.nv(), bnorm,
+                     prev, dt);
   PCF_HIP(hipGetLastError());
   *x0 = st->guess_x;
   return PCF_OK;

@@ -1,10 +1,12 @@
 """The learned initial guess of the mixed-precision solve.
 
 A small 3D convolutional network (six 3x3x3 layers, 32 channels, GELU; the
-bf16 matrix-core kernels of csrc/net.hip) maps the Poisson right-hand side
-b, the previous time step's solution of the same implicit stage and the
-terrain Jacobian to an initial guess x0 = |b| net(b/|b|, prev/|b|, jac-1)
-(|b| the RMS norm).  BiCGSTAB then starts from x0; its residual, measured
+bf16 matrix-core kernels of csrc/net.hip) corrects the warm start: from the
+Poisson right-hand side b, the previous time step's solution of the same
+implicit stage rescaled to this step (prev) and the terrain Jacobian it
+gives x0 = prev + |b| net(b/|b|, prev/|b|, jac-1) (|b| the RMS norm; prev = 0
+on a run's first step).  An untrained network (last layer zero) proposes the
+warm start itself.  BiCGSTAB then starts from x0; its residual, measured
 against |b| as for a zero start, still decides convergence, so the guess can
 only change the iteration count, never the accuracy.
 
@@ -149,7 +151,7 @@ def _interior_fcscal(st: State) -> np.ndarray:
 def collect_samples(st: State, steps: int) -> List[Tuple[np.ndarray, np.ndarray]]:
     """Run `steps` time steps and return, per step after the first, the
     network input and target of the step's second implicit-stage solve:
-    (input n x 3 FP32 channels-last, target n FP32 = x / |b|)."""
+    (input n x 3 FP32 channels-last, target n FP32 = (x - prev) / |b|)."""
     fcs = _interior_fcscal(st)
     n = st.nx * st.ny * st.nz
     d_b, d_prev = DeviceBuffer(8 * n), DeviceBuffer(8 * n)
@@ -161,15 +163,16 @@ def collect_samples(st: State, steps: int) -> List[Tuple[np.ndarray, np.ndarray]
         b = st.get_vector("RHS")
         x = st.get_vector("SOLUTION") * fcs
         if prev is not None:
+            warm = np.asfortranarray(prev * (info.dt / prev_dt)).ravel(order="F")
             d_b.upload(np.asfortranarray(b).ravel(order="F"))
-            d_prev.upload(np.asfortranarray(prev * (info.dt / prev_dt)).ravel(order="F"))
+            d_prev.upload(warm)
             bnorm = ctypes.c_double(0.0)
             rc = st.lib.pcf_guess_input(st.h, ctypes.c_void_p(d_b.ptr), ctypes.c_void_p(d_prev.ptr),
                                         ctypes.c_void_p(d_in.ptr), ctypes.byref(bnorm))
             if rc:
                 raise PinCFlowError(rc, st.lib.pcf_last_error().decode())
             inp = d_in.download(np.float32, (n * 3,))
-            target = (np.asfortranarray(x).ravel(order="F") / bnorm.value).astype(np.float32)
+            target = ((np.asfortranarray(x).ravel(order="F") - warm) / bnorm.value).astype(np.float32)
             out.append((inp, target))
         prev, prev_dt = x, info.dt
     return out
@@ -203,7 +206,7 @@ def train(net: GuessNetwork, samples, epochs: int, lr: float = 3e-4, log=None) -
 
 def evaluate(net: GuessNetwork, samples) -> float:
     """Mean squared error of the network's output against the targets,
-    relative to the zero output's (1 = no better than a zero start)."""
+    relative to the zero output's (1 = no better than the warm start)."""
     n = samples[0][1].size
     d_in, d_out = DeviceBuffer(12 * n), DeviceBuffer(4 * n)
     num = den = 0.0

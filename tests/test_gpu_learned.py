@@ -1,8 +1,9 @@
 """The learned initial guess: trained on the solver's own solves with the
 device's forward and backward kernels, used as BiCGSTAB's starting point in
-the mixed-precision solve, and compared with the zero start on held-out runs.
-The solve's stopping test is unchanged (|r| / |b| against the reference
-tolerance), so the guess affects the iteration count only."""
+the mixed-precision solve (multigrid preconditioner), and compared on held-out
+runs with the zero start and with the warm start it corrects.  The solve's
+stopping test is unchanged (|r| / |b| against the reference tolerance), so
+the guess affects the iteration count only."""
 import numpy as np
 import pytest
 
@@ -13,6 +14,7 @@ def namelists(pcf, seed, guess, steps):
     nl = pcf.examples.gaussian_hills_3d(x_size=32, y_size=32, z_size=16, iterations=steps, seed=seed, n_hills=6,
                                         precision="mixed")
     nl.poisson.initial_guess = guess
+    nl.poisson.preconditioner_type = "multigrid"
     return nl
 
 
@@ -44,14 +46,19 @@ def test_learned_guess_gains_over_zero_start(pcf):
     st.initial_cleaning()
     held = pcf.learned.collect_samples(st, 10)
     rel = pcf.learned.evaluate(infer, held)
-    print("held-out error of the guess relative to a zero start", rel)
-    assert rel < 1.0
-    zero = stage_iterations(pcf, namelists(pcf, held_out, "zero", 8), steps=8)
-    learned = stage_iterations(pcf, namelists(pcf, held_out, "learned", 8), net=infer, steps=8)
+    print("held-out error of the learned guess relative to the warm start's", rel)
+    # the trained network at least halves the squared error of the warm start
+    # it corrects, on runs it has not seen
+    assert rel < 0.5
+    steps = 8
+    zero = stage_iterations(pcf, namelists(pcf, held_out, "zero", steps), steps=steps)
+    warm = stage_iterations(pcf, namelists(pcf, held_out, "previous", steps), steps=steps)
+    learned = stage_iterations(pcf, namelists(pcf, held_out, "learned", steps), net=infer, steps=steps)
     # the first step has no previous solution; compare the steps after it
-    z, lrn = sum(zero[2:]), sum(learned[2:])
-    print("stage-2 iterations, zero start:", zero, "learned start:", learned)
-    # a guess that does not lower the residual falls back to the zero start,
-    # so the learned start costs at most the odd iteration of BiCGSTAB's
-    # irregular convergence
-    assert lrn <= z + len(zero[2:]), (zero, learned)
+    z, w, lrn = sum(zero[1:]), sum(warm[1:]), sum(learned[1:])
+    print("stage-2 iterations, zero start:", zero, "warm start:", warm, "learned start:", learned)
+    # BiCGSTAB's count depends on the initial residual, which the smaller
+    # error of the guess lowers little (its remaining error is of small
+    # scale, which the operator amplifies): the learned start may not cost
+    # more than an iteration per solve over either start
+    assert lrn <= min(z, w) + len(zero) - 1, (zero, warm, learned)
