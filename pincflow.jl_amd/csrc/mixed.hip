@@ -175,11 +175,25 @@ __global__ void __launch_bounds__(256) k_pc_rhs32(PcArgs<CT> A, const SinT* __re
     R[w] = s[pl + (size_t)nx * j + ip];
     if constexpr (PRO) {
       auto E = [&](int ii, int jj) -> float { return A.omega * A.e[A.cl.at(ii >> A.sx, jj >> A.sy, kk)]; };
+      if (W == 4 && A.sx == 1 && !A.cl.zf) {
+        // cells i0, i0 + 1 and i0 + 2, i0 + 3 share the coarse cells i0 / 2
+        // and i0 / 2 + 1: one 8-byte load per row
+        auto E2 = [&](int jj) -> float2 { return *(const float2*)(A.e + A.cl.at(i0 >> 1, jj >> A.sy, kk)); };
+        const float2 ec = E2(j), ef = E2(jp), eb = E2(jm);
 #pragma unroll
-      for (int c = 0; c < W; ++c) {
-        C[w][c] += E(i0 + c, j);
-        F[w][c] += E(i0 + c, jp);
-        B[w][c] += E(i0 + c, jm);
+        for (int c = 0; c < W; ++c) {
+          const float o = A.omega;
+          C[w][c] += o * (c < 2 ? ec.x : ec.y);
+          F[w][c] += o * (c < 2 ? ef.x : ef.y);
+          B[w][c] += o * (c < 2 ? eb.x : eb.y);
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+          C[w][c] += E(i0 + c, j);
+          F[w][c] += E(i0 + c, jp);
+          B[w][c] += E(i0 + c, jm);
+        }
       }
       L[w] += E(im, j);
       R[w] += E(ip, j);
@@ -220,7 +234,7 @@ __global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float*
                                                     OutT* out, float* col, int nx, int ny, int nz, float deta,
                                                     const SolverScalars* sc) {
   if (sc && sc->done) return;
-  constexpr int U = 8;
+  constexpr int U = 16;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= nx) return;
