@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCF_ABI_VERSION 3
+#define PCF_ABI_VERSION 4
 
 /* ---- return codes ------------------------------------------------------ */
 #define PCF_OK 0
@@ -125,6 +125,14 @@ enum pcf_preconditioner_storage {
   PCF_STORAGE_BF16 = 1
 };
 
+/* Tracer transport (TracerNamelist.jl: tracer_setup :NoTracer / :TracerOn). */
+enum pcf_tracer_setup { PCF_NO_TRACER = 0, PCF_TRACER_ON = 1 };
+
+/* Parametric forms of TracerNamelist.initial_tracer (a closure in the
+ * reference).  DIPOLE: vortex.jl:50-58, sign(x) (1 + cos(pi r)) / 2 for
+ * r = sqrt(((|x| - rx) / rx)^2 + (y / ry)^2) <= 1, else 0. */
+enum pcf_initial_tracer { PCF_TRACER_ZERO = 0, PCF_TRACER_DIPOLE = 1 };
+
 enum pcf_precision {
   PCF_PRECISION_FP64 = 0, /* reference arithmetic                            */
   PCF_PRECISION_MIXED = 1 /* FP64 BiCGSTAB with the preconditioner in FP32   */
@@ -211,6 +219,10 @@ typedef struct pcf_config {
   int32_t mg_smoothing_steps;             /* multigrid pre- and post-smoothing steps (2) */
   int32_t mg_coarse_steps;                /* multigrid steps on the coarsest level (20)  */
   int32_t preconditioner_storage;         /* enum pcf_preconditioner_storage            */
+  /* TracerNamelist.jl (leading_order_impact needs MS-GWaM: not supported) */
+  int32_t tracer_setup;                   /* enum pcf_tracer_setup        */
+  int32_t initial_tracer;                 /* enum pcf_initial_tracer      */
+  double tracer_rx, tracer_ry;            /* m, DIPOLE radii              */
 } pcf_config;
 
 /* Fields reachable through pcf_get_field / pcf_set_field (haloed, FP64). */
@@ -229,6 +241,10 @@ enum pcf_field {
   PCF_F_PHIU_X, PCF_F_PHIU_Y, PCF_F_PHIU_Z,
   PCF_F_PHIV_X, PCF_F_PHIV_Y, PCF_F_PHIV_Z,
   PCF_F_PHIW_X, PCF_F_PHIW_Y, PCF_F_PHIW_Z,
+  /* tracer (TracerPredictands, TracerIncrements, TracerAuxiliaries, the
+   * chi0 backup of integrate.jl:276 and TracerFluxes) */
+  PCF_F_CHI, PCF_F_DCHI, PCF_F_INITIALTRACER, PCF_F_CHI0,
+  PCF_F_PHICHI_X, PCF_F_PHICHI_Y, PCF_F_PHICHI_Z,
   PCF_F_COUNT
 };
 
@@ -246,7 +262,7 @@ enum pcf_vector {
 enum pcf_boundaries { PCF_BOUNDARY_PREDICTANDS = 0, PCF_BOUNDARY_FLUXES = 2 };
 
 /* Variables of update! / apply_lhs_sponge! (Types.jl singletons). */
-enum pcf_variable { PCF_VAR_RHO = 0, PCF_VAR_RHOP, PCF_VAR_U, PCF_VAR_V, PCF_VAR_W, PCF_VAR_PIP };
+enum pcf_variable { PCF_VAR_RHO = 0, PCF_VAR_RHOP, PCF_VAR_U, PCF_VAR_V, PCF_VAR_W, PCF_VAR_PIP, PCF_VAR_CHI };
 
 /* Operator kinds of apply_operator! (apply_operator!.jl:48 / :275). */
 enum pcf_operator { PCF_OP_TOTAL = 0, PCF_OP_HORIZONTAL = 1 };

@@ -98,6 +98,13 @@ struct State {
   std::vector<double> p_pc;                  // (nx, ny)
   Arr3 bp, br0, brold, br, bs, bt, bv, bmatvec, bv_pc;  // BiCGSTAB.jl
   Arr3 corx, cory;                           // Correction.jl (haloed)
+  // Tracer.jl (tracer_setup = TracerOn): chi = rho chi, its increment,
+  // initial state (TracerAuxiliaries), the chi0 backup of integrate.jl:276,
+  // reconstructions and fluxes
+  bool tracer = false;
+  Arr3 chi, dchi, initialtracer, chi0;
+  Arr5 chitilde;
+  Arr4 phichi;
   double last_residual = 0.0;
   // integrate bookkeeping (integrate.jl:68-115)
   double time = 0.0;

@@ -95,6 +95,11 @@ void set_boundaries_predictands(State& st) {
   vertical_bc(st, p.v, +1);
   vertical_bc(st, p.w, -1, true);
   vertical_bc(st, p.pip, +1);
+  if (st.tracer) {  // set_tracer_*_boundaries!.jl, BoundaryPredictands
+    zonal_bc(st, st.chi);
+    meridional_bc(st, st.chi);
+    vertical_bc(st, st.chi, -1);
+  }
 }
 
 // set_boundaries!(state, BoundaryReconstructions()): periodic copies of the
@@ -104,6 +109,7 @@ void set_boundaries_reconstructions(State& st) {
   std::vector<Arr5*> rec;
   if (pi) rec.push_back(&st.rhotilde);
   for (Arr5* a : {&st.rhoptilde, &st.utilde, &st.vtilde, &st.wtilde}) rec.push_back(a);
+  if (st.tracer) rec.push_back(&st.chitilde);
   for (Arr5* a : rec)
     for (auto& d : a->c)
       for (auto& q : d) zonal_bc(st, q);
@@ -123,6 +129,7 @@ void set_boundaries_fluxes(State& st) {
         st.phiu(i, j, st.k0 - 1, 3) = 0.0;
         st.phiv(i, j, st.k0 - 1, 3) = 0.0;
         st.phiw(i, j, st.k0 - 2, 3) = 0.0;
+        if (st.tracer) st.phichi(i, j, st.k0 - 1, 3) = 0.0;
       }
       if (st.ko + st.nz == st.cfg.z_size) {
         if (pi) st.phirho(i, j, st.k1, 3) = 0.0;
@@ -130,6 +137,7 @@ void set_boundaries_fluxes(State& st) {
         st.phiu(i, j, st.k1, 3) = 0.0;
         st.phiv(i, j, st.k1, 3) = 0.0;
         st.phiw(i, j, st.k1, 3) = 0.0;
+        if (st.tracer) st.phichi(i, j, st.k1, 3) = 0.0;
       }
     }
 }

@@ -110,6 +110,12 @@ void reconstruct(State& st) {
         phi(i, j, k) *= rhoedgeu / pedgeu;
       }
   muscl_3d(st, phi, st.wtilde);
+  if (st.tracer) {  // reconstruct!.jl:292-315
+    for (int k = ka; k <= kb; ++k)
+      for (int j = 1; j <= st.nyy; ++j)
+        for (int i = 1; i <= st.nxx; ++i) phi(i, j, k) = st.chi(i, j, k) / pbar(i, j, k);
+    muscl_3d(st, phi, st.chitilde);
+  }
 }
 
 // compute_flux.jl:22-32
@@ -126,11 +132,14 @@ void compute_fluxes(State& st, Predictands& p0) {
   const double re = st.c.re;
   const bool viscous = !(1 / re <= DBL_EPSILON && true);
 
-  // Rho (compute_fluxes!.jl:359-448) and RhoP (:450-520)
-  for (int pass = 0; pass < 2; ++pass) {
+  // Rho (compute_fluxes!.jl:359-448), RhoP (:450-520) and the tracer
+  // (:1513-1570, the carrier of RhoP); the tracer's come last in the
+  // reference, which changes nothing as no pass reads another's output
+  for (int pass = 0; pass < 3; ++pass) {
     if (pass == 0 && !pi) continue;
-    Arr5& t = pass == 0 ? st.rhotilde : st.rhoptilde;
-    Arr4& f = pass == 0 ? st.phirho : st.phirhop;
+    if (pass == 2 && !st.tracer) continue;
+    Arr5& t = pass == 0 ? st.rhotilde : pass == 1 ? st.rhoptilde : st.chitilde;
+    Arr4& f = pass == 0 ? st.phirho : pass == 1 ? st.phirhop : st.phichi;
 #pragma omp parallel for schedule(static)
     for (int k = k0; k <= k1; ++k)
       for (int j = j0; j <= j1; ++j)

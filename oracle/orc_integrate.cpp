@@ -68,6 +68,8 @@ int explicit_integration_lhs(State& st, Predictands& p0, double dt, double time)
     apply_lhs_sponge(st, sdt, time, PCF_VAR_RHO);
     update_lhs(st, dt, m, PCF_VAR_RHOP);
     apply_lhs_sponge(st, sdt, time, PCF_VAR_RHOP);
+    update_lhs(st, dt, m, PCF_VAR_CHI);  // no-ops without a tracer
+    apply_lhs_sponge(st, sdt, time, PCF_VAR_CHI);
     set_boundaries_predictands(st);
     update_lhs(st, dt, m, PCF_VAR_U);
     update_lhs(st, dt, m, PCF_VAR_V);
@@ -180,6 +182,7 @@ int orc_step(State* st, orc_step_info* info) {
   if (n.model == PCF_PSEUDO_INCOMPRESSIBLE) st->pr.rhop.a = st->pr.rho.a;
   set_boundaries_predictands(*st);
   copy_predictands(st->p0, st->pr);
+  if (st->tracer) st->chi0.a = st->chi.a;  // backup_predictands.jl:18
   // (1)
   explicit_integration_lhs(*st, st->p0, 0.5 * dt, time);
   // (2)
@@ -194,6 +197,7 @@ int orc_step(State* st, orc_step_info* info) {
   st->pr.u.a = st->p0.u.a;
   st->pr.v.a = st->p0.v.a;
   st->pr.w.a = st->p0.w.a;
+  if (st->tracer) st->chi.a = st->chi0.a;  // reset_predictands!.jl:63-67
   explicit_integration_rhs(*st, 0.5 * dt, time);
   // (4)
   copy_predictands(st->p0, st->p1);
@@ -264,6 +268,12 @@ static Arr3* field_ptr(State* st, int f) {
       return &st->phiv.c[f - PCF_F_PHIV_X];
     case PCF_F_PHIW_X: case PCF_F_PHIW_Y: case PCF_F_PHIW_Z:
       return &st->phiw.c[f - PCF_F_PHIW_X];
+    case PCF_F_CHI: return st->tracer ? &st->chi : nullptr;
+    case PCF_F_DCHI: return st->tracer ? &st->dchi : nullptr;
+    case PCF_F_INITIALTRACER: return st->tracer ? &st->initialtracer : nullptr;
+    case PCF_F_CHI0: return st->tracer ? &st->chi0 : nullptr;
+    case PCF_F_PHICHI_X: case PCF_F_PHICHI_Y: case PCF_F_PHICHI_Z:
+      return st->tracer ? &st->phichi.c[f - PCF_F_PHICHI_X] : nullptr;
     default: return nullptr;
   }
 }
@@ -348,7 +358,10 @@ void orc_set_boundaries(State* st, int which) {
   if (which == PCF_BOUNDARY_PREDICTANDS) set_boundaries_predictands(*st);
   else if (which == PCF_BOUNDARY_FLUXES) set_boundaries_fluxes(*st);
 }
-void orc_backup_predictands(State* st) { copy_predictands(st->p0, st->pr); }
+void orc_backup_predictands(State* st) {
+  copy_predictands(st->p0, st->pr);
+  if (st->tracer) st->chi0.a = st->chi.a;
+}
 // reconstruct! + set_boundaries!(Reconstructions) + compute_fluxes!(p0) + set_boundaries!(Fluxes)
 void orc_compute_fluxes(State* st) {
   reconstruct(*st);

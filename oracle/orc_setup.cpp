@@ -349,6 +349,43 @@ static int make_atmosphere(State& st) {
   }
 }
 
+// initial_tracer of TracerNamelist.jl as parametrised by pcf_initial_tracer
+// (DIPOLE: vortex.jl:50-58)
+static double initial_tracer(const pcf_config& n, double x, double y, double z) {
+  (void)z;
+  if (n.initial_tracer == PCF_TRACER_DIPOLE) {
+    const double rx = n.tracer_rx, ry = n.tracer_ry;
+    const double ax = (std::fabs(x) - rx) / rx, ay = y / ry;
+    const double r = std::sqrt(ax * ax + ay * ay);
+    if (r <= 1) {
+      const double sg = x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0);
+      return sg * (1 + std::cos(M_PI * r)) / 2;
+    }
+  }
+  return 0.0;
+}
+
+// TracerPredictands.jl:113-139 (TracerOn) and TracerAuxiliaries.jl:18-28
+static void make_tracer(State& st) {
+  const pcf_config& n = st.cfg;
+  st.tracer = n.tracer_setup == PCF_TRACER_ON;
+  if (!st.tracer) return;
+  const double lref = st.c.lref;
+  st.chi.init(st.nxx, st.nyy, st.nzz);
+  for (int k = 1; k <= st.nzz; ++k)
+    for (int j = st.j0; j <= st.j1; ++j)
+      for (int i = st.i0; i <= st.i1; ++i)
+        st.chi(i, j, k) = initial_tracer(n, st.x[i] * lref, st.y[j] * lref, st.zc(i, j, k) * lref);
+  zonal_bc(st, st.chi);
+  meridional_bc(st, st.chi);
+  for (size_t q = 0; q < st.chi.a.size(); ++q) st.chi.a[q] *= st.pr.rho.a[q] + st.rhobar.a[q];
+  st.initialtracer = st.chi;
+  st.dchi.init(st.nxx, st.nyy, st.nzz);
+  st.chi0.init(st.nxx, st.nyy, st.nzz);
+  st.chitilde.init(st.nxx, st.nyy, st.nzz);
+  st.phichi.init(st.nxx, st.nyy, st.nzz);
+}
+
 // Predictands.jl:70-157
 static int make_predictands(State& st) {
   const pcf_config& n = st.cfg;
@@ -445,6 +482,7 @@ int setup(State& st, const pcf_config& cfg) {
   st.horizontal_mean.assign(st.nz, 0.0);
   rc = make_predictands(st);
   if (rc) return rc;
+  make_tracer(st);
   for (Arr3* a : {&st.drho, &st.drhop, &st.du, &st.dv, &st.dw, &st.dpip, &st.rhoold, &st.rhopold,
                   &st.uold, &st.vold, &st.wold, &st.phi, &st.s_op, &st.corx, &st.cory})
     a->init(st.nxx, st.nyy, st.nzz);

@@ -257,6 +257,25 @@ void update_lhs(State& st, double dt, int m, int var) {
     return (f(i, j, k, 1) - f(i - 1, j, k, 1)) / dx + (f(i, j, k, 2) - f(i, j - 1, k, 2)) / dy +
            (f(i, j, k, 3) - f(i, j, k - 1, 3)) / dz;
   };
+  if (var == PCF_VAR_CHI) {  // update!.jl:1460-1500
+    if (!st.tracer) return;
+    if (m == 1) st.dchi.fill(0.0);
+    Arr3 &q = st.chi, &dq = st.dchi;
+    Arr4& f = st.phichi;
+#pragma omp parallel for schedule(static)
+    for (int k = st.k0; k <= st.k1; ++k)
+      for (int j = st.j0; j <= st.j1; ++j)
+        for (int i = st.i0; i <= st.i1; ++i) {
+          double fluxdiff = (f(i, j, k, 1) - f(i - 1, j, k, 1)) / dx + (f(i, j, k, 2) - f(i, j - 1, k, 2)) / dy +
+                            (f(i, j, k, 3) - f(i, j, k - 1, 3)) / dz;
+          fluxdiff /= jac(i, j, k);
+          // compute_volume_force(..., Chi()) is 0 without MS-GWaM
+          double fr = -fluxdiff + 0.0;
+          dq(i, j, k) = dt * fr + am * dq(i, j, k);
+          q(i, j, k) += bm * dq(i, j, k);
+        }
+    return;
+  }
   if (var == PCF_VAR_RHO || var == PCF_VAR_RHOP) {
     if (var == PCF_VAR_RHO && !pi) return;  // update!.jl:511-519
     Arr3& q = var == PCF_VAR_RHO ? st.pr.rho : st.pr.rhop;
@@ -549,6 +568,18 @@ void apply_lhs_sponge(State& st, double dt, double time, int var) {
     return;
   }
   if (var == PCF_VAR_PIP) return;
+  if (var == PCF_VAR_CHI) {  // apply_lhs_sponge!.jl:615-636
+    if (!st.tracer) return;
+    for (int k = st.k0; k <= st.k1; ++k)
+      for (int j = st.j0; j <= st.j1; ++j)
+        for (int i = st.i0; i <= st.i1; ++i) {
+          double alpha = alphar(i, j, k);
+          double old = st.chi(i, j, k);
+          double beta = 1.0 / (1.0 + alpha * dt);
+          st.chi(i, j, k) = (1.0 - beta) * st.initialtracer(i, j, k) + beta * old;
+        }
+    return;
+  }
   Arr3& f = var == PCF_VAR_U ? st.pr.u : var == PCF_VAR_V ? st.pr.v : st.pr.w;
   if (n.relax_to_mean) {
     for (int k = st.k0; k <= st.k1; ++k) {

@@ -8,8 +8,8 @@ field resident in HBM.
 from . import _abi
 from .namelists import (AgnesiHill, AtmosphereNamelist, CosineBubble, CosineHill, DiscretizationNamelist,
                         DomainNamelist, GaussianHills, GridNamelist, Namelists, OutputNamelist, PoissonNamelist,
-                        Sin2Sponge, Sin2TopSponge, SinhStretching, SpongeNamelist, TracerNamelist, Vortex,
-                        WKBNamelist, to_config)
+                        Sin2Sponge, Sin2TopSponge, SinhStretching, SpongeNamelist, TracerDipole, TracerNamelist,
+                        Vortex, WKBNamelist, to_config)
 from . import examples, learned, output
 from .state import PinCFlowError, PoissonNotConverged, State, integrate
 
@@ -17,6 +17,7 @@ __all__ = [
     "State", "integrate", "PinCFlowError", "PoissonNotConverged",
     "AgnesiHill", "AtmosphereNamelist", "CosineBubble", "CosineHill", "DiscretizationNamelist", "DomainNamelist",
     "GaussianHills", "GridNamelist", "Namelists", "OutputNamelist", "PoissonNamelist", "Sin2Sponge",
-    "Sin2TopSponge", "SinhStretching", "SpongeNamelist", "TracerNamelist", "Vortex", "WKBNamelist", "to_config",
+    "Sin2TopSponge", "SinhStretching", "SpongeNamelist", "TracerDipole", "TracerNamelist", "Vortex", "WKBNamelist",
+    "to_config",
     "examples", "learned", "output",
 ]

@@ -13,7 +13,7 @@ from pathlib import Path
 PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "lib" / "libpincflow_hip.so"
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAX_HILLS = 32
 
 # return codes
@@ -55,6 +55,8 @@ FIELDS = [
     "PHIU_X", "PHIU_Y", "PHIU_Z",
     "PHIV_X", "PHIV_Y", "PHIV_Z",
     "PHIW_X", "PHIW_Y", "PHIW_Z",
+    "CHI", "DCHI", "INITIALTRACER", "CHI0",
+    "PHICHI_X", "PHICHI_Y", "PHICHI_Z",
 ]
 FIELD = {name: i for i, name in enumerate(FIELDS)}
 
@@ -68,7 +70,9 @@ VECTORS = [
 VECTOR = {name: i for i, name in enumerate(VECTORS)}
 
 BOUNDARY_PREDICTANDS, BOUNDARY_FLUXES = 0, 2
-VAR_RHO, VAR_RHOP, VAR_U, VAR_V, VAR_W, VAR_PIP = range(6)
+VAR_RHO, VAR_RHOP, VAR_U, VAR_V, VAR_W, VAR_PIP, VAR_CHI = range(7)
+NO_TRACER, TRACER_ON = 0, 1
+TRACER_ZERO, TRACER_DIPOLE = 0, 1
 OP_TOTAL, OP_HORIZONTAL = 0, 1
 
 
@@ -137,6 +141,10 @@ class PcfConfig(ctypes.Structure):
         ("mg_smoothing_steps", ctypes.c_int32),
         ("mg_coarse_steps", ctypes.c_int32),
         ("preconditioner_storage", ctypes.c_int32),
+        ("tracer_setup", ctypes.c_int32),
+        ("initial_tracer", ctypes.c_int32),
+        ("tracer_rx", ctypes.c_double),
+        ("tracer_ry", ctypes.c_double),
     ]
 
 

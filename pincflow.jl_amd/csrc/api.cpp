@@ -68,6 +68,12 @@ double* field_ptr(pcf_state* st, int f) {
     case PCF_F_P0_U: return st->p0[2];
     case PCF_F_P0_V: return st->p0[3];
     case PCF_F_P0_W: return st->p0[4];
+    case PCF_F_CHI: return st->chi;
+    case PCF_F_DCHI: return st->dchi;
+    case PCF_F_INITIALTRACER: return st->chi_init;
+    case PCF_F_CHI0: return st->chi0;
+    case PCF_F_PHICHI_X: case PCF_F_PHICHI_Y: case PCF_F_PHICHI_Z:
+      return st->flux[5][f - PCF_F_PHICHI_X];
     default:
       if (f >= PCF_F_PHIRHO_X && f <= PCF_F_PHIW_Z) {
         int q = (f - PCF_F_PHIRHO_X) / 3, d = (f - PCF_F_PHIRHO_X) % 3;
@@ -115,6 +121,8 @@ int explicit_lhs(pcf_state* st, double dt, double time) {
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_RHO));
     TRY(update_lhs(st, dt, m, PCF_VAR_RHOP));
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_RHOP));
+    TRY(update_lhs(st, dt, m, PCF_VAR_CHI));  // no-ops without a tracer
+    TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_CHI));
     TRY(set_boundaries_predictands(st));
     TRY(update_lhs(st, dt, m, PCF_VAR_U));
     TRY(update_lhs(st, dt, m, PCF_VAR_V));
@@ -263,11 +271,14 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
     if ((rc = alloc(st, &st->p0[q], NH))) return fail(rc);
     if ((rc = alloc(st, &st->p1[q], NH))) return fail(rc);
   }
-  for (int q = 0; q < 5; ++q) {
+  for (int q = 0; q < (st->tracer ? 6 : 5); ++q) {
     if ((rc = alloc(st, &st->phi[q], NH))) return fail(rc);
     for (int d = 0; d < 3; ++d)
       if ((rc = alloc(st, &st->flux[q][d], NH))) return fail(rc);
   }
+  if (st->tracer)
+    for (double** p : {&st->dchi, &st->chi0})
+      if ((rc = alloc(st, p, NH))) return fail(rc);
   for (int q = 0; q < 6; ++q)
     if ((rc = alloc(st, &st->stress[q], NH))) return fail(rc);
   for (int q = 0; q < T_COUNT; ++q)
@@ -369,7 +380,9 @@ int pcf_backup_predictands(pcf_state* st) {
   if (!st) return PCF_E_ARG;
   double* cur[6];
   predictand_slots(st, cur);
-  return copy_set(st, st->p0, cur, 6);
+  TRY(copy_set(st, st->p0, cur, 6));
+  if (st->tracer) TRY(copy_field(st, st->chi0, st->chi));  // backup_predictands.jl:18
+  return PCF_OK;
 }
 
 int pcf_explicit_integration_lhs(pcf_state* st, double dt, double time) {
@@ -412,6 +425,7 @@ int pcf_step(pcf_state* st, pcf_step_info* info) {
   double* cur[6];
   predictand_slots(st, cur);
   TRY(copy_set(st, st->p0, cur, 6));  // backup_predictands
+  if (st->tracer) TRY(copy_field(st, st->chi0, st->chi));
   TRY(explicit_lhs(st, 0.5 * dt, time));                      // (1)
   int it1 = 0, it2 = 0;
   st->solve_slot = 0;
@@ -426,6 +440,7 @@ int pcf_step(pcf_state* st, pcf_step_info* info) {
   TRY(copy_field(st, st->u, st->p0[2]));
   TRY(copy_field(st, st->v, st->p0[3]));
   TRY(copy_field(st, st->w, st->p0[4]));
+  if (st->tracer) TRY(copy_field(st, st->chi, st->chi0));     // reset_predictands!.jl:63-67
   TRY(explicit_rhs(st, 0.5 * dt, time));                      // (3)
   TRY(copy_set(st, st->p0, st->p1, 6));                       // p0 = p1
   TRY(explicit_lhs(st, dt, time));                            // (4)

@@ -70,9 +70,8 @@ __global__ void k_bc_z(double* f, Geo g, double sign, int staggered) {
 }
 
 // set_vertical_boundaries!(state, BoundaryFluxes()) (set_vertical_boundaries!.jl:274-347)
-__global__ void k_bc_fluxes(pcf_state* unused, Geo g, double* rho3, double* rhop3, double* u3, double* v3,
+__global__ void k_bc_fluxes(double* chi3, Geo g, double* rho3, double* rhop3, double* u3, double* v3,
                             double* w3) {
-  (void)unused;
   int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   int j = 1 + blockIdx.y;
   if (i > g.nxx) return;
@@ -82,6 +81,7 @@ __global__ void k_bc_fluxes(pcf_state* unused, Geo g, double* rho3, double* rhop
   u3[b] = 0.0; u3[t] = 0.0;
   v3[b] = 0.0; v3[t] = 0.0;
   w3[g.h(i, j, g.k0 - 2)] = 0.0; w3[t] = 0.0;
+  if (chi3) { chi3[b] = 0.0; chi3[t] = 0.0; }  // set_tracer_vertical_boundaries!.jl, BoundaryFluxes
 }
 
 }  // namespace
@@ -128,13 +128,19 @@ int set_boundaries_predictands(pcf_state* st) {
   if ((rc = bc_z(st, st->u, +1, false))) return rc;
   if ((rc = bc_z(st, st->v, +1, false))) return rc;
   if ((rc = bc_z(st, st->w, -1, true))) return rc;
-  return bc_z(st, st->pip, +1, false);
+  if ((rc = bc_z(st, st->pip, +1, false))) return rc;
+  if (st->tracer) {  // set_tracer_*_boundaries!.jl, BoundaryPredictands
+    if ((rc = bc_xy(st, &st->chi, 1))) return rc;
+    if ((rc = bc_z(st, st->chi, -1, false))) return rc;
+  }
+  return PCF_OK;
 }
 
 int set_boundaries_fluxes(pcf_state* st) {
   const Geo& g = st->g;
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
-  hipLaunchKernelGGL(k_bc_fluxes, dim3(blocks_for(g.nxx, 64), g.nyy), dim3(64), 0, st->stream, nullptr, g,
+  hipLaunchKernelGGL(k_bc_fluxes, dim3(blocks_for(g.nxx, 64), g.nyy), dim3(64), 0, st->stream,
+                     st->tracer ? st->flux[5][2] : nullptr, g,
                      pi ? st->flux[0][2] : nullptr, st->flux[1][2], st->flux[2][2], st->flux[3][2], st->flux[4][2]);
   PCF_HIP(hipGetLastError());
   return PCF_OK;

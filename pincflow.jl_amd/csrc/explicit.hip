@@ -19,16 +19,16 @@ namespace pcf {
 
 namespace {
 
-enum { V_RHO = 0, V_RHOP, V_U, V_V, V_W };
+enum { V_RHO = 0, V_RHOP, V_U, V_V, V_W, V_CHI };
 
 struct FluxArgs {
   Fields F;
-  const double* phi[5];
+  const double* phi[6];
   const double* u0;
   const double* v0;
   const double* w0;
   const double* st[6];  // stress 11, 12, 13, 22, 23, 33
-  double* flux[5][3];
+  double* flux[6][3];
   Geo g;
   double inv_re;        // 1 / re
   int viscous;
@@ -39,7 +39,8 @@ __device__ __forceinline__ Fields fields_at(const FluxArgs& A) { return A.F; }
 // ---------------------------------------------------------------------------
 // reconstruction inputs (reconstruct!.jl:126-286) on layers k0-1..k1+1
 // ---------------------------------------------------------------------------
-__global__ void k_phi(Fields F, Geo g, double* phr, double* phrp, double* phu, double* phv, int pi) {
+__global__ void k_phi(Fields F, Geo g, double* phr, double* phrp, double* phu, double* phv, int pi,
+                      const double* chi, double* phchi) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y;
   const int k = g.k0 - 1 + blockIdx.z;
@@ -48,6 +49,7 @@ __global__ void k_phi(Fields F, Geo g, double* phr, double* phrp, double* phu, d
   const double pb = F.pbar[c];
   if (pi) phr[c] = F.rho[c] / pb;
   phrp[c] = F.rhop[c] / pb;  // reconstruct!.jl:154-176
+  if (phchi) phchi[c] = chi[c] / pb;  // reconstruct!.jl:292-315
   if (i < g.nxx) {           // reconstruct!.jl:178-208
     const size_t r = g.h(i + 1, j, k);
     double rhoedge = 0.5 * (F.rho[c] + F.rho[r] + F.rhobar[c] + F.rhobar[r]);
@@ -167,7 +169,7 @@ __global__ void k_flux(FluxArgs A, Range3 R) {
   double right_of_c, left_of_next;
   faces(A.phi[VAR], c, stride, right_of_c, left_of_next);
   double val;
-  if (VAR == V_RHO || VAR == V_RHOP) {
+  if (VAR == V_RHO || VAR == V_RHOP || VAR == V_CHI) {  // the tracer: RhoP's carrier (compute_fluxes!.jl:1513-1570)
     double hi = left_of_next, lo = right_of_c;  // value at c + e, value at c
     if (DIR == 0) {
       if (VAR == V_RHO) {
@@ -327,6 +329,7 @@ struct UpdArgs {
   const double* vold;
   double dt, am, bm;
   int first;
+  int force0;  // the tracer adds its (zero) volume force, update!.jl:1490-1492
 };
 
 __device__ __forceinline__ double divergence(const double* const* f, const Geo& g, int i, int j, int k) {
@@ -342,6 +345,7 @@ __global__ void k_update_scalar(UpdArgs A, Range3 R) {  // update!.jl:521-597
   double fluxdiff = divergence(A.f, A.g, i, j, k);
   fluxdiff /= A.F.jac[c];
   double fr = -fluxdiff;
+  if (A.force0) fr = fr + 0.0;
   double dqv = A.first ? 0.0 : A.dq[c];
   dqv = A.dt * fr + A.am * dqv;
   A.dq[c] = dqv;
@@ -641,6 +645,17 @@ __global__ void k_lhs_sponge(double* q, const double* alphar, const double* jac,
   q[c] = (1.0 - beta) * bg + beta * old;
 }
 
+// the tracer's LHS sponge towards its initial state (apply_lhs_sponge!.jl:615-636)
+__global__ void k_lhs_sponge_field(double* q, const double* alphar, const double* bg, Geo g, Range3 R, double dt) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const size_t c = g.h(i, j, k);
+  double alpha = alphar[c];
+  double old = q[c];
+  double beta = 1.0 / (1.0 + alpha * dt);
+  q[c] = (1.0 - beta) * bg[c] + beta * old;
+}
+
 // horizontal mean of one layer for relax_to_mean (sequential sum over the layer)
 __global__ void k_layer_mean(const double* f, double* mean, Geo g, double nxy_x, double nxy_y) {
   const int k = g.k0 + blockIdx.x;
@@ -731,7 +746,7 @@ int compute_fluxes(pcf_state* st) {
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
   const int nlev = g.nz + 2;  // k0-1 .. k1+1
   hipLaunchKernelGGL(k_phi, dim3(blocks_for(g.nxx, 64), g.nyy, nlev), dim3(64), 0, st->stream, F, g, st->phi[0],
-                     st->phi[1], st->phi[2], st->phi[3], pi ? 1 : 0);
+                     st->phi[1], st->phi[2], st->phi[3], pi ? 1 : 0, st->chi, st->tracer ? st->phi[V_CHI] : nullptr);
   hipLaunchKernelGGL(k_wcart, dim3(blocks_for(g.nxx - 1, 64), g.nyy - 1, g.nzz - 1), dim3(64), 0, st->stream, F, g,
                      st->wcart);
   hipLaunchKernelGGL(k_phi_w, dim3(blocks_for(g.nxx - 1, 64), g.nyy - 1, nlev), dim3(64), 0, st->stream, F, g,
@@ -743,12 +758,12 @@ int compute_fluxes(pcf_state* st) {
   }
   FluxArgs A;
   A.F = F;
-  for (int q = 0; q < 5; ++q) A.phi[q] = st->phi[q];
+  for (int q = 0; q < 6; ++q) A.phi[q] = st->phi[q];
   A.u0 = st->p0[2];
   A.v0 = st->p0[3];
   A.w0 = st->p0[4];
   for (int q = 0; q < 6; ++q) A.st[q] = st->stress[q];
-  for (int q = 0; q < 5; ++q)
+  for (int q = 0; q < 6; ++q)
     for (int d = 0; d < 3; ++d) A.flux[q][d] = st->flux[q][d];
   A.g = g;
   A.inv_re = 1 / st->ph.re;
@@ -773,6 +788,11 @@ int compute_fluxes(pcf_state* st) {
   launch_flux<V_W, 0>(st, A, Range3{i0 - 1, nx + 1, j0, ny, k0, nz - 1});
   launch_flux<V_W, 1>(st, A, Range3{i0, nx, j0 - 1, ny + 1, k0, nz - 1});
   launch_flux<V_W, 2>(st, A, Range3{i0, nx, j0, ny, k0 - 1, nz});
+  if (st->tracer) {
+    launch_flux<V_CHI, 0>(st, A, Range3{i0 - 1, nx + 1, j0, ny, k0, nz});
+    launch_flux<V_CHI, 1>(st, A, Range3{i0, nx, j0 - 1, ny + 1, k0, nz});
+    launch_flux<V_CHI, 2>(st, A, Range3{i0, nx, j0, ny, k0, nz - 1});
+  }
   PCF_HIP(hipGetLastError());
   return set_boundaries_fluxes(st);
 }
@@ -789,6 +809,7 @@ int update_lhs(pcf_state* st, double dt, int m, int var) {
   A.am = st->alphark[m - 1];
   A.bm = st->betark[m - 1];
   A.first = m == 1;
+  A.force0 = 0;
   A.rhoold = st->rhoold;
   A.uold = st->uold;
   A.vold = st->vold;
@@ -805,6 +826,16 @@ int update_lhs(pcf_state* st, double dt, int m, int var) {
       A.q = var == PCF_VAR_RHO ? st->rho : st->rhop;
       A.dq = var == PCF_VAR_RHO ? st->drho : st->drhop;
       for (int d = 0; d < 3; ++d) A.f[d] = st->flux[q][d];
+      R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
+      hipLaunchKernelGGL(k_update_scalar, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      break;
+    }
+    case PCF_VAR_CHI: {  // update!.jl:1460-1500
+      if (!st->tracer) return PCF_OK;
+      A.q = st->chi;
+      A.dq = st->dchi;
+      A.force0 = 1;
+      for (int d = 0; d < 3; ++d) A.f[d] = st->flux[V_CHI][d];
       R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
       hipLaunchKernelGGL(k_update_scalar, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
       break;
@@ -888,6 +919,14 @@ int apply_lhs_sponge(pcf_state* st, double dt, double time, int var) {
   const pcf_config& n = st->cfg;
   if (var == PCF_VAR_PIP) return PCF_OK;
   if (var == PCF_VAR_RHO && n.model == PCF_BOUSSINESQ) return PCF_OK;
+  if (var == PCF_VAR_CHI) {
+    if (!st->tracer) return PCF_OK;
+    Range3 R{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
+    hipLaunchKernelGGL(k_lhs_sponge_field, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, st->chi,
+                       st->alphar, st->chi_init, g, R, dt);
+    PCF_HIP(hipGetLastError());
+    return PCF_OK;
+  }
   double* q = var == PCF_VAR_RHO ? st->rho : var == PCF_VAR_RHOP ? st->rhop : var == PCF_VAR_U ? st->u
             : var == PCF_VAR_V ? st->v : st->w;
   Range3 R{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};

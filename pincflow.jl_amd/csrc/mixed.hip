@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float*
                                                     OutT* out, float* col, int nx, int ny, int nz, float deta,
                                                     const SolverScalars* sc) {
   if (sc && sc->done) return;
-  constexpr int U = 16;
+  constexpr int U = 8;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= nx) return;

@@ -129,13 +129,17 @@ struct pcf_state {
   double *p0[6], *p1[6];  // rho, rhop, u, v, w, pip
   double *drho, *drhop, *du, *dv, *dw;
   double *rhoold, *rhopold, *uold, *vold, *wold;
+  // tracer (cfg.tracer_setup == PCF_TRACER_ON): rho chi, its increment, its
+  // initial state (the sponge's background) and the chi0 backup of a step
+  bool tracer = false;
+  double *chi = nullptr, *dchi = nullptr, *chi_init = nullptr, *chi0 = nullptr;
   // background and grid (haloed, static)
   double *pbar, *thetabar, *rhobar, *n2, *jac, *met13, *met23, *met33, *zc, *zctilde, *fcscal;
   // sponges
   double *alphar, *betar, *betar_s;  // betar_s: dt-independent part of betar
   // fluxes (haloed): [var][dir]
-  double* flux[5][3];                 // rho, rhop, u, v, w
-  double* phi[5];                     // reconstruction inputs
+  double* flux[6][3] = {};            // rho, rhop, u, v, w, chi
+  double* phi[6] = {};                // reconstruction inputs
   double* wcart;                      // Cartesian vertical wind
   double* stress[6];                  // 11, 12, 13, 22, 23, 33
   // corrector

@@ -101,6 +101,22 @@ double stretching(const pcf_config& c, double zhat) {
   return zhat;
 }
 
+// initial_tracer of TracerNamelist.jl as parametrised by pcf_initial_tracer
+// (DIPOLE: vortex.jl:50-58)
+double initial_tracer(const pcf_config& c, double x, double y, double z) {
+  (void)z;
+  if (c.initial_tracer == PCF_TRACER_DIPOLE) {
+    const double rx = c.tracer_rx, ry = c.tracer_ry;
+    const double ax = (std::fabs(x) - rx) / rx, ay = y / ry;
+    const double r = std::sqrt(ax * ax + ay * ay);
+    if (r <= 1) {
+      const double sg = x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0);
+      return sg * (1 + std::cos(M_PI * r)) / 2;
+    }
+  }
+  return 0.0;
+}
+
 double bubble_radius(const pcf_config& c, double x, double y, double z) {
   double r2 = 0.0;
   if (c.pert_rx > 0) { double q = (x - c.pert_x0) / c.pert_rx; r2 += q * q; }
@@ -179,6 +195,16 @@ int host_setup(pcf_state* st, std::string& err) {
   }
   if (n.preconditioner_storage != PCF_STORAGE_FP32 && (n.precision != PCF_PRECISION_MIXED || !n.preconditioner)) {
     err = "preconditioner_storage = bf16 needs precision = mixed and preconditioner = true";
+    return PCF_E_CONFIG;
+  }
+  if (n.tracer_setup != PCF_NO_TRACER && n.tracer_setup != PCF_TRACER_ON) { err = "Invalid tracer_setup"; return PCF_E_OPTION; }
+  if (n.initial_tracer != PCF_TRACER_ZERO && n.initial_tracer != PCF_TRACER_DIPOLE) {
+    err = "Invalid initial_tracer";
+    return PCF_E_OPTION;
+  }
+  if (n.tracer_setup == PCF_TRACER_ON && n.initial_tracer == PCF_TRACER_DIPOLE &&
+      !(n.tracer_rx > 0.0 && n.tracer_ry > 0.0)) {
+    err = "initial_tracer: the dipole radii must be positive";
     return PCF_E_CONFIG;
   }
   if (n.initial_guess != PCF_GUESS_ZERO && n.precision != PCF_PRECISION_MIXED) {
@@ -451,6 +477,20 @@ int host_setup(pcf_state* st, std::string& err) {
         w(i, j, k) = (jac(i, j, k + 1) * w(i, j, k) + jac(i, j, k) * w(i, j, k + 1)) / (jac(i, j, k) + jac(i, j, k + 1));
   walls_z(g, w, -1.0, true);
 
+  // TracerPredictands.jl:113-139 (TracerOn): the initial tracer times the
+  // density, kept also as the sponge's background (TracerAuxiliaries.jl:18-28)
+  st->tracer = n.tracer_setup == PCF_TRACER_ON;
+  Host3 chi(st->tracer ? g.nxx : 1, st->tracer ? g.nyy : 1, st->tracer ? g.nzz : 1);
+  if (st->tracer) {
+    for (int k = 1; k <= g.nzz; ++k)
+      for (int j = g.j0; j <= g.j1; ++j)
+        for (int i = g.i0; i <= g.i1; ++i)
+          chi(i, j, k) = initial_tracer(n, x[i] * lref, y[j] * lref, zc(i, j, k) * lref);
+    periodic_x(g, chi);
+    periodic_y(g, chi);
+    for (size_t q = 0; q < N; ++q) chi.a[q] *= rho.a[q] + rhb.a[q];
+  }
+
   // Sponge profiles (compute_sponges!.jl:26-65): alphar does not depend on
   // time or dt for the supported forms; betar = (numerator / dt) tref.
   Host3 alphar(g.nxx, g.nyy, g.nzz), bnum(g.nxx, g.nyy, g.nzz);
@@ -493,6 +533,10 @@ int host_setup(pcf_state* st, std::string& err) {
   if ((rc = upload(st, &st->v, v.a))) return rc;
   if ((rc = upload(st, &st->w, w.a))) return rc;
   if ((rc = upload(st, &st->pip, pip.a))) return rc;
+  if (st->tracer) {
+    if ((rc = upload(st, &st->chi, chi.a))) return rc;
+    if ((rc = upload(st, &st->chi_init, chi.a))) return rc;
+  }
   if ((rc = upload(st, &st->alphar, alphar.a))) return rc;
   if ((rc = upload(st, &st->betar_s, bnum.a))) return rc;
   return PCF_OK;

@@ -11,7 +11,8 @@ import math
 
 from .namelists import (AgnesiHill, AtmosphereNamelist, CosineBubble, CosineHill, DiscretizationNamelist,
                         DomainNamelist, GaussianHills, GridNamelist, Namelists, OutputNamelist, PoissonNamelist,
-                        Sin2Sponge, Sin2TopSponge, SinhStretching, SpongeNamelist, Vortex)
+                        Sin2Sponge, Sin2TopSponge, SinhStretching, SpongeNamelist, TracerDipole, TracerNamelist,
+                        Vortex)
 
 
 def periodic_hill(*, x_size: int = 40, z_size: int = 40, output_steps: bool = False,
@@ -57,7 +58,8 @@ def cold_bubble(*, x_size: int = 40, z_size: int = 40) -> Namelists:
 
 
 def vortex(*, x_size: int = 40, y_size: int = 40) -> Namelists:
-    """src/Examples/vortex.jl:3-78 without the passive tracer (Boussinesq, neutral)."""
+    """src/Examples/vortex.jl:3-78 (Boussinesq, neutral) with its passive
+    tracer dipole."""
     lx, ly = 20000.0, 20000.0
     v = Vortex(5.0, lx / 4, ly / 4)
     return Namelists(
@@ -65,6 +67,7 @@ def vortex(*, x_size: int = 40, y_size: int = 40) -> Namelists:
                                       initial_u=v, initial_v=v),
         domain=DomainNamelist(x_size=x_size, y_size=y_size, lx=lx, ly=ly),
         output=OutputNamelist(prepare_restart=True),
+        tracer=TracerNamelist(tracer_setup="TracerOn", initial_tracer=TracerDipole(lx / 4, ly / 4)),
     )
 
 

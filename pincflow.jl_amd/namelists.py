@@ -73,6 +73,14 @@ class Vortex:
 
 
 @dataclass(frozen=True)
+class TracerDipole:
+    """initial_tracer of vortex.jl:50-58: sign(x) (1 + cos(pi r)) / 2 for
+    r = sqrt(((|x| - rx) / rx)^2 + (y / ry)^2) <= 1, else 0."""
+    rx: float
+    ry: float
+
+
+@dataclass(frozen=True)
 class Sin2Sponge:
     """lhs_sponge of mountain_wave.jl:40-53: alpharmax (ax + ay + az) / divisor."""
     alpharmax: float
@@ -210,6 +218,8 @@ class WKBNamelist:
 @dataclass
 class TracerNamelist:
     tracer_setup: str = "NoTracer"
+    leading_order_impact: bool = False
+    initial_tracer: Union[float, TracerDipole] = 0.0
 
 
 @dataclass
@@ -252,8 +262,12 @@ def to_config(nl: Namelists) -> _abi.PcfConfig:
         raise ValueError(f"Invalid limiter_type: {di.limiter_type!r}")
     if nl.wkb.wkb_mode != "NoWKB":
         raise ValueError("MS-GWaM (wkb_mode != NoWKB) is not part of this library")
-    if nl.tracer.tracer_setup != "NoTracer":
-        raise ValueError("tracer transport (tracer_setup != NoTracer) is not part of this library")
+    if nl.tracer.tracer_setup not in ("NoTracer", "TracerOn"):
+        raise ValueError(f"Invalid tracer_setup: {nl.tracer.tracer_setup!r}")
+    if nl.tracer.leading_order_impact:
+        raise ValueError("leading_order_impact needs MS-GWaM, which is not part of this library")
+    if not isinstance(nl.tracer.initial_tracer, TracerDipole) and nl.tracer.initial_tracer != 0.0:
+        raise ValueError("initial_tracer: 0 or a TracerDipole")
     if a.kinematic_diffusivity != 0.0:
         raise ValueError("kinematic_diffusivity != 0 is not part of this library")
     c = _abi.PcfConfig()
@@ -360,6 +374,11 @@ def to_config(nl: Namelists) -> _abi.PcfConfig:
     c.mg_coarse_steps = int(po.mg_coarse_steps)
     c.preconditioner_storage = _option({"fp32": _abi.STORAGE_FP32, "bf16": _abi.STORAGE_BF16},
                                        po.preconditioner_storage, "preconditioner_storage")
+    tr = nl.tracer
+    c.tracer_setup = _abi.TRACER_ON if tr.tracer_setup == "TracerOn" else _abi.NO_TRACER
+    if isinstance(tr.initial_tracer, TracerDipole):
+        c.initial_tracer = _abi.TRACER_DIPOLE
+        c.tracer_rx, c.tracer_ry = tr.initial_tracer.rx, tr.initial_tracer.ry
     return c
 
 

@@ -94,6 +94,9 @@ def record_datasets(nl: Namelists, get: Callable[[str], np.ndarray], time_s: flo
                          * c["thetaref"]).astype(np.float32)
     if o.prepare_restart or "pip" in o.output_variables:
         out["pip"] = get("PIP")[ii, jj, kk].astype(np.float32)
+    if nl.tracer.tracer_setup == "TracerOn":  # write_output.jl:274-286
+        rhobar = get("RHOBAR")
+        out["chi"] = (get("CHI")[ii, jj, kk] / (rhobar[ii, jj, kk] + rho[ii, jj, kk])).astype(np.float32)
     return out
 
 
@@ -140,12 +143,12 @@ def write_file(path: str, records: List[Dict[str, np.ndarray]]) -> None:
 
 def read_input(path: str, iin: int = -1):
     """read_input! (read_input!.jl:13-105) for the datasets this library
-    integrates: returns (time in s, {rhop, us, vs, wts, pip}) of record iin
-    (1-based as in the reference; -1 = the last record)."""
+    integrates: returns (time in s, {rhop, us, vs, wts, pip[, chi]}) of record
+    iin (1-based as in the reference; -1 = the last record)."""
     with np.load(path, allow_pickle=False) as data:
         idx = -1 if iin == -1 else iin - 1
         time_s = float(data["t"][idx])
-        fields = {k: np.array(data[k][..., idx]) for k in ("rhop", "us", "vs", "wts", "pip") if k in data}
+        fields = {k: np.array(data[k][..., idx]) for k in ("rhop", "us", "vs", "wts", "pip", "chi") if k in data}
     missing = {"rhop", "us", "vs", "wts", "pip"} - set(fields)
     if missing:
         raise ValueError(f"restart file {path} lacks {sorted(missing)} (write it with prepare_restart = true)")

@@ -211,8 +211,10 @@ class State:
     def restart(self, time_s: float, fields: dict):
         """Continue from a stored record (integrate.jl:168-186, read_input!.jl):
         fields are the dimensional datasets rhop, us, vs, wts and pip of
-        write_output; the interior of the predictands is replaced, rho follows
-        rhop in the pseudo-incompressible model, then the boundaries are set."""
+        write_output (and chi with a tracer); the interior of the predictands
+        is replaced, rho follows rhop in the pseudo-incompressible model, chi
+        is the dataset times rhobar + rho (read_input!.jl:79-84), then the
+        boundaries are set."""
         from . import output
 
         nl = self.namelists
@@ -230,6 +232,12 @@ class State:
                 r = self.get("RHO")
                 r[ii, jj, kk] = a[ii, jj, kk]
                 self.set("RHO", r)
+        if nl.tracer.tracer_setup == "TracerOn":
+            if "chi" not in fields:
+                raise ValueError("the restart record has no chi dataset, but the tracer is on")
+            chi, rho, rhobar = self.get("CHI"), self.get("RHO"), self.get("RHOBAR")
+            chi[ii, jj, kk] = np.asarray(fields["chi"], dtype=np.float64) * (rhobar[ii, jj, kk] + rho[ii, jj, kk])
+            self.set("CHI", chi)
         self.set_boundaries(_abi.BOUNDARY_PREDICTANDS)
         self._check(self.lib.pcf_set_time(self.h, time_s))
 

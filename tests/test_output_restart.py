@@ -39,14 +39,15 @@ def test_read_input_needs_the_restart_datasets(pcf, oracle_mod, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("example", ["periodic_hill", "mountain_wave"])
+@pytest.mark.parametrize("example", ["periodic_hill", "mountain_wave", "vortex"])
 def test_restart_continues_the_run(pcf, tmp_path, example):
     """A run restarted from its stored record tracks the uninterrupted run.
     The record holds FP32 values (as the reference's file), so the two agree
     to RESTART_RTOL, not bitwise."""
     RESTART_RTOL = 1e-4
     make = {"periodic_hill": lambda: pcf.examples.periodic_hill(x_size=24, z_size=20, output_steps=True),
-            "mountain_wave": lambda: pcf.examples.mountain_wave(x_size=12, y_size=10, z_size=9)}[example]
+            "mountain_wave": lambda: pcf.examples.mountain_wave(x_size=12, y_size=10, z_size=9),
+            "vortex": lambda: pcf.examples.vortex(x_size=12, y_size=10)}[example]
 
     def namelists(iterations):
         nl = make()
@@ -66,7 +67,8 @@ def test_restart_continues_the_run(pcf, tmp_path, example):
     rest, recs, steps_rest = pcf.integrate(nl)
     assert recs[0]["t"][0] == pytest.approx(t3, rel=1e-6)
     assert rest.time()[0] == pytest.approx(full.time()[0], rel=1e-6)
-    for f in ("RHOP", "U", "V", "W", "PIP"):
+    tracer = ("CHI",) if example == "vortex" else ()
+    for f in ("RHOP", "U", "V", "W", "PIP") + tracer:
         a, b = rest.get(f), full.get(f)
         scale = max(float(np.max(np.abs(b))), 1e-30)
         assert float(np.max(np.abs(a - b))) / scale <= RESTART_RTOL, f
