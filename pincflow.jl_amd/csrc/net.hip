@@ -98,52 +98,71 @@ __global__ void k_first(Dims d, const float* __restrict__ x, const float* __rest
 }
 
 // ---- layers 2..5: 32 -> 32 implicit GEMM on the matrix cores.  One
-// wavefront computes 32 output channels x 32 voxels consecutive in x (nx is
-// a multiple of 32).  Operand layout of v_mfma_f32_32x32x16_bf16: lane l
+// wavefront computes 32 output channels x kMidTiles tiles of 32 voxels
+// consecutive in x (nx is a multiple of 32).  Operand layout of v_mfma_f32_32x32x16_bf16: lane l
 // holds A[l % 32][8 (l / 32) + e] and B[8 (l / 32) + e][l % 32], e = 0..7,
 // and D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32], q = 0..15.
 // FWD: out = W x + bias -> z (FP32, may be null) and gelu -> a (bf16).
 // DATA: out = W x (the input gradient, with transposed tap-flipped W) -> g (FP32).
 enum { MID_FWD = 0, MID_DATA = 1 };
+constexpr int kMidTiles = 4;  // 32-voxel tiles per wavefront: each weight load feeds 4 MFMAs
 
 template <int MODE>
 __global__ void __launch_bounds__(256) k_mid(Dims d, const bf16* __restrict__ in, const bf16* __restrict__ W,
                                              const float* __restrict__ bias, float* z, bf16* a, float* g) {
   const int lane = threadIdx.x & 63;
-  const size_t tile = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;  // 32-voxel tile
-  const size_t p0 = tile * 32;
-  if (p0 >= d.n) return;
+  const size_t wave = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
+  const size_t ntiles = d.n / 32;
+  const size_t tile0 = wave * kMidTiles;
+  if (tile0 >= ntiles) return;  // uniform over the wavefront
   const int col = lane & 31, half = lane >> 5;
-  int i, j, k;
-  coords(d, p0 + col, i, j, k);
-  f16v acc = {};
+  int ci[kMidTiles], cj[kMidTiles], ck[kMidTiles];
+  bool valid[kMidTiles];
+#pragma unroll
+  for (int u = 0; u < kMidTiles; ++u) {
+    valid[u] = tile0 + u < ntiles;
+    coords(d, (tile0 + u) * 32 + col, ci[u], cj[u], ck[u]);
+  }
+  f16v acc[kMidTiles];
+#pragma unroll
+  for (int u = 0; u < kMidTiles; ++u) acc[u] = f16v{};
   for (int t = 0; t < 27; ++t) {
-    const long long q = neighbour(d, i, j, k, t);
+    long long q[kMidTiles];
+#pragma unroll
+    for (int u = 0; u < kMidTiles; ++u) q[u] = valid[u] ? neighbour(d, ci[u], cj[u], ck[u], t) : -1;
 #pragma unroll
     for (int kc = 0; kc < 2; ++kc) {
       const int c0 = 16 * kc + 8 * half;
       const bf8 av = *(const bf8*)(W + ((size_t)t * kC + col) * kC + c0);
-      bf8 bv;
-      if (q >= 0) {
-        bv = *(const bf8*)(in + (size_t)q * kC + c0);
-      } else {
+      bf8 bv[kMidTiles];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) bv[e] = (bf16)0.0f;
+      for (int u = 0; u < kMidTiles; ++u) {
+        if (q[u] >= 0) {
+          bv[u] = *(const bf8*)(in + (size_t)q[u] * kC + c0);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) bv[u][e] = (bf16)0.0f;
+        }
       }
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+#pragma unroll
+      for (int u = 0; u < kMidTiles; ++u) acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv[u], acc[u], 0, 0, 0);
     }
   }
-  const size_t p = p0 + col;
 #pragma unroll
-  for (int qi = 0; qi < 16; ++qi) {
-    const int co = 8 * (qi / 4) + 4 * half + (qi % 4);
-    float v = acc[qi];
-    if (MODE == MID_FWD) {
-      v += bias[co];
-      if (z) z[p * kC + co] = v;
-      a[p * kC + co] = (bf16)gelu(v);
-    } else {
-      g[p * kC + co] = v;
+  for (int u = 0; u < kMidTiles; ++u) {
+    if (!valid[u]) continue;
+    const size_t p = (tile0 + u) * 32 + col;
+#pragma unroll
+    for (int qi = 0; qi < 16; ++qi) {
+      const int co = 8 * (qi / 4) + 4 * half + (qi % 4);
+      float v = acc[u][qi];
+      if (MODE == MID_FWD) {
+        v += bias[co];
+        if (z) z[p * kC + co] = v;
+        a[p * kC + co] = (bf16)gelu(v);
+      } else {
+        g[p * kC + co] = v;
+      }
     }
   }
 }
@@ -206,24 +225,45 @@ __global__ void k_last_data(Dims d, const float* dout, const float* W, float* g)
   for (int c = 0; c < kC; ++c) g[p * kC + c] = acc[c];
 }
 
-// ---- layer 6 weight gradient: per chunk of voxels, W[t][c] += dout(p) in(p + tap_t)[c], b += dout(p)
-__global__ void k_last_weights(Dims d, const float* dout, const bf16* in, float* partials) {
-  const int slot = threadIdx.x;  // 0..863: (t, c); 864: the bias
+// ---- layer 6 weight gradient: per chunk of voxels, W[t][c] += dout(p) in(p + tap_t)[c], b += dout(p).
+// Thread (t, 4 channels) of 216, thread 216 the bias; the chunk's dout in LDS.
+constexpr int kLastWThreads = 27 * kC / 4 + 1;
+__global__ void __launch_bounds__(256) k_last_weights(Dims d, const float* dout, const bf16* in, float* partials) {
+  __shared__ float ld[256];
+  const int slot = threadIdx.x;
   const size_t p0 = (size_t)blockIdx.x * kChunk, p1 = p0 + kChunk < d.n ? p0 + kChunk : d.n;
-  if (slot > 27 * kC) return;
-  const int t = slot / kC, c = slot % kC;
-  float s = 0.0f;
-  for (size_t p = p0; p < p1; ++p) {
-    if (slot == 27 * kC) {
-      s += dout[p];
-      continue;
+  const int t = slot / (kC / 4), c0 = 4 * (slot % (kC / 4));
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (size_t pb = p0; pb < p1; pb += 256) {
+    const size_t pl = pb + threadIdx.x;
+    ld[threadIdx.x] = pl < p1 ? dout[pl] : 0.0f;
+    __syncthreads();
+    const int m = (int)(p1 - pb < 256 ? p1 - pb : 256);
+    if (slot < kLastWThreads - 1) {
+      for (int v = 0; v < m; ++v) {
+        const size_t p = pb + v;
+        int i, j, k;
+        coords(d, p, i, j, k);
+        const long long q = neighbour(d, i, j, k, t);
+        if (q < 0) continue;
+        const float dv = ld[v];
+        typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+        const bf4 x = *(const bf4*)(in + (size_t)q * kC + c0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s[e] += dv * (float)x[e];
+      }
+    } else if (slot == kLastWThreads - 1) {
+      for (int v = 0; v < m; ++v) s[0] += ld[v];
     }
-    int i, j, k;
-    coords(d, p, i, j, k);
-    const long long q = neighbour(d, i, j, k, t);
-    if (q >= 0) s += dout[p] * (float)in[(size_t)q * kC + c];
+    __syncthreads();
   }
-  partials[(size_t)blockIdx.x * (27 * kC + 1) + slot] = s;
+  float* out = partials + (size_t)blockIdx.x * (27 * kC + 1);
+  if (slot < kLastWThreads - 1) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) out[t * kC + c0 + e] = s[e];
+  } else if (slot == kLastWThreads - 1) {
+    out[27 * kC] = s[0];
+  }
 }
 
 // ---- GELU backward: dz = g * gelu'(z) (bf16 for the matrix cores), and per
@@ -295,28 +335,82 @@ __global__ void __launch_bounds__(64) k_mid_weights(Dims d, const bf16* __restri
   }
 }
 
-// ---- layer 1 weight gradient: W[t][co][ci] += dz(p)[co] x(p + tap_t)[ci]
-__global__ void k_first_weights(Dims d, const bf16* dz, const float* x, float* partials) {
-  const int slot = blockIdx.y * blockDim.x + threadIdx.x;  // 0..2591
-  if (slot >= 27 * kC * 3) return;
-  const int t = slot / (kC * 3), co = (slot / 3) % kC, ci = slot % 3;
+// ---- layer 1 weight gradient: W[t][co][ci] += dz(p)[co] x(p + tap_t)[ci].
+// A batch of 64 voxels of the chunk is staged in LDS (dz and the 27 x 3
+// neighbour inputs); thread (t, 4 output channels) of 216 accumulates 12
+// weights over the batch, voxel by voxel in order.
+constexpr int kFirstWBatch = 64;
+__global__ void __launch_bounds__(256) k_first_weights(Dims d, const bf16* dz, const float* x, float* partials) {
+  __shared__ float ldz[kFirstWBatch][kC];
+  __shared__ float lx[kFirstWBatch][27 * 3];
   const size_t p0 = (size_t)blockIdx.x * kChunk, p1 = p0 + kChunk < d.n ? p0 + kChunk : d.n;
-  float s = 0.0f;
-  for (size_t p = p0; p < p1; ++p) {
-    int i, j, k;
-    coords(d, p, i, j, k);
-    const long long q = neighbour(d, i, j, k, t);
-    if (q >= 0) s += (float)dz[p * kC + co] * x[3 * q + ci];
+  const int slot = threadIdx.x;
+  const int t = slot / (kC / 4), co0 = 4 * (slot % (kC / 4));
+  float s[4][3] = {};
+  for (size_t pb = p0; pb < p1; pb += kFirstWBatch) {
+    const int m = (int)(p1 - pb < kFirstWBatch ? p1 - pb : kFirstWBatch);
+    for (int e = threadIdx.x; e < kFirstWBatch * kC; e += blockDim.x) {
+      const int v = e / kC, c = e % kC;
+      ldz[v][c] = v < m ? (float)dz[(pb + v) * kC + c] : 0.0f;
+    }
+    for (int e = threadIdx.x; e < kFirstWBatch * 27; e += blockDim.x) {
+      const int v = e / 27, tt = e % 27;
+      float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f;
+      if (v < m) {
+        int i, j, k;
+        coords(d, pb + v, i, j, k);
+        const long long q = neighbour(d, i, j, k, tt);
+        if (q >= 0) {
+          x0 = x[3 * q];
+          x1 = x[3 * q + 1];
+          x2 = x[3 * q + 2];
+        }
+      }
+      lx[v][3 * tt] = x0;
+      lx[v][3 * tt + 1] = x1;
+      lx[v][3 * tt + 2] = x2;
+    }
+    __syncthreads();
+    if (slot < 27 * kC / 4) {
+      for (int v = 0; v < m; ++v) {
+        const float x0 = lx[v][3 * t], x1 = lx[v][3 * t + 1], x2 = lx[v][3 * t + 2];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const float dv = ldz[v][co0 + c];
+          s[c][0] += dv * x0;
+          s[c][1] += dv * x1;
+          s[c][2] += dv * x2;
+        }
+      }
+    }
+    __syncthreads();
   }
-  partials[(size_t)blockIdx.x * (27 * kC * 3) + slot] = s;
+  if (slot < 27 * kC / 4) {
+    float* out = partials + (size_t)blockIdx.x * (27 * kC * 3) + (size_t)t * kC * 3;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int ci = 0; ci < 3; ++ci) out[(co0 + c) * 3 + ci] = s[c][ci];
+  }
 }
 
-// sum over chunks, in chunk order: out[s] = sum_c partials[c][s]
-__global__ void k_sum_chunks(const float* partials, int nchunks, int size, float* out) {
+// sum over chunks in a fixed order, in two levels: groups of kSumGroup
+// consecutive chunks (blockIdx.y) into group sums, then the group sums in
+// group order (k_sum_groups): out[s] = sum_g sum_{c in g} partials[c][s]
+constexpr int kSumGroup = 64;
+__global__ void k_sum_chunks(const float* partials, int nchunks, int size, float* groups) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= size) return;
+  const int c0 = blockIdx.y * kSumGroup, c1 = c0 + kSumGroup < nchunks ? c0 + kSumGroup : nchunks;
+  float acc = 0.0f;
+  for (int c = c0; c < c1; ++c) acc += partials[(size_t)c * size + s];
+  groups[(size_t)blockIdx.y * size + s] = acc;
+}
+__global__ void k_sum_groups(const float* groups, int ngroups, int size, float* out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= size) return;
   float acc = 0.0f;
-  for (int c = 0; c < nchunks; ++c) acc += partials[(size_t)c * size + s];
+  for (int c = 0; c < ngroups; ++c) acc += groups[(size_t)c * size + s];
   out[s] = acc;
 }
 
@@ -384,6 +478,7 @@ struct pcf_net {
   float *dout = nullptr, *g = nullptr;
   bf16* dzb = nullptr;
   float* partials = nullptr;
+  float* groups = nullptr;  // group sums of the chunk partials
   double* loss_partials = nullptr;
   double* loss_host = nullptr;
   int step = 0;
@@ -402,11 +497,24 @@ int net_alloc(pcf_net* net, void** p, size_t bytes) {
 int nchunks(const pcf_net* net) { return (int)((net->d.n + kChunk - 1) / kChunk); }
 
 // forward pass over the FP32 input (n x 3); output in net->out
+// blocks of 4 wavefronts for the k_mid launches
+unsigned mid_blocks(const Dims& d) {
+  const size_t waves = (d.n / 32 + kMidTiles - 1) / kMidTiles;
+  return (unsigned)((waves + 3) / 4);
+}
+
+// out[0..size) = sum over the nc chunk partials (fixed order, two levels)
+void sum_chunks(pcf_net* net, int nc, int size, float* out) {
+  const unsigned ng = (unsigned)((nc + kSumGroup - 1) / kSumGroup);
+  const unsigned blocks = (unsigned)((size + 255) / 256);
+  hipLaunchKernelGGL(k_sum_chunks, dim3(blocks, ng), dim3(256), 0, net->stream, net->partials, nc, size, net->groups);
+  hipLaunchKernelGGL(k_sum_groups, dim3(blocks), dim3(256), 0, net->stream, net->groups, (int)ng, size, out);
+}
+
 int net_forward(pcf_net* net, const float* input, hipStream_t s) {
   const Dims& d = net->d;
   const unsigned vb = 256, vg = (unsigned)((d.n + vb - 1) / vb);
-  const unsigned tiles = (unsigned)((d.n + 31) / 32);
-  const unsigned mg = (tiles + 3) / 4;  // 4 wavefronts per block
+  const unsigned mg = mid_blocks(d);
   const float* P = net->params;
   bf16* cur = net->a[0];
   hipLaunchKernelGGL(k_first, dim3(vg), dim3(vb), 0, s, d, input, P + layer_offset(0),
@@ -428,36 +536,30 @@ int net_backward(pcf_net* net, const float* input, const float* target, double* 
   const Dims& d = net->d;
   hipStream_t s = net->stream;
   const unsigned vb = 256, vg = (unsigned)((d.n + vb - 1) / vb);
-  const unsigned tiles = (unsigned)((d.n + 31) / 32);
-  const unsigned mg = (tiles + 3) / 4;
+  const unsigned mg = mid_blocks(d);
   const int nc = nchunks(net);
   const float* P = net->params;
   float* G = net->grads;
   const unsigned lg = 1024;
   hipLaunchKernelGGL(k_loss, dim3(lg), dim3(256), 0, s, d, net->out, target, net->dout, net->loss_partials);
   // layer 6
-  hipLaunchKernelGGL(k_last_weights, dim3(nc), dim3(27 * kC + 1 + 31), 0, s, d, net->dout, net->a[4], net->partials);
-  hipLaunchKernelGGL(k_sum_chunks, dim3((27 * kC + 1 + 255) / 256), dim3(256), 0, s, net->partials, nc, 27 * kC + 1,
-                     G + layer_offset(5));
+  hipLaunchKernelGGL(k_last_weights, dim3(nc), dim3(256), 0, s, d, net->dout, net->a[4], net->partials);
+  sum_chunks(net, nc, 27 * kC + 1, G + layer_offset(5));
   hipLaunchKernelGGL(k_last_data, dim3(vg), dim3(vb), 0, s, d, net->dout, P + layer_offset(5), net->g);
   // layers 5..2
   for (int l = 4; l >= 1; --l) {
     hipLaunchKernelGGL(k_gelu_back, dim3(nc), dim3(256), 0, s, d, net->g, net->z[l], net->dzb, net->partials);
-    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(kC), 0, s, net->partials, nc, kC,
-                       G + layer_offset(l) + 27 * kC * kC);
+    sum_chunks(net, nc, kC, G + layer_offset(l) + 27 * kC * kC);
     hipLaunchKernelGGL(k_mid_weights, dim3(nc, 27), dim3(64), 0, s, d, net->dzb, net->a[l - 1], net->partials);
-    hipLaunchKernelGGL(k_sum_chunks, dim3((27 * kC * kC + 255) / 256), dim3(256), 0, s, net->partials, nc,
-                       27 * kC * kC, G + layer_offset(l));
+    sum_chunks(net, nc, 27 * kC * kC, G + layer_offset(l));
     hipLaunchKernelGGL(k_mid<MID_DATA>, dim3(mg), dim3(256), 0, s, d, net->dzb,
                        net->wt16 + (size_t)(l - 1) * 27 * kC * kC, nullptr, nullptr, nullptr, net->g);
   }
   // layer 1
   hipLaunchKernelGGL(k_gelu_back, dim3(nc), dim3(256), 0, s, d, net->g, net->z[0], net->dzb, net->partials);
-  hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(kC), 0, s, net->partials, nc, kC, G + layer_offset(0) + 27 * kC * 3);
-  hipLaunchKernelGGL(k_first_weights, dim3(nc, (27 * kC * 3 + 255) / 256), dim3(256), 0, s, d, net->dzb, input,
-                     net->partials);
-  hipLaunchKernelGGL(k_sum_chunks, dim3((27 * kC * 3 + 255) / 256), dim3(256), 0, s, net->partials, nc, 27 * kC * 3,
-                     G + layer_offset(0));
+  sum_chunks(net, nc, kC, G + layer_offset(0) + 27 * kC * 3);
+  hipLaunchKernelGGL(k_first_weights, dim3(nc), dim3(256), 0, s, d, net->dzb, input, net->partials);
+  sum_chunks(net, nc, 27 * kC * 3, G + layer_offset(0));
   PCF_HIP(hipGetLastError());
   std::vector<double> lp(lg);
   PCF_HIP(hipMemcpyAsync(lp.data(), net->loss_partials, lg * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -591,6 +693,8 @@ int pcf_net_create(int32_t nx, int32_t ny, int32_t nz, int32_t training, pcf_net
     if ((rc = net_alloc(net, (void**)&net->dzb, N * kC * sizeof(bf16)))) return fail(rc);
     const size_t part = (size_t)nchunks(net) * 27 * kC * kC;
     if ((rc = net_alloc(net, (void**)&net->partials, part * sizeof(float)))) return fail(rc);
+    const size_t ngroups = (size_t)(nchunks(net) + kSumGroup - 1) / kSumGroup;
+    if ((rc = net_alloc(net, (void**)&net->groups, ngroups * 27 * kC * kC * sizeof(float)))) return fail(rc);
     if ((rc = net_alloc(net, (void**)&net->loss_partials, 1024 * sizeof(double)))) return fail(rc);
     if (hipMemsetAsync(net->adam_m, 0, kParams * sizeof(float), net->stream) != hipSuccess) return fail(PCF_E_HIP);
     if (hipMemsetAsync(net->adam_v, 0, kParams * sizeof(float), net->stream) != hipSuccess) return fail(PCF_E_HIP);
