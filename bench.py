@@ -207,6 +207,17 @@ def main():
                 entry["guess_fallbacks"] = r["state"].guess_fallbacks()
             if guess == "learned":
                 entry["learned_training"] = vtrain
+                net = r["breakdown"].get("net")
+                if net:
+                    # 2 x 27 taps x (3 x 32 + 4 x 32 x 32 + 32 x 1) multiply-adds per voxel
+                    flop = 2 * 27 * (3 * 32 + 4 * 32 * 32 + 32) * n_cells
+                    t = net[1] / net[0] * 1e-3
+                    entry["network_roofline"] = {
+                        "bound": "mfma", "kernel": "network forward (32-channel layers on bf16 MFMA, first and "
+                                                   "last layers on the vector units)",
+                        "achieved": round(flop / t / 1e12, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": round(flop / t / 1e12 / 2500.0, 4), "flop_per_forward": flop,
+                        "avg_forward_ms": round(net[1] / net[0], 3)}
             r["state"].close()
             variants.append(entry)
     # the strict criterion |r| / |b| <= 1e-8 (tolerance_is_relative = true)

@@ -378,6 +378,7 @@ enum pcf_kernel_group {
   PCF_K_BICG_VECTOR,         /* BiCGSTAB vector updates and reductions     */
   PCF_K_FLUXES,              /* reconstruction, stress and flux kernels    */
   PCF_K_UPDATES,             /* RK3 and RHS updates, sponges, boundaries   */
+  PCF_K_NET,                 /* the learned guess: input, network, output  */
   PCF_K_COUNT
 };
 /* Record HIP events around every launch of the groups above (on = 1) or stop

@@ -607,6 +607,7 @@ int learned_guess(pcf_state* st, double dt, const double* prev, const double** x
   double bnorm = 0.0;
   int rc = guess_input(st, st->rhs, prev, dt, st->guess_in, &bnorm);
   if (rc) return rc;
+  ProfScope prof(st, PCF_K_NET);
   if ((rc = net_forward(net, st->guess_in, st->stream))) return rc;
   hipLaunchKernelGGL(k_guess_output, dim3(2048), dim3(256), 0, st->stream, net->out, st->guess_x, st->g.nv(), bnorm,
                      prev, dt);
