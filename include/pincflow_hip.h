@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCF_ABI_VERSION 4
+#define PCF_ABI_VERSION 5
 
 /* ---- return codes ------------------------------------------------------ */
 #define PCF_OK 0
@@ -45,7 +45,8 @@ extern "C" {
 /* ---- namelist options (Types/NamelistTypes) ---------------------------- */
 enum pcf_model {
   PCF_BOUSSINESQ = 0,
-  PCF_PSEUDO_INCOMPRESSIBLE = 1
+  PCF_PSEUDO_INCOMPRESSIBLE = 1,
+  PCF_COMPRESSIBLE = 2 /* P predicted (set_p.jl), pbar follows it */
 };
 
 enum pcf_background {
@@ -223,6 +224,9 @@ typedef struct pcf_config {
   int32_t tracer_setup;                   /* enum pcf_tracer_setup        */
   int32_t initial_tracer;                 /* enum pcf_initial_tracer      */
   double tracer_rx, tracer_ry;            /* m, DIPOLE radii              */
+  /* AtmosphereNamelist.thermal_conductivity (AtmosphereNamelist.jl:113),
+   * m^2/s at the surface; heat conduction of the compressible model */
+  double thermal_conductivity;
 } pcf_config;
 
 /* Fields reachable through pcf_get_field / pcf_set_field (haloed, FP64). */
@@ -245,6 +249,11 @@ enum pcf_field {
    * chi0 backup of integrate.jl:276 and TracerFluxes) */
   PCF_F_CHI, PCF_F_DCHI, PCF_F_INITIALTRACER, PCF_F_CHI0,
   PCF_F_PHICHI_X, PCF_F_PHICHI_Y, PCF_F_PHICHI_Z,
+  /* compressible model: P (Predictands.jl:67), its increment, its p0 copy,
+   * the P fluxes and the heat-conduction fluxes phitheta (Fluxes.jl:58-66) */
+  PCF_F_P, PCF_F_DP, PCF_F_P0_P,
+  PCF_F_PHIP_X, PCF_F_PHIP_Y, PCF_F_PHIP_Z,
+  PCF_F_PHITHETA_X, PCF_F_PHITHETA_Y, PCF_F_PHITHETA_Z,
   PCF_F_COUNT
 };
 
@@ -262,7 +271,7 @@ enum pcf_vector {
 enum pcf_boundaries { PCF_BOUNDARY_PREDICTANDS = 0, PCF_BOUNDARY_FLUXES = 2 };
 
 /* Variables of update! / apply_lhs_sponge! (Types.jl singletons). */
-enum pcf_variable { PCF_VAR_RHO = 0, PCF_VAR_RHOP, PCF_VAR_U, PCF_VAR_V, PCF_VAR_W, PCF_VAR_PIP, PCF_VAR_CHI };
+enum pcf_variable { PCF_VAR_RHO = 0, PCF_VAR_RHOP, PCF_VAR_U, PCF_VAR_V, PCF_VAR_W, PCF_VAR_PIP, PCF_VAR_CHI, PCF_VAR_P };
 
 /* Operator kinds of apply_operator! (apply_operator!.jl:48 / :275). */
 enum pcf_operator { PCF_OP_TOTAL = 0, PCF_OP_HORIZONTAL = 1 };
@@ -330,6 +339,16 @@ int pcf_set_boundaries(pcf_state* st, int which);
  * set_boundaries!(state, BoundaryFluxes()) (explicit_integration!.jl:53-58);
  * the MUSCL reconstruction is evaluated inside the flux kernels. */
 int pcf_compute_fluxes(pcf_state* st);
+/* compute_fluxes!(state, p0, Theta()) (compute_fluxes!.jl:1592-1756): the
+ * heat-conduction fluxes of the compressible model from the p0 copy; nothing
+ * in the other models or with thermal_conductivity = 0. */
+int pcf_compute_theta_fluxes(pcf_state* st);
+/* modify_compressible_wind!(state, *) (multiply = 1) or (state, /) (0),
+ * modify_compressible_wind!.jl:52-80; nothing outside the compressible model. */
+int pcf_modify_compressible_wind(pcf_state* st, int multiply);
+/* synchronize_compressible_atmosphere!(state, state.variables.predictands)
+ * (synchronize_compressible_atmosphere!.jl:61-72): pbar := P. */
+int pcf_synchronize_compressible_atmosphere(pcf_state* st);
 
 /* ---- updates (Update/) --------------------------------------------------- */
 /* update!(state, dt, m, var[, LHS()]) (update!.jl:505-1192), m in 1..3. */

@@ -72,6 +72,9 @@ def lib() -> ctypes.CDLL:
             "orc_backup_predictands": (None, [P]),
             "orc_compute_fluxes": (None, [P]),
             "orc_save_backups": (None, [P]),
+            "orc_compute_theta_fluxes": (None, [P]),
+            "orc_modify_compressible_wind": (None, [P, I]),
+            "orc_synchronize_compressible_atmosphere": (None, [P]),
             "orc_update_lhs": (None, [P, D, I, I]),
             "orc_update_rhs": (None, [P, D, I, I, D]),
             "orc_apply_lhs_sponge": (None, [P, D, D, I]),
@@ -220,6 +223,16 @@ class Oracle:
 
     def save_backups(self):
         lib().orc_save_backups(self.h)
+
+    def compute_theta_fluxes(self):
+        """compute_fluxes!(state, p0, Theta()) with the p0 copy."""
+        lib().orc_compute_theta_fluxes(self.h)
+
+    def modify_compressible_wind(self, multiply: bool):
+        lib().orc_modify_compressible_wind(self.h, int(multiply))
+
+    def synchronize_compressible_atmosphere(self):
+        lib().orc_synchronize_compressible_atmosphere(self.h)
 
     def update_lhs(self, dt: float, m: int, var: int):
         lib().orc_update_lhs(self.h, dt, m, var)

@@ -1,7 +1,7 @@
 // ORACLE -- test infrastructure only.
 //
 // CPU FP64 restatement of PinCFlow.jl's semi-implicit dynamical-core step
-// (Boussinesq and pseudo-incompressible, npx = npy = npz = 1).  It exists to
+// (Boussinesq, pseudo-incompressible and compressible, npx = npy = npz = 1).  It exists to
 // check the HIP library and to time the CPU baseline; nothing in the product
 // (pincflow.jl_amd/) includes, links or calls it.  Only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
@@ -53,8 +53,8 @@ struct Arr5 {
   double& operator()(int i, int j, int k, int d, int s) { return c[d - 1][s - 1](i, j, k); }
 };
 
-struct Predictands {  // Predictands.jl:60-68 (p unused outside compressible)
-  Arr3 rho, rhop, u, v, w, pip;
+struct Predictands {  // Predictands.jl:60-68 (p is empty outside compressible, set_p.jl)
+  Arr3 rho, rhop, u, v, w, pip, p;
 };
 
 struct Constants {  // Constants.jl:108-182
@@ -90,6 +90,10 @@ struct State {
   Arr3 phi;                                  // Auxiliaries.jl
   Arr5 rhotilde, rhoptilde, utilde, vtilde, wtilde;
   Arr4 phirho, phirhop, phiu, phiv, phiw;
+  // compressible model: P increment and the P / heat-conduction fluxes
+  bool compressible = false;
+  Arr3 dp;
+  Arr4 phip, phitheta;
   // Poisson.jl
   Arr3 rhs, solution;                        // interior
   Arr3 tensor[25];                           // interior, order of enum pcf_vector - 2
@@ -149,6 +153,11 @@ double dot(const Arr3& a, const Arr3& b);
 double sumsq(const Arr3& a);
 
 // ---- pointwise helpers (orc_update.cpp) ----
+double wind_factor_u(State& st, int i, int j, int k);
+double wind_factor_v(State& st, int i, int j, int k);
+double wind_factor_w(State& st, int i, int j, int k);
+double conductive_heating(State& st, int i, int j, int k);
+double dpdpi(State& st, double p);
 double buoyancy_factor_rhop(State& st, int i, int j, int k);
 double buoyancy_factor_w(State& st, int i, int j, int k);
 double pressure_gradient_u(State& st, Arr3& pip, int i, int j, int k);
@@ -170,6 +179,7 @@ void apply_lhs_sponge(State& st, double dt, double time, int var);
 // ---- advection (orc_flux.cpp) ----
 void reconstruct(State& st);
 void compute_fluxes(State& st, Predictands& p0);
+void compute_theta_fluxes(State& st, Predictands& p0);
 
 // ---- integration (orc_integrate.cpp) ----
 int compute_time_step(State& st, double& dt);
@@ -177,5 +187,7 @@ int explicit_integration_lhs(State& st, Predictands& p0, double dt, double time)
 int explicit_integration_rhs(State& st, double dt, double time);
 int implicit_integration(State& st, double dt, double time, double rf, int& niter);
 void copy_predictands(Predictands& dst, const Predictands& src);
+void modify_compressible_wind(State& st, bool multiply);
+void synchronize_compressible_atmosphere(State& st, const Predictands& p);
 
 }  // namespace orc

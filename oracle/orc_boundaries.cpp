@@ -87,6 +87,7 @@ void set_boundaries_predictands(State& st) {
   std::vector<Arr3*> fields;
   if (pi) fields.push_back(&p.rho);
   for (Arr3* a : {&p.rhop, &p.u, &p.v, &p.w, &p.pip}) fields.push_back(a);
+  if (st.compressible) fields.push_back(&p.p);
   for (Arr3* a : fields) zonal_bc(st, *a);
   for (Arr3* a : fields) meridional_bc(st, *a);
   if (pi) vertical_bc(st, p.rho, -1);
@@ -95,6 +96,7 @@ void set_boundaries_predictands(State& st) {
   vertical_bc(st, p.v, +1);
   vertical_bc(st, p.w, -1, true);
   vertical_bc(st, p.pip, +1);
+  if (st.compressible) vertical_bc(st, p.p, +1);  // set_vertical_boundaries!.jl:217-234
   if (st.tracer) {  // set_tracer_*_boundaries!.jl, BoundaryPredictands
     zonal_bc(st, st.chi);
     meridional_bc(st, st.chi);
@@ -130,6 +132,10 @@ void set_boundaries_fluxes(State& st) {
         st.phiv(i, j, st.k0 - 1, 3) = 0.0;
         st.phiw(i, j, st.k0 - 2, 3) = 0.0;
         if (st.tracer) st.phichi(i, j, st.k0 - 1, 3) = 0.0;
+        if (st.compressible) {  // set_vertical_boundaries!.jl:324-347
+          st.phitheta(i, j, st.k0 - 1, 3) = 0.0;
+          st.phip(i, j, st.k0 - 1, 3) = 0.0;
+        }
       }
       if (st.ko + st.nz == st.cfg.z_size) {
         if (pi) st.phirho(i, j, st.k1, 3) = 0.0;
@@ -138,6 +144,10 @@ void set_boundaries_fluxes(State& st) {
         st.phiv(i, j, st.k1, 3) = 0.0;
         st.phiw(i, j, st.k1, 3) = 0.0;
         if (st.tracer) st.phichi(i, j, st.k1, 3) = 0.0;
+        if (st.compressible) {
+          st.phitheta(i, j, st.k1, 3) = 0.0;
+          st.phip(i, j, st.k1, 3) = 0.0;
+        }
       }
     }
 }

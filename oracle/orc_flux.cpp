@@ -123,7 +123,7 @@ static inline double upwind(double usurf, double phiup, double phidown) {
   return usurf > 0.0 ? usurf * phiup : usurf * phidown;
 }
 
-// compute_fluxes!.jl:326-1503 (Boussinesq / PI; diffusion off)
+// compute_fluxes!.jl:326-1503 (diffusion off)
 void compute_fluxes(State& st, Predictands& p0) {
   const int i0 = st.i0, i1 = st.i1, j0 = st.j0, j1 = st.j1, k0 = st.k0, k1 = st.k1;
   Arr3 &jac = st.jac, &pbar = st.pbar, &rb = st.rhobar, &m13 = st.met13, &m23 = st.met23;
@@ -192,6 +192,27 @@ void compute_fluxes(State& st, Predictands& p0) {
           double pedgeu = jc * ju * (pbar(i, j, k) + pbar(i, j, k + 1)) / (jc + ju);
           f(i, j, k, 3) = upwind(pedgeu * w0(i, j, k), dn, up);
         }
+  }
+
+  // P (compute_fluxes!.jl:531-585, compressible only): the carrier flux J P u
+  if (st.compressible) {
+    Arr4& f = st.phip;
+#pragma omp parallel for schedule(static)
+    for (int k = k0; k <= k1; ++k)
+      for (int j = j0; j <= j1; ++j)
+        for (int i = i0 - 1; i <= i1; ++i)
+          f(i, j, k, 1) = 0.5 * (jac(i, j, k) * pbar(i, j, k) + jac(i + 1, j, k) * pbar(i + 1, j, k)) * u0(i, j, k);
+#pragma omp parallel for schedule(static)
+    for (int k = k0; k <= k1; ++k)
+      for (int j = j0 - 1; j <= j1; ++j)
+        for (int i = i0; i <= i1; ++i)
+          f(i, j, k, 2) = 0.5 * (jac(i, j, k) * pbar(i, j, k) + jac(i, j + 1, k) * pbar(i, j + 1, k)) * v0(i, j, k);
+#pragma omp parallel for schedule(static)
+    for (int k = k0 - 1; k <= k1; ++k)
+      for (int j = j0; j <= j1; ++j)
+        for (int i = i0; i <= i1; ++i)
+          f(i, j, k, 3) = jac(i, j, k) * jac(i, j, k + 1) * (pbar(i, j, k) + pbar(i, j, k + 1)) /
+                          (jac(i, j, k) + jac(i, j, k + 1)) * w0(i, j, k);
   }
 
   const int kmin = k0, kmax = st.ko + st.nz == st.cfg.z_size ? k1 : k1 + 1;
@@ -384,6 +405,75 @@ void compute_fluxes(State& st, Predictands& p0) {
           }
     }
   }
+}
+
+// compute_fluxes!(state, p0, Theta()) (compute_fluxes!.jl:1592-1756):
+// heat conduction of the compressible model, theta = pbar / (rho0 + rhobar)
+// with rho0 of the step's p0; nothing when thermal_conductivity = 0.
+void compute_theta_fluxes(State& st, Predictands& p0) {
+  if (!st.compressible || st.cfg.thermal_conductivity == 0.0) return;
+  const int i0 = st.i0, i1 = st.i1, j0 = st.j0, j1 = st.j1, k0 = st.k0, k1 = st.k1;
+  const double dx = st.dx, dy = st.dy, dz = st.dz;
+  Arr3 &jac = st.jac, &pbar = st.pbar, &rb = st.rhobar, &rho = p0.rho;
+  Arr3 &m13 = st.met13, &m23 = st.met23, &m33 = st.met33;
+  Arr4& f = st.phitheta;
+  const double mu_conduct = st.cfg.thermal_conductivity / st.c.uref / st.c.lref;
+  auto th = [&](int i, int j, int k) { return pbar(i, j, k) / (rho(i, j, k) + rb(i, j, k)); };
+#pragma omp parallel for schedule(static)
+  for (int k = k0; k <= k1; ++k)
+    for (int j = j0; j <= j1; ++j)
+      for (int i = i0 - 1; i <= i1; ++i) {
+        double coef_t = mu_conduct * 0.5 * (rb(i, j, k0) / rb(i, j, k) + rb(i + 1, j, k0) / rb(i + 1, j, k));
+        double thetal = th(i, j, k);
+        double thetar = th(i + 1, j, k);
+        double thetad = 0.5 * (th(i, j, k - 1) + th(i + 1, j, k - 1));
+        double thetau = 0.5 * (th(i, j, k + 1) + th(i + 1, j, k + 1));
+        double dtht_dxi = 0.5 * (jac(i, j, k) + jac(i + 1, j, k)) * (thetar - thetal) / dx +
+                          0.5 * (jac(i, j, k) * m13(i, j, k) + jac(i + 1, j, k) * m13(i + 1, j, k)) *
+                              (thetau - thetad) / (2.0 * dz);
+        f(i, j, k, 1) = -coef_t * dtht_dxi;
+      }
+#pragma omp parallel for schedule(static)
+  for (int k = k0; k <= k1; ++k)
+    for (int j = j0 - 1; j <= j1; ++j)
+      for (int i = i0; i <= i1; ++i) {
+        double coef_t = mu_conduct * 0.5 * (rb(i, j, k0) / rb(i, j, k) + rb(i, j + 1, k0) / rb(i, j + 1, k));
+        double thetab = th(i, j, k);
+        double thetaf = th(i, j + 1, k);
+        double thetad = 0.5 * (th(i, j, k - 1) + th(i, j + 1, k - 1));
+        double thetau = 0.5 * (th(i, j, k + 1) + th(i, j + 1, k + 1));
+        double dtht_dyi = 0.5 * (jac(i, j, k) + jac(i, j + 1, k)) * (thetaf - thetab) / dy +
+                          0.5 * (jac(i, j, k) * m23(i, j, k) + jac(i, j + 1, k) * m23(i, j + 1, k)) *
+                              (thetau - thetad) / (2.0 * dz);
+        f(i, j, k, 2) = -coef_t * dtht_dyi;
+      }
+  // the vertical coefficient takes rhobar of the lowest halo layer (index 1),
+  // the horizontal ones that of k0, as the reference does
+#pragma omp parallel for schedule(static)
+  for (int k = k0 - 1; k <= k1; ++k)
+    for (int j = j0; j <= j1; ++j)
+      for (int i = i0; i <= i1; ++i) {
+        const double jc = jac(i, j, k), ju = jac(i, j, k + 1);
+        double coef_t = mu_conduct * (ju * rb(i, j, 1) / rb(i, j, k) + jc * rb(i, j, 1) / rb(i, j, k + 1)) / (ju + jc);
+        // (J pbar) / (rho + rhobar) at k and k + 1, Jacobian-weighted to k + 1/2
+        auto edge = [&](int ii, int jj) {
+          return (jac(ii, jj, k + 1) * pbar(ii, jj, k) / (rho(ii, jj, k) + rb(ii, jj, k)) +
+                  jac(ii, jj, k) * pbar(ii, jj, k + 1) / (rho(ii, jj, k + 1) + rb(ii, jj, k + 1))) /
+                 (jac(ii, jj, k + 1) + jac(ii, jj, k));
+        };
+        double thetal = edge(i - 1, j);
+        double thetar = edge(i + 1, j);
+        double thetab = edge(i, j - 1);
+        double thetaf = edge(i, j + 1);
+        double thetad = th(i, j, k);
+        double thetau = th(i, j, k + 1);
+        double dtht_dzi = jc * ju *
+                          ((m13(i, j, k) + m13(i, j, k + 1)) * (thetar - thetal) / (2.0 * dx) +
+                           (m23(i, j, k) + m23(i, j, k + 1)) * (thetaf - thetab) / (2.0 * dy) +
+                           (m33(i, j, k) + m33(i, j, k + 1)) * (thetau - thetad) / dz) /
+                          (jc + ju);
+        f(i, j, k, 3) = -coef_t * dtht_dzi;
+      }
 }
 
 }  // namespace orc

@@ -16,6 +16,28 @@ void copy_predictands(Predictands& d, const Predictands& s) {
   d.v.a = s.v.a;
   d.w.a = s.w.a;
   d.pip.a = s.pip.a;
+  d.p.a = s.p.a;
+}
+
+// modify_compressible_wind!.jl:52-80: the winds times (multiply) or divided
+// by J P interpolated to their points; nothing outside the compressible model
+void modify_compressible_wind(State& st, bool multiply) {
+  if (!st.compressible) return;
+  Arr3 &u = st.pr.u, &v = st.pr.v, &w = st.pr.w, &p = st.pr.p, &jac = st.jac;
+  auto op = [multiply](double a, double b) { return multiply ? a * b : a / b; };
+  for (int k = st.k0; k <= st.k1; ++k)
+    for (int j = st.j0; j <= st.j1; ++j)
+      for (int i = st.i0; i <= st.i1; ++i) {
+        u(i, j, k) = op(u(i, j, k), (jac(i, j, k) * p(i, j, k) + jac(i + 1, j, k) * p(i + 1, j, k)) / 2);
+        v(i, j, k) = op(v(i, j, k), (jac(i, j, k) * p(i, j, k) + jac(i, j + 1, k) * p(i, j + 1, k)) / 2);
+        w(i, j, k) = op(w(i, j, k), jac(i, j, k) * jac(i, j, k + 1) * (p(i, j, k) + p(i, j, k + 1)) /
+                                        (jac(i, j, k) + jac(i, j, k + 1)));
+      }
+}
+
+// synchronize_compressible_atmosphere!.jl:61-72: pbar .= p (compressible only)
+void synchronize_compressible_atmosphere(State& st, const Predictands& p) {
+  if (st.compressible) st.pbar.a = p.p.a;
 }
 
 // compute_time_step.jl:46-153 (NoWKB)
@@ -68,6 +90,8 @@ int explicit_integration_lhs(State& st, Predictands& p0, double dt, double time)
     apply_lhs_sponge(st, sdt, time, PCF_VAR_RHO);
     update_lhs(st, dt, m, PCF_VAR_RHOP);
     apply_lhs_sponge(st, sdt, time, PCF_VAR_RHOP);
+    update_lhs(st, dt, m, PCF_VAR_P);  // no-ops outside the compressible model
+    apply_lhs_sponge(st, sdt, time, PCF_VAR_P);
     update_lhs(st, dt, m, PCF_VAR_CHI);  // no-ops without a tracer
     apply_lhs_sponge(st, sdt, time, PCF_VAR_CHI);
     set_boundaries_predictands(st);
@@ -79,18 +103,24 @@ int explicit_integration_lhs(State& st, Predictands& p0, double dt, double time)
     apply_lhs_sponge(st, sdt, time, PCF_VAR_W);
     set_boundaries_predictands(st);
   }
+  synchronize_compressible_atmosphere(st, st.pr);
+  apply_lhs_sponge(st, dt, time, PCF_VAR_PIP);
   return PCF_OK;
 }
 
 // explicit_integration!.jl:103-131
 int explicit_integration_rhs(State& st, double dt, double time) {
   (void)time;
+  synchronize_compressible_atmosphere(st, st.pr);
+  modify_compressible_wind(st, true);
   set_boundaries_predictands(st);
   save_backups(st, false, true, true, true, true);
   update_rhs(st, dt, PCF_VAR_RHOP, false, 0.0);
   update_rhs(st, dt, PCF_VAR_U, false, 0.0);
   update_rhs(st, dt, PCF_VAR_V, false, 0.0);
   update_rhs(st, dt, PCF_VAR_W, false, 0.0);
+  update_rhs(st, dt, PCF_VAR_PIP, false, 0.0);
+  modify_compressible_wind(st, false);
   set_boundaries_predictands(st);
   return PCF_OK;
 }
@@ -98,6 +128,7 @@ int explicit_integration_rhs(State& st, double dt, double time) {
 // implicit_integration!.jl:37-79
 int implicit_integration(State& st, double dt, double time, double rf, int& niter) {
   (void)time;
+  modify_compressible_wind(st, true);
   set_boundaries_predictands(st);
   save_backups(st, false, false, false, false, true);
   update_rhs(st, dt, PCF_VAR_U, true, rf);
@@ -108,6 +139,7 @@ int implicit_integration(State& st, double dt, double time, double rf, int& nite
   set_boundaries_predictands(st);
   int rc = apply_corrector(st, dt, rf, niter);
   if (rc) return rc;
+  modify_compressible_wind(st, false);
   set_boundaries_predictands(st);
   return PCF_OK;
 }
@@ -152,11 +184,13 @@ void orc_dims(State* st, int32_t* d) {
 
 // integrate.jl:136-156
 int orc_initial_cleaning(State* st, int32_t* niter) {
+  modify_compressible_wind(*st, true);
   set_boundaries_predictands(*st);
   int it = 0;
   int rc = apply_corrector(*st, 1.0, 1.0, it);
   *niter = it;
   if (rc) return rc;
+  modify_compressible_wind(*st, false);
   set_boundaries_predictands(*st);
   return PCF_OK;
 }
@@ -178,11 +212,15 @@ int orc_step(State* st, orc_step_info* info) {
   st->time += dt;
   const double time = st->time;
   compute_sponges(*st, dt, time);
-  // synchronize_density_fluctuations! (PI: rhop .= rho)
+  // synchronize_density_fluctuations! (synchronize_density_fluctuations!.jl:65-86)
   if (n.model == PCF_PSEUDO_INCOMPRESSIBLE) st->pr.rhop.a = st->pr.rho.a;
+  if (n.model == PCF_COMPRESSIBLE)
+    for (size_t q = 0; q < st->pr.rhop.a.size(); ++q)
+      st->pr.rhop.a[q] = st->pr.rho.a[q] + st->rhobar.a[q] - st->pbar.a[q] / st->thetabar.a[q];
   set_boundaries_predictands(*st);
   copy_predictands(st->p0, st->pr);
   if (st->tracer) st->chi0.a = st->chi.a;  // backup_predictands.jl:18
+  compute_theta_fluxes(*st, st->p0);  // integrate.jl:279
   // (1)
   explicit_integration_lhs(*st, st->p0, 0.5 * dt, time);
   // (2)
@@ -191,16 +229,22 @@ int orc_step(State* st, orc_step_info* info) {
   info->niter[0] = it1;
   if (rc) return rc;
   copy_predictands(st->p1, st->pr);
-  // (3) reset_predictands! (rho, rhop, u, v, w; pip keeps its new value)
+  // (3) reset_predictands! (reset_predictands!.jl:71-118: rho, rhop, u, v, w;
+  // pip and p too in the compressible model, else pip keeps its new value)
   if (n.model != PCF_BOUSSINESQ) st->pr.rho.a = st->p0.rho.a;
   st->pr.rhop.a = st->p0.rhop.a;
   st->pr.u.a = st->p0.u.a;
   st->pr.v.a = st->p0.v.a;
   st->pr.w.a = st->p0.w.a;
+  if (n.model == PCF_COMPRESSIBLE) {
+    st->pr.pip.a = st->p0.pip.a;
+    st->pr.p.a = st->p0.p.a;
+  }
   if (st->tracer) st->chi.a = st->chi0.a;  // reset_predictands!.jl:63-67
   explicit_integration_rhs(*st, 0.5 * dt, time);
   // (4)
   copy_predictands(st->p0, st->p1);
+  synchronize_compressible_atmosphere(*st, st->p0);
   explicit_integration_lhs(*st, st->p0, dt, time);
   // (5)
   rc = implicit_integration(*st, 0.5 * dt, time, 2.0, it2);
@@ -274,6 +318,13 @@ static Arr3* field_ptr(State* st, int f) {
     case PCF_F_CHI0: return st->tracer ? &st->chi0 : nullptr;
     case PCF_F_PHICHI_X: case PCF_F_PHICHI_Y: case PCF_F_PHICHI_Z:
       return st->tracer ? &st->phichi.c[f - PCF_F_PHICHI_X] : nullptr;
+    case PCF_F_P: return st->compressible ? &p.p : nullptr;
+    case PCF_F_DP: return st->compressible ? &st->dp : nullptr;
+    case PCF_F_P0_P: return st->compressible ? &st->p0.p : nullptr;
+    case PCF_F_PHIP_X: case PCF_F_PHIP_Y: case PCF_F_PHIP_Z:
+      return st->compressible ? &st->phip.c[f - PCF_F_PHIP_X] : nullptr;
+    case PCF_F_PHITHETA_X: case PCF_F_PHITHETA_Y: case PCF_F_PHITHETA_Z:
+      return st->compressible ? &st->phitheta.c[f - PCF_F_PHITHETA_X] : nullptr;
     default: return nullptr;
   }
 }
@@ -370,6 +421,10 @@ void orc_compute_fluxes(State* st) {
   set_boundaries_fluxes(*st);
 }
 void orc_save_backups(State* st) { save_backups(*st, true, true, true, true, true); }
+// compute_fluxes!(state, p0, Theta()) with the p0 copy
+void orc_compute_theta_fluxes(State* st) { compute_theta_fluxes(*st, st->p0); }
+void orc_modify_compressible_wind(State* st, int multiply) { modify_compressible_wind(*st, multiply != 0); }
+void orc_synchronize_compressible_atmosphere(State* st) { synchronize_compressible_atmosphere(*st, st->pr); }
 void orc_update_lhs(State* st, double dt, int m, int var) { update_lhs(*st, dt, m, var); }
 void orc_update_rhs(State* st, double dt, int var, int implicit, double rf) {
   update_rhs(*st, dt, var, implicit != 0, rf);

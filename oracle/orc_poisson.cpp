@@ -51,7 +51,7 @@ double sumsq(const Arr3& a) {
   return preduce([&](size_t q) { return x[q] * x[q]; }, a.a.size());
 }
 
-// compute_lhs!.jl:68-160 (Boussinesq / pseudo-incompressible)
+// compute_lhs!.jl:68-223
 int compute_lhs(State& st, double& tolref) {
   const double ma = st.c.ma, kappa = st.c.kappa, dx = st.dx, dy = st.dy, dz = st.dz;
   Arr3 &jac = st.jac, &pbar = st.pbar, &rhobar = st.rhobar;
@@ -65,6 +65,21 @@ int compute_lhs(State& st, double& tolref) {
         double fcscal = std::sqrt(pbar(i, j, k) * pbar(i, j, k) / rhobar(i, j, k));
         double ur = u(i, j, k), ul = u(i - 1, j, k), vf = v(i, j, k), vb = v(i, j - 1, k);
         double wu = w(i, j, k), wd = w(i, j, k - 1);
+        if (st.compressible) {  // compute_lhs!.jl:162-223: the winds carry J P already
+          double heating = conductive_heating(st, i, j, k) * (ma * ma) * kappa;
+          double bu = (ur - ul) / dx / jac(i, j, k) * (ma * ma) * kappa;
+          double bv = (vf - vb) / dy / jac(i, j, k) * (ma * ma) * kappa;
+          double bw = (wu - wd) / dz / jac(i, j, k) * (ma * ma) * kappa;
+          bu /= fcscal;
+          bv /= fcscal;
+          bw /= fcscal;
+          heating /= fcscal;
+          double b = bu + bv + bw - heating;
+          st.rhs(i - st.i0 + 1, j - st.j0 + 1, k - st.k0 + 1) = b;
+          divl2 += b * b;
+          divl2_norm += bu * bu + bv * bv + bw * bw + heating * heating;
+          continue;
+        }
         double pedger = 0.5 * (jac(i, j, k) * pbar(i, j, k) + jac(i + 1, j, k) * pbar(i + 1, j, k));
         double pedgel = 0.5 * (jac(i, j, k) * pbar(i, j, k) + jac(i - 1, j, k) * pbar(i - 1, j, k));
         double pedgef = 0.5 * (jac(i, j, k) * pbar(i, j, k) + jac(i, j + 1, k) * pbar(i, j + 1, k));
@@ -103,7 +118,7 @@ static inline double fcs(State& st, int i, int j, int k) {
   return std::sqrt(st.pbar(i, j, k) * st.pbar(i, j, k) / st.rhobar(i, j, k));
 }
 
-// compute_operator!.jl:40-1606 (Boussinesq / pseudo-incompressible)
+// compute_operator!.jl:40-1606
 void compute_operator(State& st, double dt, double rf) {
   const int z_size = st.cfg.z_size, nbz = st.nbz, ko = st.ko, k0 = st.k0;
   const bool damp = st.cfg.damp_horizontal_wind_on_rhs != 0;
@@ -311,6 +326,10 @@ void compute_operator(State& st, double dt, double rf) {
           ac = -R / dx + L / dx - F / dy + B / dy - U * met33edgeu / dz + D * met33edged / dz -
                q4(UR, met13uedger) - q4(UL, met13uedgel) - q4(UF, met23uedgef) - q4(UB, met23uedgeb) +
                q4(DR, met13dedger) + q4(DL, met13dedgel) + q4(DF, met23dedgef) + q4(DB, met23dedgeb);
+        }
+        if (st.compressible) {  // compute_operator!.jl:854-860
+          double dpdpi_ = dpdpi(st, st.pr.p(i, j, k));
+          ac -= (dpdpi_ / dt) / (dt * st.c.rsp / st.c.kappa);
         }
         // A(i+1,j,k)
         if (kbot) {
@@ -756,7 +775,8 @@ void correct(State& st, double dt, double rf) {
         if (damp) factor += dt * 0.5 * (betar(i, j, k) + betar(i + 1, j, k)) * rf;
         double gradient = pressure_gradient_u(st, dpip, i, j, k);
         corx(i, j, k) = dt / factor * gradient;
-        u(i, j, k) -= 1.0 * corx(i, j, k);
+        double jpedger = wind_factor_u(st, i, j, k);
+        u(i, j, k) -= jpedger * corx(i, j, k);
       }
   // V (correct!.jl:217-255)
   for (int k = st.k0; k <= kmax_uv; ++k)
@@ -766,7 +786,8 @@ void correct(State& st, double dt, double rf) {
         if (damp) factor += dt * 0.5 * (betar(i, j, k) + betar(i, j + 1, k)) * rf;
         double gradient = pressure_gradient_v(st, dpip, i, j, k);
         cory(i, j, k) = dt / factor * gradient;
-        v(i, j, k) -= 1.0 * cory(i, j, k);
+        double jpedgef = wind_factor_v(st, i, j, k);
+        v(i, j, k) -= jpedgef * cory(i, j, k);
       }
   // W (correct!.jl:257-316)
   const int kmin_w = st.ko == 0 ? st.k0 : st.k0 - 1;
@@ -780,7 +801,7 @@ void correct(State& st, double dt, double rf) {
         double n2edgeu = (jac(i, j, k + 1) * n2(i, j, k) + jac(i, j, k) * n2(i, j, k + 1)) /
                          (jac(i, j, k) + jac(i, j, k + 1));
         double gradient = pressure_gradient_w(st, dpip, i, j, k);
-        double jpedgeu = 1.0;
+        double jpedgeu = wind_factor_w(st, i, j, k);
         double fw = buoyancy_factor_w(st, i, j, k);
         w(i, j, k) +=
             -dt / (factor + fw * n2edgeu * dt2) * jpedgeu * gradient -

@@ -258,7 +258,7 @@ static int make_atmosphere(State& st) {
     }
     return PCF_OK;
   }
-  if (n.model != PCF_PSEUDO_INCOMPRESSIBLE) {
+  if (n.model != PCF_PSEUDO_INCOMPRESSIBLE && n.model != PCF_COMPRESSIBLE) {
     st.err = "Invalid model";
     return PCF_E_OPTION;
   }
@@ -442,6 +442,7 @@ static int make_predictands(State& st) {
         p.w(i, j, k) = (st.jac(i, j, k + 1) * p.w(i, j, k) + st.jac(i, j, k) * p.w(i, j, k + 1)) /
                        (st.jac(i, j, k) + st.jac(i, j, k + 1));
   vertical_bc(st, p.w, -1, true);
+  if (n.model == PCF_COMPRESSIBLE) p.p = st.pbar;  // set_p.jl:36-41
   return PCF_OK;
 }
 
@@ -462,6 +463,7 @@ int setup(State& st, const pcf_config& cfg) {
     return PCF_E_CONFIG;
   }
   make_constants(st);
+  st.compressible = n.model == PCF_COMPRESSIBLE;
   // Time.jl:31-38
   st.alphark[0] = 0.0; st.alphark[1] = -5.0 / 9.0; st.alphark[2] = -153.0 / 128.0;
   st.betark[0] = 1.0 / 3.0; st.betark[1] = 15.0 / 16.0; st.betark[2] = 8.0 / 15.0;
@@ -496,6 +498,13 @@ int setup(State& st, const pcf_config& cfg) {
   st.p_pc.assign((size_t)st.nx * st.ny, 0.0);
   for (Predictands* q : {&st.p0, &st.p1})
     for (Arr3* a : {&q->rho, &q->rhop, &q->u, &q->v, &q->w, &q->pip}) a->init(st.nxx, st.nyy, st.nzz);
+  if (st.compressible) {  // Increments.jl:91-94, Fluxes.jl:89-92
+    st.dp.init(st.nxx, st.nyy, st.nzz);
+    st.phip.init(st.nxx, st.nyy, st.nzz);
+    st.phitheta.init(st.nxx, st.nyy, st.nzz);
+    st.p0.p.init(st.nxx, st.nyy, st.nzz);
+    st.p1.p.init(st.nxx, st.nyy, st.nzz);
+  }
   return PCF_OK;
 }
 

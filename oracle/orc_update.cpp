@@ -1,19 +1,60 @@
 // ORACLE -- test infrastructure only (see orc.hpp).
-// src/Update/*.jl for the Boussinesq and pseudo-incompressible models with
-// wkb_mode = :NoWKB and no tracers (compute_volume_force = 0,
-// compute_compressible_wind_factor = 1).
+// src/Update/*.jl for the Boussinesq, pseudo-incompressible and compressible
+// models with wkb_mode = :NoWKB (compute_volume_force = 0 for the winds and
+// the conductive heating for P).
 #include "orc.hpp"
 
 namespace orc {
 
-// compute_buoyancy_factor.jl:122-133 (RhoP, Boussinesq / PI)
+// compute_compressible_wind_factor.jl:102-157: 1 outside the compressible
+// model, else J P interpolated to the staggered wind points.
+double wind_factor_u(State& st, int i, int j, int k) {
+  if (!st.compressible) return 1.0;
+  Arr3 &jac = st.jac, &p = st.pr.p;
+  return (jac(i, j, k) * p(i, j, k) + jac(i + 1, j, k) * p(i + 1, j, k)) / 2;
+}
+double wind_factor_v(State& st, int i, int j, int k) {
+  if (!st.compressible) return 1.0;
+  Arr3 &jac = st.jac, &p = st.pr.p;
+  return (jac(i, j, k) * p(i, j, k) + jac(i, j + 1, k) * p(i, j + 1, k)) / 2;
+}
+double wind_factor_w(State& st, int i, int j, int k) {
+  if (!st.compressible) return 1.0;
+  Arr3 &jac = st.jac, &p = st.pr.p;
+  return jac(i, j, k) * jac(i, j, k + 1) * (p(i, j, k) + p(i, j, k + 1)) / (jac(i, j, k) + jac(i, j, k + 1));
+}
+
+// conductive_heating.jl:67-117 (0 outside the compressible model)
+double conductive_heating(State& st, int i, int j, int k) {
+  if (!st.compressible) return 0.0;
+  Arr4& f = st.phitheta;
+  double rhotot = (st.pr.rho(i, j, k) + st.rhobar(i, j, k)) / st.jac(i, j, k);
+  return -rhotot * ((f(i, j, k, 1) - f(i - 1, j, k, 1)) / st.dx + (f(i, j, k, 2) - f(i, j - 1, k, 2)) / st.dy +
+                    (f(i, j, k, 3) - f(i, j, k - 1, 3)) / st.dz);
+}
+
+// dP/dpi of update!.jl:1385-1386, apply_lhs_sponge!.jl:556-557 and
+// compute_operator!.jl:855-858, with Constants.jl's dimensional gamma, rsp, pref
+double dpdpi(State& st, double p) {
+  const double gamma = st.c.gamma;
+  return 1 / (gamma - 1) * std::pow(st.c.rsp / st.c.pref, 1 - gamma) * std::pow(p, 2 - gamma);
+}
+
+// compute_buoyancy_factor.jl:108-133 (RhoP)
 double buoyancy_factor_rhop(State& st, int i, int j, int k) {
+  if (st.compressible)
+    return st.pbar(i, j, k) / st.thetabar(i, j, k) / (st.pr.rho(i, j, k) + st.rhobar(i, j, k));
   return st.rhobar(i, j, k) / (st.pr.rho(i, j, k) + st.rhobar(i, j, k));
 }
 
-// compute_buoyancy_factor.jl:155-172 (W, Boussinesq / PI)
+// compute_buoyancy_factor.jl:135-172 (W)
 double buoyancy_factor_w(State& st, int i, int j, int k) {
   Arr3 &jac = st.jac, &rb = st.rhobar, &rho = st.pr.rho;
+  if (st.compressible) {
+    Arr3 &pb = st.pbar, &tb = st.thetabar;
+    return (jac(i, j, k + 1) * pb(i, j, k) / tb(i, j, k) + jac(i, j, k) * pb(i, j, k + 1) / tb(i, j, k + 1)) /
+           (jac(i, j, k + 1) * (rho(i, j, k) + rb(i, j, k)) + jac(i, j, k) * (rho(i, j, k + 1) + rb(i, j, k + 1)));
+  }
   return (jac(i, j, k + 1) * rb(i, j, k) + jac(i, j, k) * rb(i, j, k + 1)) /
          (jac(i, j, k + 1) * (rho(i, j, k) + rb(i, j, k)) + jac(i, j, k) * (rho(i, j, k + 1) + rb(i, j, k + 1)));
 }
@@ -276,11 +317,12 @@ void update_lhs(State& st, double dt, int m, int var) {
         }
     return;
   }
-  if (var == PCF_VAR_RHO || var == PCF_VAR_RHOP) {
-    if (var == PCF_VAR_RHO && !pi) return;  // update!.jl:511-519
-    Arr3& q = var == PCF_VAR_RHO ? st.pr.rho : st.pr.rhop;
-    Arr3& dq = var == PCF_VAR_RHO ? st.drho : st.drhop;
-    Arr4& f = var == PCF_VAR_RHO ? st.phirho : st.phirhop;
+  if (var == PCF_VAR_RHO || var == PCF_VAR_RHOP || var == PCF_VAR_P) {
+    if (var == PCF_VAR_RHO && !pi) return;            // update!.jl:511-519
+    if (var == PCF_VAR_P && !st.compressible) return;  // update!.jl:1400-1408
+    Arr3& q = var == PCF_VAR_RHO ? st.pr.rho : var == PCF_VAR_RHOP ? st.pr.rhop : st.pr.p;
+    Arr3& dq = var == PCF_VAR_RHO ? st.drho : var == PCF_VAR_RHOP ? st.drhop : st.dp;
+    Arr4& f = var == PCF_VAR_RHO ? st.phirho : var == PCF_VAR_RHOP ? st.phirhop : st.phip;
     if (m == 1) dq.fill(0.0);
 #pragma omp parallel for schedule(static)
     for (int k = st.k0; k <= st.k1; ++k)
@@ -289,8 +331,16 @@ void update_lhs(State& st, double dt, int m, int var) {
           double fluxdiff = (f(i, j, k, 1) - f(i - 1, j, k, 1)) / dx + (f(i, j, k, 2) - f(i, j - 1, k, 2)) / dy +
                             (f(i, j, k, 3) - f(i, j, k - 1, 3)) / dz;
           fluxdiff /= jac(i, j, k);
-          // update!.jl:582-590: heating = 0 for Boussinesq / PI
-          double fr = var == PCF_VAR_RHO ? -fluxdiff : -fluxdiff - 0.0 / st.thetabar(i, j, k);
+          double fr;
+          if (var == PCF_VAR_RHO) {
+            fr = -fluxdiff;
+          } else if (var == PCF_VAR_RHOP) {  // update!.jl:586-594
+            double heating = conductive_heating(st, i, j, k);
+            fr = -fluxdiff - heating / st.thetabar(i, j, k);
+          } else {  // update!.jl:1410-1448
+            double heating = conductive_heating(st, i, j, k);
+            fr = -fluxdiff + heating;
+          }
           dq(i, j, k) = dt * fr + am * dq(i, j, k);
           q(i, j, k) += bm * dq(i, j, k);
         }
@@ -409,6 +459,22 @@ void update_rhs(State& st, double dt, int var, bool implicit, double rf) {
   Arr3 &jac = st.jac, &rb = st.rhobar, &n2 = st.n2, &betar = st.betar, &m13 = st.met13, &m23 = st.met23;
   Arr3 &rho = st.pr.rho, &rhop = st.pr.rhop, &u = st.pr.u, &v = st.pr.v, &w = st.pr.w, &pip = st.pr.pip;
   const int kmax_uv = implicit && st.ko + st.nz != st.cfg.z_size ? st.k1 + 1 : st.k1;
+  if (var == PCF_VAR_PIP) {  // update!.jl:1360-1392 (explicit; compressible only)
+    if (!st.compressible) return;
+    Arr3 &uold = st.uold, &vold = st.vold, &wold = st.wold, &p = st.pr.p;
+#pragma omp parallel for schedule(static)
+    for (int k = st.k0; k <= st.k1; ++k)
+      for (int j = st.j0; j <= st.j1; ++j)
+        for (int i = st.i0; i <= st.i1; ++i) {
+          double fluxdiff = (uold(i, j, k) - uold(i - 1, j, k)) / st.dx + (vold(i, j, k) - vold(i, j - 1, k)) / st.dy +
+                            (wold(i, j, k) - wold(i, j, k - 1)) / st.dz;
+          fluxdiff /= jac(i, j, k);
+          double heating = conductive_heating(st, i, j, k);
+          double dpdpi_ = dpdpi(st, p(i, j, k));
+          pip(i, j, k) -= dt * (fluxdiff - heating) / dpdpi_;
+        }
+    return;
+  }
   if (var == PCF_VAR_U) {  // update!.jl:809-881
 #pragma omp parallel for schedule(static)
     for (int k = st.k0; k <= kmax_uv; ++k)
@@ -419,12 +485,13 @@ void update_rhs(State& st, double dt, int var, bool implicit, double rf) {
           rhoedger += rhobaredger;
           double gradient = pressure_gradient_u(st, pip, i, j, k);
           double force = 0.0;
+          double jpedger = wind_factor_u(st, i, j, k);
           if (!implicit) {
-            u(i, j, k) += dt * (-gradient + force / rhoedger) * 1.0;
+            u(i, j, k) += dt * (-gradient + force / rhoedger) * jpedger;
           } else {
             double factor = 1.0;
             if (damp) factor += dt * 0.5 * (betar(i, j, k) + betar(i + 1, j, k)) * rf;
-            u(i, j, k) = 1.0 / factor * (u(i, j, k) + dt * (-gradient + force / rhoedger) * 1.0);
+            u(i, j, k) = 1.0 / factor * (u(i, j, k) + dt * (-gradient + force / rhoedger) * jpedger);
           }
         }
     return;
@@ -439,12 +506,13 @@ void update_rhs(State& st, double dt, int var, bool implicit, double rf) {
           rhoedgef += rhobaredgef;
           double gradient = pressure_gradient_v(st, pip, i, j, k);
           double force = 0.0;
+          double jpedgef = wind_factor_v(st, i, j, k);
           if (!implicit) {
-            v(i, j, k) += dt * (-gradient + force / rhoedgef) * 1.0;
+            v(i, j, k) += dt * (-gradient + force / rhoedgef) * jpedgef;
           } else {
             double factor = 1.0;
             if (damp) factor += dt * 0.5 * (betar(i, j, k) + betar(i, j + 1, k)) * rf;
-            v(i, j, k) = 1.0 / factor * (v(i, j, k) + dt * (-gradient + force / rhoedgef) * 1.0);
+            v(i, j, k) = 1.0 / factor * (v(i, j, k) + dt * (-gradient + force / rhoedgef) * jpedgef);
           }
         }
     return;
@@ -468,17 +536,22 @@ void update_rhs(State& st, double dt, int var, bool implicit, double rf) {
           double force = 0.0;
           if (!implicit) {
             double b = -g * (ju * rhopold(i, j, k) / rhoc / jc + jc * rhopold(i, j, k + 1) / rhou / ju) / (jc + ju);
-            w(i, j, k) += dt * (b - gradient + force / rhoedgeu) * 1.0;
+            double jpedgeu = wind_factor_w(st, i, j, k);
+            w(i, j, k) += dt * (b - gradient + force / rhoedgeu) * jpedgeu;
           } else {
             double n2edgeu = (ju * n2(i, j, k) + jc * n2(i, j, k + 1)) / (jc + ju);
             double factor = 1.0;
             factor += dt * (ju * betar(i, j, k) + jc * betar(i, j, k + 1)) / (jc + ju) * rf;
             double b = -g * (ju * rhop(i, j, k) / rhoc / jc + jc * rhop(i, j, k + 1) / rhou / ju) / (jc + ju);
-            double uc = 0.5 * (u(i, j, k) / 1.0 + u(i - 1, j, k) / 1.0);
-            double uu = 0.5 * (u(i, j, k + 1) / 1.0 + u(i - 1, j, k + 1) / 1.0);
-            double vc = 0.5 * (v(i, j, k) / 1.0 + v(i, j - 1, k) / 1.0);
-            double vu = 0.5 * (v(i, j, k + 1) / 1.0 + v(i, j - 1, k + 1) / 1.0);
-            double jpedgeu = 1.0;
+            double jpedger = wind_factor_u(st, i, j, k), jpedgel = wind_factor_u(st, i - 1, j, k);
+            double jpedgef = wind_factor_v(st, i, j, k), jpedgeb = wind_factor_v(st, i, j - 1, k);
+            double jpuedger = wind_factor_u(st, i, j, k + 1), jpuedgel = wind_factor_u(st, i - 1, j, k + 1);
+            double jpuedgef = wind_factor_v(st, i, j, k + 1), jpuedgeb = wind_factor_v(st, i, j - 1, k + 1);
+            double uc = 0.5 * (u(i, j, k) / jpedger + u(i - 1, j, k) / jpedgel);
+            double uu = 0.5 * (u(i, j, k + 1) / jpuedger + u(i - 1, j, k + 1) / jpuedgel);
+            double vc = 0.5 * (v(i, j, k) / jpedgef + v(i, j - 1, k) / jpedgeb);
+            double vu = 0.5 * (v(i, j, k + 1) / jpuedgef + v(i, j - 1, k + 1) / jpuedgeb);
+            double jpedgeu = wind_factor_w(st, i, j, k);
             double fw = buoyancy_factor_w(st, i, j, k);
             w(i, j, k) = 1.0 / (factor + fw * n2edgeu * dt2) *
                          (w(i, j, k) - dt * gradient * jpedgeu + dt * b * jpedgeu + dt * force / rhoedgeu * jpedgeu +
@@ -495,7 +568,7 @@ void update_rhs(State& st, double dt, int var, bool implicit, double rf) {
       for (int k = st.k0; k <= st.k1; ++k)
         for (int j = st.j0; j <= st.j1; ++j)
           for (int i = st.i0; i <= st.i1; ++i) {
-            double jpu = 1.0, jpd = 1.0;
+            double jpu = wind_factor_w(st, i, j, k), jpd = wind_factor_w(st, i, j, k - 1);
             double wvrt = 0.5 * (vertical_wind(st, i, j, k) / jpu + vertical_wind(st, i, j, k - 1) / jpd);
             double buoy = -g * rhop(i, j, k) / (rho(i, j, k) + rb(i, j, k));
             double fb = buoyancy_factor_rhop(st, i, j, k);
@@ -516,7 +589,7 @@ void update_rhs(State& st, double dt, int var, bool implicit, double rf) {
           rhoedgeu += (ju * rb(i, j, k) + jc * rb(i, j, k + 1)) / (jc + ju);
           double rhoedged = (jd * rho(i, j, k) + jc * rho(i, j, k - 1)) / (jc + jd);
           rhoedged += (jd * rb(i, j, k) + jc * rb(i, j, k - 1)) / (jc + jd);
-          double jpedgeu = 1.0, jpedged = 1.0;
+          double jpedgeu = wind_factor_w(st, i, j, k), jpedged = wind_factor_w(st, i, j, k - 1);
           double wv = 0.5 * (wold(i, j, k) / jpedgeu + wold(i, j, k - 1) / jpedged);
           double lower_gradient = pressure_gradient_w(st, pip, i, j, k - 1);
           double lower_force = 0.0;
@@ -534,12 +607,14 @@ void update_rhs(State& st, double dt, int var, bool implicit, double rf) {
           double factor = 1.0;
           factor += dt * betar(i, j, k) * rf;
           double b = -g * rhop(i, j, k) / (rho(i, j, k) + rb(i, j, k));
+          double jpedger = wind_factor_u(st, i, j, k), jpedgel = wind_factor_u(st, i - 1, j, k);
+          double jpedgef = wind_factor_v(st, i, j, k), jpedgeb = wind_factor_v(st, i, j - 1, k);
           double fb = buoyancy_factor_rhop(st, i, j, k);
           b = 1.0 / (factor + fb * n2(i, j, k) * dt2) *
               (-fb * n2(i, j, k) * dt * jac(i, j, k) * (wv + dt * (-gradient + force / rhoc)) + factor * b +
                fb * n2(i, j, k) * dt * jac(i, j, k) * factor * 0.5 *
-                   (m13(i, j, k) * (u(i, j, k) / 1.0 + u(i - 1, j, k) / 1.0) +
-                    m23(i, j, k) * (v(i, j, k) / 1.0 + v(i, j - 1, k) / 1.0)));
+                   (m13(i, j, k) * (u(i, j, k) / jpedger + u(i - 1, j, k) / jpedgel) +
+                    m23(i, j, k) * (v(i, j, k) / jpedgef + v(i, j - 1, k) / jpedgeb)));
           rhop(i, j, k) = -b * (rho(i, j, k) + rb(i, j, k)) / g;
         }
   }
@@ -553,6 +628,47 @@ void apply_lhs_sponge(State& st, double dt, double time, int var) {
   const double lref = st.c.lref, uref = st.c.uref;
   const pcf_config& n = st.cfg;
   const bool pi = n.model != PCF_BOUSSINESQ;
+  if (var == PCF_VAR_RHOP && st.compressible) {  // apply_lhs_sponge!.jl:301-328
+    Arr3 &rho = st.pr.rho, &rhop = st.pr.rhop, &p = st.pr.p, &rb = st.rhobar, &tb = st.thetabar;
+    for (int k = st.k0; k <= st.k1; ++k)
+      for (int j = st.j0; j <= st.j1; ++j)
+        for (int i = st.i0; i <= st.i1; ++i) {
+          double rhopbg = rb(i, j, k) * (1.0 - p(i, j, k) / tb(i, j, k) / (rho(i, j, k) + rb(i, j, k)));
+          double alpha = alphar(i, j, k);
+          double rhopold = rhop(i, j, k);
+          double beta = 1.0 / (1.0 + alpha * dt);
+          rhop(i, j, k) = (1.0 - beta) * rhopbg + beta * rhopold;
+        }
+    return;
+  }
+  if (var == PCF_VAR_P) {  // apply_lhs_sponge!.jl:580-602 (compressible only)
+    if (!st.compressible) return;
+    Arr3 &rho = st.pr.rho, &p = st.pr.p, &rb = st.rhobar;
+    for (int k = st.k0; k <= st.k1; ++k)
+      for (int j = st.j0; j <= st.j1; ++j)
+        for (int i = st.i0; i <= st.i1; ++i) {
+          double pb = rb(i, j, k) * p(i, j, k) / (rho(i, j, k) + rb(i, j, k));
+          double alpha = alphar(i, j, k);
+          double pold = p(i, j, k);
+          double beta = 1 / (1 + alpha * dt);
+          p(i, j, k) = (1 - beta) * pb + beta * pold;
+        }
+    return;
+  }
+  if (var == PCF_VAR_PIP) {  // apply_lhs_sponge!.jl:542-568 (compressible only)
+    if (!st.compressible) return;
+    Arr3 &rho = st.pr.rho, &pip = st.pr.pip, &p = st.pr.p, &rb = st.rhobar;
+    for (int k = st.k0; k <= st.k1; ++k)
+      for (int j = st.j0; j <= st.j1; ++j)
+        for (int i = st.i0; i <= st.i1; ++i) {
+          double dpdpi_ = dpdpi(st, p(i, j, k));
+          double pib = rb(i, j, k) * p(i, j, k) / (rho(i, j, k) + rb(i, j, k)) / dpdpi_;
+          double alpha = alphar(i, j, k);
+          double pipold = pip(i, j, k);
+          pip(i, j, k) = pipold - alpha * dt * (p(i, j, k) / dpdpi_ - pib);
+        }
+    return;
+  }
   if (var == PCF_VAR_RHO || var == PCF_VAR_RHOP) {
     if (var == PCF_VAR_RHO && !pi) return;
     Arr3& q = var == PCF_VAR_RHO ? st.pr.rho : st.pr.rhop;
@@ -567,7 +683,6 @@ void apply_lhs_sponge(State& st, double dt, double time, int var) {
         }
     return;
   }
-  if (var == PCF_VAR_PIP) return;
   if (var == PCF_VAR_CHI) {  // apply_lhs_sponge!.jl:615-636
     if (!st.tracer) return;
     for (int k = st.k0; k <= st.k1; ++k)

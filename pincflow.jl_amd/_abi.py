@@ -13,7 +13,7 @@ from pathlib import Path
 PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "lib" / "libpincflow_hip.so"
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 MAX_HILLS = 32
 
 # return codes
@@ -28,7 +28,7 @@ E_HIP = -7
 E_ARG = -8
 
 # enums
-BOUSSINESQ, PSEUDO_INCOMPRESSIBLE = 0, 1
+BOUSSINESQ, PSEUDO_INCOMPRESSIBLE, COMPRESSIBLE = 0, 1, 2
 NEUTRAL_STRATIFICATION, STABLE_STRATIFICATION, ISOTHERMAL, ISENTROPIC, REALISTIC, LAPSE_RATES = range(6)
 TOPO_FLAT, TOPO_COSINE_HILL, TOPO_AGNESI, TOPO_GAUSSIANS = range(4)
 STRETCH_NONE, STRETCH_SINH = 0, 1
@@ -57,6 +57,9 @@ FIELDS = [
     "PHIW_X", "PHIW_Y", "PHIW_Z",
     "CHI", "DCHI", "INITIALTRACER", "CHI0",
     "PHICHI_X", "PHICHI_Y", "PHICHI_Z",
+    "P", "DP", "P0_P",
+    "PHIP_X", "PHIP_Y", "PHIP_Z",
+    "PHITHETA_X", "PHITHETA_Y", "PHITHETA_Z",
 ]
 FIELD = {name: i for i, name in enumerate(FIELDS)}
 
@@ -70,7 +73,7 @@ VECTORS = [
 VECTOR = {name: i for i, name in enumerate(VECTORS)}
 
 BOUNDARY_PREDICTANDS, BOUNDARY_FLUXES = 0, 2
-VAR_RHO, VAR_RHOP, VAR_U, VAR_V, VAR_W, VAR_PIP, VAR_CHI = range(7)
+VAR_RHO, VAR_RHOP, VAR_U, VAR_V, VAR_W, VAR_PIP, VAR_CHI, VAR_P = range(8)
 NO_TRACER, TRACER_ON = 0, 1
 TRACER_ZERO, TRACER_DIPOLE = 0, 1
 OP_TOTAL, OP_HORIZONTAL = 0, 1
@@ -145,6 +148,7 @@ class PcfConfig(ctypes.Structure):
         ("initial_tracer", ctypes.c_int32),
         ("tracer_rx", ctypes.c_double),
         ("tracer_ry", ctypes.c_double),
+        ("thermal_conductivity", ctypes.c_double),
     ]
 
 
@@ -180,6 +184,9 @@ SIGNATURES = [
     ("pcf_backup_predictands", _I, [_P]),
     ("pcf_set_boundaries", _I, [_P, _I]),
     ("pcf_compute_fluxes", _I, [_P]),
+    ("pcf_compute_theta_fluxes", _I, [_P]),
+    ("pcf_modify_compressible_wind", _I, [_P, _I]),
+    ("pcf_synchronize_compressible_atmosphere", _I, [_P]),
     ("pcf_update_lhs", _I, [_P, _D, ctypes.c_int32, _I]),
     ("pcf_update_rhs", _I, [_P, _D, _I, _I, _D]),
     ("pcf_apply_lhs_sponge", _I, [_P, _D, _D, _I]),

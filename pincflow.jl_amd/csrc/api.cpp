@@ -74,6 +74,13 @@ double* field_ptr(pcf_state* st, int f) {
     case PCF_F_CHI0: return st->chi0;
     case PCF_F_PHICHI_X: case PCF_F_PHICHI_Y: case PCF_F_PHICHI_Z:
       return st->flux[5][f - PCF_F_PHICHI_X];
+    case PCF_F_P: return st->p;
+    case PCF_F_DP: return st->dp;
+    case PCF_F_P0_P: return st->p0[6];
+    case PCF_F_PHIP_X: case PCF_F_PHIP_Y: case PCF_F_PHIP_Z:
+      return st->flux[6][f - PCF_F_PHIP_X];
+    case PCF_F_PHITHETA_X: case PCF_F_PHITHETA_Y: case PCF_F_PHITHETA_Z:
+      return st->phitheta[f - PCF_F_PHITHETA_X];
     default:
       if (f >= PCF_F_PHIRHO_X && f <= PCF_F_PHIW_Z) {
         int q = (f - PCF_F_PHIRHO_X) / 3, d = (f - PCF_F_PHIRHO_X) % 3;
@@ -90,9 +97,12 @@ double* vector_ptr(pcf_state* st, int v) {
   return nullptr;
 }
 
-double** predictand_slots(pcf_state* st, double* out[6]) {
+// the predictands in the order of p0 / p1; returns how many there are (P
+// only in the compressible model)
+int predictand_slots(pcf_state* st, double* out[7]) {
   out[0] = st->rho; out[1] = st->rhop; out[2] = st->u; out[3] = st->v; out[4] = st->w; out[5] = st->pip;
-  return out;
+  out[6] = st->p;
+  return st->compressible ? 7 : 6;
 }
 
 int save_backups(pcf_state* st, bool rho, bool rhop, bool u, bool v, bool w) {  // save_backups!.jl
@@ -121,6 +131,8 @@ int explicit_lhs(pcf_state* st, double dt, double time) {
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_RHO));
     TRY(update_lhs(st, dt, m, PCF_VAR_RHOP));
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_RHOP));
+    TRY(update_lhs(st, dt, m, PCF_VAR_P));  // no-ops outside the compressible model
+    TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_P));
     TRY(update_lhs(st, dt, m, PCF_VAR_CHI));  // no-ops without a tracer
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_CHI));
     TRY(set_boundaries_predictands(st));
@@ -132,24 +144,30 @@ int explicit_lhs(pcf_state* st, double dt, double time) {
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_W));
     TRY(set_boundaries_predictands(st));
   }
-  return PCF_OK;
+  TRY(synchronize_compressible_atmosphere(st, st->p));
+  return apply_lhs_sponge(st, dt, time, PCF_VAR_PIP);
 }
 
 // explicit_integration!.jl:103-131
 int explicit_rhs(pcf_state* st, double dt, double time) {
   (void)time;
+  TRY(synchronize_compressible_atmosphere(st, st->p));
+  TRY(modify_compressible_wind(st, true));
   TRY(set_boundaries_predictands(st));
   TRY(save_backups(st, false, true, true, true, true));
   TRY(update_rhs(st, dt, PCF_VAR_RHOP, false, 0.0));
   TRY(update_rhs(st, dt, PCF_VAR_U, false, 0.0));
   TRY(update_rhs(st, dt, PCF_VAR_V, false, 0.0));
   TRY(update_rhs(st, dt, PCF_VAR_W, false, 0.0));
+  TRY(update_rhs(st, dt, PCF_VAR_PIP, false, 0.0));
+  TRY(modify_compressible_wind(st, false));
   return set_boundaries_predictands(st);
 }
 
 // implicit_integration!.jl:37-79
 int implicit(pcf_state* st, double dt, double time, double rf, int* niter) {
   (void)time;
+  TRY(modify_compressible_wind(st, true));
   TRY(set_boundaries_predictands(st));
   TRY(save_backups(st, false, false, false, false, true));
   TRY(update_rhs(st, dt, PCF_VAR_U, true, rf));
@@ -159,6 +177,7 @@ int implicit(pcf_state* st, double dt, double time, double rf, int* niter) {
   TRY(update_rhs(st, dt, PCF_VAR_RHOP, true, rf));
   TRY(set_boundaries_predictands(st));
   TRY(apply_corrector(st, dt, rf, niter));
+  TRY(modify_compressible_wind(st, false));
   return set_boundaries_predictands(st);
 }
 
@@ -221,6 +240,7 @@ void pcf_config_default(pcf_config* c) {
   c->model = PCF_PSEUDO_INCOMPRESSIBLE;
   c->background = PCF_ISOTHERMAL;
   c->kinematic_viscosity = 1.5e-5;
+  c->thermal_conductivity = 3.0e-5;
   c->buoyancy_frequency = 1.0e-2;
   c->potential_temperature = 3.0e2;
   c->temperature = 3.0e2;
@@ -267,9 +287,16 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
                        &st->vold, &st->wold, &st->betar, &st->dpip, &st->corx, &st->cory, &st->wcart};
   for (double** p : haloed)
     if ((rc = alloc(st, p, NH))) return fail(rc);
-  for (int q = 0; q < 6; ++q) {
+  for (int q = 0; q < (st->compressible ? 7 : 6); ++q) {
     if ((rc = alloc(st, &st->p0[q], NH))) return fail(rc);
     if ((rc = alloc(st, &st->p1[q], NH))) return fail(rc);
+  }
+  if (st->compressible) {  // Increments.jl:91-94, Fluxes.jl:89-92
+    if ((rc = alloc(st, &st->dp, NH))) return fail(rc);
+    for (int d = 0; d < 3; ++d) {
+      if ((rc = alloc(st, &st->flux[6][d], NH))) return fail(rc);
+      if ((rc = alloc(st, &st->phitheta[d], NH))) return fail(rc);
+    }
   }
   for (int q = 0; q < (st->tracer ? 6 : 5); ++q) {
     if ((rc = alloc(st, &st->phi[q], NH))) return fail(rc);
@@ -356,11 +383,13 @@ int pcf_set_time(pcf_state* st, double time_s) {  // integrate.jl:168-196
 
 int pcf_initial_cleaning(pcf_state* st, int32_t* niter) {  // integrate.jl:136-156
   if (!st) return PCF_E_ARG;
+  TRY(modify_compressible_wind(st, true));
   TRY(set_boundaries_predictands(st));
   int it = 0;
   int rc = apply_corrector(st, 1.0, 1.0, &it);
   if (niter) *niter = it;
   if (rc) return rc;
+  TRY(modify_compressible_wind(st, false));
   TRY(set_boundaries_predictands(st));
   if (hipStreamSynchronize(st->stream) != hipSuccess) return PCF_E_HIP;
   return PCF_OK;
@@ -378,9 +407,9 @@ int pcf_compute_sponges(pcf_state* st, double dt, double time) {
 
 int pcf_backup_predictands(pcf_state* st) {
   if (!st) return PCF_E_ARG;
-  double* cur[6];
-  predictand_slots(st, cur);
-  TRY(copy_set(st, st->p0, cur, 6));
+  double* cur[7];
+  const int np = predictand_slots(st, cur);
+  TRY(copy_set(st, st->p0, cur, np));
   if (st->tracer) TRY(copy_field(st, st->chi0, st->chi));  // backup_predictands.jl:18
   return PCF_OK;
 }
@@ -420,12 +449,13 @@ int pcf_step(pcf_state* st, pcf_step_info* info) {
   st->time += dt;
   const double time = st->time;
   TRY(compute_sponges(st, dt, time));
-  if (n.model == PCF_PSEUDO_INCOMPRESSIBLE) TRY(copy_field(st, st->rhop, st->rho));  // synchronize_density_fluctuations!
+  TRY(synchronize_density_fluctuations(st));
   TRY(set_boundaries_predictands(st));
-  double* cur[6];
-  predictand_slots(st, cur);
-  TRY(copy_set(st, st->p0, cur, 6));  // backup_predictands
+  double* cur[7];
+  const int np = predictand_slots(st, cur);
+  TRY(copy_set(st, st->p0, cur, np));  // backup_predictands
   if (st->tracer) TRY(copy_field(st, st->chi0, st->chi));
+  TRY(compute_theta_fluxes(st));                              // integrate.jl:279
   TRY(explicit_lhs(st, 0.5 * dt, time));                      // (1)
   int it1 = 0, it2 = 0;
   st->solve_slot = 0;
@@ -433,16 +463,23 @@ int pcf_step(pcf_state* st, pcf_step_info* info) {
   st->solve_slot = -1;
   info->niter[0] = it1;
   if (rc) return rc;
-  TRY(copy_set(st, st->p1, cur, 6));                          // p1 = predictands
-  // reset_predictands!: rho (PI), rhop, u, v, w; pip keeps its new value
+  TRY(copy_set(st, st->p1, cur, np));                         // p1 = predictands
+  // reset_predictands! (reset_predictands!.jl:71-118): rho (PI and
+  // compressible), rhop, u, v, w; pip and p too in the compressible model,
+  // else pip keeps its new value
   if (n.model != PCF_BOUSSINESQ) TRY(copy_field(st, st->rho, st->p0[0]));
   TRY(copy_field(st, st->rhop, st->p0[1]));
   TRY(copy_field(st, st->u, st->p0[2]));
   TRY(copy_field(st, st->v, st->p0[3]));
   TRY(copy_field(st, st->w, st->p0[4]));
+  if (st->compressible) {
+    TRY(copy_field(st, st->pip, st->p0[5]));
+    TRY(copy_field(st, st->p, st->p0[6]));
+  }
   if (st->tracer) TRY(copy_field(st, st->chi, st->chi0));     // reset_predictands!.jl:63-67
   TRY(explicit_rhs(st, 0.5 * dt, time));                      // (3)
-  TRY(copy_set(st, st->p0, st->p1, 6));                       // p0 = p1
+  TRY(copy_set(st, st->p0, st->p1, np));                      // p0 = p1
+  TRY(synchronize_compressible_atmosphere(st, st->p0[6]));   // pbar = p1.p
   TRY(explicit_lhs(st, dt, time));                            // (4)
   st->solve_slot = 1;
   rc = implicit(st, 0.5 * dt, time, 2.0, &it2);               // (5)
@@ -473,6 +510,21 @@ int pcf_set_boundaries(pcf_state* st, int which) {
 int pcf_compute_fluxes(pcf_state* st) {
   if (!st) return PCF_E_ARG;
   return compute_fluxes(st);
+}
+
+int pcf_compute_theta_fluxes(pcf_state* st) {
+  if (!st) return PCF_E_ARG;
+  return compute_theta_fluxes(st);
+}
+
+int pcf_modify_compressible_wind(pcf_state* st, int multiply) {
+  if (!st) return PCF_E_ARG;
+  return modify_compressible_wind(st, multiply != 0);
+}
+
+int pcf_synchronize_compressible_atmosphere(pcf_state* st) {
+  if (!st) return PCF_E_ARG;
+  return synchronize_compressible_atmosphere(st, st->p);
 }
 
 int pcf_update_lhs(pcf_state* st, double dt, int32_t m, int var) {
@@ -549,6 +601,7 @@ int pcf_set_field(pcf_state* st, int field, const double* host_in) {
   if (!p) { set_error("unknown field"); return PCF_E_ARG; }
   PCF_HIP(hipMemcpyAsync(p, host_in, st->g.nh() * sizeof(double), hipMemcpyHostToDevice, st->stream));
   PCF_HIP(hipStreamSynchronize(st->stream));
+  if (field == PCF_F_PBAR || field == PCF_F_RHOBAR) TRY(refresh_fcscal(st));
   return PCF_OK;
 }
 

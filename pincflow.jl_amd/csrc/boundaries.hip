@@ -71,7 +71,7 @@ __global__ void k_bc_z(double* f, Geo g, double sign, int staggered) {
 
 // set_vertical_boundaries!(state, BoundaryFluxes()) (set_vertical_boundaries!.jl:274-347)
 __global__ void k_bc_fluxes(double* chi3, Geo g, double* rho3, double* rhop3, double* u3, double* v3,
-                            double* w3) {
+                            double* w3, double* p3, double* theta3) {
   int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   int j = 1 + blockIdx.y;
   if (i > g.nxx) return;
@@ -82,6 +82,8 @@ __global__ void k_bc_fluxes(double* chi3, Geo g, double* rho3, double* rhop3, do
   v3[b] = 0.0; v3[t] = 0.0;
   w3[g.h(i, j, g.k0 - 2)] = 0.0; w3[t] = 0.0;
   if (chi3) { chi3[b] = 0.0; chi3[t] = 0.0; }  // set_tracer_vertical_boundaries!.jl, BoundaryFluxes
+  if (p3) { p3[b] = 0.0; p3[t] = 0.0; }        // compressible: set_vertical_boundaries!.jl:324-347
+  if (theta3) { theta3[b] = 0.0; theta3[t] = 0.0; }
 }
 
 }  // namespace
@@ -117,10 +119,11 @@ int bc_z(pcf_state* st, double* f, int mode, bool staggered) {
 int set_boundaries_predictands(pcf_state* st) {
   ProfScope prof(st, PCF_K_UPDATES);
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
-  double* f[6];
+  double* f[7];
   int n = 0;
   if (pi) f[n++] = st->rho;
   f[n++] = st->rhop; f[n++] = st->u; f[n++] = st->v; f[n++] = st->w; f[n++] = st->pip;
+  if (st->compressible) f[n++] = st->p;
   int rc = bc_xy(st, f, n);
   if (rc) return rc;
   if (pi && (rc = bc_z(st, st->rho, -1, false))) return rc;
@@ -129,6 +132,7 @@ int set_boundaries_predictands(pcf_state* st) {
   if ((rc = bc_z(st, st->v, +1, false))) return rc;
   if ((rc = bc_z(st, st->w, -1, true))) return rc;
   if ((rc = bc_z(st, st->pip, +1, false))) return rc;
+  if (st->compressible && (rc = bc_z(st, st->p, +1, false))) return rc;  // set_vertical_boundaries!.jl:217-234
   if (st->tracer) {  // set_tracer_*_boundaries!.jl, BoundaryPredictands
     if ((rc = bc_xy(st, &st->chi, 1))) return rc;
     if ((rc = bc_z(st, st->chi, -1, false))) return rc;
@@ -141,7 +145,8 @@ int set_boundaries_fluxes(pcf_state* st) {
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
   hipLaunchKernelGGL(k_bc_fluxes, dim3(blocks_for(g.nxx, 64), g.nyy), dim3(64), 0, st->stream,
                      st->tracer ? st->flux[5][2] : nullptr, g,
-                     pi ? st->flux[0][2] : nullptr, st->flux[1][2], st->flux[2][2], st->flux[3][2], st->flux[4][2]);
+                     pi ? st->flux[0][2] : nullptr, st->flux[1][2], st->flux[2][2], st->flux[3][2], st->flux[4][2],
+                     st->compressible ? st->flux[6][2] : nullptr, st->compressible ? st->phitheta[2] : nullptr);
   PCF_HIP(hipGetLastError());
   return PCF_OK;
 }

@@ -1,6 +1,7 @@
 // Explicit part of the step on the device: src/FluxCalculator (reconstruct!,
 // apply_3d_muscl!, compute_fluxes!) and src/Update (update!, apply_lhs_sponge!,
-// compute_sponges!) for the Boussinesq and pseudo-incompressible models.
+// compute_sponges!) for the Boussinesq, pseudo-incompressible and
+// compressible models.
 //
 // The MUSCL reconstruction is evaluated inside the flux kernels from the
 // reconstruction inputs phi (reconstruct!.jl:126-286), so the 30 haloed
@@ -19,7 +20,7 @@ namespace pcf {
 
 namespace {
 
-enum { V_RHO = 0, V_RHOP, V_U, V_V, V_W, V_CHI };
+enum { V_RHO = 0, V_RHOP, V_U, V_V, V_W, V_CHI, V_P };
 
 struct FluxArgs {
   Fields F;
@@ -28,7 +29,7 @@ struct FluxArgs {
   const double* v0;
   const double* w0;
   const double* st[6];  // stress 11, 12, 13, 22, 23, 33
-  double* flux[6][3];
+  double* flux[7][3];
   Geo g;
   double inv_re;        // 1 / re
   int viscous;
@@ -312,6 +313,92 @@ __global__ void k_flux(FluxArgs A, Range3 R) {
   A.flux[VAR][DIR][c] = val;
 }
 
+// compute_fluxes!(..., P()) (compute_fluxes!.jl:531-585): the carrier flux
+// J P u of the compressible model
+template <int DIR>
+__global__ void k_flux_p(FluxArgs A, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const Geo& g = A.g;
+  const size_t c = g.h(i, j, k);
+  double val;
+  if (DIR == 0) val = pe_x(A.F, g, i, j, k) * A.u0[c];
+  else if (DIR == 1) val = pe_y(A.F, g, i, j, k) * A.v0[c];
+  else val = pe_z(A.F, g, i, j, k) * A.w0[c];
+  A.flux[V_P][DIR][c] = val;
+}
+
+// compute_fluxes!(state, p0, Theta()) (compute_fluxes!.jl:1592-1756): heat
+// conduction with theta = pbar / (rho0 + rhobar); the vertical wall faces,
+// which set_boundaries!(state, BoundaryFluxes()) zeroes before any use, are
+// written as zeros.
+struct ThetaArgs {
+  const double *rho0, *rhobar, *pbar, *jac, *m13, *m23, *m33;
+  double* f[3];
+  Geo g;
+  double mu;  // thermal_conductivity / uref / lref
+};
+
+template <int DIR>
+__global__ void k_flux_theta(ThetaArgs A, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const Geo& g = A.g;
+  const double *rb = A.rhobar, *pb = A.pbar, *rho = A.rho0, *jac = A.jac;
+  auto th = [&](int ii, int jj, int kk) {
+    const size_t q = g.h(ii, jj, kk);
+    return pb[q] / (rho[q] + rb[q]);
+  };
+  const size_t c = g.h(i, j, k);
+  double val;
+  if (DIR == 0) {
+    const size_t r = g.h(i + 1, j, k);
+    double coef_t = A.mu * 0.5 * (rb[g.h(i, j, g.k0)] / rb[c] + rb[g.h(i + 1, j, g.k0)] / rb[r]);
+    double thetal = th(i, j, k);
+    double thetar = th(i + 1, j, k);
+    double thetad = 0.5 * (th(i, j, k - 1) + th(i + 1, j, k - 1));
+    double thetau = 0.5 * (th(i, j, k + 1) + th(i + 1, j, k + 1));
+    double d = 0.5 * (jac[c] + jac[r]) * (thetar - thetal) / g.dx +
+               0.5 * (jac[c] * A.m13[c] + jac[r] * A.m13[r]) * (thetau - thetad) / (2.0 * g.dz);
+    val = -coef_t * d;
+  } else if (DIR == 1) {
+    const size_t f = g.h(i, j + 1, k);
+    double coef_t = A.mu * 0.5 * (rb[g.h(i, j, g.k0)] / rb[c] + rb[g.h(i, j + 1, g.k0)] / rb[f]);
+    double thetab = th(i, j, k);
+    double thetaf = th(i, j + 1, k);
+    double thetad = 0.5 * (th(i, j, k - 1) + th(i, j + 1, k - 1));
+    double thetau = 0.5 * (th(i, j, k + 1) + th(i, j + 1, k + 1));
+    double d = 0.5 * (jac[c] + jac[f]) * (thetaf - thetab) / g.dy +
+               0.5 * (jac[c] * A.m23[c] + jac[f] * A.m23[f]) * (thetau - thetad) / (2.0 * g.dz);
+    val = -coef_t * d;
+  } else if (k == g.k0 - 1 || k == g.k1) {
+    val = 0.0;
+  } else {
+    const size_t u = g.h(i, j, k + 1);
+    const double jc = jac[c], ju = jac[u];
+    // rhobar of the lowest halo layer (index 1) as the reference takes it
+    const double rb1 = rb[g.h(i, j, 1)];
+    double coef_t = A.mu * (ju * rb1 / rb[c] + jc * rb1 / rb[u]) / (ju + jc);
+    auto edge = [&](int ii, int jj) {  // (J pbar) / (rho + rhobar), weighted to k + 1/2
+      const size_t q = g.h(ii, jj, k), qu = g.h(ii, jj, k + 1);
+      return (jac[qu] * pb[q] / (rho[q] + rb[q]) + jac[q] * pb[qu] / (rho[qu] + rb[qu])) / (jac[qu] + jac[q]);
+    };
+    double thetal = edge(i - 1, j);
+    double thetar = edge(i + 1, j);
+    double thetab = edge(i, j - 1);
+    double thetaf = edge(i, j + 1);
+    double thetad = th(i, j, k);
+    double thetau = th(i, j, k + 1);
+    double d = jc * ju *
+               ((A.m13[c] + A.m13[u]) * (thetar - thetal) / (2.0 * g.dx) +
+                (A.m23[c] + A.m23[u]) * (thetaf - thetab) / (2.0 * g.dy) +
+                (A.m33[c] + A.m33[u]) * (thetau - thetad) / g.dz) /
+               (jc + ju);
+    val = -coef_t * d;
+  }
+  A.f[DIR][c] = val;
+}
+
 // ---------------------------------------------------------------------------
 // RK3 updates of the left-hand side (update!.jl:505-1192)
 // ---------------------------------------------------------------------------
@@ -330,6 +417,7 @@ struct UpdArgs {
   double dt, am, bm;
   int first;
   int force0;  // the tracer adds its (zero) volume force, update!.jl:1490-1492
+  int heat;    // 1: RhoP, -heating / thetabar (update!.jl:586-594); 2: P, +heating (update!.jl:1436-1442)
 };
 
 __device__ __forceinline__ double divergence(const double* const* f, const Geo& g, int i, int j, int k) {
@@ -346,6 +434,8 @@ __global__ void k_update_scalar(UpdArgs A, Range3 R) {  // update!.jl:521-597
   fluxdiff /= A.F.jac[c];
   double fr = -fluxdiff;
   if (A.force0) fr = fr + 0.0;
+  if (A.heat == 1) fr = fr - conductive_heating(A.F, A.g, i, j, k) / A.F.thetabar[c];
+  if (A.heat == 2) fr = fr + conductive_heating(A.F, A.g, i, j, k);
   double dqv = A.first ? 0.0 : A.dq[c];
   dqv = A.dt * fr + A.am * dqv;
   A.dq[c] = dqv;
@@ -482,12 +572,13 @@ __global__ void k_rhs_u(RhsArgs A, Range3 R) {
   rhoedger += rhobaredger;
   double gradient = pressure_gradient_u(F, A.ph, g, A.pip, i, j, k);
   double force = 0.0;
+  double jpedger = wind_factor_u(F, g, i, j, k);
   if (!A.implicit) {
-    A.u[c] += A.dt * (-gradient + force / rhoedger) * 1.0;
+    A.u[c] += A.dt * (-gradient + force / rhoedger) * jpedger;
   } else {
     double factor = 1.0;
     if (A.ph.damp) factor += A.dt * 0.5 * (A.betar[c] + A.betar[r]) * A.rf;
-    A.u[c] = 1.0 / factor * (A.u[c] + A.dt * (-gradient + force / rhoedger) * 1.0);
+    A.u[c] = 1.0 / factor * (A.u[c] + A.dt * (-gradient + force / rhoedger) * jpedger);
   }
 }
 
@@ -502,12 +593,13 @@ __global__ void k_rhs_v(RhsArgs A, Range3 R) {
   rhoedgef += rhobaredgef;
   double gradient = pressure_gradient_v(F, A.ph, g, A.pip, i, j, k);
   double force = 0.0;
+  double jpedgef = wind_factor_v(F, g, i, j, k);
   if (!A.implicit) {
-    A.v[c] += A.dt * (-gradient + force / rhoedgef) * 1.0;
+    A.v[c] += A.dt * (-gradient + force / rhoedgef) * jpedgef;
   } else {
     double factor = 1.0;
     if (A.ph.damp) factor += A.dt * 0.5 * (A.betar[c] + A.betar[f]) * A.rf;
-    A.v[c] = 1.0 / factor * (A.v[c] + A.dt * (-gradient + force / rhoedgef) * 1.0);
+    A.v[c] = 1.0 / factor * (A.v[c] + A.dt * (-gradient + force / rhoedgef) * jpedgef);
   }
 }
 
@@ -528,18 +620,21 @@ __global__ void k_rhs_w(RhsArgs A, Range3 R) {
   double force = 0.0;
   if (!A.implicit) {
     double b = -gnd * (ju * A.rhopold[c] / rhoc / jc + jc * A.rhopold[up] / rhou / ju) / (jc + ju);
-    A.w[c] += dt * (b - gradient + force / rhoedgeu) * 1.0;
+    double jpedgeu = wind_factor_w(F, g, i, j, k);
+    A.w[c] += dt * (b - gradient + force / rhoedgeu) * jpedgeu;
   } else {
     const double dt2 = dt * dt;
     double n2edgeu = (ju * F.n2[c] + jc * F.n2[up]) / (jc + ju);
     double factor = 1.0;
     factor += dt * (ju * A.betar[c] + jc * A.betar[up]) / (jc + ju) * A.rf;
     double b = -gnd * (ju * A.rhop[c] / rhoc / jc + jc * A.rhop[up] / rhou / ju) / (jc + ju);
-    double uc = 0.5 * (A.u[c] / 1.0 + A.u[g.h(i - 1, j, k)] / 1.0);
-    double uu = 0.5 * (A.u[up] / 1.0 + A.u[g.h(i - 1, j, k + 1)] / 1.0);
-    double vc = 0.5 * (A.v[c] / 1.0 + A.v[g.h(i, j - 1, k)] / 1.0);
-    double vu = 0.5 * (A.v[up] / 1.0 + A.v[g.h(i, j - 1, k + 1)] / 1.0);
-    double jpedgeu = 1.0;
+    double uc = 0.5 * (A.u[c] / wind_factor_u(F, g, i, j, k) + A.u[g.h(i - 1, j, k)] / wind_factor_u(F, g, i - 1, j, k));
+    double uu = 0.5 * (A.u[up] / wind_factor_u(F, g, i, j, k + 1) +
+                       A.u[g.h(i - 1, j, k + 1)] / wind_factor_u(F, g, i - 1, j, k + 1));
+    double vc = 0.5 * (A.v[c] / wind_factor_v(F, g, i, j, k) + A.v[g.h(i, j - 1, k)] / wind_factor_v(F, g, i, j - 1, k));
+    double vu = 0.5 * (A.v[up] / wind_factor_v(F, g, i, j, k + 1) +
+                       A.v[g.h(i, j - 1, k + 1)] / wind_factor_v(F, g, i, j - 1, k + 1));
+    double jpedgeu = wind_factor_w(F, g, i, j, k);
     double fw = buoyancy_factor_w(F, g, i, j, k);
     A.w[c] = 1.0 / (factor + fw * n2edgeu * dt2) *
              (A.w[c] - dt * gradient * jpedgeu + dt * b * jpedgeu + dt * force / rhoedgeu * jpedgeu +
@@ -557,7 +652,7 @@ __global__ void k_rhs_rhop_explicit(RhsArgs A, Range3 R) {
   const Fields& F = A.F;
   const size_t c = g.h(i, j, k);
   const double gnd = A.ph.g_ndim;
-  double jpu = 1.0, jpd = 1.0;
+  double jpu = wind_factor_w(F, g, i, j, k), jpd = wind_factor_w(F, g, i, j, k - 1);
   double wvrt = 0.5 * (vertical_wind(F, g, i, j, k) / jpu + vertical_wind(F, g, i, j, k - 1) / jpd);
   double buoy = -gnd * A.rhop[c] / (F.rho[c] + F.rhobar[c]);
   double fb = buoyancy_factor_rhop(F, g, i, j, k);
@@ -579,7 +674,7 @@ __global__ void k_rhs_rhop_implicit(RhsArgs A, Range3 R) {
   rhoedgeu += (ju * F.rhobar[c] + jc * F.rhobar[up]) / (jc + ju);
   double rhoedged = (jd * F.rho[c] + jc * F.rho[dn]) / (jc + jd);
   rhoedged += (jd * F.rhobar[c] + jc * F.rhobar[dn]) / (jc + jd);
-  double jpedgeu = 1.0, jpedged = 1.0;
+  double jpedgeu = wind_factor_w(F, g, i, j, k), jpedged = wind_factor_w(F, g, i, j, k - 1);
   double wv = 0.5 * (A.wold[c] / jpedgeu + A.wold[dn] / jpedged);
   double lower_gradient = pressure_gradient_w(F, A.ph, g, A.pip, i, j, k - 1);
   double lower_force = 0.0;
@@ -597,13 +692,99 @@ __global__ void k_rhs_rhop_implicit(RhsArgs A, Range3 R) {
   double factor = 1.0;
   factor += dt * A.betar[c] * A.rf;
   double b = -gnd * A.rhop[c] / (F.rho[c] + F.rhobar[c]);
+  double jpedger = wind_factor_u(F, g, i, j, k), jpedgel = wind_factor_u(F, g, i - 1, j, k);
+  double jpedgef = wind_factor_v(F, g, i, j, k), jpedgeb = wind_factor_v(F, g, i, j - 1, k);
   double fb = buoyancy_factor_rhop(F, g, i, j, k);
   const double n2c = F.n2[c];
   b = 1.0 / (factor + fb * n2c * dt2) *
       (-fb * n2c * dt * jc * (wv + dt * (-gradient + force / rhoc)) + factor * b +
        fb * n2c * dt * jc * factor * 0.5 *
-           (F.m13[c] * (A.u[c] / 1.0 + A.u[g.h(i - 1, j, k)] / 1.0) + F.m23[c] * (A.v[c] / 1.0 + A.v[g.h(i, j - 1, k)] / 1.0)));
+           (F.m13[c] * (A.u[c] / jpedger + A.u[g.h(i - 1, j, k)] / jpedgel) +
+            F.m23[c] * (A.v[c] / jpedgef + A.v[g.h(i, j - 1, k)] / jpedgeb)));
   A.rhop[c] = -b * (F.rho[c] + F.rhobar[c]) / gnd;
+}
+
+// update!(state, dt, PiP()) (update!.jl:1360-1392, compressible): pi' follows
+// the divergence of the J P-weighted winds saved before the explicit RHS step
+__global__ void k_rhs_pip(RhsArgs A, double* pip, const double* uold, const double* vold, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const Geo& g = A.g;
+  const size_t c = g.h(i, j, k);
+  double fluxdiff = (uold[c] - uold[g.h(i - 1, j, k)]) / g.dx + (vold[c] - vold[g.h(i, j - 1, k)]) / g.dy +
+                    (A.wold[c] - A.wold[g.h(i, j, k - 1)]) / g.dz;
+  fluxdiff /= A.F.jac[c];
+  double heating = conductive_heating(A.F, g, i, j, k);
+  double d = dpdpi(A.ph, A.F.p[c]);
+  pip[c] -= A.dt * (fluxdiff - heating) / d;
+}
+
+// modify_compressible_wind!.jl:52-80 over the interior: u, v, w times
+// (multiply) or divided by J P at their points
+__global__ void k_modify_wind(double* u, double* v, double* w, const double* p, const double* jac, Geo g, Range3 R,
+                              int multiply) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const size_t c = g.h(i, j, k), r = g.h(i + 1, j, k), f = g.h(i, j + 1, k), up = g.h(i, j, k + 1);
+  double fu = (jac[c] * p[c] + jac[r] * p[r]) / 2;
+  double fv = (jac[c] * p[c] + jac[f] * p[f]) / 2;
+  double fw = jac[c] * jac[up] * (p[c] + p[up]) / (jac[c] + jac[up]);
+  if (multiply) {
+    u[c] = u[c] * fu;
+    v[c] = v[c] * fv;
+    w[c] = w[c] * fw;
+  } else {
+    u[c] = u[c] / fu;
+    v[c] = v[c] / fv;
+    w[c] = w[c] / fw;
+  }
+}
+
+// synchronize_compressible_atmosphere!.jl:61-72 (pbar .= p) over the whole
+// haloed array, with the Poisson scaling sqrt(pbar^2 / rhobar) that follows pbar
+__global__ void k_sync_pbar(double* pbar, double* fcscal, const double* p, const double* rhobar, size_t n) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    const double v = p[q];
+    pbar[q] = v;
+    fcscal[q] = sqrt(v * v / rhobar[q]);
+  }
+}
+
+// synchronize_density_fluctuations!.jl:76-86 (compressible): rhop = rho + rhobar - pbar / thetabar
+__global__ void k_sync_rhop(double* rhop, const double* rho, const double* rhobar, const double* pbar,
+                            const double* thetabar, size_t n) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    rhop[q] = rho[q] + rhobar[q] - pbar[q] / thetabar[q];
+}
+
+// apply_lhs_sponge! of the compressible model: RhoP towards
+// rhobar (1 - P / thetabar / (rho + rhobar)) (apply_lhs_sponge!.jl:301-328),
+// P towards rhobar P / (rho + rhobar) (:580-602) and pi' (:542-568)
+__global__ void k_lhs_sponge_comp(Fields F, Phys ph, double* rhop, double* p, double* pip, const double* alphar, Geo g,
+                                  Range3 R, double dt, int var) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  const size_t c = g.h(i, j, k);
+  const double rhoc = F.rho[c], rb = F.rhobar[c];
+  if (var == PCF_VAR_RHOP) {
+    double rhopbg = rb * (1.0 - p[c] / F.thetabar[c] / (rhoc + rb));
+    double alpha = alphar[c];
+    double old = rhop[c];
+    double beta = 1.0 / (1.0 + alpha * dt);
+    rhop[c] = (1.0 - beta) * rhopbg + beta * old;
+  } else if (var == PCF_VAR_P) {
+    double pb = rb * p[c] / (rhoc + rb);
+    double alpha = alphar[c];
+    double old = p[c];
+    double beta = 1 / (1 + alpha * dt);
+    p[c] = (1 - beta) * pb + beta * old;
+  } else {
+    double d = dpdpi(ph, p[c]);
+    double pib = rb * p[c] / (rhoc + rb) / d;
+    double alpha = alphar[c];
+    double old = pip[c];
+    pip[c] = old - alpha * dt * (p[c] / d - pib);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -763,7 +944,7 @@ int compute_fluxes(pcf_state* st) {
   A.v0 = st->p0[3];
   A.w0 = st->p0[4];
   for (int q = 0; q < 6; ++q) A.st[q] = st->stress[q];
-  for (int q = 0; q < 6; ++q)
+  for (int q = 0; q < 7; ++q)
     for (int d = 0; d < 3; ++d) A.flux[q][d] = st->flux[q][d];
   A.g = g;
   A.inv_re = 1 / st->ph.re;
@@ -793,8 +974,74 @@ int compute_fluxes(pcf_state* st) {
     launch_flux<V_CHI, 1>(st, A, Range3{i0, nx, j0 - 1, ny + 1, k0, nz});
     launch_flux<V_CHI, 2>(st, A, Range3{i0, nx, j0, ny, k0, nz - 1});
   }
+  if (st->compressible) {
+    Range3 rx{i0 - 1, nx + 1, j0, ny, k0, nz}, ry{i0, nx, j0 - 1, ny + 1, k0, nz}, rz{i0, nx, j0, ny, k0, nz - 1};
+    hipLaunchKernelGGL(k_flux_p<0>, dim3(blocks_for(rx.count(), 256)), dim3(256), 0, st->stream, A, rx);
+    hipLaunchKernelGGL(k_flux_p<1>, dim3(blocks_for(ry.count(), 256)), dim3(256), 0, st->stream, A, ry);
+    if (rz.nk > 0) hipLaunchKernelGGL(k_flux_p<2>, dim3(blocks_for(rz.count(), 256)), dim3(256), 0, st->stream, A, rz);
+  }
   PCF_HIP(hipGetLastError());
   return set_boundaries_fluxes(st);
+}
+
+// compute_fluxes!(state, p0, Theta()) (compute_fluxes!.jl:1592-1756),
+// compressible model with a nonzero thermal conductivity
+int compute_theta_fluxes(pcf_state* st) {
+  if (!st->compressible || st->cfg.thermal_conductivity == 0.0) return PCF_OK;
+  ProfScope prof(st, PCF_K_FLUXES);
+  const Geo& g = st->g;
+  ThetaArgs A;
+  A.rho0 = st->p0[0]; A.rhobar = st->rhobar; A.pbar = st->pbar; A.jac = st->jac;
+  A.m13 = st->met13; A.m23 = st->met23; A.m33 = st->met33;
+  for (int d = 0; d < 3; ++d) A.f[d] = st->phitheta[d];
+  A.g = g;
+  A.mu = st->cfg.thermal_conductivity / st->uref / st->lref;
+  Range3 rx{g.i0 - 1, g.nx + 1, g.j0, g.ny, g.k0, g.nz}, ry{g.i0, g.nx, g.j0 - 1, g.ny + 1, g.k0, g.nz};
+  Range3 rz{g.i0, g.nx, g.j0, g.ny, g.k0 - 1, g.nz + 1};
+  hipLaunchKernelGGL(k_flux_theta<0>, dim3(blocks_for(rx.count(), 256)), dim3(256), 0, st->stream, A, rx);
+  hipLaunchKernelGGL(k_flux_theta<1>, dim3(blocks_for(ry.count(), 256)), dim3(256), 0, st->stream, A, ry);
+  hipLaunchKernelGGL(k_flux_theta<2>, dim3(blocks_for(rz.count(), 256)), dim3(256), 0, st->stream, A, rz);
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+int modify_compressible_wind(pcf_state* st, bool multiply) {
+  if (!st->compressible) return PCF_OK;
+  ProfScope prof(st, PCF_K_UPDATES);
+  const Geo& g = st->g;
+  Range3 R{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
+  hipLaunchKernelGGL(k_modify_wind, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, st->u, st->v, st->w,
+                     st->p, st->jac, g, R, multiply ? 1 : 0);
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+int synchronize_compressible_atmosphere(pcf_state* st, const double* p) {
+  if (!st->compressible) return PCF_OK;
+  ProfScope prof(st, PCF_K_UPDATES);
+  hipLaunchKernelGGL(k_sync_pbar, dim3(2048), dim3(256), 0, st->stream, st->pbar, st->fcscal, p, st->rhobar,
+                     st->g.nh());
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+// sqrt(pbar^2 / rhobar) after pbar or rhobar changed from the host
+int refresh_fcscal(pcf_state* st) {
+  hipLaunchKernelGGL(k_sync_pbar, dim3(2048), dim3(256), 0, st->stream, st->pbar, st->fcscal, st->pbar, st->rhobar,
+                     st->g.nh());
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+int synchronize_density_fluctuations(pcf_state* st) {
+  ProfScope prof(st, PCF_K_UPDATES);
+  if (st->cfg.model == PCF_PSEUDO_INCOMPRESSIBLE) return copy_field(st, st->rhop, st->rho);
+  if (st->cfg.model == PCF_COMPRESSIBLE) {
+    hipLaunchKernelGGL(k_sync_rhop, dim3(2048), dim3(256), 0, st->stream, st->rhop, st->rho, st->rhobar, st->pbar,
+                       st->thetabar, st->g.nh());
+    PCF_HIP(hipGetLastError());
+  }
+  return PCF_OK;
 }
 
 int update_lhs(pcf_state* st, double dt, int m, int var) {
@@ -810,6 +1057,7 @@ int update_lhs(pcf_state* st, double dt, int m, int var) {
   A.bm = st->betark[m - 1];
   A.first = m == 1;
   A.force0 = 0;
+  A.heat = 0;
   A.rhoold = st->rhoold;
   A.uold = st->uold;
   A.vold = st->vold;
@@ -825,7 +1073,18 @@ int update_lhs(pcf_state* st, double dt, int m, int var) {
       const int q = var == PCF_VAR_RHO ? V_RHO : V_RHOP;
       A.q = var == PCF_VAR_RHO ? st->rho : st->rhop;
       A.dq = var == PCF_VAR_RHO ? st->drho : st->drhop;
+      A.heat = var == PCF_VAR_RHOP && st->compressible ? 1 : 0;
       for (int d = 0; d < 3; ++d) A.f[d] = st->flux[q][d];
+      R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
+      hipLaunchKernelGGL(k_update_scalar, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      break;
+    }
+    case PCF_VAR_P: {  // update!.jl:1400-1448
+      if (!st->compressible) return PCF_OK;
+      A.q = st->p;
+      A.dq = st->dp;
+      A.heat = 2;
+      for (int d = 0; d < 3; ++d) A.f[d] = st->flux[V_P][d];
       R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
       hipLaunchKernelGGL(k_update_scalar, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
       break;
@@ -898,6 +1157,12 @@ int update_rhs(pcf_state* st, double dt, int var, bool implicit, double rf) {
       R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz - 1};
       if (R.nk > 0) hipLaunchKernelGGL(k_rhs_w, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
       break;
+    case PCF_VAR_PIP:  // update!.jl:1351-1392: explicit, compressible only
+      if (!st->compressible || implicit) return PCF_OK;
+      R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
+      hipLaunchKernelGGL(k_rhs_pip, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, st->pip, st->uold,
+                         st->vold, R);
+      break;
     case PCF_VAR_RHOP:
       R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
       if (implicit)
@@ -917,7 +1182,14 @@ int apply_lhs_sponge(pcf_state* st, double dt, double time, int var) {
   (void)time;
   const Geo& g = st->g;
   const pcf_config& n = st->cfg;
-  if (var == PCF_VAR_PIP) return PCF_OK;
+  if (var == PCF_VAR_PIP || var == PCF_VAR_P || (var == PCF_VAR_RHOP && st->compressible)) {
+    if (!st->compressible) return PCF_OK;
+    Range3 R{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
+    hipLaunchKernelGGL(k_lhs_sponge_comp, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, fields_of(st),
+                       st->ph, st->rhop, st->p, st->pip, st->alphar, g, R, dt, var);
+    PCF_HIP(hipGetLastError());
+    return PCF_OK;
+  }
   if (var == PCF_VAR_RHO && n.model == PCF_BOUSSINESQ) return PCF_OK;
   if (var == PCF_VAR_CHI) {
     if (!st->tracer) return PCF_OK;

@@ -11,6 +11,10 @@ namespace pcf {
 struct Fields {
   const double *rho, *rhobar, *pbar, *jac, *m13, *m23, *m33, *n2;
   const double *u, *v, *w, *rhop;
+  // compressible model (comp = 1): thetabar, P and the heat-conduction fluxes
+  const double *thetabar, *p;
+  const double* phith[3];
+  int comp;
 };
 
 inline Fields make_fields(const pcf_state* st) {
@@ -18,21 +22,61 @@ inline Fields make_fields(const pcf_state* st) {
   F.rho = st->rho; F.rhobar = st->rhobar; F.pbar = st->pbar; F.jac = st->jac;
   F.m13 = st->met13; F.m23 = st->met23; F.m33 = st->met33; F.n2 = st->n2;
   F.u = st->u; F.v = st->v; F.w = st->w; F.rhop = st->rhop;
+  F.thetabar = st->thetabar; F.p = st->p;
+  for (int d = 0; d < 3; ++d) F.phith[d] = st->phitheta[d];
+  F.comp = st->compressible ? 1 : 0;
   return F;
 }
 
 __device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
 __device__ __forceinline__ double dmax(double a, double b) { return a < b ? b : a; }
 
-// compute_buoyancy_factor.jl:122-133 (RhoP, Boussinesq / PI)
+// compute_compressible_wind_factor.jl:102-157: 1 outside the compressible
+// model, else J P interpolated to the staggered wind points
+__device__ __forceinline__ double wind_factor_u(const Fields& F, const Geo& g, int i, int j, int k) {
+  if (!F.comp) return 1.0;
+  size_t c = g.h(i, j, k), r = g.h(i + 1, j, k);
+  return (F.jac[c] * F.p[c] + F.jac[r] * F.p[r]) / 2;
+}
+__device__ __forceinline__ double wind_factor_v(const Fields& F, const Geo& g, int i, int j, int k) {
+  if (!F.comp) return 1.0;
+  size_t c = g.h(i, j, k), f = g.h(i, j + 1, k);
+  return (F.jac[c] * F.p[c] + F.jac[f] * F.p[f]) / 2;
+}
+__device__ __forceinline__ double wind_factor_w(const Fields& F, const Geo& g, int i, int j, int k) {
+  if (!F.comp) return 1.0;
+  size_t c = g.h(i, j, k), u = g.h(i, j, k + 1);
+  return F.jac[c] * F.jac[u] * (F.p[c] + F.p[u]) / (F.jac[c] + F.jac[u]);
+}
+
+// conductive_heating.jl:67-117 (0 outside the compressible model)
+__device__ __forceinline__ double conductive_heating(const Fields& F, const Geo& g, int i, int j, int k) {
+  if (!F.comp) return 0.0;
+  const size_t c = g.h(i, j, k);
+  double rhotot = (F.rho[c] + F.rhobar[c]) / F.jac[c];
+  return -rhotot * ((F.phith[0][c] - F.phith[0][g.h(i - 1, j, k)]) / g.dx +
+                    (F.phith[1][c] - F.phith[1][g.h(i, j - 1, k)]) / g.dy +
+                    (F.phith[2][c] - F.phith[2][g.h(i, j, k - 1)]) / g.dz);
+}
+
+// dP/dpi (update!.jl:1385-1386) with the host-evaluated constant factor
+__device__ __forceinline__ double dpdpi(const Phys& ph, double p) {
+  return ph.dpdpi_c * pow(p, 2 - ph.gamma);
+}
+
+// compute_buoyancy_factor.jl:108-133 (RhoP)
 __device__ __forceinline__ double buoyancy_factor_rhop(const Fields& F, const Geo& g, int i, int j, int k) {
   size_t c = g.h(i, j, k);
+  if (F.comp) return F.pbar[c] / F.thetabar[c] / (F.rho[c] + F.rhobar[c]);
   return F.rhobar[c] / (F.rho[c] + F.rhobar[c]);
 }
 
-// compute_buoyancy_factor.jl:155-172 (W, Boussinesq / PI)
+// compute_buoyancy_factor.jl:135-172 (W)
 __device__ __forceinline__ double buoyancy_factor_w(const Fields& F, const Geo& g, int i, int j, int k) {
   size_t c = g.h(i, j, k), u = g.h(i, j, k + 1);
+  if (F.comp)
+    return (F.jac[u] * F.pbar[c] / F.thetabar[c] + F.jac[c] * F.pbar[u] / F.thetabar[u]) /
+           (F.jac[u] * (F.rho[c] + F.rhobar[c]) + F.jac[c] * (F.rho[u] + F.rhobar[u]));
   return (F.jac[u] * F.rhobar[c] + F.jac[c] * F.rhobar[u]) /
          (F.jac[u] * (F.rho[c] + F.rhobar[c]) + F.jac[c] * (F.rho[u] + F.rhobar[u]));
 }

@@ -46,6 +46,9 @@ struct Phys {
   int model;        // enum pcf_model
   int damp;         // damp_horizontal_wind_on_rhs
   int viscous;      // 1 / re > eps
+  // compressible model: gamma and dP/dpi = dpdpi_c p^(2 - gamma) with
+  // dpdpi_c = 1 / (gamma - 1) (rsp / pref)^(1 - gamma) (update!.jl:1385-1386)
+  double gamma, rsp, dpdpi_c;
 };
 
 // Index range of a kernel launch (1-based, inclusive start, counts).
@@ -124,9 +127,14 @@ struct pcf_state {
   hipStream_t stream = nullptr;
   int device = 0;
 
-  // predictands and copies (haloed)
+  // predictands and copies (haloed); P and its copies only in the
+  // compressible model (set_p.jl), else null
   double *rho, *rhop, *u, *v, *w, *pip;
-  double *p0[6], *p1[6];  // rho, rhop, u, v, w, pip
+  double* p = nullptr;
+  double *p0[7] = {}, *p1[7] = {};  // rho, rhop, u, v, w, pip, p
+  bool compressible = false;
+  double *dp = nullptr;
+  double* phitheta[3] = {};           // heat-conduction fluxes (compressible)
   double *drho, *drhop, *du, *dv, *dw;
   double *rhoold, *rhopold, *uold, *vold, *wold;
   // tracer (cfg.tracer_setup == PCF_TRACER_ON): rho chi, its increment, its
@@ -138,7 +146,7 @@ struct pcf_state {
   // sponges
   double *alphar, *betar, *betar_s;  // betar_s: dt-independent part of betar
   // fluxes (haloed): [var][dir]
-  double* flux[6][3] = {};            // rho, rhop, u, v, w, chi
+  double* flux[7][3] = {};            // rho, rhop, u, v, w, chi, p
   double* phi[6] = {};                // reconstruction inputs
   double* wcart;                      // Cartesian vertical wind
   double* stress[6];                  // 11, 12, 13, 22, 23, 33
@@ -263,6 +271,11 @@ int apply_lhs_sponge(pcf_state* st, double dt, double time, int var);
 int compute_sponges(pcf_state* st, double dt, double time);
 int cfl_maxima(pcf_state* st, double out[3]);
 int copy_field(pcf_state* st, double* dst, const double* src);
+int compute_theta_fluxes(pcf_state* st);
+int modify_compressible_wind(pcf_state* st, bool multiply);
+int synchronize_compressible_atmosphere(pcf_state* st, const double* p);
+int synchronize_density_fluctuations(pcf_state* st);
+int refresh_fcscal(pcf_state* st);
 
 // ---- launch helpers -------------------------------------------------------
 inline unsigned blocks_for(long long n, int threads) {

@@ -19,7 +19,7 @@ namespace pcf {
 namespace {
 
 // ---------------------------------------------------------------------------
-// compute_lhs! (compute_lhs!.jl:68-160)
+// compute_lhs! (compute_lhs!.jl:68-223)
 // ---------------------------------------------------------------------------
 __global__ void k_compute_lhs(Fields F, Phys ph, Geo g, double* rhs, double* partials) {
   int i = g.i0 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -32,6 +32,20 @@ __global__ void k_compute_lhs(Fields F, Phys ph, Geo g, double* rhs, double* par
     double fcscal = sqrt(pbar[c] * pbar[c] / F.rhobar[c]);
     double ur = F.u[c], ul = F.u[g.h(i - 1, j, k)], vf = F.v[c], vb = F.v[g.h(i, j - 1, k)];
     double wu = F.w[c], wd = F.w[g.h(i, j, k - 1)];
+    if (F.comp) {  // compute_lhs!.jl:162-223: the winds carry J P already
+      double heating = conductive_heating(F, g, i, j, k) * ph.ma2 * ph.kappa;
+      double bu = (ur - ul) / g.dx / jac[c] * ph.ma2 * ph.kappa;
+      double bv = (vf - vb) / g.dy / jac[c] * ph.ma2 * ph.kappa;
+      double bw = (wu - wd) / g.dz / jac[c] * ph.ma2 * ph.kappa;
+      bu /= fcscal;
+      bv /= fcscal;
+      bw /= fcscal;
+      heating /= fcscal;
+      double bb = bu + bv + bw - heating;
+      rhs[g.v(i - g.i0 + 1, j - g.j0 + 1, k - g.k0 + 1)] = bb;
+      val[0] = bb * bb;
+      val[1] = bu * bu + bv * bv + bw * bw + heating * heating;
+    } else {
     size_t r = g.h(i + 1, j, k), l = g.h(i - 1, j, k), f = g.h(i, j + 1, k), b = g.h(i, j - 1, k);
     size_t up = g.h(i, j, k + 1), dn = g.h(i, j, k - 1);
     double pedger = 0.5 * (jac[c] * pbar[c] + jac[r] * pbar[r]);
@@ -50,6 +64,7 @@ __global__ void k_compute_lhs(Fields F, Phys ph, Geo g, double* rhs, double* par
     rhs[g.v(i - g.i0 + 1, j - g.j0 + 1, k - g.k0 + 1)] = bb;
     val[0] = bb * bb;
     val[1] = bu * bu + bv * bv + bw * bw;
+    }
   }
   int block = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   block_partials<2>(val, partials, block);
@@ -65,10 +80,10 @@ __global__ void k_finalize_lhs(const double* partials, int nparts, SolverScalars
 }
 
 // ---------------------------------------------------------------------------
-// compute_operator! (compute_operator!.jl:40-1606), Boussinesq / PI
+// compute_operator! (compute_operator!.jl:40-1606)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_compute_operator(Fields F, const double* betar, const double* fcs, Geo g,
-                                                          Tensor T, double dt, double rf, int damp) {
+                                                          Tensor T, double dt, double rf, int damp, Phys ph) {
   int i = g.i0 + blockIdx.x * blockDim.x + threadIdx.x;
   int j = g.j0 + blockIdx.y * blockDim.y + threadIdx.y;
   int k = g.k0 + blockIdx.z;
@@ -256,6 +271,10 @@ __global__ void __launch_bounds__(256) k_compute_operator(Fields F, const double
     ac = -Rr / dx + Ll / dx - Ff / dy + Bb / dy - Uu * met33edgeu / dz + Dd * met33edged / dz - q4(UR, met13uedger) -
          q4(UL, met13uedgel) - q4(UF, met23uedgef) - q4(UB, met23uedgeb) + q4(DR, met13dedger) + q4(DL, met13dedgel) +
          q4(DF, met23dedgef) + q4(DB, met23dedgeb);
+  }
+  if (F.comp) {  // compute_operator!.jl:854-860
+    double d = dpdpi(ph, F.p[C]);
+    ac -= (d / dt) / (dt * ph.rsp / ph.kappa);
   }
   if (kbot) {
     ar = Rr * (1.0 / dx - 0.75 * met13edger / dz) + hx(Uu, met13edgeu, dx, jru, jr + jru) - q4(UR, met13uedger);
@@ -878,7 +897,8 @@ __global__ void k_correct_u(Fields F, Phys ph, Geo g, const double* betar, const
   double gradient = pressure_gradient_u(F, ph, g, dpip, i, j, k);
   size_t c = g.h(i, j, k);
   corx[c] = dt / factor * gradient;
-  u[c] -= 1.0 * corx[c];
+  double jpedger = wind_factor_u(F, g, i, j, k);
+  u[c] -= jpedger * corx[c];
 }
 
 __global__ void k_correct_v(Fields F, Phys ph, Geo g, const double* betar, const double* dpip, double* cory, double* v,
@@ -890,7 +910,8 @@ __global__ void k_correct_v(Fields F, Phys ph, Geo g, const double* betar, const
   double gradient = pressure_gradient_v(F, ph, g, dpip, i, j, k);
   size_t c = g.h(i, j, k);
   cory[c] = dt / factor * gradient;
-  v[c] -= 1.0 * cory[c];
+  double jpedgef = wind_factor_v(F, g, i, j, k);
+  v[c] -= jpedgef * cory[c];
 }
 
 __global__ void k_correct_w(Fields F, Phys ph, Geo g, const double* betar, const double* dpip, const double* corx,
@@ -903,7 +924,7 @@ __global__ void k_correct_w(Fields F, Phys ph, Geo g, const double* betar, const
   factor += dt * (ju * betar[c] + jc * betar[up]) / (jc + ju) * rf;
   double n2edgeu = (ju * F.n2[c] + jc * F.n2[up]) / (jc + ju);
   double gradient = pressure_gradient_w(F, ph, g, dpip, i, j, k);
-  double jpedgeu = 1.0;
+  double jpedgeu = wind_factor_w(F, g, i, j, k);
   double fw = buoyancy_factor_w(F, g, i, j, k);
   w[c] += -dt / (factor + fw * n2edgeu * dt2) * jpedgeu * gradient -
           1.0 / (factor + fw * n2edgeu * dt2) * fw * n2edgeu * dt2 * jpedgeu * 0.5 *
@@ -979,7 +1000,7 @@ int compute_operator(pcf_state* st, double dt, double rf) {
   dim3 blk(64, 4), grd(blocks_for(g.nx, 64), blocks_for(g.ny, 4), g.nz);
   ProfScope prof(st, PCF_K_COMPUTE_OPERATOR);
   hipLaunchKernelGGL(k_compute_operator, grd, blk, 0, st->stream, fields_of(st), st->betar, st->fcscal, g, st->T, dt,
-                     rf, st->ph.damp);
+                     rf, st->ph.damp, st->ph);
   PCF_HIP(hipGetLastError());
   st->pc_factors_valid = false;
   st->t32_valid = false;

@@ -170,7 +170,11 @@ int host_setup(pcf_state* st, std::string& err) {
     err = "Error in Domain: nb > size!";
     return PCF_E_CONFIG;
   }
-  if (n.model != PCF_BOUSSINESQ && n.model != PCF_PSEUDO_INCOMPRESSIBLE) { err = "Invalid model"; return PCF_E_OPTION; }
+  if (n.model != PCF_BOUSSINESQ && n.model != PCF_PSEUDO_INCOMPRESSIBLE && n.model != PCF_COMPRESSIBLE) {
+    err = "Invalid model";
+    return PCF_E_OPTION;
+  }
+  if (!(n.thermal_conductivity >= 0.0)) { err = "thermal_conductivity < 0"; return PCF_E_CONFIG; }
   if (n.preconditioner_iterations < 1 && n.preconditioner) { err = "preconditioner_iterations < 1"; return PCF_E_CONFIG; }
   if (n.precision != PCF_PRECISION_FP64 && n.precision != PCF_PRECISION_MIXED) { err = "Invalid precision"; return PCF_E_OPTION; }
   if (n.initial_guess != PCF_GUESS_ZERO && n.initial_guess != PCF_GUESS_PREVIOUS && n.initial_guess != PCF_GUESS_LEARNED) {
@@ -233,6 +237,10 @@ int host_setup(pcf_state* st, std::string& err) {
   ph.tref = tref; ph.lref = lref; ph.uref = uref; ph.model = n.model;
   ph.damp = n.damp_horizontal_wind_on_rhs != 0;
   ph.viscous = !(1 / re <= DBL_EPSILON);
+  ph.gamma = gamma;
+  ph.rsp = rsp;
+  ph.dpdpi_c = 1 / (gamma - 1) * std::pow(rsp / pref, 1 - gamma);
+  st->compressible = n.model == PCF_COMPRESSIBLE;
   // Time.jl:31-38
   st->alphark[0] = 0.0; st->alphark[1] = -5.0 / 9.0; st->alphark[2] = -153.0 / 128.0;
   st->betark[0] = 1.0 / 3.0; st->betark[1] = 15.0 / 16.0; st->betark[2] = 8.0 / 15.0;
@@ -533,6 +541,7 @@ int host_setup(pcf_state* st, std::string& err) {
   if ((rc = upload(st, &st->v, v.a))) return rc;
   if ((rc = upload(st, &st->w, w.a))) return rc;
   if ((rc = upload(st, &st->pip, pip.a))) return rc;
+  if (st->compressible && (rc = upload(st, &st->p, pbar.a))) return rc;  // set_p.jl:36-41
   if (st->tracer) {
     if ((rc = upload(st, &st->chi, chi.a))) return rc;
     if ((rc = upload(st, &st->chi_init, chi.a))) return rc;

@@ -1,6 +1,6 @@
 """Example configurations (src/Examples of PinCFlow.jl) as Namelists.
 
-``periodic_hill``, ``mountain_wave``, ``cold_bubble`` and ``vortex`` return the
+``periodic_hill``, ``mountain_wave``, ``cold_bubble``, ``hot_bubble`` and ``vortex`` return the
 Namelists their reference namesake passes to ``integrate``; the keyword
 arguments keep the reference names and defaults.  ``mountain_wave_3d`` and
 ``gaussian_hills_3d`` are the production configurations measured by bench.py.
@@ -51,6 +51,19 @@ def cold_bubble(*, x_size: int = 40, z_size: int = 40) -> Namelists:
     return Namelists(
         atmosphere=AtmosphereNamelist(background="Isentropic",
                                       initial_rhop=CosineBubble(0.005, x0=0.0, z0=3 * rz, rx=rx, rz=rz)),
+        discretization=DiscretizationNamelist(dtmax=60.0),
+        domain=DomainNamelist(x_size=x_size, z_size=z_size, lx=lx, lz=lz),
+        output=OutputNamelist(output_variables=("thetap",), prepare_restart=True),
+    )
+
+
+def hot_bubble(*, x_size: int = 40, z_size: int = 40) -> Namelists:
+    """src/Examples/hot_bubble.jl:3-52 (compressible, isentropic)."""
+    lx, lz = 20000.0, 20000.0
+    rx, rz = lx / 8, lz / 8
+    return Namelists(
+        atmosphere=AtmosphereNamelist(model="Compressible", background="Isentropic",
+                                      initial_rhop=CosineBubble(-0.005, x0=0.0, z0=5 * rz, rx=rx, rz=rz)),
         discretization=DiscretizationNamelist(dtmax=60.0),
         domain=DomainNamelist(x_size=x_size, z_size=z_size, lx=lx, lz=lz),
         output=OutputNamelist(output_variables=("thetap",), prepare_restart=True),

@@ -235,7 +235,8 @@ class Namelists:
     tracer: TracerNamelist = field(default_factory=TracerNamelist)
 
 
-_MODELS = {"Boussinesq": _abi.BOUSSINESQ, "PseudoIncompressible": _abi.PSEUDO_INCOMPRESSIBLE}
+_MODELS = {"Boussinesq": _abi.BOUSSINESQ, "PseudoIncompressible": _abi.PSEUDO_INCOMPRESSIBLE,
+           "Compressible": _abi.COMPRESSIBLE}
 _BACKGROUNDS = {
     "NeutralStratification": _abi.NEUTRAL_STRATIFICATION,
     "StableStratification": _abi.STABLE_STRATIFICATION,
@@ -289,6 +290,7 @@ def to_config(nl: Namelists) -> _abi.PcfConfig:
     c.specify_reynolds_number = int(a.specify_reynolds_number)
     c.inverse_reynolds_number = a.inverse_reynolds_number
     c.kinematic_viscosity = a.kinematic_viscosity
+    c.thermal_conductivity = a.thermal_conductivity
     c.buoyancy_frequency = a.buoyancy_frequency
     c.potential_temperature = a.potential_temperature
     c.temperature = a.temperature

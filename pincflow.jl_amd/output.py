@@ -70,7 +70,10 @@ def record_datasets(nl: Namelists, get: Callable[[str], np.ndarray], time_s: flo
             out["rhobar"] = (get("RHOBAR")[ii, jj, kk] * c["rhoref"]).astype(np.float32)
             out["thetabar"] = (get("THETABAR")[ii, jj, kk] * c["thetaref"]).astype(np.float32)
             out["n2"] = (get("N2")[ii, jj, kk] / c["tref"] ** 2).astype(np.float32)
-            out["p"] = (get("PBAR")[ii, jj, kk] * c["rhoref"] * c["thetaref"]).astype(np.float32)
+            if model != "Compressible":
+                out["p"] = (get("PBAR")[ii, jj, kk] * c["rhoref"] * c["thetaref"]).astype(np.float32)
+    if model == "Compressible":  # P is predicted: written with every record
+        out["p"] = (get("P")[ii, jj, kk] * c["rhoref"] * c["thetaref"]).astype(np.float32)
     o = nl.output
     rho, rhop = get("RHO"), get("RHOP")
     if o.prepare_restart or "rhop" in o.output_variables:
@@ -114,7 +117,8 @@ def norms(records: List[Dict[str, np.ndarray]]):
     return l2, linf
 
 
-# datasets written once, with the first record (write_output.jl:73-165)
+# datasets written once, with the first record (write_output.jl:73-165);
+# "p" is time-dependent in the compressible model
 STATIC = ("x", "y", "z", "ztilde", "rhobar", "thetabar", "n2", "p")
 
 
@@ -131,7 +135,7 @@ def write_file(path: str, records: List[Dict[str, np.ndarray]]) -> None:
         keys += [k for k in r if k not in keys]
     for k in keys:
         parts = [r[k] for r in records if k in r]
-        if k in STATIC:
+        if k in STATIC and len(parts) == 1:
             out[k] = parts[0]
         elif k == "t":
             out[k] = np.concatenate(parts).astype(np.float32)
@@ -143,12 +147,14 @@ def write_file(path: str, records: List[Dict[str, np.ndarray]]) -> None:
 
 def read_input(path: str, iin: int = -1):
     """read_input! (read_input!.jl:13-105) for the datasets this library
-    integrates: returns (time in s, {rhop, us, vs, wts, pip[, chi]}) of record
+    integrates: returns (time in s, {rhop, us, vs, wts, pip[, p][, chi]}) of record
     iin (1-based as in the reference; -1 = the last record)."""
     with np.load(path, allow_pickle=False) as data:
         idx = -1 if iin == -1 else iin - 1
         time_s = float(data["t"][idx])
         fields = {k: np.array(data[k][..., idx]) for k in ("rhop", "us", "vs", "wts", "pip", "chi") if k in data}
+        if "p" in data:  # the compressible model's P (read_input!.jl:61-65); one record is stored 3-D
+            fields["p"] = np.array(data["p"][..., idx] if data["p"].ndim == 4 else data["p"])
     missing = {"rhop", "us", "vs", "wts", "pip"} - set(fields)
     if missing:
         raise ValueError(f"restart file {path} lacks {sorted(missing)} (write it with prepare_restart = true)")

@@ -127,6 +127,18 @@ class State:
     def compute_fluxes(self):
         self._check(self.lib.pcf_compute_fluxes(self.h))
 
+    def compute_theta_fluxes(self):
+        """compute_fluxes!(state, p0, Theta()): heat conduction (compressible model)."""
+        self._check(self.lib.pcf_compute_theta_fluxes(self.h))
+
+    def modify_compressible_wind(self, multiply: bool):
+        """modify_compressible_wind!(state, * or /) (compressible model)."""
+        self._check(self.lib.pcf_modify_compressible_wind(self.h, int(multiply)))
+
+    def synchronize_compressible_atmosphere(self):
+        """synchronize_compressible_atmosphere!(state, predictands): pbar := P."""
+        self._check(self.lib.pcf_synchronize_compressible_atmosphere(self.h))
+
     def update_lhs(self, dt: float, m: int, var: int):
         self._check(self.lib.pcf_update_lhs(self.h, dt, m, var))
 
@@ -211,10 +223,10 @@ class State:
     def restart(self, time_s: float, fields: dict):
         """Continue from a stored record (integrate.jl:168-186, read_input!.jl):
         fields are the dimensional datasets rhop, us, vs, wts and pip of
-        write_output (and chi with a tracer); the interior of the predictands
-        is replaced, rho follows rhop in the pseudo-incompressible model, chi
-        is the dataset times rhobar + rho (read_input!.jl:79-84), then the
-        boundaries are set."""
+        write_output (p in the compressible model, chi with a tracer); the
+        interior of the predictands is replaced, rho follows rhop outside the
+        Boussinesq model, chi is the dataset times rhobar + rho
+        (read_input!.jl:79-84), then the boundaries are set and pbar follows P."""
         from . import output
 
         nl = self.namelists
@@ -232,6 +244,12 @@ class State:
                 r = self.get("RHO")
                 r[ii, jj, kk] = a[ii, jj, kk]
                 self.set("RHO", r)
+        if nl.atmosphere.model == "Compressible":  # read_input!.jl:61-65
+            if "p" not in fields:
+                raise ValueError("the restart record has no p dataset, but the model is compressible")
+            pp = self.get("P")
+            pp[ii, jj, kk] = np.asarray(fields["p"], dtype=np.float64) / c["rhoref"] / c["thetaref"]
+            self.set("P", pp)
         if nl.tracer.tracer_setup == "TracerOn":
             if "chi" not in fields:
                 raise ValueError("the restart record has no chi dataset, but the tracer is on")
@@ -239,6 +257,8 @@ class State:
             chi[ii, jj, kk] = np.asarray(fields["chi"], dtype=np.float64) * (rhobar[ii, jj, kk] + rho[ii, jj, kk])
             self.set("CHI", chi)
         self.set_boundaries(_abi.BOUNDARY_PREDICTANDS)
+        # integrate.jl:123-126: pbar follows the restored P
+        self._check(self.lib.pcf_synchronize_compressible_atmosphere(self.h))
         self._check(self.lib.pcf_set_time(self.h, time_s))
 
     def dot_device(self, d_a: int, d_b: int) -> float:

@@ -44,7 +44,7 @@ def test_oracle_config_layout(abi, oracle_mod):
 
 def test_invalid_options_raise(pcf):
     nl = pcf.Namelists()
-    nl.atmosphere.model = "Compressible"
+    nl.atmosphere.model = "Anelastic"
     with pytest.raises(ValueError):
         pcf.to_config(nl)
     nl = pcf.Namelists()

@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 GOLD = json.loads((Path(__file__).parent / "golden" / "reference_norms.json").read_text())
 
 
-@pytest.mark.parametrize("example", ["periodic_hill", "mountain_wave", "cold_bubble", "vortex"])
+@pytest.mark.parametrize("example", ["periodic_hill", "mountain_wave", "cold_bubble", "hot_bubble", "vortex"])
 def test_reference_norms_gpu(pcf, example):
     case = GOLD[example]
     nl = getattr(pcf.examples, example)(**case["keywords"])

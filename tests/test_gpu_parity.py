@@ -50,12 +50,25 @@ def configs(pcf):
     vx = pcf.examples.vortex(x_size=12, y_size=10)
     mwt = pcf.examples.mountain_wave_3d(x_size=16, y_size=14, z_size=12, iterations=2)
     mwt.tracer = pcf.TracerNamelist(tracer_setup="TracerOn", initial_tracer=pcf.TracerDipole(50000.0, 50000.0))
+    # the compressible model: the hot bubble, and the mountain wave (terrain,
+    # LHS sponge) with a conductivity large enough for the heat conduction to
+    # matter and the tracer on
+    hb = pcf.examples.hot_bubble(x_size=12, z_size=10)
+    mwc = pcf.examples.mountain_wave(x_size=12, y_size=10, z_size=9)
+    mwc.atmosphere.model = "Compressible"
+    mwc.atmosphere.thermal_conductivity = 50.0
+    mwc.tracer = pcf.TracerNamelist(tracer_setup="TracerOn", initial_tracer=pcf.TracerDipole(5000.0, 5000.0))
     return {"periodic_hill": ph, "mountain_wave": mw, "mountain_wave_3d": mw3, "cold_bubble": cb,
-            "mountain_wave_odd": mwo, "vortex": vx, "mountain_wave_tracer": mwt}
+            "mountain_wave_odd": mwo, "vortex": vx, "mountain_wave_tracer": mwt, "hot_bubble": hb,
+            "mountain_wave_compressible": mwc}
 
 
 CASES = ["periodic_hill", "mountain_wave", "mountain_wave_3d", "cold_bubble", "mountain_wave_odd", "vortex",
-         "mountain_wave_tracer"]
+         "mountain_wave_tracer", "hot_bubble", "mountain_wave_compressible"]
+
+
+def compressible(nl):
+    return nl.atmosphere.model == "Compressible"
 
 
 def predictands(nl, pip=True):
@@ -66,6 +79,8 @@ def predictands(nl, pip=True):
         names.append("RHO")
     if nl.tracer.tracer_setup == "TracerOn":
         names.append("CHI")
+    if compressible(nl):
+        names.append("P")
     return names
 
 
@@ -94,8 +109,9 @@ def perturb(gpu, orc, names, seed, amp=0.05):
 def test_setup_fields(pcf, abi, oracle_mod, name):
     nl, gpu, orc = pair(pcf, abi, oracle_mod, name)
     tracer = ("CHI", "INITIALTRACER") if nl.tracer.tracer_setup == "TracerOn" else ()
+    comp = ("P",) if compressible(nl) else ()
     for f in ("PBAR", "THETABAR", "RHOBAR", "N2", "JAC", "MET13", "MET23", "MET33", "ZC", "ZCTILDE",
-              "RHO", "RHOP", "U", "V", "W", "PIP") + tracer:
+              "RHO", "RHOP", "U", "V", "W", "PIP") + tracer + comp:
         a, b = gpu.get(f), orc.get(f)
         assert np.array_equal(a, b), (f, float(np.max(np.abs(a - b))))
 
@@ -112,8 +128,15 @@ def test_sponges(pcf, abi, oracle_mod, name):
 
 @pytest.mark.parametrize("name", CASES)
 def test_compute_lhs(pcf, abi, oracle_mod, name):
-    _, gpu, orc = pair(pcf, abi, oracle_mod, name)
+    nl, gpu, orc = pair(pcf, abi, oracle_mod, name)
     perturb(gpu, orc, ["U", "V", "W"], seed=1)
+    if compressible(nl):  # the heating term: conduction fluxes of a perturbed p0
+        perturb(gpu, orc, ["RHO"], seed=11, amp=0.01)
+        for s in (gpu, orc):
+            s.set_boundaries(abi.BOUNDARY_PREDICTANDS)
+            s.backup_predictands()
+            s.compute_theta_fluxes()
+            s.set_boundaries(abi.BOUNDARY_FLUXES)
     for s in (gpu, orc):
         s.set_boundaries(abi.BOUNDARY_PREDICTANDS)
     t1, t2 = gpu.compute_lhs(), orc.compute_lhs()
@@ -125,8 +148,8 @@ def test_compute_lhs(pcf, abi, oracle_mod, name):
 @pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("dt,rf", [(1.0, 1.0), (0.37, 2.0)])
 def test_compute_operator(pcf, abi, oracle_mod, name, dt, rf):
-    _, gpu, orc = pair(pcf, abi, oracle_mod, name)
-    perturb(gpu, orc, ["RHO"], seed=2, amp=0.01)
+    nl, gpu, orc = pair(pcf, abi, oracle_mod, name)
+    perturb(gpu, orc, ["RHO"] + (["P"] if compressible(nl) else []), seed=2, amp=0.01)
     for s in (gpu, orc):
         s.compute_sponges(0.5, 3.0)
         s.compute_operator(dt, rf)
@@ -205,6 +228,7 @@ def test_fluxes(pcf, abi, oracle_mod, name):
     perturb(gpu, orc, ["U", "V", "W"], seed=6, amp=0.02)  # current winds differ from p0
     for s in (gpu, orc):
         s.set_boundaries(abi.BOUNDARY_PREDICTANDS)
+        s.compute_theta_fluxes()  # before compute_fluxes, which zeroes its wall faces
         s.compute_fluxes()
     nb = 3
     i0, j0, k0 = nb + 1, nb + 1, nb + 1
@@ -227,7 +251,7 @@ def test_fluxes(pcf, abi, oracle_mod, name):
         "PHIW_Z": (sl(i0, i1), sl(j0, j1), sl(k0 - 1, k1)),
     }
     fl = ["PHIRHOP", "PHIU", "PHIV", "PHIW"] + (["PHIRHO"] if "RHO" in names else []) + \
-        (["PHICHI"] if "CHI" in names else [])
+        (["PHICHI"] if "CHI" in names else []) + (["PHIP", "PHITHETA"] if compressible(nl) else [])
     for f in fl:
         for d in "XYZ":
             key = f"{f}_{d}"
@@ -247,6 +271,7 @@ def test_explicit_stages(pcf, abi, oracle_mod, name):
         s.set_boundaries(abi.BOUNDARY_PREDICTANDS)
         s.compute_sponges(0.3, 2.0)
         s.backup_predictands()
+        s.compute_theta_fluxes()
         s.explicit_integration_lhs(0.3, 2.0)
     for f in names:
         err = rel(gpu.get(f), orc.get(f))

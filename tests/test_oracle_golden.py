@@ -1,12 +1,14 @@
 """The oracle against the reference's own end-to-end fixtures.
 
-test/test_{periodic_hill,mountain_wave,cold_bubble,vortex}.jl run each example
+test/test_{periodic_hill,mountain_wave,cold_bubble,hot_bubble,vortex}.jl run each example
 at reduced size to t = 3600 s and compare the L2 and L-infinity norms of every
 output dataset (t = 0 after the initial cleaning, and t = 3600 s) with Float32
 literals at rtol = sqrt(eps(Float32)).  The literals are in
 tests/golden/reference_norms.json.  They pin the whole FP64 path of the oracle
 in aggregate: grid, background, initial cleaning, the 25-point operator, the
-preconditioner, BiCGSTAB, the corrector, MUSCL fluxes, RK3 and boundaries.
+preconditioner, BiCGSTAB, the corrector, MUSCL fluxes, RK3 and boundaries; hot_bubble
+adds the compressible model (P fluxes, heat conduction, the pi' update, the
+J P wind weighting and the dP/dpi operator term).
 """
 import json
 from pathlib import Path
@@ -16,7 +18,7 @@ import pytest
 GOLD = json.loads((Path(__file__).parent / "golden" / "reference_norms.json").read_text())
 
 
-@pytest.mark.parametrize("example", ["periodic_hill", "mountain_wave", "cold_bubble", "vortex"])
+@pytest.mark.parametrize("example", ["periodic_hill", "mountain_wave", "cold_bubble", "hot_bubble", "vortex"])
 def test_reference_norms(pcf, abi, oracle_mod, example):
     case = GOLD[example]
     nl = getattr(pcf.examples, example)(**case["keywords"])

@@ -27,6 +27,22 @@ def test_output_file_layout_and_read_input(pcf, oracle_mod, tmp_path):
             np.testing.assert_array_equal(fields[k], rec[k])
 
 
+def test_compressible_records_carry_p(pcf, oracle_mod, tmp_path):
+    """write_output.jl:159-165: the compressible model writes P with every
+    record; read_input! restores it (read_input!.jl:61-65)."""
+    nl = pcf.examples.hot_bubble(x_size=8, z_size=8)
+    nl.output.output_steps = True
+    nl.output.iterations = 2
+    _, recs, _ = oracle_mod.integrate(nl, pcf._abi, pcf.output)
+    assert all("p" in r for r in recs)
+    path = tmp_path / "hb.npz"
+    pcf.output.write_file(str(path), recs)
+    with np.load(path, allow_pickle=False) as f:
+        assert f["p"].shape == recs[0]["p"].shape + (len(recs),)
+    _, fields = pcf.output.read_input(str(path), 2)
+    np.testing.assert_array_equal(fields["p"], recs[1]["p"])
+
+
 def test_read_input_needs_the_restart_datasets(pcf, oracle_mod, tmp_path):
     nl = pcf.examples.periodic_hill(x_size=12, z_size=10, output_steps=True, iterations=1)
     nl.output.prepare_restart = False
@@ -39,7 +55,7 @@ def test_read_input_needs_the_restart_datasets(pcf, oracle_mod, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("example", ["periodic_hill", "mountain_wave", "vortex"])
+@pytest.mark.parametrize("example", ["periodic_hill", "mountain_wave", "vortex", "hot_bubble"])
 def test_restart_continues_the_run(pcf, tmp_path, example):
     """A run restarted from its stored record tracks the uninterrupted run.
     The record holds FP32 values (as the reference's file), so the two agree
@@ -47,7 +63,8 @@ def test_restart_continues_the_run(pcf, tmp_path, example):
     RESTART_RTOL = 1e-4
     make = {"periodic_hill": lambda: pcf.examples.periodic_hill(x_size=24, z_size=20, output_steps=True),
             "mountain_wave": lambda: pcf.examples.mountain_wave(x_size=12, y_size=10, z_size=9),
-            "vortex": lambda: pcf.examples.vortex(x_size=12, y_size=10)}[example]
+            "vortex": lambda: pcf.examples.vortex(x_size=12, y_size=10),
+            "hot_bubble": lambda: pcf.examples.hot_bubble(x_size=12, z_size=10)}[example]
 
     def namelists(iterations):
         nl = make()
@@ -68,7 +85,8 @@ def test_restart_continues_the_run(pcf, tmp_path, example):
     assert recs[0]["t"][0] == pytest.approx(t3, rel=1e-6)
     assert rest.time()[0] == pytest.approx(full.time()[0], rel=1e-6)
     tracer = ("CHI",) if example == "vortex" else ()
-    for f in ("RHOP", "U", "V", "W", "PIP") + tracer:
+    comp = ("P", "PBAR") if example == "hot_bubble" else ()  # P restored, pbar synchronized
+    for f in ("RHOP", "U", "V", "W", "PIP") + tracer + comp:
         a, b = rest.get(f), full.get(f)
         scale = max(float(np.max(np.abs(b))), 1e-30)
         assert float(np.max(np.abs(a - b))) / scale <= RESTART_RTOL, f
