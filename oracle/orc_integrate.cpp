@@ -220,7 +220,7 @@ int orc_step(State* st, orc_step_info* info) {
   set_boundaries_predictands(*st);
   copy_predictands(st->p0, st->pr);
   if (st->tracer) st->chi0.a = st->chi.a;  // backup_predictands.jl:18
-  compute_theta_fluxes(*st, st->p0);  // integrate.jl:279
+  compute_theta_fluxes(*st, st->p0);  // integrate.jl:277
   // (1)
   explicit_integration_lhs(*st, st->p0, 0.5 * dt, time);
   // (2)

@@ -455,7 +455,7 @@ int pcf_step(pcf_state* st, pcf_step_info* info) {
   const int np = predictand_slots(st, cur);
   TRY(copy_set(st, st->p0, cur, np));  // backup_predictands
   if (st->tracer) TRY(copy_field(st, st->chi0, st->chi));
-  TRY(compute_theta_fluxes(st));                              // integrate.jl:279
+  TRY(compute_theta_fluxes(st));                              // integrate.jl:277
   TRY(explicit_lhs(st, 0.5 * dt, time));                      // (1)
   int it1 = 0, it2 = 0;
   st->solve_slot = 0;

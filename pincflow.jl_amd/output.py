@@ -153,7 +153,7 @@ def read_input(path: str, iin: int = -1):
         idx = -1 if iin == -1 else iin - 1
         time_s = float(data["t"][idx])
         fields = {k: np.array(data[k][..., idx]) for k in ("rhop", "us", "vs", "wts", "pip", "chi") if k in data}
-        if "p" in data:  # the compressible model's P (read_input!.jl:61-65); one record is stored 3-D
+        if "p" in data:  # the compressible model's P (read_input!.jl:73-77); one record is stored 3-D
             fields["p"] = np.array(data["p"][..., idx] if data["p"].ndim == 4 else data["p"])
     missing = {"rhop", "us", "vs", "wts", "pip"} - set(fields)
     if missing:

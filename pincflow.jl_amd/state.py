@@ -244,7 +244,7 @@ class State:
                 r = self.get("RHO")
                 r[ii, jj, kk] = a[ii, jj, kk]
                 self.set("RHO", r)
-        if nl.atmosphere.model == "Compressible":  # read_input!.jl:61-65
+        if nl.atmosphere.model == "Compressible":  # read_input!.jl:73-77
             if "p" not in fields:
                 raise ValueError("the restart record has no p dataset, but the model is compressible")
             pp = self.get("P")
@@ -257,7 +257,7 @@ class State:
             chi[ii, jj, kk] = np.asarray(fields["chi"], dtype=np.float64) * (rhobar[ii, jj, kk] + rho[ii, jj, kk])
             self.set("CHI", chi)
         self.set_boundaries(_abi.BOUNDARY_PREDICTANDS)
-        # integrate.jl:123-126: pbar follows the restored P
+        # integrate.jl:182-185: pbar follows the restored P
         self._check(self.lib.pcf_synchronize_compressible_atmosphere(self.h))
         self._check(self.lib.pcf_set_time(self.h, time_s))
 

@@ -29,7 +29,7 @@ def test_output_file_layout_and_read_input(pcf, oracle_mod, tmp_path):
 
 def test_compressible_records_carry_p(pcf, oracle_mod, tmp_path):
     """write_output.jl:159-165: the compressible model writes P with every
-    record; read_input! restores it (read_input!.jl:61-65)."""
+    record; read_input! restores it (read_input!.jl:73-77)."""
     nl = pcf.examples.hot_bubble(x_size=8, z_size=8)
     nl.output.output_steps = True
     nl.output.iterations = 2
