@@ -8,6 +8,10 @@ A step is one pass of the time loop of integrate (two Poisson solves).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
                        [--no-cpu-baseline] [--no-fp64-check]
+The mixed-precision configuration (16 seeded hills at 512 x 512 x 128, error
+against FP64 after 100 steps):
+  python bench.py --workload gaussian_hills --nx 512 --precision mixed --preconditioner multigrid
+                  --storage bf16 --mg-coarse-steps 2 --initial-guess previous --steps 100 --no-cpu-baseline
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -40,6 +44,9 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", default="mountain_wave", choices=["mountain_wave", "gaussian_hills"],
+                   help="mountain_wave: one Gaussian hill (the metric's workload); gaussian_hills: the same "
+                        "physics over 16 seeded hills (the mixed-precision configuration, run at --nx 512)")
     p.add_argument("--nx", type=int, default=256)
     p.add_argument("--nz", type=int, default=128)
     p.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
@@ -68,6 +75,15 @@ def parse():
     return p.parse_args()
 
 
+def workload(pcf, args, iterations, precision):
+    """Namelists of the benchmarked workload at --nx x --nx x --nz."""
+    if args.workload == "gaussian_hills":
+        return pcf.examples.gaussian_hills_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz, iterations=iterations,
+                                              precision=precision)
+    return pcf.examples.mountain_wave_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz, iterations=iterations,
+                                         precision=precision)
+
+
 def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False, storage=None, mg_coarse=None,
                relative=False):
     """Setup, initial cleaning, warmup, then the timed steps of one method,
@@ -75,8 +91,7 @@ def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False
     average duration gives the roofline figure; events around every launch
     would delay the launch-heavy preconditioners), then one more step with
     events around every kernel group for the breakdown."""
-    nl = pcf.examples.mountain_wave_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz,
-                                       iterations=args.steps + args.warmup + 1, precision=precision)
+    nl = workload(pcf, args, args.steps + args.warmup + 1, precision)
     nl.poisson.initial_guess = guess
     nl.poisson.preconditioner_type = preconditioner
     nl.poisson.tolerance_is_relative = relative
@@ -236,8 +251,9 @@ def main():
             strict.append({"method": method, "value": round(r["value"], 4), "ms_per_step": round(r["ms_per_step"], 2),
                            "bicgstab_iterations_per_solve": round(r["iterations_per_solve"], 1)})
     error_vs_fp64 = None
+    fp64_run = None
     if not plain and not args.no_fp64_check:
-        error_vs_fp64 = fp64_difference(pcf, args, st)
+        error_vs_fp64, fp64_run = fp64_difference(pcf, args, st)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(pcf, main_run["nl"], main_run["iters"], args)
@@ -254,7 +270,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if plain else "f64 (preconditioner f32)",
         "data": "synthetic",
-        "config": {"workload": (f"pseudo-incompressible mountain wave {args.nx}x{args.nx}x{args.nz}, " +
+        "config": {"workload": (("pseudo-incompressible flow over 16 seeded Gaussian hills "
+                                 if args.workload == "gaussian_hills" else "pseudo-incompressible mountain wave ") +
+                                f"{args.nx}x{args.nx}x{args.nz}, " +
                                 (f"FP64 BiCGSTAB with the {args.preconditioner.replace('_', ' ')} preconditioner "
                                  f"evaluated in FP32, initial guess {args.initial_guess}" if not plain
                                  else "plain preconditioned BiCGSTAB (FP64)")),
@@ -272,6 +290,7 @@ def main():
         "variants": variants,
         "strict_criterion": strict,
         "error_vs_fp64": error_vs_fp64,
+        "fp64_reference_run": fp64_run,
         "learned_training": training,
         "kernel_ms_one_step": {k: [v[0], round(v[1], 3)] for k, v in sorted(main_run["breakdown"].items())},
     }
@@ -321,12 +340,16 @@ def fp64_difference(pcf, args, st_mixed):
     mixed-precision fields after warmup + steps and an FP64 run of the same
     steps (the FP64 library path agrees with the FP64 oracle to 1e-9); the
     Exner pressure is compared up to a constant."""
-    nl = pcf.examples.mountain_wave_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz,
-                                       iterations=args.steps + args.warmup + 1, precision="fp64")
+    nl = workload(pcf, args, args.steps + args.warmup + 1, "fp64")
     ref = pcf.State(nl)
     ref.initial_cleaning()
-    for _ in range(args.steps + args.warmup + 1):
-        ref.step()
+    t0 = time.perf_counter()
+    its = []
+    for s in range(args.steps + args.warmup + 1):
+        info = ref.step()
+        its += list(info.niter)
+        print(f"# [fp64 check] step {s + 1}: iterations {list(info.niter)}", file=sys.stderr, flush=True)
+    fp64_rate = (args.steps + args.warmup + 1) / (time.perf_counter() - t0)
     out = {}
     for f in ("RHO", "RHOP", "U", "V", "W", "PIP"):
         a, b = st_mixed.get(f), ref.get(f)
@@ -336,7 +359,8 @@ def fp64_difference(pcf, args, st_mixed):
         scale = float(abs(b).max()) or 1.0
         out[f] = float(abs(d).max()) / scale
     ref.close()
-    return out
+    return out, {"value": round(fp64_rate, 4), "bicgstab_iterations_per_solve": round(sum(its) / len(its), 1),
+                 "steps": args.steps + args.warmup + 1}
 
 
 def cpu_baseline(pcf, nl, iters, args):
