@@ -369,6 +369,10 @@ int pcf_compute_operator(pcf_state* st, double dt, double rayleigh_factor);
 /* apply_operator!(sin, ls, Total()|Horizontal(), state) (apply_operator!.jl:48-479);
  * d_in/d_out are device pointers to interior-sized FP64 vectors. */
 int pcf_apply_operator(pcf_state* st, const double* d_in, double* d_out, int kind);
+/* d_out = A^T d_in for the total operator of the last compute_operator
+ * (no reference counterpart: the gradient of the residual loss that trains
+ * the initial-guess network).  Device pointers, interior-sized FP64. */
+int pcf_apply_operator_transpose(pcf_state* st, const double* d_in, double* d_out);
 /* apply_preconditioner!(sin, sout, state) (apply_preconditioner!.jl:40-124);
  * device pointers, interior-sized. */
 int pcf_apply_preconditioner(pcf_state* st, const double* d_in, double* d_out);
@@ -432,6 +436,10 @@ int pcf_net_forward(pcf_net* net, const float* d_input, float* d_output);
 int pcf_net_gradients(pcf_net* net, const float* d_input, const float* d_target, float* host_grads, double* loss);
 /* one Adam step (learning rate lr) on the loss against d_target */
 int pcf_net_train_step(pcf_net* net, const float* d_input, const float* d_target, float lr, double* loss);
+/* one Adam step (learning rate lr) for a loss whose gradient with respect to
+ * the network output is d_dout (device, FP32, one value per voxel), e.g. the
+ * Poisson residual of the guess it proposes */
+int pcf_net_train_step_dout(pcf_net* net, const float* d_input, const float* d_dout, float lr);
 /* Input channels of the learned initial guess for the right-hand side d_b
  * and the previous solution d_prev (interior FP64 device vectors; d_prev may
  * be null = zero): b / |b|, prev / |b| and jac - 1 (RMS norm |b|), written

@@ -194,6 +194,7 @@ SIGNATURES = [
     ("pcf_compute_lhs", _I, [_P, _PD]),
     ("pcf_compute_operator", _I, [_P, _D, _D]),
     ("pcf_apply_operator", _I, [_P, _P, _P, _I]),
+    ("pcf_apply_operator_transpose", _I, [_P, _P, _P]),
     ("pcf_apply_preconditioner", _I, [_P, _P, _P]),
     ("pcf_apply_bicgstab", _I, [_P, _D, _PI32]),
     ("pcf_dot", _I, [_P, _P, _P, _PD]),
@@ -210,6 +211,7 @@ SIGNATURES = [
     ("pcf_net_forward", _I, [_P, _P, _P]),
     ("pcf_net_gradients", _I, [_P, _P, _P, ctypes.POINTER(ctypes.c_float), _PD]),
     ("pcf_net_train_step", _I, [_P, _P, _P, ctypes.c_float, _PD]),
+    ("pcf_net_train_step_dout", _I, [_P, _P, _P, ctypes.c_float]),
     ("pcf_guess_input", _I, [_P, _P, _P, _P, _PD]),
     ("pcf_set_guess_network", _I, [_P, _P]),
     ("pcf_guess_fallbacks", _I, [_P, _PI32]),

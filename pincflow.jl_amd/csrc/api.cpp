@@ -567,6 +567,12 @@ int pcf_apply_operator(pcf_state* st, const double* d_in, double* d_out, int kin
   return hipStreamSynchronize(st->stream) == hipSuccess ? PCF_OK : PCF_E_HIP;
 }
 
+int pcf_apply_operator_transpose(pcf_state* st, const double* d_in, double* d_out) {
+  if (!st || !d_in || !d_out || d_in == d_out) return PCF_E_ARG;
+  TRY(apply_operator_transpose(st, d_in, d_out));
+  return hipStreamSynchronize(st->stream) == hipSuccess ? PCF_OK : PCF_E_HIP;
+}
+
 int pcf_apply_preconditioner(pcf_state* st, const double* d_in, double* d_out) {
   if (!st || !d_in || !d_out || d_in == d_out) return PCF_E_ARG;
   TRY(apply_preconditioner(st, d_in, d_out));

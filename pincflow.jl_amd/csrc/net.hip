@@ -531,8 +531,8 @@ int net_forward(pcf_net* net, const float* input, hipStream_t s) {
   return PCF_OK;
 }
 
-// backward pass for the loss against target; gradients in net->grads, loss on the host
-int net_backward(pcf_net* net, const float* input, const float* target, double* loss) {
+// backward pass from the output gradient in net->dout; gradients in net->grads
+int net_backward_dout(pcf_net* net, const float* input) {
   const Dims& d = net->d;
   hipStream_t s = net->stream;
   const unsigned vb = 256, vg = (unsigned)((d.n + vb - 1) / vb);
@@ -540,8 +540,6 @@ int net_backward(pcf_net* net, const float* input, const float* target, double* 
   const int nc = nchunks(net);
   const float* P = net->params;
   float* G = net->grads;
-  const unsigned lg = 1024;
-  hipLaunchKernelGGL(k_loss, dim3(lg), dim3(256), 0, s, d, net->out, target, net->dout, net->loss_partials);
   // layer 6
   hipLaunchKernelGGL(k_last_weights, dim3(nc), dim3(256), 0, s, d, net->dout, net->a[4], net->partials);
   sum_chunks(net, nc, 27 * kC + 1, G + layer_offset(5));
@@ -561,6 +559,17 @@ int net_backward(pcf_net* net, const float* input, const float* target, double* 
   hipLaunchKernelGGL(k_first_weights, dim3(nc), dim3(256), 0, s, d, net->dzb, input, net->partials);
   sum_chunks(net, nc, 27 * kC * 3, G + layer_offset(0));
   PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+// backward pass for the mean squared error against target; loss on the host
+int net_backward(pcf_net* net, const float* input, const float* target, double* loss) {
+  const Dims& d = net->d;
+  hipStream_t s = net->stream;
+  const unsigned lg = 1024;
+  hipLaunchKernelGGL(k_loss, dim3(lg), dim3(256), 0, s, d, net->out, target, net->dout, net->loss_partials);
+  int rc = net_backward_dout(net, input);
+  if (rc) return rc;
   std::vector<double> lp(lg);
   PCF_HIP(hipMemcpyAsync(lp.data(), net->loss_partials, lg * sizeof(double), hipMemcpyDeviceToHost, s));
   PCF_HIP(hipStreamSynchronize(s));
@@ -756,18 +765,35 @@ int pcf_net_gradients(pcf_net* net, const float* d_input, const float* d_target,
   return PCF_OK;
 }
 
-int pcf_net_train_step(pcf_net* net, const float* d_input, const float* d_target, float lr, double* loss) {
+// one Adam step on the gradients in net->grads
+static int adam_step(pcf_net* net, float lr) {
   using namespace pcf;
-  int rc = pcf_net_gradients(net, d_input, d_target, nullptr, loss);
-  if (rc) return rc;
   net->step += 1;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.0f - std::pow(b1, (float)net->step), c2 = 1.0f - std::pow(b2, (float)net->step);
   hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, net->stream, net->params, net->grads,
                      net->adam_m, net->adam_v, kParams, lr, b1, b2, 1e-8f, c1, c2);
-  if ((rc = net_pack(net))) return rc;
+  int rc = net_pack(net);
+  if (rc) return rc;
   PCF_HIP(hipStreamSynchronize(net->stream));
   return PCF_OK;
+}
+
+int pcf_net_train_step(pcf_net* net, const float* d_input, const float* d_target, float lr, double* loss) {
+  using namespace pcf;
+  int rc = pcf_net_gradients(net, d_input, d_target, nullptr, loss);
+  if (rc) return rc;
+  return adam_step(net, lr);
+}
+
+int pcf_net_train_step_dout(pcf_net* net, const float* d_input, const float* d_dout, float lr) {
+  using namespace pcf;
+  if (!net || !d_input || !d_dout || !net->training) return PCF_E_ARG;
+  int rc = net_forward(net, d_input, net->stream);
+  if (rc) return rc;
+  PCF_HIP(hipMemcpyAsync(net->dout, d_dout, net->d.n * sizeof(float), hipMemcpyDeviceToDevice, net->stream));
+  if ((rc = net_backward_dout(net, d_input))) return rc;
+  return adam_step(net, lr);
 }
 
 }  // extern "C"

@@ -246,6 +246,7 @@ int set_boundaries_fluxes(pcf_state* st);
 int compute_lhs(pcf_state* st, double* tolref_out);
 int compute_operator(pcf_state* st, double dt, double rf);
 int apply_operator(pcf_state* st, const double* sin, double* ls, bool total);
+int apply_operator_transpose(pcf_state* st, const double* r, double* out);
 int apply_preconditioner(pcf_state* st, const double* sin, double* sout);
 int apply_bicgstab(pcf_state* st, double tolref, int* niter);
 int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess, double guess_scale, int* niter);

@@ -1043,6 +1043,46 @@ static void launch_total(pcf_state* st, const double* s, double* ls, const doubl
                        nullptr, st->partials, sc);
 }
 
+// out = A^T r for the total operator: the tap t of cell p, T[t][p] r[p],
+// lands on cell p + offset_t (periodic in x and y, dropped beyond the z
+// walls exactly where the apply drops it), so cell q gathers T[t][q - off_t]
+// r[q - off_t] over the 25 taps.  Used to train the initial-guess network on
+// the residual; the reference has no transposed apply.
+__constant__ int kTapOffset[T_COUNT][3] = {
+    {0, 0, 0},   {-1, 0, 0},  {1, 0, 0},   {0, -1, 0},  {0, 1, 0},   {0, 0, -1},  {0, 0, 1},
+    {-1, 0, -1}, {-1, 0, 1},  {1, 0, -1},  {1, 0, 1},   {0, -1, -1}, {0, -1, 1},  {0, 1, -1},
+    {0, 1, 1},   {0, 0, -2},  {0, 0, 2},   {-1, 0, -2}, {-1, 0, 2},  {1, 0, -2},  {1, 0, 2},
+    {0, -1, -2}, {0, -1, 2},  {0, 1, -2},  {0, 1, 2}};
+
+__global__ void __launch_bounds__(256) k_apply_transpose(ApplyArgs A) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  if (i >= nx || j >= ny) return;
+  const size_t sxy = (size_t)nx * ny;
+  double v = 0.0;
+#pragma unroll
+  for (int t = 0; t < T_COUNT; ++t) {
+    const int pk = k - kTapOffset[t][2];
+    if (pk < 0 || pk >= nz) continue;
+    int pi = i - kTapOffset[t][0], pj = j - kTapOffset[t][1];
+    pi = pi < 0 ? pi + nx : (pi >= nx ? pi - nx : pi);
+    pj = pj < 0 ? pj + ny : (pj >= ny ? pj - ny : pj);
+    const size_t p = (size_t)pi + (size_t)nx * pj + sxy * pk;
+    v += A.T.a[t][p] * A.s[p];
+  }
+  A.ls[(size_t)i + (size_t)nx * j + sxy * k] = v;
+}
+
+int apply_operator_transpose(pcf_state* st, const double* r, double* out) {
+  const Geo& g = st->g;
+  hipLaunchKernelGGL(k_apply_transpose, dim3(blocks_for(g.nx, 64), blocks_for(g.ny, 4), g.nz), dim3(64, 4), 0,
+                     st->stream, apply_args(st, r, out));
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
 int apply_operator(pcf_state* st, const double* sin, double* ls, bool total) {
   const Geo& g = st->g;
   if (total) {

@@ -13,7 +13,9 @@ only change the iteration count, never the accuracy.
 Training data are the solver's own solves: after every time step the RHS
 vector and the solution of the step's second implicit stage are read back
 (the solution times sqrt(pbar^2/rhobar) undoes the scaling solve_poisson!
-applies in place, solve_poisson!.jl:69).
+applies in place, solve_poisson!.jl:69).  ``train`` fits the solution change
+(mean squared error); ``train_residual`` fits the Poisson residual of the
+guess, with the gradient through the operator's transpose.
 """
 from __future__ import annotations
 
@@ -202,6 +204,111 @@ def train(net: GuessNetwork, samples, epochs: int, lr: float = 3e-4, log=None) -
     p[-(27 * 32 + 1):] *= scale  # last layer: W[27][1][32] and b[1]
     net.set_params(p)
     return losses
+
+
+def train_residual(net: GuessNetwork, states, steps: int, inner: int = 6, lr: float = 1e-3, log=None
+                   ) -> List[Tuple[float, float]]:
+    """Online training on the residual of the guess the network proposes,
+    the quantity BiCGSTAB's stopping test and iteration count follow:
+
+        L = |A x0 - b|^2 / |b|^2,  x0 = warm + |b| net(b / |b|, warm / |b|, J - 1)
+
+    with A and b those of the solve.  Every time step of each state (after
+    the first, which has no warm start) gives the solve of the step's second
+    implicit stage, whose operator and right-hand side stay in the state, and
+    `inner` Adam steps on it; the gradient with respect to the network output
+    is dL/dnet = 2 |b| A^T (A x0 - b) / |b|^2 (A^T: the transposed apply).
+    Returns, per solve, (L of the warm start, L after training on it)."""
+    lib = states[0].lib
+    out = []
+    for st in states:
+        n = st.nx * st.ny * st.nz
+        fcs = _interior_fcscal(st).ravel(order="F")
+        d_b, d_prev, d_x, d_ax = DeviceBuffer(8 * n), DeviceBuffer(8 * n), DeviceBuffer(8 * n), DeviceBuffer(8 * n)
+        d_in, d_y = DeviceBuffer(12 * n), DeviceBuffer(4 * n)
+        prev, prev_dt = None, None
+
+        def loss_and_residual(x0, b, bb):
+            d_x.upload(x0)
+            st.apply_operator_device(d_x.ptr, d_ax.ptr, True)
+            r = d_ax.download(np.float64, (n,)) - b
+            return float(np.dot(r, r)) / bb, r
+
+        for _ in range(steps):
+            info = st.step()
+            b = np.asfortranarray(st.get_vector("RHS")).ravel(order="F")
+            x = np.asfortranarray(st.get_vector("SOLUTION")).ravel(order="F") * fcs
+            if prev is not None:
+                warm = prev * (info.dt / prev_dt)
+                bb = float(np.dot(b, b))
+                d_b.upload(b)
+                d_prev.upload(warm)
+                bnorm = ctypes.c_double(0.0)
+                rc = lib.pcf_guess_input(st.h, ctypes.c_void_p(d_b.ptr), ctypes.c_void_p(d_prev.ptr),
+                                         ctypes.c_void_p(d_in.ptr), ctypes.byref(bnorm))
+                if rc:
+                    raise PinCFlowError(rc, lib.pcf_last_error().decode())
+                l_warm, _ = loss_and_residual(warm, b, bb)
+                l = l_warm
+                for _ in range(inner):
+                    rc = net.lib.pcf_net_forward(net.h, ctypes.c_void_p(d_in.ptr), ctypes.c_void_p(d_y.ptr))
+                    if rc:
+                        raise PinCFlowError(rc, net.lib.pcf_last_error().decode())
+                    y = d_y.download(np.float32, (n,)).astype(np.float64)
+                    l, r = loss_and_residual(warm + bnorm.value * y, b, bb)
+                    d_x.upload(r)
+                    rc = lib.pcf_apply_operator_transpose(st.h, ctypes.c_void_p(d_x.ptr), ctypes.c_void_p(d_ax.ptr))
+                    if rc:
+                        raise PinCFlowError(rc, lib.pcf_last_error().decode())
+                    g = d_ax.download(np.float64, (n,)) * (2.0 * bnorm.value / bb)
+                    d_y.upload(g.astype(np.float32))
+                    rc = net.lib.pcf_net_train_step_dout(net.h, ctypes.c_void_p(d_in.ptr), ctypes.c_void_p(d_y.ptr),
+                                                         ctypes.c_float(lr))
+                    if rc:
+                        raise PinCFlowError(rc, net.lib.pcf_last_error().decode())
+                out.append((l_warm, l))
+                if log:
+                    log(f"solve {len(out)}: residual loss of the warm start {l_warm:.4e}, of the guess {l:.4e}")
+            prev, prev_dt = x, info.dt
+    return out
+
+
+def residual_loss(net: GuessNetwork, st: State, steps: int) -> List[Tuple[float, float]]:
+    """Per solve of `steps` time steps of st (after the first): the residual
+    loss |A x0 - b|^2 / |b|^2 of the warm start and of the network's guess."""
+    n = st.nx * st.ny * st.nz
+    lib = st.lib
+    fcs = _interior_fcscal(st).ravel(order="F")
+    d_b, d_prev, d_x, d_ax = DeviceBuffer(8 * n), DeviceBuffer(8 * n), DeviceBuffer(8 * n), DeviceBuffer(8 * n)
+    d_in, d_y = DeviceBuffer(12 * n), DeviceBuffer(4 * n)
+    out, prev, prev_dt = [], None, None
+    for _ in range(steps):
+        info = st.step()
+        b = np.asfortranarray(st.get_vector("RHS")).ravel(order="F")
+        x = np.asfortranarray(st.get_vector("SOLUTION")).ravel(order="F") * fcs
+        if prev is not None:
+            warm = prev * (info.dt / prev_dt)
+            bb = float(np.dot(b, b))
+            d_b.upload(b)
+            d_prev.upload(warm)
+            bnorm = ctypes.c_double(0.0)
+            rc = lib.pcf_guess_input(st.h, ctypes.c_void_p(d_b.ptr), ctypes.c_void_p(d_prev.ptr),
+                                     ctypes.c_void_p(d_in.ptr), ctypes.byref(bnorm))
+            if rc:
+                raise PinCFlowError(rc, lib.pcf_last_error().decode())
+            rc = net.lib.pcf_net_forward(net.h, ctypes.c_void_p(d_in.ptr), ctypes.c_void_p(d_y.ptr))
+            if rc:
+                raise PinCFlowError(rc, net.lib.pcf_last_error().decode())
+            y = d_y.download(np.float32, (n,)).astype(np.float64)
+            res = []
+            for x0 in (warm, warm + bnorm.value * y):
+                d_x.upload(x0)
+                st.apply_operator_device(d_x.ptr, d_ax.ptr, True)
+                r = d_ax.download(np.float64, (n,)) - b
+                res.append(float(np.dot(r, r)) / bb)
+            out.append(tuple(res))
+        prev, prev_dt = x, info.dt
+    return out
 
 
 def evaluate(net: GuessNetwork, samples) -> float:

@@ -4,6 +4,9 @@ the mixed-precision solve (multigrid preconditioner), and compared on held-out
 runs with the zero start and with the warm start it corrects.  The solve's
 stopping test is unchanged (|r| / |b| against the reference tolerance), so
 the guess affects the iteration count only."""
+import ctypes
+
+import hipmem
 import numpy as np
 import pytest
 
@@ -62,3 +65,50 @@ def test_learned_guess_gains_over_zero_start(pcf):
     # scale, which the operator amplifies): the learned start may not cost
     # more than an iteration per solve over either start
     assert lrn <= min(z, w) + len(zero) - 1, (zero, warm, learned)
+
+
+def test_transposed_operator(pcf):
+    """<A x, y> = <x, A^T y> for the operator of a solve (the gradient of the
+    residual loss goes through A^T)."""
+    st = pcf.State(namelists(pcf, 5, "zero", 2))
+    st.initial_cleaning()
+    st.step()
+    n = st.nx * st.ny * st.nz
+    rng = np.random.default_rng(0)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    dx, dy, da = hipmem.DeviceArray((n,)).upload(x), hipmem.DeviceArray((n,)).upload(y), hipmem.DeviceArray((n,))
+    st.apply_operator_device(dx.ptr, da.ptr, True)
+    axy = float(np.dot(da.download(), y))
+    assert st.lib.pcf_apply_operator_transpose(st.h, ctypes.c_void_p(dy.ptr), ctypes.c_void_p(da.ptr)) == 0
+    xaty = float(np.dot(x, da.download()))
+    assert abs(axy - xaty) <= 1e-12 * abs(axy), (axy, xaty)
+
+
+def test_residual_trained_guess(pcf):
+    """Online training on the residual |A x0 - b|^2 / |b|^2 of the guess
+    (learned.train_residual) on two runs; on a held-out run the guess leaves
+    a smaller residual than the warm start it corrects.  The iteration
+    counts stay those of the warm start: the guess removes the fast residual
+    components, which BiCGSTAB removes in its first iterations anyway."""
+    net = pcf.learned.GuessNetwork(32, 32, 16, training=True, seed=0)
+    states = []
+    for seed in (11, 12):
+        st = pcf.State(namelists(pcf, seed, "zero", 14))
+        st.initial_cleaning()
+        states.append(st)
+    hist = pcf.learned.train_residual(net, states, 14, inner=6, lr=1e-3)
+    print("training solves (warm, guess):", [(round(a, 4), round(b, 4)) for a, b in hist])
+    assert all(b < a for a, b in hist[-6:])
+    infer = pcf.learned.GuessNetwork(32, 32, 16, training=False, params=net.params())
+    held_out = 21
+    st = pcf.State(namelists(pcf, held_out, "zero", 10))
+    st.initial_cleaning()
+    rl = pcf.learned.residual_loss(infer, st, 8)
+    print("held-out residual loss (warm, learned):", [(round(a, 4), round(b, 4)) for a, b in rl])
+    ratio = float(np.mean([b / a for a, b in rl]))
+    assert ratio < 0.85, ratio
+    steps = 8
+    warm = stage_iterations(pcf, namelists(pcf, held_out, "previous", steps), steps=steps)
+    learned = stage_iterations(pcf, namelists(pcf, held_out, "learned", steps), net=infer, steps=steps)
+    print("stage-2 iterations, warm start:", warm, "learned start:", learned)
+    assert sum(learned[1:]) <= sum(warm[1:]) + len(warm) - 1, (warm, learned)
