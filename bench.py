@@ -72,6 +72,8 @@ def parse():
                    help="learned guess: residual |A x0 - b|^2 (online, --train-inner Adam steps per solve) or "
                         "the squared error of the solution change")
     p.add_argument("--train-inner", type=int, default=6, help="learned guess: Adam steps per solve (residual loss)")
+    p.add_argument("--train-lr", type=float, default=1e-6,
+                   help="learned guess: Adam learning rate of the residual loss (1e-5 and above diverge at 256^2 x 128)")
     p.add_argument("--no-variants", action="store_true",
                    help="plain FP64 run: skip the mixed-precision, multigrid and learned lines")
     p.add_argument("--no-fp64-check", action="store_true",
@@ -354,7 +356,7 @@ def train_guess_residual(pcf, args, preconditioner=None):
         nl.poisson.preconditioner_type = preconditioner or args.preconditioner
         st = pcf.State(nl)
         st.initial_cleaning()
-        hist += pcf.learned.train_residual(net, [st], args.train_steps, inner=args.train_inner,
+        hist += pcf.learned.train_residual(net, [st], args.train_steps, inner=args.train_inner, lr=args.train_lr,
                                            log=lambda m: print("# " + m, file=sys.stderr, flush=True))
         st.close()
     params = net.params()
@@ -369,6 +371,7 @@ def train_guess_residual(pcf, args, preconditioner=None):
     rl = pcf.learned.residual_loss(infer, st, 5)
     st.close()
     return {"net": infer, "loss": "residual", "solves": len(hist), "adam_steps_per_solve": args.train_inner,
+            "learning_rate": args.train_lr,
             "train_s": round(t_train, 1),
             "training_residual_vs_warm_start_last": round(hist[-1][1] / hist[-1][0], 4),
             "held_out_residual_vs_warm_start": [round(b / a, 4) for a, b in rl]}
