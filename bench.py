@@ -44,9 +44,11 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="mountain_wave", choices=["mountain_wave", "gaussian_hills"],
+    p.add_argument("--workload", default="mountain_wave", choices=["mountain_wave", "gaussian_hills", "ensemble"],
                    help="mountain_wave: one Gaussian hill (the metric's workload); gaussian_hills: the same "
-                        "physics over 16 seeded hills (the mixed-precision configuration, run at --nx 512)")
+                        "physics over 16 seeded hills (the mixed-precision configuration, run at --nx 512); "
+                        "ensemble: --members seeded bubble members sharing the GPU, against one at a time")
+    p.add_argument("--members", type=int, default=16, help="ensemble workload: number of members")
     p.add_argument("--nx", type=int, default=256)
     p.add_argument("--nz", type=int, default=128)
     p.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
@@ -169,6 +171,9 @@ def main():
               "(the domain is not decomposed across GPUs in this build)", file=sys.stderr)
         sys.exit(2)
     pcf = load_package()
+    if args.workload == "ensemble":
+        ensemble_bench(pcf, args)
+        return
     plain = args.precision == "fp64"
     training = None
     net = None
@@ -301,6 +306,42 @@ def main():
         "kernel_ms_one_step": {k: [v[0], round(v[1], 3)] for k, v in sorted(main_run["breakdown"].items())},
     }
     print(json.dumps(out), flush=True)
+
+
+def ensemble_bench(pcf, args):
+    """Member time steps per second of the seeded bubble ensemble
+    (examples.bubble_member, seeds 0 .. members-1, their own grid sizes) run
+    by ensemble() with all members sharing the GPU, against the same members
+    one at a time; the members' records are kept in memory, not written."""
+    def sim(seed):
+        nl = pcf.examples.bubble_member(seed=seed, iterations=args.steps, precision=args.precision)
+        if args.precision == "mixed":
+            nl.poisson.preconditioner_type = args.preconditioner
+            nl.poisson.preconditioner_storage = args.storage
+            nl.poisson.mg_coarse_steps = args.mg_coarse_steps
+        return nl
+    params = {"seed": tuple(range(args.members)),
+              "output_file": tuple(f"member{m}.npz" for m in range(args.members))}
+    cells = [sim(m).domain for m in range(args.members)]
+    cells = sum(d.x_size * d.y_size * d.z_size for d in cells)
+    runs = {}
+    for label, conc in (("one at a time", 1), ("sharing the GPU", args.members)):
+        t0 = time.perf_counter()
+        res = pcf.ensemble(sim, params, {}, max_concurrent=conc, write=False)
+        elapsed = time.perf_counter() - t0
+        its = [sum(s[2] + s[3] for s in r[2]) / (2 * len(r[2])) for r in res]
+        for r in res:
+            r[0].close()
+        runs[label] = {"member_steps_per_s": round(args.members * args.steps / elapsed, 3),
+                       "wall_s": round(elapsed, 2), "bicgstab_iterations_per_solve": round(sum(its) / len(its), 1)}
+        print(f"# [ensemble] {label}: {elapsed:.1f}s", file=sys.stderr, flush=True)
+    print(json.dumps({"workload": f"seeded bubble ensemble, {args.members} members, {args.steps} steps each "
+                                  "(setup and initial cleaning included)",
+                      "cells": cells, "precision": args.precision,
+                      **({"preconditioner": args.preconditioner} if args.precision == "mixed" else {}),
+                      "runs": runs,
+                      "speedup": round(runs["sharing the GPU"]["member_steps_per_s"] /
+                                       runs["one at a time"]["member_steps_per_s"], 3)}), flush=True)
 
 
 def train_guess(pcf, args, preconditioner=None):

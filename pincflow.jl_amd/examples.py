@@ -3,7 +3,8 @@
 ``periodic_hill``, ``mountain_wave``, ``cold_bubble``, ``hot_bubble`` and ``vortex`` return the
 Namelists their reference namesake passes to ``integrate``; the keyword
 arguments keep the reference names and defaults.  ``mountain_wave_3d`` and
-``gaussian_hills_3d`` are the production configurations measured by bench.py.
+``gaussian_hills_3d`` are the production configurations measured by bench.py;
+``bubble_member`` builds the members of the seeded bubble ensemble.
 """
 from __future__ import annotations
 
@@ -118,3 +119,47 @@ def gaussian_hills_3d(*, x_size: int = 512, y_size: int = 512, z_size: int = 128
     nl = mountain_wave_3d(x_size=x_size, y_size=y_size, z_size=z_size, iterations=iterations, precision=precision)
     nl.grid.resolved_topography = GaussianHills(hills)
     return nl
+
+
+def bubble_member(*, seed: int, x_size: int = 0, z_size: int = 0, iterations: int = 50,
+                  precision: str = "fp64") -> Namelists:
+    """One member of the seeded bubble ensemble: from numpy's PCG64 generator
+    with the given seed, a horizontal grid of 64, 96, 128 or 192 cells squared
+    and 64 or 96 levels (x_size / z_size override them when positive), the
+    Boussinesq (stable stratification) or pseudo-incompressible (isentropic)
+    model, a hot or cold cosine bubble of potential-temperature amplitude
+    U(-2, 2) K and radius U(1, 3) km at a random position, over flat ground
+    or one Gaussian hill (U(100, 500) m, half-width U(2, 5) km), in a 60 km x
+    60 km x 10 km box (cells at least three times wider than high, which the
+    reference's z-line preconditioner needs to converge); `iterations` output
+    steps."""
+    import numpy as np
+
+    lx = ly = 60000.0
+    lz = 10000.0
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nxy = int(rng.choice([64, 96, 128, 192]))
+    nz = int(rng.choice([64, 96]))
+    boussinesq = bool(rng.uniform() < 0.5)
+    amp = float(rng.uniform(-2.0, 2.0))
+    r = float(rng.uniform(1000.0, 3000.0))
+    x0 = float(rng.uniform(-lx / 2 + r, lx / 2 - r))
+    y0 = float(rng.uniform(-ly / 2 + r, ly / 2 - r))
+    z0 = float(rng.uniform(r + 500.0, lz - r))
+    hill = None
+    if rng.uniform() < 0.5:
+        hill = GaussianHills(((float(rng.uniform(100.0, 500.0)), 0.0, 0.0, float(rng.uniform(2000.0, 5000.0))),))
+    return Namelists(
+        atmosphere=AtmosphereNamelist(
+            model="Boussinesq" if boussinesq else "PseudoIncompressible",
+            background="StableStratification" if boussinesq else "Isentropic",
+            buoyancy_initialization="initial_thetap",
+            # A (1 + cos(pi r)) peaks at 2 A
+            initial_thetap=CosineBubble(amp / 2, x0=x0, y0=y0, z0=z0, rx=r, ry=r, rz=r)),
+        # dtmax of the bubble examples (cold_bubble.jl, hot_bubble.jl): at rest the CFL limit allows any step
+        discretization=DiscretizationNamelist(dtmax=60.0),
+        domain=DomainNamelist(x_size=x_size or nxy, y_size=x_size or nxy, z_size=z_size or nz, lx=lx, ly=ly, lz=lz),
+        grid=GridNamelist(resolved_topography=hill),
+        output=OutputNamelist(output_variables=("thetap",), output_steps=True, iterations=iterations),
+        poisson=PoissonNamelist(precision=precision),
+    )
