@@ -607,6 +607,7 @@ int pcf_set_field(pcf_state* st, int field, const double* host_in) {
   if (!p) { set_error("unknown field"); return PCF_E_ARG; }
   PCF_HIP(hipMemcpyAsync(p, host_in, st->g.nh() * sizeof(double), hipMemcpyHostToDevice, st->stream));
   PCF_HIP(hipStreamSynchronize(st->stream));
+  st->lhs_div_valid[0] = st->lhs_div_valid[1] = false;  // fluxes or the Jacobian may have changed
   if (field == PCF_F_PBAR || field == PCF_F_RHOBAR) TRY(refresh_fcscal(st));
   return PCF_OK;
 }

@@ -418,6 +418,9 @@ struct UpdArgs {
   int first;
   int force0;  // the tracer adds its (zero) volume force, update!.jl:1490-1492
   int heat;    // 1: RhoP, -heating / thetabar (update!.jl:586-594); 2: P, +heating (update!.jl:1436-1442)
+  double* div_out;        // U / V updates: their flux divergence / J_edge, kept for the W update (or null)
+  const double* div_u;    // W update: those of the U and V updates of this stage (or null: recompute)
+  const double* div_v;
 };
 
 __device__ __forceinline__ double divergence(const double* const* f, const Geo& g, int i, int j, int k) {
@@ -451,6 +454,7 @@ __global__ void k_update_u(UpdArgs A, Range3 R) {  // update!.jl:730-806
   double fluxdiff = divergence(A.f, g, i, j, k);
   double jacedger = 0.5 * (F.jac[c] + F.jac[r]);
   fluxdiff /= jacedger;
+  if (A.div_out) A.div_out[c] = fluxdiff;
   double vc = 0.5 * (F.v[c] + F.v[g.h(i, j - 1, k)]);
   double vr = 0.5 * (F.v[r] + F.v[g.h(i + 1, j - 1, k)]);
   double volforce = 0.5 * A.ph.fc * ((A.rhoold[c] + F.rhobar[c]) * vc + (A.rhoold[r] + F.rhobar[r]) * vr);
@@ -478,6 +482,7 @@ __global__ void k_update_v(UpdArgs A, Range3 R) {  // update!.jl:884-957
   double fluxdiff = divergence(A.f, g, i, j, k);
   double jacedgef = 0.5 * (F.jac[c] + F.jac[f]);
   fluxdiff /= jacedgef;
+  if (A.div_out) A.div_out[c] = fluxdiff;
   double uc = 0.5 * (A.uold[c] + A.uold[g.h(i - 1, j, k)]);
   double uf = 0.5 * (A.uold[f] + A.uold[g.h(i - 1, j + 1, k)]);
   double volforce = -0.5 * A.ph.fc * ((A.rhoold[c] + F.rhobar[c]) * uc + (A.rhoold[f] + F.rhobar[f]) * uf);
@@ -507,18 +512,26 @@ __global__ void k_update_w(UpdArgs A, Range3 R) {  // update!.jl:1035-1192
   double jacedgeu = 2.0 * jc * ju / (jc + ju);
   fluxdiff /= jacedgeu;
   double fdu[2][2], fdv[2][2];
-  for (int ll = 0; ll <= 1; ++ll)
-    for (int mm = 0; mm <= 1; ++mm) {
-      fdu[ll][mm] = divergence(A.fu, g, i - ll, j, k + mm);
-      double jacedger = 0.5 * (F.jac[g.h(i - ll, j, k + mm)] + F.jac[g.h(i + 1 - ll, j, k + mm)]);
-      fdu[ll][mm] /= jacedger;
-    }
-  for (int ll = 0; ll <= 1; ++ll)
-    for (int mm = 0; mm <= 1; ++mm) {
-      fdv[ll][mm] = divergence(A.fv, g, i, j - ll, k + mm);
-      double jacedgef = 0.5 * (F.jac[g.h(i, j - ll, k + mm)] + F.jac[g.h(i, j + 1 - ll, k + mm)]);
-      fdv[ll][mm] /= jacedgef;
-    }
+  if (A.div_u) {  // the same values, as the U and V updates of this stage computed them
+    for (int ll = 0; ll <= 1; ++ll)
+      for (int mm = 0; mm <= 1; ++mm) {
+        fdu[ll][mm] = A.div_u[g.h(i - ll, j, k + mm)];
+        fdv[ll][mm] = A.div_v[g.h(i, j - ll, k + mm)];
+      }
+  } else {
+    for (int ll = 0; ll <= 1; ++ll)
+      for (int mm = 0; mm <= 1; ++mm) {
+        fdu[ll][mm] = divergence(A.fu, g, i - ll, j, k + mm);
+        double jacedger = 0.5 * (F.jac[g.h(i - ll, j, k + mm)] + F.jac[g.h(i + 1 - ll, j, k + mm)]);
+        fdu[ll][mm] /= jacedger;
+      }
+    for (int ll = 0; ll <= 1; ++ll)
+      for (int mm = 0; mm <= 1; ++mm) {
+        fdv[ll][mm] = divergence(A.fv, g, i, j - ll, k + mm);
+        double jacedgef = 0.5 * (F.jac[g.h(i, j - ll, k + mm)] + F.jac[g.h(i, j + 1 - ll, k + mm)]);
+        fdv[ll][mm] /= jacedgef;
+      }
+  }
   fluxdiff = transform_transformed(F, g, i, j, k, fdu[0][0], fdu[0][1], fdu[1][0], fdu[1][1], fdv[0][0], fdv[0][1],
                                    fdv[1][0], fdv[1][1], fluxdiff);
   const double fc = A.ph.fc;
@@ -923,6 +936,7 @@ void launch_flux(pcf_state* st, const FluxArgs& A, Range3 R) {
 int compute_fluxes(pcf_state* st) {
   ProfScope prof(st, PCF_K_FLUXES);
   const Geo& g = st->g;
+  st->lhs_div_valid[0] = st->lhs_div_valid[1] = false;  // the stress arrays are rewritten below
   Fields F = fields_of(st);
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
   const int nlev = g.nz + 2;  // k0-1 .. k1+1
@@ -1058,6 +1072,8 @@ int update_lhs(pcf_state* st, double dt, int m, int var) {
   A.first = m == 1;
   A.force0 = 0;
   A.heat = 0;
+  A.div_out = nullptr;
+  A.div_u = A.div_v = nullptr;
   A.rhoold = st->rhoold;
   A.uold = st->uold;
   A.vold = st->vold;
@@ -1103,19 +1119,29 @@ int update_lhs(pcf_state* st, double dt, int m, int var) {
       A.q = st->u;
       A.dq = st->du;
       for (int d = 0; d < 3; ++d) A.f[d] = st->flux[V_U][d];
+      // the stress arrays are free between flux computations: keep the
+      // divergences there for the W update of this stage
+      A.div_out = st->stress[0];
       R = Range3{g.i0 - 1, g.nx + 1, g.j0, g.ny, g.k0, g.nz};
       hipLaunchKernelGGL(k_update_u, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      st->lhs_div_valid[0] = true;
       break;
     case PCF_VAR_V:
       A.q = st->v;
       A.dq = st->dv;
       for (int d = 0; d < 3; ++d) A.f[d] = st->flux[V_V][d];
+      A.div_out = st->stress[1];
       R = Range3{g.i0, g.nx, g.j0 - 1, g.ny + 1, g.k0, g.nz};
       hipLaunchKernelGGL(k_update_v, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      st->lhs_div_valid[1] = true;
       break;
     case PCF_VAR_W:
       A.q = st->w;
       A.dq = st->dw;
+      if (st->lhs_div_valid[0] && st->lhs_div_valid[1]) {
+        A.div_u = st->stress[0];
+        A.div_v = st->stress[1];
+      }
       for (int d = 0; d < 3; ++d) A.f[d] = st->flux[V_W][d];
       R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz - 1};
       if (R.nk > 0)

@@ -150,6 +150,9 @@ struct pcf_state {
   double* phi[6] = {};                // reconstruction inputs
   double* wcart;                      // Cartesian vertical wind
   double* stress[6];                  // 11, 12, 13, 22, 23, 33
+  // between flux computations stress[0] and stress[1] hold the flux
+  // divergences of the stage's U and V updates, which the W update reuses
+  bool lhs_div_valid[2] = {false, false};
   // corrector
   double *dpip, *corx, *cory;
   // Poisson (interior)
