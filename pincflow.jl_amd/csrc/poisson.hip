@@ -82,8 +82,12 @@ __global__ void k_finalize_lhs(const double* partials, int nparts, SolverScalars
 // ---------------------------------------------------------------------------
 // compute_operator! (compute_operator!.jl:40-1606)
 // ---------------------------------------------------------------------------
+// COMP: the compressible model (its buoyancy factor and centre shift); a
+// template parameter so that each model's kernel carries only its own branch
+template <bool COMP>
 __global__ void __launch_bounds__(256) k_compute_operator(Fields F, const double* betar, const double* fcs, Geo g,
-                                                          Tensor T, double dt, double rf, int damp, Phys ph) {
+                                                          Tensor T, double dt, double rf, int damp,
+                                                          const double* __restrict__ shift) {
   int i = g.i0 + blockIdx.x * blockDim.x + threadIdx.x;
   int j = g.j0 + blockIdx.y * blockDim.y + threadIdx.y;
   int k = g.k0 + blockIdx.z;
@@ -131,6 +135,7 @@ __global__ void __launch_bounds__(256) k_compute_operator(Fields F, const double
   const double rhoedgeb = 0.5 * (rho[C] + rho[B]) + 0.5 * (rhobar[C] + rhobar[B]);
   const double rhoedgeu = (ju * rho[C] + jc * rho[U]) / (jc + ju) + (ju * rhobar[C] + jc * rhobar[U]) / (jc + ju);
   const double rhoedged = (jd * rho[C] + jc * rho[D]) / (jc + jd) + (jd * rhobar[C] + jc * rhobar[D]) / (jc + jd);
+  F.comp = COMP ? 1 : 0;
   const double fwu = buoyancy_factor_w(F, g, i, j, k);
   const double fwd = buoyancy_factor_w(F, g, i, j, k - 1);
   const double rhouedger = 0.5 * (rho[U] + rho[RU] + rhobar[U] + rhobar[RU]);
@@ -272,10 +277,7 @@ __global__ void __launch_bounds__(256) k_compute_operator(Fields F, const double
          q4(UL, met13uedgel) - q4(UF, met23uedgef) - q4(UB, met23uedgeb) + q4(DR, met13dedger) + q4(DL, met13dedgel) +
          q4(DF, met23dedgef) + q4(DB, met23dedgeb);
   }
-  if (F.comp) {  // compute_operator!.jl:854-860
-    double d = dpdpi(ph, F.p[C]);
-    ac -= (d / dt) / (dt * ph.rsp / ph.kappa);
-  }
+  if (COMP) ac -= shift[g.v(i - g.i0 + 1, j - g.j0 + 1, k - g.k0 + 1)];  // compute_operator!.jl:854-860
   if (kbot) {
     ar = Rr * (1.0 / dx - 0.75 * met13edger / dz) + hx(Uu, met13edgeu, dx, jru, jr + jru) - q4(UR, met13uedger);
     al = -Ll * (1.0 / dx + 0.75 * met13edgel / dz) - hx(Uu, met13edgeu, dx, jlu, jl + jlu) - q4(UL, met13uedgel);
@@ -995,12 +997,34 @@ int compute_lhs(pcf_state* st, double* tolref_out) {
   return PCF_OK;
 }
 
+// The compressible model's centre-coefficient term (dP/dpi / dt) / (dt R / kappa)
+// (compute_operator!.jl:854-860) per interior cell; evaluated apart so that
+// its pow does not raise the register count of the operator kernel.
+__global__ void k_operator_shift(const double* p, Phys ph, Geo g, double dt, double* shift) {
+  const int i = g.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = g.j0 + blockIdx.y;
+  const int k = g.k0 + blockIdx.z;
+  if (i > g.i1) return;
+  double d = dpdpi(ph, p[g.h(i, j, k)]);
+  shift[g.v(i - g.i0 + 1, j - g.j0 + 1, k - g.k0 + 1)] = (d / dt) / (dt * ph.rsp / ph.kappa);
+}
+
 int compute_operator(pcf_state* st, double dt, double rf) {
   const Geo& g = st->g;
   dim3 blk(64, 4), grd(blocks_for(g.nx, 64), blocks_for(g.ny, 4), g.nz);
   ProfScope prof(st, PCF_K_COMPUTE_OPERATOR);
-  hipLaunchKernelGGL(k_compute_operator, grd, blk, 0, st->stream, fields_of(st), st->betar, st->fcscal, g, st->T, dt,
-                     rf, st->ph.damp, st->ph);
+  const double* shift = nullptr;
+  if (st->compressible) {  // pc_tmp is free outside the preconditioner
+    hipLaunchKernelGGL(k_operator_shift, dim3(blocks_for(g.nx, 64), g.ny, g.nz), dim3(64), 0, st->stream, st->p,
+                       st->ph, g, dt, st->pc_tmp);
+    shift = st->pc_tmp;
+  }
+  if (st->compressible)
+    hipLaunchKernelGGL(k_compute_operator<true>, grd, blk, 0, st->stream, fields_of(st), st->betar, st->fcscal, g,
+                       st->T, dt, rf, st->ph.damp, shift);
+  else
+    hipLaunchKernelGGL(k_compute_operator<false>, grd, blk, 0, st->stream, fields_of(st), st->betar, st->fcscal, g,
+                       st->T, dt, rf, st->ph.damp, shift);
   PCF_HIP(hipGetLastError());
   st->pc_factors_valid = false;
   st->t32_valid = false;
