@@ -49,6 +49,8 @@ def parse():
                         "physics over 16 seeded hills (the mixed-precision configuration, run at --nx 512); "
                         "ensemble: --members seeded bubble members sharing the GPU, against one at a time")
     p.add_argument("--members", type=int, default=16, help="ensemble workload: number of members")
+    p.add_argument("--model", default="PseudoIncompressible", choices=["PseudoIncompressible", "Compressible"],
+                   help="dynamic equations of the mountain-wave / hill workloads")
     p.add_argument("--nx", type=int, default=256)
     p.add_argument("--nz", type=int, default=128)
     p.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
@@ -86,10 +88,13 @@ def parse():
 def workload(pcf, args, iterations, precision):
     """Namelists of the benchmarked workload at --nx x --nx x --nz."""
     if args.workload == "gaussian_hills":
-        return pcf.examples.gaussian_hills_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz, iterations=iterations,
-                                              precision=precision)
-    return pcf.examples.mountain_wave_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz, iterations=iterations,
-                                         precision=precision)
+        nl = pcf.examples.gaussian_hills_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz, iterations=iterations,
+                                            precision=precision)
+    else:
+        nl = pcf.examples.mountain_wave_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz, iterations=iterations,
+                                           precision=precision)
+    nl.atmosphere.model = args.model
+    return nl
 
 
 def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False, storage=None, mg_coarse=None,
@@ -281,8 +286,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if plain else "f64 (preconditioner f32)",
         "data": "synthetic",
-        "config": {"workload": (("pseudo-incompressible flow over 16 seeded Gaussian hills "
-                                 if args.workload == "gaussian_hills" else "pseudo-incompressible mountain wave ") +
+        "config": {"workload": (("compressible " if args.model == "Compressible" else "pseudo-incompressible ") +
+                                ("flow over 16 seeded Gaussian hills " if args.workload == "gaussian_hills"
+                                 else "mountain wave ") +
                                 f"{args.nx}x{args.nx}x{args.nz}, " +
                                 (f"FP64 BiCGSTAB with the {args.preconditioner.replace('_', ' ')} preconditioner "
                                  f"evaluated in FP32, initial guess {args.initial_guess}" if not plain

@@ -19,13 +19,29 @@ namespace pcf {
 void set_error(const std::string& msg) { g_last_error = msg; }
 }  // namespace pcf
 
+namespace pcf {
+
+constexpr size_t kStaggerStep = 2304;  // 9 x 256 B
+constexpr unsigned kStaggerSlots = 28;
+
+int device_alloc(pcf_state* st, void** p, size_t bytes) {
+  const size_t off = (size_t)(st->alloc_count++ % kStaggerSlots) * kStaggerStep;
+  char* base = nullptr;
+  PCF_HIP(hipMalloc((void**)&base, bytes + off));
+  st->allocations.push_back(base);
+  *p = base + off;
+  return PCF_OK;
+}
+
+}  // namespace pcf
+
 using namespace pcf;
 
 namespace {
 
 int alloc(pcf_state* st, double** p, size_t n) {
-  PCF_HIP(hipMalloc((void**)p, n * sizeof(double)));
-  st->allocations.push_back(*p);
+  int rc = device_alloc(st, p, n);
+  if (rc) return rc;
   PCF_HIP(hipMemsetAsync(*p, 0, n * sizeof(double), st->stream));
   return PCF_OK;
 }

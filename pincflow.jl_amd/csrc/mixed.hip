@@ -1092,27 +1092,19 @@ int apply_bicgstab_mixed(pcf_state* st, double tolref, double dt, int* niter_out
 // Allocation of the mixed-precision buffers (pcf_create, precision MIXED).
 int alloc_mixed(pcf_state* st) {
   const size_t N = st->g.nv();
-  auto af = [&](float** p) -> int {
-    PCF_HIP(hipMalloc((void**)p, N * sizeof(float)));
-    st->allocations.push_back(*p);
-    return PCF_OK;
-  };
+  auto af = [&](float** p) -> int { return device_alloc(st, p, N); };
   int rc;
   for (int t = 0; t < T_COUNT; ++t)
     if ((rc = af(&st->T32[t]))) return rc;
   if (st->cfg.preconditioner_storage == PCF_STORAGE_BF16)
-    for (int t = 0; t < T_COUNT; ++t) {
-      PCF_HIP(hipMalloc((void**)&st->T16[t], N * sizeof(uint16_t)));
-      st->allocations.push_back(st->T16[t]);
-    }
+    for (int t = 0; t < T_COUNT; ++t)
+      if ((rc = device_alloc(st, &st->T16[t], N))) return rc;
   float** fl[] = {&st->pc32_P, &st->pc32_E, &st->pc32_Q, &st->m_pc_tmp, &st->m_pc_cur, &st->m_pc_rhs};
   for (float** p : fl)
     if ((rc = af(p))) return rc;
   double** dl[] = {&st->bzpc, &st->m_prev[0], &st->m_prev[1]};
-  for (double** p : dl) {
-    PCF_HIP(hipMalloc((void**)p, N * sizeof(double)));
-    st->allocations.push_back(*p);
-  }
+  for (double** p : dl)
+    if ((rc = device_alloc(st, p, N))) return rc;
   if (st->cfg.preconditioner_type != PCF_PC_MULTIGRID) return PCF_OK;
   // level 0 works on the buffers above; coarser levels get their own
   const Geo& g = st->g;
@@ -1153,11 +1145,7 @@ int alloc_mixed(pcf_state* st) {
     cy *= fx;
     C.deta = (float)(st->cfg.dtau / (2.0 * (cx + cy)));
     const size_t Nc = (size_t)C.nx * C.ny * C.nz;
-    auto ac = [&](float** p) -> int {
-      PCF_HIP(hipMalloc((void**)p, Nc * sizeof(float)));
-      st->allocations.push_back(*p);
-      return PCF_OK;
-    };
+    auto ac = [&](float** p) -> int { return device_alloc(st, p, Nc); };
     for (int t = 0; t < T_COUNT; ++t)
       if ((rc = ac(&C.a[t]))) return rc;
     float** v[] = {&C.P, &C.E, &C.Q, &C.b, &C.s, &C.rhs, &C.col};
@@ -1165,10 +1153,8 @@ int alloc_mixed(pcf_state* st) {
       if ((rc = ac(p))) return rc;
     C.zf = column_path(C);  // coarse levels smoothed column by column: z fastest
     if (!C.zf && st->cfg.preconditioner_storage == PCF_STORAGE_BF16)
-      for (int t = 0; t < T_COUNT; ++t) {
-        PCF_HIP(hipMalloc((void**)&C.a16[t], Nc * sizeof(uint16_t)));
-        st->allocations.push_back(C.a16[t]);
-      }
+      for (int t = 0; t < T_COUNT; ++t)
+        if ((rc = device_alloc(st, &C.a16[t], Nc))) return rc;
     st->mg.push_back(C);
   }
   return PCF_OK;

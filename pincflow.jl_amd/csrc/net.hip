@@ -654,10 +654,9 @@ int pcf_set_guess_network(pcf_state* st, pcf_net* net) {
   }
   if (net && !st->guess_in) {
     const size_t N = st->g.nv();
-    PCF_HIP(hipMalloc((void**)&st->guess_in, 3 * N * sizeof(float)));
-    st->allocations.push_back(st->guess_in);
-    PCF_HIP(hipMalloc((void**)&st->guess_x, N * sizeof(double)));
-    st->allocations.push_back(st->guess_x);
+    int rc = device_alloc(st, &st->guess_in, 3 * N);
+    if (rc == PCF_OK) rc = device_alloc(st, &st->guess_x, N);
+    if (rc) return rc;
   }
   st->guess_net = net;
   return PCF_OK;

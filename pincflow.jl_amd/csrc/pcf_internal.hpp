@@ -205,7 +205,8 @@ struct pcf_state {
   double prof_ms[PCF_K_COUNT] = {};
   long long prof_n[PCF_K_COUNT] = {};
 
-  std::vector<void*> allocations;
+  std::vector<void*> allocations;  // hipMalloc bases, freed by pcf_destroy
+  unsigned alloc_count = 0;         // staggers successive buffers (device_alloc)
 };
 
 namespace pcf {
@@ -263,6 +264,17 @@ int dot(pcf_state* st, const double* a, const double* b, double* out);
 int apply_bicgstab_mixed(pcf_state* st, double tolref, double dt, int* niter);
 int precondition_mixed(pcf_state* st, const double* sin, double* sout, const SolverScalars* sc);
 int alloc_mixed(pcf_state* st);
+// Device buffer of `bytes` owned by the state (freed by pcf_destroy).  The
+// start of the n-th buffer is offset by (n mod 28) x 2304 B within its
+// allocation: buffers that kernels stream side by side (the 25 operator
+// coefficients, the solver vectors) then start on different HBM channels
+// instead of all at the same offset from a large power of two, which made
+// the streams of one wavefront contend for the same channels.
+int device_alloc(pcf_state* st, void** p, size_t bytes);
+template <class T>
+int device_alloc(pcf_state* st, T** p, size_t count) {
+  return device_alloc(st, (void**)p, count * sizeof(T));
+}
 
 // ---- net.hip -----------------------------------------------------------------
 int learned_guess(pcf_state* st, double dt, const double* prev, const double** x0);

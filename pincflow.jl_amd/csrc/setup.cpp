@@ -153,8 +153,8 @@ double rhs_sponge_numerator(const pcf_config& n, double z) {
 
 int upload(pcf_state* st, double** dst, const std::vector<double>& src) {
   size_t bytes = src.size() * sizeof(double);
-  PCF_HIP(hipMalloc((void**)dst, bytes));
-  st->allocations.push_back(*dst);
+  int rc = device_alloc(st, dst, src.size());
+  if (rc) return rc;
   PCF_HIP(hipMemcpy(*dst, src.data(), bytes, hipMemcpyHostToDevice));
   return PCF_OK;
 }
