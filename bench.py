@@ -363,6 +363,7 @@ def train_guess(pcf, args, preconditioner=None):
         nl = pcf.examples.gaussian_hills_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz,
                                             iterations=args.train_steps, seed=100 + r, n_hills=8, precision="mixed")
         nl.poisson.preconditioner_type = preconditioner or args.preconditioner
+        nl.atmosphere.model = args.model
         st = pcf.State(nl)
         st.initial_cleaning()
         samples += pcf.learned.collect_samples(st, args.train_steps)
@@ -381,6 +382,7 @@ def train_guess(pcf, args, preconditioner=None):
     nl = pcf.examples.mountain_wave_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz, iterations=4,
                                        precision="mixed")
     nl.poisson.preconditioner_type = preconditioner or args.preconditioner
+    nl.atmosphere.model = args.model
     st = pcf.State(nl)
     st.initial_cleaning()
     held_out = pcf.learned.evaluate(infer, pcf.learned.collect_samples(st, 4))
@@ -401,6 +403,7 @@ def train_guess_residual(pcf, args, preconditioner=None):
         nl = pcf.examples.gaussian_hills_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz,
                                             iterations=args.train_steps, seed=100 + r, n_hills=8, precision="mixed")
         nl.poisson.preconditioner_type = preconditioner or args.preconditioner
+        nl.atmosphere.model = args.model
         st = pcf.State(nl)
         st.initial_cleaning()
         hist += pcf.learned.train_residual(net, [st], args.train_steps, inner=args.train_inner, lr=args.train_lr,
@@ -413,6 +416,7 @@ def train_guess_residual(pcf, args, preconditioner=None):
     nl = pcf.examples.mountain_wave_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz, iterations=5,
                                        precision="mixed")
     nl.poisson.preconditioner_type = preconditioner or args.preconditioner
+    nl.atmosphere.model = args.model
     st = pcf.State(nl)
     st.initial_cleaning()
     rl = pcf.learned.residual_loss(infer, st, 5)
@@ -466,6 +470,7 @@ def cpu_baseline(pcf, nl, iters, args):
     full_cells = nl.domain.x_size * nl.domain.y_size * nl.domain.z_size
     sub = pcf.examples.mountain_wave_3d(x_size=sx, y_size=sx, z_size=nl.domain.z_size, iterations=1,
                                         precision="fp64")
+    sub.atmosphere.model = nl.atmosphere.model
     scale = full_cells / (sx * sx * nl.domain.z_size)
     k = 10  # BiCGSTAB iterations timed (the solve is stopped there)
     cfg = pcf.to_config(sub)
