@@ -226,23 +226,35 @@ __global__ void __launch_bounds__(256) k_pc_rhs32(PcArgs<CT> A, const SinT* __re
 
 // The preconditioner's line solve in z with FP32 factors (the recurrence of
 // the FP64 k_pc_thomas); SrcT / OutT: FP64 or FP32 input and output vectors.
-// The upward sweep keeps its values in the FP32 scratch `col`.
-// FIRST: rhs = deta (0 - src).  Not __restrict__: keeps each batch of loads
-// ahead of the batch's stores.
+// One column per thread, blockDim.x = 64.  The upward sweep keeps its values
+// of the lowest kThomas32Lds levels in LDS (32 KiB per wavefront) and those
+// of higher levels in the FP32 scratch `col`.  FIRST: rhs = deta (0 - src).
+// The compiler barriers keep each batch of loads ahead of its recurrence.
+constexpr int kThomas32Lds = 128;
+
 template <bool FIRST, typename SrcT, typename OutT>
 __global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float* E, const float* Q, const SrcT* src,
                                                     OutT* out, float* col, int nx, int ny, int nz, float deta,
                                                     const SolverScalars* sc) {
+  __shared__ float up[kThomas32Lds][64];
   if (sc && sc->done) return;
   constexpr int U = 8;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + tid;
   const int j = blockIdx.y;
   if (i >= nx) return;
   const size_t sxy = (size_t)nx * ny;
   const size_t c = (size_t)i + (size_t)nx * j;
   auto rhs_of = [&](SrcT v) -> float { return FIRST ? deta * (0.0f - (float)v) : (float)v; };
+  auto keep = [&](int k, float v) {
+    if (k < kThomas32Lds)
+      up[k][tid] = v;
+    else
+      col[c + sxy * k] = v;
+  };
+  auto kept = [&](int k) -> float { return k < kThomas32Lds ? up[k][tid] : col[c + sxy * k]; };
   float sk = rhs_of(src[c]) / P[c];
-  col[c] = sk;
+  keep(0, sk);
   int k = 1;
   for (; k + U <= nz; k += U) {
     float r[U], e[U], p[U];
@@ -253,16 +265,17 @@ __global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float*
       e[u] = E[o];
       p[u] = P[o];
     }
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       sk = (r[u] + e[u] * sk) * p[u];
-      col[c + sxy * (k + u)] = sk;
+      keep(k + u, sk);
     }
   }
   for (; k < nz; ++k) {
     const size_t o = c + sxy * k;
     sk = (rhs_of(src[o]) + E[o] * sk) * P[o];
-    col[o] = sk;
+    keep(k, sk);
   }
   out[c + sxy * (nz - 1)] = (OutT)sk;
   k = nz - 2;
@@ -270,10 +283,10 @@ __global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float*
     float r[U], q[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const size_t o = c + sxy * (k - u);
-      r[u] = col[o];
-      q[u] = Q[o];
+      r[u] = kept(k - u);
+      q[u] = Q[c + sxy * (k - u)];
     }
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       sk = r[u] + q[u] * sk;
@@ -281,9 +294,8 @@ __global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float*
     }
   }
   for (; k >= 0; --k) {
-    const size_t o = c + sxy * k;
-    sk = col[o] + Q[o] * sk;
-    out[o] = (OutT)sk;
+    sk = kept(k) + Q[c + sxy * k] * sk;
+    out[c + sxy * k] = (OutT)sk;
   }
 }
 

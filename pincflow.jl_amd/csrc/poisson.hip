@@ -558,27 +558,39 @@ __global__ void __launch_bounds__(64) k_pc_factors(const double* ac, const doubl
 }
 
 // The line solve with precomputed factors, one column per thread and 64
-// adjacent columns per wavefront; the loads of 8 levels are issued before
-// their (sequential) recurrence steps.  The pointers are deliberately not
-// __restrict__: the possible aliasing of `out` keeps each batch of loads
-// ahead of the batch's stores, which is what keeps 24 loads in flight per
-// thread (with __restrict__ the compiler sinks them next to their uses).
+// adjacent columns per wavefront (blockDim.x must be 64); the loads of 8
+// levels are issued before their (sequential) recurrence steps, and a
+// compiler barrier keeps them there.  The upward sweep's values of the
+// lowest kThomasLds levels stay in LDS (kThomasLds x 64 x 8 B = 32 KiB per
+// wavefront); those of higher levels go through `out`, which the downward
+// sweep overwrites with the solution.
 // FIRST: rhs = 0 + deta (0 - src), the first pseudo-time step from s = 0,
 // whose horizontal apply is zero.
+constexpr int kThomasLds = 64;
+
 template <bool FIRST>
 __global__ void __launch_bounds__(64) k_pc_thomas(const double* P, const double* E, const double* Q, const double* src,
                                                   double* out, int nx, int ny, int nz, double deta,
                                                   const SolverScalars* sc) {
+  __shared__ double up[kThomasLds][64];
   if (sc && sc->done) return;
   constexpr int U = 8;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + tid;
   const int j = blockIdx.y;
   if (i >= nx) return;
   const size_t sxy = (size_t)nx * ny;
   const size_t c = (size_t)i + (size_t)nx * j;
   auto rhs_of = [&](double v) -> double { return FIRST ? 0.0 + deta * (0.0 - v) : v; };
+  auto keep = [&](int k, double v) {
+    if (k < kThomasLds)
+      up[k][tid] = v;
+    else
+      out[c + sxy * k] = v;
+  };
+  auto kept = [&](int k) -> double { return k < kThomasLds ? up[k][tid] : out[c + sxy * k]; };
   double sk = rhs_of(src[c]) / P[c];
-  out[c] = sk;
+  keep(0, sk);
   int k = 1;
   for (; k + U <= nz; k += U) {
     double r[U], e[U], p[U];
@@ -589,27 +601,30 @@ __global__ void __launch_bounds__(64) k_pc_thomas(const double* P, const double*
       e[u] = E[o];
       p[u] = P[o];
     }
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       sk = (rhs_of(r[u]) + e[u] * sk) * p[u];
-      out[c + sxy * (k + u)] = sk;
+      keep(k + u, sk);
     }
   }
   for (; k < nz; ++k) {
     const size_t o = c + sxy * k;
     sk = (rhs_of(src[o]) + E[o] * sk) * P[o];
-    out[o] = sk;
+    keep(k, sk);
   }
+  // the top level's upward value is its solution
+  out[c + sxy * (nz - 1)] = sk;
   // downward sweep
   k = nz - 2;
   for (; k - (U - 1) >= 0; k -= U) {
     double r[U], q[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const size_t o = c + sxy * (k - u);
-      r[u] = out[o];
-      q[u] = Q[o];
+      r[u] = kept(k - u);
+      q[u] = Q[c + sxy * (k - u)];
     }
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       sk = r[u] + q[u] * sk;
@@ -617,9 +632,8 @@ __global__ void __launch_bounds__(64) k_pc_thomas(const double* P, const double*
     }
   }
   for (; k >= 0; --k) {
-    const size_t o = c + sxy * k;
-    sk = out[o] + Q[o] * sk;
-    out[o] = sk;
+    sk = kept(k) + Q[c + sxy * k] * sk;
+    out[c + sxy * k] = sk;
   }
 }
 
