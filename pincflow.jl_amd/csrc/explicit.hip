@@ -616,11 +616,15 @@ __global__ void k_rhs_v(RhsArgs A, Range3 R) {
   }
 }
 
+// COMP: the model is compressible (folds the model's branches of the wind
+// and buoyancy factors at compile time; the runtime F.comp is overridden).
+template <bool COMP>
 __global__ void k_rhs_w(RhsArgs A, Range3 R) {
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
   const Geo& g = A.g;
-  const Fields& F = A.F;
+  Fields F = A.F;
+  F.comp = COMP ? 1 : 0;
   const size_t c = g.h(i, j, k), up = g.h(i, j, k + 1);
   const double jc = F.jac[c], ju = F.jac[up], dt = A.dt, gnd = A.ph.g_ndim;
   double rhoc = F.rho[c];
@@ -674,11 +678,13 @@ __global__ void k_rhs_rhop_explicit(RhsArgs A, Range3 R) {
 }
 
 // update!(state, dt, RhoP(), RHS(), Implicit(), rf) (update!.jl:632-727)
+template <bool COMP>
 __global__ void k_rhs_rhop_implicit(RhsArgs A, Range3 R) {
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
   const Geo& g = A.g;
-  const Fields& F = A.F;
+  Fields F = A.F;
+  F.comp = COMP ? 1 : 0;
   const size_t c = g.h(i, j, k), up = g.h(i, j, k + 1), dn = g.h(i, j, k - 1);
   const double dt = A.dt, dt2 = dt * dt, gnd = A.ph.g_ndim;
   double rhoc = F.rho[c] + F.rhobar[c];
@@ -1181,7 +1187,12 @@ int update_rhs(pcf_state* st, double dt, int var, bool implicit, double rf) {
       break;
     case PCF_VAR_W:
       R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz - 1};
-      if (R.nk > 0) hipLaunchKernelGGL(k_rhs_w, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      if (R.nk > 0) {
+        if (st->compressible)
+          hipLaunchKernelGGL(k_rhs_w<true>, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+        else
+          hipLaunchKernelGGL(k_rhs_w<false>, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      }
       break;
     case PCF_VAR_PIP:  // update!.jl:1351-1392: explicit, compressible only
       if (!st->compressible || implicit) return PCF_OK;
@@ -1191,8 +1202,12 @@ int update_rhs(pcf_state* st, double dt, int var, bool implicit, double rf) {
       break;
     case PCF_VAR_RHOP:
       R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
-      if (implicit)
-        hipLaunchKernelGGL(k_rhs_rhop_implicit, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      if (implicit && st->compressible)
+        hipLaunchKernelGGL(k_rhs_rhop_implicit<true>, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A,
+                           R);
+      else if (implicit)
+        hipLaunchKernelGGL(k_rhs_rhop_implicit<false>, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A,
+                           R);
       else
         hipLaunchKernelGGL(k_rhs_rhop_explicit, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
       break;

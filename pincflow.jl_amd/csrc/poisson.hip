@@ -930,8 +930,10 @@ __global__ void k_correct_v(Fields F, Phys ph, Geo g, const double* betar, const
   v[c] -= jpedgef * cory[c];
 }
 
+template <bool COMP>
 __global__ void k_correct_w(Fields F, Phys ph, Geo g, const double* betar, const double* dpip, const double* corx,
                             const double* cory, double* w, Range3 R, double dt, double rf) {
+  F.comp = COMP ? 1 : 0;  // folds the model's branches
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
   const size_t c = g.h(i, j, k), up = g.h(i, j, k + 1);
@@ -950,8 +952,10 @@ __global__ void k_correct_w(Fields F, Phys ph, Geo g, const double* betar, const
               (jc + ju);
 }
 
+template <bool COMP>
 __global__ void k_correct_rhop(Fields F, Phys ph, Geo g, const double* betar, const double* dpip, const double* corx,
                                const double* cory, double* rhop, Range3 R, double dt, double rf) {
+  F.comp = COMP ? 1 : 0;  // folds the model's branches
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
   const size_t c = g.h(i, j, k);
@@ -1391,10 +1395,12 @@ int correct(pcf_state* st, double dt, double rf) {
   hipLaunchKernelGGL(k_correct_v, dim3(blocks_for(rv.count(), 256)), dim3(256), 0, st->stream, F, st->ph, g, st->betar,
                      st->dpip, st->cory, st->v, rv, dt, rf);
   if (rw.nk > 0)
-    hipLaunchKernelGGL(k_correct_w, dim3(blocks_for(rw.count(), 256)), dim3(256), 0, st->stream, F, st->ph, g,
-                       st->betar, st->dpip, st->corx, st->cory, st->w, rw, dt, rf);
-  hipLaunchKernelGGL(k_correct_rhop, dim3(blocks_for(rr.count(), 256)), dim3(256), 0, st->stream, F, st->ph, g,
-                     st->betar, st->dpip, st->corx, st->cory, st->rhop, rr, dt, rf);
+    hipLaunchKernelGGL(st->compressible ? k_correct_w<true> : k_correct_w<false>, dim3(blocks_for(rw.count(), 256)),
+                       dim3(256), 0, st->stream, F, st->ph, g, st->betar, st->dpip, st->corx, st->cory, st->w, rw, dt,
+                       rf);
+  hipLaunchKernelGGL(st->compressible ? k_correct_rhop<true> : k_correct_rhop<false>, dim3(blocks_for(rr.count(), 256)),
+                     dim3(256), 0, st->stream, F, st->ph, g, st->betar, st->dpip, st->corx, st->cory, st->rhop, rr, dt,
+                     rf);
   hipLaunchKernelGGL(k_correct_pip, dim3(blocks_for(rp.count(), 256)), dim3(256), 0, st->stream, st->pip, st->dpip,
                      rp, g);
   PCF_HIP(hipGetLastError());
