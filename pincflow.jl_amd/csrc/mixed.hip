@@ -227,16 +227,20 @@ __global__ void __launch_bounds__(256) k_pc_rhs32(PcArgs<CT> A, const SinT* __re
 // The preconditioner's line solve in z with FP32 factors (the recurrence of
 // the FP64 k_pc_thomas); SrcT / OutT: FP64 or FP32 input and output vectors.
 // One column per thread, blockDim.x = 64.  The upward sweep keeps its values
-// of the lowest kThomas32Lds levels in LDS (32 KiB per wavefront) and those
+// of the lowest NLDS levels in LDS (NLDS x 64 x 4 B per wavefront) and those
 // of higher levels in the FP32 scratch `col`.  FIRST: rhs = deta (0 - src).
-// The compiler barriers keep each batch of loads ahead of its recurrence.
+// The compiler barriers keep each batch of loads ahead of its recurrence
+// (without LDS, the possible aliasing of `col` does).  LDS pays where a level
+// has enough columns for several wavefronts per CU (kThomas32LdsColumns);
+// with fewer, the LDS round trips are not hidden and NLDS = 0 is faster.
 constexpr int kThomas32Lds = 128;
+constexpr size_t kThomas32LdsColumns = 32768;
 
-template <bool FIRST, typename SrcT, typename OutT>
+template <bool FIRST, typename SrcT, typename OutT, int NLDS = kThomas32Lds>
 __global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float* E, const float* Q, const SrcT* src,
                                                     OutT* out, float* col, int nx, int ny, int nz, float deta,
                                                     const SolverScalars* sc) {
-  __shared__ float up[kThomas32Lds][64];
+  __shared__ float up[NLDS > 0 ? NLDS : 1][64];
   if (sc && sc->done) return;
   constexpr int U = 8;
   const int tid = threadIdx.x;
@@ -247,12 +251,12 @@ __global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float*
   const size_t c = (size_t)i + (size_t)nx * j;
   auto rhs_of = [&](SrcT v) -> float { return FIRST ? deta * (0.0f - (float)v) : (float)v; };
   auto keep = [&](int k, float v) {
-    if (k < kThomas32Lds)
+    if (k < NLDS)
       up[k][tid] = v;
     else
       col[c + sxy * k] = v;
   };
-  auto kept = [&](int k) -> float { return k < kThomas32Lds ? up[k][tid] : col[c + sxy * k]; };
+  auto kept = [&](int k) -> float { return k < NLDS ? up[k][tid] : col[c + sxy * k]; };
   float sk = rhs_of(src[c]) / P[c];
   keep(0, sk);
   int k = 1;
@@ -265,7 +269,7 @@ __global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float*
       e[u] = E[o];
       p[u] = P[o];
     }
-    asm volatile("" ::: "memory");
+    if constexpr (NLDS > 0) asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       sk = (r[u] + e[u] * sk) * p[u];
@@ -286,7 +290,7 @@ __global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float*
       r[u] = kept(k - u);
       q[u] = Q[c + sxy * (k - u)];
     }
-    asm volatile("" ::: "memory");
+    if constexpr (NLDS > 0) asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       sk = r[u] + q[u] * sk;
@@ -873,8 +877,12 @@ void line_solve(pcf_state* st, const MgLevel& L, const SrcT* src, OutT* out, con
                          0, st->stream, L.P, L.E, L.Q, src, out, (int)nc, L.nz, L.deta, sc);
     });
   } else {
-    hipLaunchKernelGGL((k_pc_thomas32<FIRST, SrcT, OutT>), dim3(blocks_for(L.nx, 64), L.ny), dim3(64), 0,
-                       st->stream, L.P, L.E, L.Q, src, out, L.col, L.nx, L.ny, L.nz, L.deta, sc);
+    if (nc >= kThomas32LdsColumns)
+      hipLaunchKernelGGL((k_pc_thomas32<FIRST, SrcT, OutT>), dim3(blocks_for(L.nx, 64), L.ny), dim3(64), 0,
+                         st->stream, L.P, L.E, L.Q, src, out, L.col, L.nx, L.ny, L.nz, L.deta, sc);
+    else
+      hipLaunchKernelGGL((k_pc_thomas32<FIRST, SrcT, OutT, 0>), dim3(blocks_for(L.nx, 64), L.ny), dim3(64), 0,
+                         st->stream, L.P, L.E, L.Q, src, out, L.col, L.nx, L.ny, L.nz, L.deta, sc);
   }
 }
 
