@@ -42,6 +42,12 @@ def configs(pcf):
         # aggregation once ny is odd
         "mg_deep": pcf.examples.mountain_wave_3d(x_size=64, y_size=64, z_size=72, iterations=3),
         "mg_wide": pcf.examples.mountain_wave_3d(x_size=256, y_size=72, z_size=8, iterations=3),
+        # 32768 columns: the finest level's line solves keep their upward
+        # sweep in LDS, the coarser levels' in their scratch column
+        "mg_lds": pcf.examples.mountain_wave_3d(x_size=256, y_size=128, z_size=8, iterations=3),
+        # 136 levels: the FP32 line solve of the line relaxation keeps 128 in
+        # LDS and the rest in its scratch column
+        "tall": pcf.examples.mountain_wave_3d(x_size=16, y_size=12, z_size=136, iterations=3),
     }
 
 
@@ -53,6 +59,10 @@ MG_CASES = CASES + ["mg_deep", "mg_wide"]
 @pytest.mark.parametrize("name", CASES)
 def test_mixed_solve_meets_reference_criterion(pcf, abi, oracle_mod, name, pc):
     solve_case(pcf, abi, oracle_mod, name, pc, "fp32")
+
+
+def test_tall_columns_line_relaxation_meets_reference_criterion(pcf, abi, oracle_mod):
+    solve_case(pcf, abi, oracle_mod, "tall", "line_relaxation", "fp32")
 
 
 @pytest.mark.parametrize("pc", ["line_relaxation", "multigrid"])
@@ -147,7 +157,7 @@ def test_fp64_rejects_a_warm_start(pcf):
 
 @pytest.mark.parametrize("name,storage,nu,coarse", [(n, "fp32", 2, 20) for n in MG_CASES] +
                          [("mg_deep", "bf16", 2, 20), ("mg_wide", "bf16", 2, 20), ("mg_deep", "fp32", 1, 7),
-                          ("mountain_wave_3d", "fp32", 3, 5)])
+                          ("mountain_wave_3d", "fp32", 3, 5), ("mg_lds", "fp32", 2, 20), ("mg_lds", "bf16", 2, 20)])
 def test_multigrid_vcycle_matches_fp64_restatement(pcf, name, storage, nu, coarse):
     """The FP32 V-cycle on the device against the FP64 numpy restatement of the
     same algorithm (tests/mg_reference.py); FP32 rounding through the levels
