@@ -561,12 +561,13 @@ __global__ void __launch_bounds__(64) k_pc_factors(const double* ac, const doubl
 // adjacent columns per wavefront (blockDim.x must be 64); the loads of 8
 // levels are issued before their (sequential) recurrence steps, and a
 // compiler barrier keeps them there.  The upward sweep's values of the
-// lowest kThomasLds levels stay in LDS (kThomasLds x 64 x 8 B = 32 KiB per
-// wavefront); those of higher levels go through `out`, which the downward
-// sweep overwrites with the solution.
+// lowest kThomasLds levels stay in LDS (kThomasLds x 64 x 8 B = 40 KiB per
+// wavefront, so the four one-wavefront blocks per CU of a 256 x 256 grid
+// fill the CU's 160 KiB); those of higher levels go through `out`, which the
+// downward sweep overwrites with the solution.
 // FIRST: rhs = 0 + deta (0 - src), the first pseudo-time step from s = 0,
 // whose horizontal apply is zero.
-constexpr int kThomasLds = 64;
+constexpr int kThomasLds = 80;
 
 template <bool FIRST>
 __global__ void __launch_bounds__(64) k_pc_thomas(const double* P, const double* E, const double* Q, const double* src,

@@ -58,7 +58,7 @@ def configs(pcf):
     mwc.atmosphere.model = "Compressible"
     mwc.atmosphere.thermal_conductivity = 50.0
     mwc.tracer = pcf.TracerNamelist(tracer_setup="TracerOn", initial_tracer=pcf.TracerDipole(5000.0, 5000.0))
-    # columns taller than the 64 levels the FP64 line solve keeps in LDS
+    # columns taller than the 80 levels the FP64 line solve keeps in LDS
     mwz = pcf.examples.mountain_wave_3d(x_size=16, y_size=14, z_size=136, iterations=2)
     return {"periodic_hill": ph, "mountain_wave": mw, "mountain_wave_3d": mw3, "cold_bubble": cb,
             "mountain_wave_odd": mwo, "vortex": vx, "mountain_wave_tracer": mwt, "hot_bubble": hb,
@@ -191,8 +191,8 @@ def test_apply_operator_and_preconditioner(pcf, abi, oracle_mod, name):
 
 
 def test_preconditioner_tall_columns(pcf, abi, oracle_mod):
-    """136 levels: the line solve's upward sweep of levels 64 and above goes
-    through HBM, the lower ones through LDS; both halves against the oracle."""
+    """136 levels: the line solve's upward sweep of levels 80 and above goes
+    through HBM, the lower ones through LDS; both parts against the oracle."""
     _, gpu, orc = pair(pcf, abi, oracle_mod, "mountain_wave_tall")
     for s in (gpu, orc):
         s.compute_sponges(0.5, 3.0)
@@ -202,7 +202,7 @@ def test_preconditioner_tall_columns(pcf, abi, oracle_mod):
     dy = hipmem.DeviceArray(x.shape)
     gpu.apply_preconditioner_device(dx.ptr, dy.ptr)
     got, want = dy.download(), orc.apply_preconditioner(x)
-    for half in (slice(0, 64), slice(64, None)):
+    for half in (slice(0, 80), slice(80, None)):
         err = rel(got[:, :, half], want[:, :, half])
         assert err <= KERNEL_RTOL, (half, err)
 
