@@ -274,7 +274,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(pcf, main_run["nl"], main_run["iters"], args)
     out = {
-        "metric": "time-steps/s at 256x256x128, Poisson to 1e-8 rel residual: learned vs plain",
+        "metric": f"time-steps/s at {args.nx}x{args.nx}x{args.nz}, Poisson to 1e-8 rel residual: learned vs plain",
         "value": round(main_run["value"], 4),
         "unit": "time-steps/s",
         "n_gpus": 1,
@@ -456,25 +456,26 @@ def fp64_difference(pcf, args, st_mixed):
                  "steps": args.steps + args.warmup + 1}
 
 
-def cpu_baseline(pcf, nl, iters, args):
-    """The FP64 oracle on the host cores, timed on a bounded sample of the
-    same workload: the components of one step (explicit LHS stage, RHS stage,
-    RHS assembly + operator assembly, BiCGSTAB iterations) on a horizontal
-    subset of the columns, scaled to the full grid by the cell count and to a
-    step by the GPU run's measured iteration count.  The component sum agrees
-    with a fully timed oracle step to a few per cent."""
-    from oracle import oracle
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    sx = min(args.cpu_sample_nx, nl.domain.x_size)
-    full_cells = nl.domain.x_size * nl.domain.y_size * nl.domain.z_size
+
+def oracle_components(pcf, oracle, nl, sx, threads):
+    """Host seconds of the components of one step of the FP64 oracle on sx x
+    sx columns of the workload: explicit LHS stage, RHS stage, RHS + operator
+    assembly, and one BiCGSTAB iteration (10 timed, the solve stopped there)."""
     sub = pcf.examples.mountain_wave_3d(x_size=sx, y_size=sx, z_size=nl.domain.z_size, iterations=1,
                                         precision="fp64")
     sub.atmosphere.model = nl.atmosphere.model
-    scale = full_cells / (sx * sx * nl.domain.z_size)
-    k = 10  # BiCGSTAB iterations timed (the solve is stopped there)
     cfg = pcf.to_config(sub)
-    cfg.poisson_iterations = k
+    cfg.poisson_iterations = 10
     o = oracle.Oracle(cfg, pcf._abi, threads=threads)
     o.compute_sponges(0.01, 0.1)
     o.backup_predictands()
@@ -491,15 +492,53 @@ def cpu_baseline(pcf, nl, iters, args):
     t0 = time.perf_counter()
     rc, n_done = o.apply_bicgstab(tolref)
     t_it = (time.perf_counter() - t0) / max(n_done, 1)
-    k = n_done
+    return {"lhs": t_lhs, "rhs": t_rhs, "assembly": t_op, "iteration": t_it, "timed_iterations": n_done}
+
+
+def cpu_baseline(pcf, nl, iters, args):
+    """The FP64 oracle on the host's cores, timed on a bounded sample of the
+    same workload.  The components of one step (explicit LHS stage, RHS
+    stage, RHS + operator assembly, one BiCGSTAB iteration) are timed on a
+    horizontal subset of the columns and scaled to the full grid by the cell
+    count and to a step by the GPU run's iteration count.  The model is
+    checked in the same run against one whole oracle step (integrate's time
+    loop body, both solves to the reference criterion) timed on a smaller
+    sample, where the model uses that step's own iteration counts.  Threads:
+    OMP_NUM_THREADS if set (the host's share of cores for one GPU), else all
+    cores."""
+    from oracle import oracle
+
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = args.cpu_threads or (int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1))
+    sx = min(args.cpu_sample_nx, nl.domain.x_size)
+    nz = nl.domain.z_size
+    full_cells = nl.domain.x_size * nl.domain.y_size * nz
+    scale = full_cells / (sx * sx * nz)
+    c = oracle_components(pcf, oracle, nl, sx, threads)
     n_it = sum(sum(i) for i in iters) / len(iters)
-    t_step = scale * (2 * t_lhs + t_rhs + 2 * t_op + n_it * t_it)
+    t_step = scale * (2 * c["lhs"] + c["rhs"] + 2 * c["assembly"] + n_it * c["iteration"])
+    # the model against a whole timed oracle step on sc x sc columns
+    scx = min(64, sx)
+    cc = oracle_components(pcf, oracle, nl, scx, threads)
+    whole = pcf.examples.mountain_wave_3d(x_size=scx, y_size=scx, z_size=nz, iterations=1, precision="fp64")
+    whole.atmosphere.model = nl.atmosphere.model
+    o = oracle.Oracle(pcf.to_config(whole), pcf._abi, threads=threads)
+    o.initial_cleaning()
+    t0 = time.perf_counter()
+    info = o.step()
+    t_whole = time.perf_counter() - t0
+    n_whole = info.niter[0] + info.niter[1]
+    t_model = 2 * cc["lhs"] + cc["rhs"] + 2 * cc["assembly"] + n_whole * cc["iteration"]
     return {"value": round(1.0 / t_step, 6), "unit": "time-steps/s", "cores": threads, "kind": "port",
-            "sample": (f"FP64 oracle on {sx}x{sx}x{nl.domain.z_size} columns of the same workload: explicit LHS "
-                       f"stage {t_lhs:.2f}s, RHS stage {t_rhs:.2f}s, RHS+operator assembly {t_op:.2f}s, "
-                       f"{t_it * 1e3:.1f} ms per BiCGSTAB iteration ({k} timed); step = 2 LHS + RHS + 2 assemblies "
-                       f"+ {n_it:.0f} iterations (the GPU run's count), scaled x{scale:.0f} to the full grid"),
-            "s_per_step": round(t_step, 1)}
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "sample": (f"FP64 oracle on {sx}x{sx}x{nz} columns of the same workload: explicit LHS stage "
+                       f"{c['lhs']:.2f}s, RHS stage {c['rhs']:.2f}s, RHS+operator assembly {c['assembly']:.2f}s, "
+                       f"{c['iteration'] * 1e3:.1f} ms per BiCGSTAB iteration ({c['timed_iterations']} timed); "
+                       f"step = 2 LHS + RHS + 2 assemblies + {n_it:.0f} iterations (the GPU run's count), scaled "
+                       f"x{scale:.0f} to the full grid"),
+            "s_per_step": round(t_step, 1),
+            "model_check": {"sample": f"{scx}x{scx}x{nz}", "timed_step_s": round(t_whole, 3),
+                            "model_step_s": round(t_model, 3), "iterations": [info.niter[0], info.niter[1]]}}
 
 
 if __name__ == "__main__":

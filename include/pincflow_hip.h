@@ -267,8 +267,13 @@ enum pcf_vector {
   PCF_V_COUNT
 };
 
-/* Boundary sets of set_boundaries!(state, ::Boundary...) (set_boundaries!.jl:43-68). */
-enum pcf_boundaries { PCF_BOUNDARY_PREDICTANDS = 0, PCF_BOUNDARY_FLUXES = 2 };
+/* Boundary sets of set_boundaries!(state, ::Boundary...) (set_boundaries!.jl:43-68).
+ * RECONSTRUCTIONS has nothing to fill here (pcf_reconstruct) and returns 0. */
+enum pcf_boundaries {
+  PCF_BOUNDARY_PREDICTANDS = 0,
+  PCF_BOUNDARY_RECONSTRUCTIONS = 1,
+  PCF_BOUNDARY_FLUXES = 2
+};
 
 /* Variables of update! / apply_lhs_sponge! (Types.jl singletons). */
 enum pcf_variable { PCF_VAR_RHO = 0, PCF_VAR_RHOP, PCF_VAR_U, PCF_VAR_V, PCF_VAR_W, PCF_VAR_PIP, PCF_VAR_CHI, PCF_VAR_P };
@@ -335,9 +340,19 @@ int pcf_backup_predictands(pcf_state* st);
 int pcf_set_boundaries(pcf_state* st, int which);
 
 /* ---- advection (FluxCalculator/) ---------------------------------------- */
-/* reconstruct! followed by compute_fluxes!(state, p0) and
- * set_boundaries!(state, BoundaryFluxes()) (explicit_integration!.jl:53-58);
- * the MUSCL reconstruction is evaluated inside the flux kernels. */
+/* reconstruct!(state) (reconstruct!.jl:102-315): the reconstruction inputs
+ * (rho / P, rho' / P, the edge-weighted momenta over P, the Cartesian
+ * vertical wind) from the current predictands.  The MUSCL limiter
+ * (apply_3d_muscl!.jl:35-75) is evaluated from them inside the flux kernels,
+ * so the reconstructed face values are never stored and
+ * set_boundaries!(state, BoundaryReconstructions()) has nothing to fill. */
+int pcf_reconstruct(pcf_state* st);
+/* compute_fluxes!(state, p0) (compute_fluxes!.jl:326-1503) followed by
+ * set_boundaries!(state, BoundaryFluxes()) (explicit_integration!.jl:53-58),
+ * with p0 = the copy made by pcf_backup_predictands.  It evaluates the
+ * reconstruction itself first, so the reference's sequence pcf_reconstruct,
+ * pcf_set_boundaries(PCF_BOUNDARY_RECONSTRUCTIONS), pcf_compute_fluxes and
+ * this call alone give the same fluxes. */
 int pcf_compute_fluxes(pcf_state* st);
 /* compute_fluxes!(state, p0, Theta()) (compute_fluxes!.jl:1592-1756): the
  * heat-conduction fluxes of the compressible model from the p0 copy; nothing

@@ -49,10 +49,12 @@ def lib() -> ctypes.CDLL:
             "orc_destroy": (None, [P]),
             "orc_last_error": (ctypes.c_char_p, [P]),
             "orc_set_threads": (None, [I]),
+            "orc_set_reduction_order": (None, [I, I]),
             "orc_dims": (None, [P, PI32]),
             "orc_initial_cleaning": (I, [P, PI32]),
             "orc_step": (I, [P, ctypes.POINTER(StepInfo)]),
             "orc_time": (D, [P]),
+            "orc_set_time": (I, [P, D]),
             "orc_get_field": (I, [P, I, PD]),
             "orc_set_field": (I, [P, I, PD]),
             "orc_get_vector": (I, [P, I, PD]),
@@ -167,6 +169,40 @@ class Oracle:
 
     def time(self) -> float:
         return lib().orc_time(self.h)
+
+    def restart(self, nl, output, time_s: float, fields: dict):
+        """Continue from a stored record: read_input! (read_input!.jl:14-84)
+        followed by integrate.jl:168-186.  fields are write_output's datasets
+        rhop, us, vs, wts, pip (p in the compressible model, chi with a
+        tracer), dimensional; the interiors of the predictands are replaced,
+        rho follows rhop outside the Boussinesq model, P is p / rhoref /
+        thetaref, chi is the dataset times rhobar + rho; then the predictand
+        boundaries, pbar := P, and the model time."""
+        c = output.constants(nl)
+        d = nl.domain
+        ii = slice(d.nbx, d.nbx + d.x_size)
+        jj = slice(d.nby, d.nby + d.y_size)
+        kk = slice(d.nbz, d.nbz + d.z_size)
+        for name, key, scale in (("RHOP", "rhop", c["rhoref"]), ("U", "us", c["uref"]), ("V", "vs", c["uref"]),
+                                 ("W", "wts", c["uref"]), ("PIP", "pip", 1.0)):
+            a = self.get(name)
+            a[ii, jj, kk] = np.asarray(fields[key], dtype=np.float64) / scale
+            self.set(name, a)
+            if name == "RHOP" and nl.atmosphere.model != "Boussinesq":
+                r = self.get("RHO")
+                r[ii, jj, kk] = a[ii, jj, kk]
+                self.set("RHO", r)
+        if nl.atmosphere.model == "Compressible":
+            pp = self.get("P")
+            pp[ii, jj, kk] = np.asarray(fields["p"], dtype=np.float64) / c["rhoref"] / c["thetaref"]
+            self.set("P", pp)
+        if nl.tracer.tracer_setup == "TracerOn":
+            chi, rho, rhobar = self.get("CHI"), self.get("RHO"), self.get("RHOBAR")
+            chi[ii, jj, kk] = np.asarray(fields["chi"], dtype=np.float64) * (rhobar[ii, jj, kk] + rho[ii, jj, kk])
+            self.set("CHI", chi)
+        self.set_boundaries(self.abi.BOUNDARY_PREDICTANDS)
+        self.synchronize_compressible_atmosphere()
+        self._check(lib().orc_set_time(self.h, time_s))
 
     def compute_lhs(self) -> float:
         t = ctypes.c_double(0.0)
@@ -298,6 +334,13 @@ def integrate(nl, abi, output, records: bool = True, threads: int = 0, max_steps
 def abi_config(nl, abi):
     from importlib import import_module
     return import_module(abi.__name__.rsplit(".", 1)[0] + ".namelists").to_config(nl)
+
+
+def set_reduction_order(block: int = 0, backwards: bool = False):
+    """Summation order of the oracle's global sums: block = 0 is Julia's
+    pairwise order (the default), block > 0 sums blocks of that many terms
+    in order and adds the block sums forwards or backwards."""
+    lib().orc_set_reduction_order(block, int(backwards))
 
 
 def muscl_line(phi: np.ndarray):

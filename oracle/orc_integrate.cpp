@@ -263,6 +263,18 @@ int orc_step(State* st, orc_step_info* info) {
 
 double orc_time(State* st) { return st->time * st->c.tref; }
 
+// Restart (integrate.jl:168-200): the model time read from the record and
+// the next output time one output_interval later; tmax < time is an error.
+int orc_set_time(State* st, double time_s) {
+  if (st->cfg.tmax < time_s) {
+    st->err = "Restart error: tmax < time!";
+    return PCF_E_CONFIG;
+  }
+  st->time = time_s / st->c.tref;
+  st->nextoutputtime = time_s + st->cfg.output_interval;
+  return PCF_OK;
+}
+
 static Arr3* field_ptr(State* st, int f) {
   Predictands& p = st->pr;
   switch (f) {

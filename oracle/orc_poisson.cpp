@@ -29,12 +29,30 @@ static double pairwise(F f, size_t lo, size_t hi) {  // [lo, hi]
   return a + b;
 }
 
+// Summation order of the global sums: 0 = Julia's pairwise order above (the
+// default); B > 0 = blocks of B terms summed in order, the block sums added
+// forwards, or backwards when g_order_backwards.  Only the iteration-count
+// spread test (tests/test_oracle_invariants.py) changes it: the count of a
+// BiCGSTAB solve near its tolerance depends on the rounding of these sums.
+static size_t g_order_block = 0;
+static bool g_order_backwards = false;
+
 template <class F>
 static double preduce(F f, size_t n) {
   double r = 0.0;
+  if (g_order_block == 0) {
 #pragma omp parallel
 #pragma omp single
-  r = pairwise(f, 0, n - 1);
+    r = pairwise(f, 0, n - 1);
+    return r;
+  }
+  const size_t B = g_order_block, nb = (n + B - 1) / B;
+  for (size_t bb = 0; bb < nb; ++bb) {
+    const size_t b = g_order_backwards ? nb - 1 - bb : bb;
+    double v = 0.0;
+    for (size_t q = b * B; q < n && q < b * B + B; ++q) v += f(q);
+    r += v;
+  }
   return r;
 }
 
@@ -838,3 +856,8 @@ void correct(State& st, double dt, double rf) {
 }
 
 }  // namespace orc
+
+extern "C" void orc_set_reduction_order(int block, int backwards) {
+  orc::g_order_block = block > 0 ? (size_t)block : 0;
+  orc::g_order_backwards = backwards != 0;
+}

@@ -72,7 +72,7 @@ VECTORS = [
 ]
 VECTOR = {name: i for i, name in enumerate(VECTORS)}
 
-BOUNDARY_PREDICTANDS, BOUNDARY_FLUXES = 0, 2
+BOUNDARY_PREDICTANDS, BOUNDARY_RECONSTRUCTIONS, BOUNDARY_FLUXES = 0, 1, 2
 VAR_RHO, VAR_RHOP, VAR_U, VAR_V, VAR_W, VAR_PIP, VAR_CHI, VAR_P = range(8)
 NO_TRACER, TRACER_ON = 0, 1
 TRACER_ZERO, TRACER_DIPOLE = 0, 1
@@ -183,6 +183,7 @@ SIGNATURES = [
     ("pcf_implicit_integration", _I, [_P, _D, _D, _D, _PI32]),
     ("pcf_backup_predictands", _I, [_P]),
     ("pcf_set_boundaries", _I, [_P, _I]),
+    ("pcf_reconstruct", _I, [_P]),
     ("pcf_compute_fluxes", _I, [_P]),
     ("pcf_compute_theta_fluxes", _I, [_P]),
     ("pcf_modify_compressible_wind", _I, [_P, _I]),

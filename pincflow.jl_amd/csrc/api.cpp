@@ -137,24 +137,28 @@ int save_backups(pcf_state* st, bool rho, bool rhop, bool u, bool v, bool w) {  
     if (rc_) return rc_;  \
   } while (0)
 
-// explicit_integration!.jl:42-101
+// explicit_integration!.jl:42-101.  Each RK stage computes the flux inputs
+// once (reconstruct!, stress tensor) and then, per variable, its fluxes and
+// update in one fused kernel (stage_update); compute_fluxes! and update!
+// stay available as separate calls through the ABI.
 int explicit_lhs(pcf_state* st, double dt, double time) {
   for (int m = 1; m <= 3; ++m) {
     const double sdt = st->stepfrac[m - 1] * dt;
-    TRY(compute_fluxes(st));
+    TRY(stage_inputs(st));
+    st->lhs_div_valid[0] = st->lhs_div_valid[1] = false;  // the stress arrays were rewritten
     TRY(save_backups(st, true, false, true, true, false));
-    TRY(update_lhs(st, dt, m, PCF_VAR_RHO));
+    TRY(stage_update(st, dt, m, PCF_VAR_RHO));
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_RHO));
-    TRY(update_lhs(st, dt, m, PCF_VAR_RHOP));
+    TRY(stage_update(st, dt, m, PCF_VAR_RHOP));
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_RHOP));
-    TRY(update_lhs(st, dt, m, PCF_VAR_P));  // no-ops outside the compressible model
+    TRY(stage_update(st, dt, m, PCF_VAR_P));  // no-ops outside the compressible model
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_P));
-    TRY(update_lhs(st, dt, m, PCF_VAR_CHI));  // no-ops without a tracer
+    TRY(stage_update(st, dt, m, PCF_VAR_CHI));  // no-ops without a tracer
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_CHI));
     TRY(set_boundaries_predictands(st));
-    TRY(update_lhs(st, dt, m, PCF_VAR_U));
-    TRY(update_lhs(st, dt, m, PCF_VAR_V));
-    TRY(update_lhs(st, dt, m, PCF_VAR_W));
+    TRY(stage_update(st, dt, m, PCF_VAR_U));
+    TRY(stage_update(st, dt, m, PCF_VAR_V));
+    TRY(stage_update(st, dt, m, PCF_VAR_W));
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_U));
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_V));
     TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_W));
@@ -298,9 +302,14 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
     return fail(rc);
   }
   const Geo& g = st->g;
+  if ((rc = check_reduction_sizes(g, err))) {
+    set_error(err);
+    return fail(rc);
+  }
   const size_t NH = g.nh(), NV = g.nv();
   double** haloed[] = {&st->drho, &st->drhop, &st->du, &st->dv, &st->dw, &st->rhoold, &st->rhopold, &st->uold,
-                       &st->vold, &st->wold, &st->betar, &st->dpip, &st->corx, &st->cory, &st->wcart};
+                       &st->vold, &st->wold, &st->betar, &st->dpip, &st->corx, &st->cory, &st->wcart,
+                       &st->div_u, &st->div_v, &st->fdiv_w};
   for (double** p : haloed)
     if ((rc = alloc(st, p, NH))) return fail(rc);
   for (int q = 0; q < (st->compressible ? 7 : 6); ++q) {
@@ -330,7 +339,7 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
                      &st->bvpc, &st->bmat, &st->pc_rhs, &st->pc_q, &st->pc_tmp, &st->pc_P, &st->pc_E, &st->pc_Q};
   for (double** p : vecs)
     if ((rc = alloc(st, p, NV > (size_t)g.nz ? NV : (size_t)g.nz))) return fail(rc);
-  if ((rc = alloc(st, &st->partials, (size_t)kMaxDots << 17))) return fail(rc);
+  if ((rc = alloc(st, &st->partials, (size_t)kMaxDots * kPartialStride))) return fail(rc);
   if (cfg->precision == PCF_PRECISION_MIXED && (rc = alloc_mixed(st))) return fail(rc);
   if ((rc = alloc(st, &st->cfl_dev, 8))) return fail(rc);
   {
@@ -518,9 +527,17 @@ int pcf_step(pcf_state* st, pcf_step_info* info) {
 int pcf_set_boundaries(pcf_state* st, int which) {
   if (!st) return PCF_E_ARG;
   if (which == PCF_BOUNDARY_PREDICTANDS) return set_boundaries_predictands(st);
+  // the reconstructions are evaluated inside the flux kernels from inputs
+  // whose halo entries are already the periodic images: nothing to fill
+  if (which == PCF_BOUNDARY_RECONSTRUCTIONS) return PCF_OK;
   if (which == PCF_BOUNDARY_FLUXES) return set_boundaries_fluxes(st);
   set_error("unknown boundary set");
   return PCF_E_OPTION;
+}
+
+int pcf_reconstruct(pcf_state* st) {
+  if (!st) return PCF_E_ARG;
+  return reconstruct(st);
 }
 
 int pcf_compute_fluxes(pcf_state* st) {

@@ -157,11 +157,11 @@ __device__ __forceinline__ double pe_z(const Fields& F, const Geo& g, int i, int
 }
 
 // One flux component of one variable at face (i, j, k)
-// (compute_fluxes!.jl:359-520, 587-1503).
+// (compute_fluxes!.jl:359-520, 587-1503): a pure function of the
+// reconstruction inputs, the p0 winds, the stress components and the static
+// fields, so every kernel that evaluates it at a face gets the same bits.
 template <int VAR, int DIR>
-__global__ void k_flux(FluxArgs A, Range3 R) {
-  int i, j, k;
-  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+__device__ __forceinline__ double flux_value(const FluxArgs& A, int i, int j, int k) {
   const Geo& g = A.g;
   const Fields& F = A.F;
   const size_t c = g.h(i, j, k);
@@ -310,7 +310,14 @@ __global__ void k_flux(FluxArgs A, Range3 R) {
       }
     }
   }
-  A.flux[VAR][DIR][c] = val;
+  return val;
+}
+
+template <int VAR, int DIR>
+__global__ void k_flux(FluxArgs A, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  A.flux[VAR][DIR][A.g.h(i, j, k)] = flux_value<VAR, DIR>(A, i, j, k);
 }
 
 // compute_fluxes!(..., P()) (compute_fluxes!.jl:531-585): the carrier flux
@@ -421,6 +428,7 @@ struct UpdArgs {
   double* div_out;        // U / V updates: their flux divergence / J_edge, kept for the W update (or null)
   const double* div_u;    // W update: those of the U and V updates of this stage (or null: recompute)
   const double* div_v;
+  double* fdiv;           // fused stage, W: its flux divergence, read by k_update_w_fdiv
 };
 
 __device__ __forceinline__ double divergence(const double* const* f, const Geo& g, int i, int j, int k) {
@@ -429,11 +437,12 @@ __device__ __forceinline__ double divergence(const double* const* f, const Geo& 
          (f[2][c] - f[2][g.h(i, j, k - 1)]) / g.dz;
 }
 
-__global__ void k_update_scalar(UpdArgs A, Range3 R) {  // update!.jl:521-597
-  int i, j, k;
-  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+// The updates at one cell from the flux divergence `fluxdiff` (the three
+// differences over dx, dy, dz, summed in that order), shared by the update
+// kernels that read stored fluxes and by the fused stage kernels.
+__device__ __forceinline__ void update_scalar_at(const UpdArgs& A, int i, int j, int k, double fluxdiff) {
+  // update!.jl:521-597
   const size_t c = A.g.h(i, j, k);
-  double fluxdiff = divergence(A.f, A.g, i, j, k);
   fluxdiff /= A.F.jac[c];
   double fr = -fluxdiff;
   if (A.force0) fr = fr + 0.0;
@@ -445,13 +454,17 @@ __global__ void k_update_scalar(UpdArgs A, Range3 R) {  // update!.jl:521-597
   A.q[c] += A.bm * dqv;
 }
 
-__global__ void k_update_u(UpdArgs A, Range3 R) {  // update!.jl:730-806
+__global__ void k_update_scalar(UpdArgs A, Range3 R) {
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  update_scalar_at(A, i, j, k, divergence(A.f, A.g, i, j, k));
+}
+
+__device__ __forceinline__ void update_u_at(const UpdArgs& A, int i, int j, int k, double fluxdiff) {
+  // update!.jl:730-806
   const Geo& g = A.g;
   const Fields& F = A.F;
   const size_t c = g.h(i, j, k), r = g.h(i + 1, j, k);
-  double fluxdiff = divergence(A.f, g, i, j, k);
   double jacedger = 0.5 * (F.jac[c] + F.jac[r]);
   fluxdiff /= jacedger;
   if (A.div_out) A.div_out[c] = fluxdiff;
@@ -473,13 +486,17 @@ __global__ void k_update_u(UpdArgs A, Range3 R) {  // update!.jl:730-806
   A.q[c] = momm / rhom;
 }
 
-__global__ void k_update_v(UpdArgs A, Range3 R) {  // update!.jl:884-957
+__global__ void k_update_u(UpdArgs A, Range3 R) {
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  update_u_at(A, i, j, k, divergence(A.f, A.g, i, j, k));
+}
+
+__device__ __forceinline__ void update_v_at(const UpdArgs& A, int i, int j, int k, double fluxdiff) {
+  // update!.jl:884-957
   const Geo& g = A.g;
   const Fields& F = A.F;
   const size_t c = g.h(i, j, k), f = g.h(i, j + 1, k);
-  double fluxdiff = divergence(A.f, g, i, j, k);
   double jacedgef = 0.5 * (F.jac[c] + F.jac[f]);
   fluxdiff /= jacedgef;
   if (A.div_out) A.div_out[c] = fluxdiff;
@@ -501,18 +518,26 @@ __global__ void k_update_v(UpdArgs A, Range3 R) {  // update!.jl:884-957
   A.q[c] = momm / rhom;
 }
 
-__global__ void k_update_w(UpdArgs A, Range3 R) {  // update!.jl:1035-1192
+__global__ void k_update_v(UpdArgs A, Range3 R) {
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  update_v_at(A, i, j, k, divergence(A.f, A.g, i, j, k));
+}
+
+// REUSE: the divergences of this stage's U and V updates are known to be in
+// A.div_u / A.div_v (the fused stage); else they are read from there when
+// given and recomputed from the stored fluxes otherwise.
+template <bool REUSE>
+__device__ __forceinline__ void update_w_at(const UpdArgs& A, int i, int j, int k, double fluxdiff) {
+  // update!.jl:1035-1192
   const Geo& g = A.g;
   const Fields& F = A.F;
   const size_t c = g.h(i, j, k), up = g.h(i, j, k + 1);
-  double fluxdiff = divergence(A.f, g, i, j, k);
   const double jc = F.jac[c], ju = F.jac[up];
   double jacedgeu = 2.0 * jc * ju / (jc + ju);
   fluxdiff /= jacedgeu;
   double fdu[2][2], fdv[2][2];
-  if (A.div_u) {  // the same values, as the U and V updates of this stage computed them
+  if (REUSE || A.div_u) {  // the same values, as the U and V updates of this stage computed them
     for (int ll = 0; ll <= 1; ++ll)
       for (int mm = 0; mm <= 1; ++mm) {
         fdu[ll][mm] = A.div_u[g.h(i - ll, j, k + mm)];
@@ -556,6 +581,66 @@ __global__ void k_update_w(UpdArgs A, Range3 R) {  // update!.jl:1035-1192
   A.dq[c] = d;
   double momm = momm_1 + A.bm * d;
   A.q[c] = momm / rhom;
+}
+
+__global__ void k_update_w(UpdArgs A, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  update_w_at<false>(A, i, j, k, divergence(A.f, A.g, i, j, k));
+}
+
+// the W update of the fused stage from its stored flux divergence
+__global__ void k_update_w_fdiv(UpdArgs A, Range3 R) {
+  int i, j, k;
+  if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
+  update_w_at<true>(A, i, j, k, A.fdiv[A.g.h(i, j, k)]);
+}
+
+// ---------------------------------------------------------------------------
+// Fused LHS stage kernels (W: the W fluxes and their divergence, stored in
+// A.fdiv, then k_update_w_fdiv, because the W update alone takes about as
+// many registers as the rest of the fused kernel): the fluxes of one variable at the faces of a cell,
+// their divergence and the RK3 update in one pass, for the stage loop of
+// explicit_integration! (explicit_integration!.jl:42-101).  No flux array is
+// stored: a thread evaluates the x and y faces on both sides of its cell
+// (flux_value) and marches up a chunk of kc levels, carrying the z face
+// between levels; the vertical wall faces, which set_boundaries!(state,
+// BoundaryFluxes()) zeroes, are zero here too.  The flux inputs (phi, the
+// Cartesian w, the stress components) are computed at the start of the stage
+// and the p0 winds are fixed, so evaluating a variable's fluxes just before
+// its update gives the bits the stored fluxes of compute_fluxes! had.  The
+// divergence is summed as divergence() sums it.  Threads map to the cells of
+// one layer of the update range R with x fastest.
+// ---------------------------------------------------------------------------
+template <int VAR>
+__device__ __forceinline__ double zface(const FluxArgs& FA, int i, int j, int k) {
+  if (VAR != V_W && (k == FA.g.k0 - 1 || k == FA.g.k1)) return 0.0;
+  return flux_value<VAR, 2>(FA, i, j, k);
+}
+
+template <int VAR>
+__global__ void __launch_bounds__(256) k_stage_update(FluxArgs FA, UpdArgs A, Range3 R, int kc) {
+  const long long t = blockIdx.x * 256LL + threadIdx.x;
+  if (t >= (long long)R.ni * R.nj) return;
+  const int jj = (int)(t / R.ni);
+  const int i = R.i0 + (int)(t - (long long)jj * R.ni);
+  const int j = R.j0 + jj;
+  const int kb = R.k0 + (int)blockIdx.y * kc;
+  const int ke = min(kb + kc, R.k0 + R.nk);
+  const Geo& g = A.g;
+  double fzd = zface<VAR>(FA, i, j, kb - 1);
+#pragma unroll 1
+  for (int k = kb; k < ke; ++k) {
+    const double fzu = zface<VAR>(FA, i, j, k);
+    const double fxr = flux_value<VAR, 0>(FA, i, j, k), fxl = flux_value<VAR, 0>(FA, i - 1, j, k);
+    const double fyf = flux_value<VAR, 1>(FA, i, j, k), fyb = flux_value<VAR, 1>(FA, i, j - 1, k);
+    const double fluxdiff = (fxr - fxl) / g.dx + (fyf - fyb) / g.dy + (fzu - fzd) / g.dz;
+    if (VAR == V_U) update_u_at(A, i, j, k, fluxdiff);
+    else if (VAR == V_V) update_v_at(A, i, j, k, fluxdiff);
+    else if (VAR == V_W) A.fdiv[g.h(i, j, k)] = fluxdiff;  // the update follows in k_update_w_fdiv
+    else update_scalar_at(A, i, j, k, fluxdiff);
+    fzd = fzu;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -937,12 +1022,14 @@ void launch_flux(pcf_state* st, const FluxArgs& A, Range3 R) {
 
 }  // namespace
 
-// reconstruct! + set_boundaries!(Reconstructions) + compute_fluxes!(state, p0)
-// + set_boundaries!(Fluxes) (explicit_integration!.jl:53-58)
-int compute_fluxes(pcf_state* st) {
+// reconstruct! (reconstruct!.jl:102-315): the reconstruction inputs phi of
+// every variable and the Cartesian vertical wind they and the stress tensor
+// need.  The limited face values themselves are evaluated by the flux
+// kernels, so set_boundaries!(state, BoundaryReconstructions()) has nothing
+// to fill: halo entries of phi are the periodic images of interior ones.
+int reconstruct(pcf_state* st) {
   ProfScope prof(st, PCF_K_FLUXES);
   const Geo& g = st->g;
-  st->lhs_div_valid[0] = st->lhs_div_valid[1] = false;  // the stress arrays are rewritten below
   Fields F = fields_of(st);
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
   const int nlev = g.nz + 2;  // k0-1 .. k1+1
@@ -952,13 +1039,32 @@ int compute_fluxes(pcf_state* st) {
                      st->wcart);
   hipLaunchKernelGGL(k_phi_w, dim3(blocks_for(g.nxx - 1, 64), g.nyy - 1, nlev), dim3(64), 0, st->stream, F, g,
                      st->wcart, st->phi[4]);
-  if (st->ph.viscous) {
-    Range3 R{g.i0 - 1, g.nx + 2, g.j0 - 1, g.ny + 2, g.k0, g.nz};
-    hipLaunchKernelGGL(k_stress, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, F, g, st->wcart,
-                       st->stress[0], st->stress[1], st->stress[2], st->stress[3], st->stress[4], st->stress[5], R);
-  }
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+// reconstruct! + compute_fluxes!(state, p0) + set_boundaries!(state,
+// BoundaryFluxes()) (explicit_integration!.jl:53-58); the reconstruction is
+// evaluated here again, so a caller of the reference's sequence (which calls
+// reconstruct first) and a caller of this function alone get the same fluxes.
+namespace {
+
+// the six stress components compute_stress_tensor gives the viscous fluxes
+// (compute_stress_tensor.jl:55-164), from the current winds
+int stress_tensor(pcf_state* st) {
+  if (!st->ph.viscous) return PCF_OK;
+  const Geo& g = st->g;
+  Range3 R{g.i0 - 1, g.nx + 2, g.j0 - 1, g.ny + 2, g.k0, g.nz};
+  hipLaunchKernelGGL(k_stress, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, fields_of(st), g,
+                     st->wcart, st->stress[0], st->stress[1], st->stress[2], st->stress[3], st->stress[4],
+                     st->stress[5], R);
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+FluxArgs flux_args(pcf_state* st) {
   FluxArgs A;
-  A.F = F;
+  A.F = fields_of(st);
   for (int q = 0; q < 6; ++q) A.phi[q] = st->phi[q];
   A.u0 = st->p0[2];
   A.v0 = st->p0[3];
@@ -966,9 +1072,50 @@ int compute_fluxes(pcf_state* st) {
   for (int q = 0; q < 6; ++q) A.st[q] = st->stress[q];
   for (int q = 0; q < 7; ++q)
     for (int d = 0; d < 3; ++d) A.flux[q][d] = st->flux[q][d];
-  A.g = g;
+  A.g = st->g;
   A.inv_re = 1 / st->ph.re;
   A.viscous = st->ph.viscous;
+  return A;
+}
+
+// arguments of update!(state, dt, m, var, LHS()) common to every variable
+UpdArgs upd_args(pcf_state* st, double dt, int m) {
+  UpdArgs A;
+  A.F = fields_of(st);
+  A.g = st->g;
+  A.ph = st->ph;
+  A.dt = dt;
+  A.am = st->alphark[m - 1];
+  A.bm = st->betark[m - 1];
+  A.first = m == 1;
+  A.force0 = 0;
+  A.heat = 0;
+  A.div_out = nullptr;
+  A.div_u = A.div_v = nullptr;
+  A.fdiv = nullptr;
+  A.rhoold = st->rhoold;
+  A.uold = st->uold;
+  A.vold = st->vold;
+  for (int d = 0; d < 3; ++d) {
+    A.fu[d] = st->flux[V_U][d];
+    A.fv[d] = st->flux[V_V][d];
+    A.f[d] = nullptr;
+  }
+  A.q = A.dq = nullptr;
+  return A;
+}
+
+}  // namespace
+
+int compute_fluxes(pcf_state* st) {
+  int rc = reconstruct(st);
+  if (rc) return rc;
+  ProfScope prof(st, PCF_K_FLUXES);
+  const Geo& g = st->g;
+  st->lhs_div_valid[0] = st->lhs_div_valid[1] = false;  // the stress arrays are rewritten below
+  const bool pi = st->cfg.model != PCF_BOUSSINESQ;
+  if ((rc = stress_tensor(st))) return rc;
+  FluxArgs A = flux_args(st);
   const int nx = g.nx, ny = g.ny, nz = g.nz, i0 = g.i0, j0 = g.j0, k0 = g.k0;
   // faces the updates consume (update!.jl loop ranges); vertical wall faces
   // are zeroed below
@@ -1068,25 +1215,7 @@ int update_lhs(pcf_state* st, double dt, int m, int var) {
   ProfScope prof(st, PCF_K_UPDATES);
   const Geo& g = st->g;
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
-  UpdArgs A;
-  A.F = fields_of(st);
-  A.g = g;
-  A.ph = st->ph;
-  A.dt = dt;
-  A.am = st->alphark[m - 1];
-  A.bm = st->betark[m - 1];
-  A.first = m == 1;
-  A.force0 = 0;
-  A.heat = 0;
-  A.div_out = nullptr;
-  A.div_u = A.div_v = nullptr;
-  A.rhoold = st->rhoold;
-  A.uold = st->uold;
-  A.vold = st->vold;
-  for (int d = 0; d < 3; ++d) {
-    A.fu[d] = st->flux[V_U][d];
-    A.fv[d] = st->flux[V_V][d];
-  }
+  UpdArgs A = upd_args(st, dt, m);
   Range3 R;
   switch (var) {
     case PCF_VAR_RHO:
@@ -1152,6 +1281,101 @@ int update_lhs(pcf_state* st, double dt, int m, int var) {
       R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz - 1};
       if (R.nk > 0)
         hipLaunchKernelGGL(k_update_w, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      break;
+    default:
+      return PCF_E_ARG;
+  }
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
+// The inputs of a stage's fluxes: reconstruct! and the stress tensor from
+// the current predictands (compute_fluxes without its flux kernels).
+int stage_inputs(pcf_state* st) {
+  int rc = reconstruct(st);
+  if (rc) return rc;
+  ProfScope prof(st, PCF_K_FLUXES);
+  return stress_tensor(st);
+}
+
+// update!(state, dt, m, var, LHS()) of one RK stage with the variable's
+// fluxes (compute_fluxes!) evaluated in the same kernel (k_stage_update),
+// after stage_inputs: the sequence of explicit_integration!.jl:53-99 without
+// storing the fluxes.  The U and V updates leave their flux divergences in
+// div_u / div_v for the W update of the same stage.
+int stage_update(pcf_state* st, double dt, int m, int var) {
+  const Geo& g = st->g;
+  const bool pi = st->cfg.model != PCF_BOUSSINESQ;
+  constexpr int kc = 8;  // levels per thread
+  UpdArgs A = upd_args(st, dt, m);
+  Range3 R{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
+  auto launch = [&](auto kernel) {
+    if (R.ni <= 0 || R.nj <= 0 || R.nk <= 0) return;
+    dim3 grd(blocks_for((long long)R.ni * R.nj, 256), (unsigned)((R.nk + kc - 1) / kc));
+    hipLaunchKernelGGL(kernel, grd, dim3(256), 0, st->stream, flux_args(st), A, R, kc);
+  };
+  if (var == PCF_VAR_P) {  // the carrier flux of P has no reconstruction: stored, then the update
+    if (!st->compressible) return PCF_OK;
+    FluxArgs FA = flux_args(st);
+    Range3 rx{g.i0 - 1, g.nx + 1, g.j0, g.ny, g.k0, g.nz}, ry{g.i0, g.nx, g.j0 - 1, g.ny + 1, g.k0, g.nz},
+        rz{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz - 1};
+    {
+      ProfScope prof(st, PCF_K_FLUXES);
+      hipLaunchKernelGGL(k_flux_p<0>, dim3(blocks_for(rx.count(), 256)), dim3(256), 0, st->stream, FA, rx);
+      hipLaunchKernelGGL(k_flux_p<1>, dim3(blocks_for(ry.count(), 256)), dim3(256), 0, st->stream, FA, ry);
+      if (rz.nk > 0)
+        hipLaunchKernelGGL(k_flux_p<2>, dim3(blocks_for(rz.count(), 256)), dim3(256), 0, st->stream, FA, rz);
+      PCF_HIP(hipGetLastError());
+      int rc = set_boundaries_fluxes(st);
+      if (rc) return rc;
+    }
+    return update_lhs(st, dt, m, PCF_VAR_P);
+  }
+  ProfScope prof(st, PCF_K_FLUXES);
+  switch (var) {
+    case PCF_VAR_RHO:
+      if (!pi) return PCF_OK;
+      A.q = st->rho;
+      A.dq = st->drho;
+      launch(k_stage_update<V_RHO>);
+      break;
+    case PCF_VAR_RHOP:
+      A.q = st->rhop;
+      A.dq = st->drhop;
+      A.heat = st->compressible ? 1 : 0;
+      launch(k_stage_update<V_RHOP>);
+      break;
+    case PCF_VAR_CHI:
+      if (!st->tracer) return PCF_OK;
+      A.q = st->chi;
+      A.dq = st->dchi;
+      A.force0 = 1;
+      launch(k_stage_update<V_CHI>);
+      break;
+    case PCF_VAR_U:
+      A.q = st->u;
+      A.dq = st->du;
+      A.div_out = st->div_u;
+      R = Range3{g.i0 - 1, g.nx + 1, g.j0, g.ny, g.k0, g.nz};
+      launch(k_stage_update<V_U>);
+      break;
+    case PCF_VAR_V:
+      A.q = st->v;
+      A.dq = st->dv;
+      A.div_out = st->div_v;
+      R = Range3{g.i0, g.nx, g.j0 - 1, g.ny + 1, g.k0, g.nz};
+      launch(k_stage_update<V_V>);
+      break;
+    case PCF_VAR_W:
+      A.q = st->w;
+      A.dq = st->dw;
+      A.div_u = st->div_u;
+      A.div_v = st->div_v;
+      A.fdiv = st->fdiv_w;
+      R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz - 1};
+      launch(k_stage_update<V_W>);
+      if (R.nk > 0)
+        hipLaunchKernelGGL(k_update_w_fdiv, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
       break;
     default:
       return PCF_E_ARG;

@@ -104,6 +104,10 @@ struct SolverScalars {
 };
 
 constexpr int kReduceBlock = 256;     // threads per reduction block
+// Partial-sum slots per dot product in pcf_state::partials: one per block of
+// the dot-fused kernels, ceil(nx / 64) ceil(ny / 4) nz at most; pcf_create
+// refuses grids that need more (check_reduction_sizes)
+constexpr int kPartialStride = 1 << 20;
 constexpr int kMaxPartials = 4096;    // partial sums per reduction
 constexpr int kMaxDots = 4;
 
@@ -153,6 +157,10 @@ struct pcf_state {
   // between flux computations stress[0] and stress[1] hold the flux
   // divergences of the stage's U and V updates, which the W update reuses
   bool lhs_div_valid[2] = {false, false};
+  // the same divergences from the fused stage kernels (stage_update), which
+  // read the stress arrays while they write these
+  double *div_u = nullptr, *div_v = nullptr;
+  double* fdiv_w = nullptr;  // the W flux divergence of a fused stage
   // corrector
   double *dpip, *corx, *cory;
   // Poisson (interior)
@@ -256,6 +264,7 @@ int apply_bicgstab(pcf_state* st, double tolref, int* niter);
 int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess, double guess_scale, int* niter);
 int solve(pcf_state* st, double tolref, double dt, int* niter);  // apply_bicgstab, configured precision
 int pc_factors(pcf_state* st);
+int check_reduction_sizes(const Geo& g, std::string& err);
 int apply_corrector(pcf_state* st, double dt, double rf, int* niter);
 int correct(pcf_state* st, double dt, double rf);
 int dot(pcf_state* st, const double* a, const double* b, double* out);
@@ -280,7 +289,10 @@ int device_alloc(pcf_state* st, T** p, size_t count) {
 int learned_guess(pcf_state* st, double dt, const double* prev, const double** x0);
 
 // ---- explicit.hip --------------------------------------------------------
+int reconstruct(pcf_state* st);
 int compute_fluxes(pcf_state* st);
+int stage_inputs(pcf_state* st);
+int stage_update(pcf_state* st, double dt, int m, int var);
 int update_lhs(pcf_state* st, double dt, int m, int var);
 int update_rhs(pcf_state* st, double dt, int var, bool implicit, double rf);
 int apply_lhs_sponge(pcf_state* st, double dt, double time, int var);

@@ -561,18 +561,19 @@ __global__ void __launch_bounds__(64) k_pc_factors(const double* ac, const doubl
 // adjacent columns per wavefront (blockDim.x must be 64); the loads of 8
 // levels are issued before their (sequential) recurrence steps, and a
 // compiler barrier keeps them there.  The upward sweep's values of the
-// lowest kThomasLds levels stay in LDS (kThomasLds x 64 x 8 B = 40 KiB per
-// wavefront, so the four one-wavefront blocks per CU of a 256 x 256 grid
-// fill the CU's 160 KiB); those of higher levels go through `out`, which the
-// downward sweep overwrites with the solution.
+// lowest NLDS levels stay in LDS; those of higher levels go through `out`,
+// which the downward sweep overwrites with the solution.  NLDS is chosen per
+// launch (launch_thomas): 80 levels (40 KiB per one-wavefront block) while
+// the grid has at most four blocks per CU, as a 256 x 256 grid has, so that
+// four blocks fill the CU's 160 KiB; 64 levels (32 KiB, five blocks per CU)
+// on larger grids, which have blocks enough to use the fifth slot.
 // FIRST: rhs = 0 + deta (0 - src), the first pseudo-time step from s = 0,
 // whose horizontal apply is zero.
-constexpr int kThomasLds = 80;
-
-template <bool FIRST>
+template <bool FIRST, int NLDS>
 __global__ void __launch_bounds__(64) k_pc_thomas(const double* P, const double* E, const double* Q, const double* src,
                                                   double* out, int nx, int ny, int nz, double deta,
                                                   const SolverScalars* sc) {
+  constexpr int kThomasLds = NLDS;
   __shared__ double up[kThomasLds][64];
   if (sc && sc->done) return;
   constexpr int U = 8;
@@ -994,7 +995,6 @@ int compute_lhs(pcf_state* st, double* tolref_out) {
   const Geo& g = st->g;
   dim3 blk(64, 4), grd(blocks_for(g.nx, 64), blocks_for(g.ny, 4), g.nz);
   int nparts = grd.x * grd.y * grd.z;
-  if (nparts > kPartialStride) return PCF_E_ARG;
   hipLaunchKernelGGL(k_compute_lhs, grd, blk, 0, st->stream, fields_of(st), st->ph, g, st->rhs, st->partials);
   hipLaunchKernelGGL(k_finalize_lhs, dim3(1), dim3(1024), 0, st->stream, st->partials, nparts, st->sc,
                      (double)st->cfg.x_size * st->cfg.y_size * st->cfg.z_size);
@@ -1072,6 +1072,19 @@ static dim3 pair_block() { return dim3(32, kPairBlockY); }
 static int apply_parts(const Geo& g) {
   const dim3 grd = use_pairs(g) ? pair_grid(g) : apply_grid(g);
   return (int)(grd.x * grd.y * grd.z);
+}
+
+// The block partials of compute_lhs and of the dot-fused applies must fit
+// the kPartialStride slots of one dot product (checked once, by pcf_create).
+int check_reduction_sizes(const Geo& g, std::string& err) {
+  const long long lhs = (long long)blocks_for(g.nx, 64) * blocks_for(g.ny, 4) * g.nz;
+  const long long app = (long long)apply_parts(g);
+  if (lhs > kPartialStride || app > kPartialStride) {
+    err = "grid too large for the reductions: ceil(x_size / 64) * ceil(y_size / 4) * z_size = " +
+          std::to_string(lhs > app ? lhs : app) + " block partials, at most " + std::to_string(kPartialStride);
+    return PCF_E_CONFIG;
+  }
+  return PCF_OK;
 }
 
 // ls = A s (TOTAL) with NDOT fused dot products: <ls, d1> (and <ls, ls>).
@@ -1152,20 +1165,36 @@ int pc_factors(pcf_state* st) {
   return PCF_OK;
 }
 
+// The line solve of one pseudo-time step (k_pc_thomas) with the LDS depth
+// for the grid: at most 4 x 256 one-wavefront blocks fill the 256 CUs at
+// four blocks each (80 levels in LDS), more blocks are better served by five
+// blocks per CU (64 levels).
+constexpr int kThomasBlocksPerCuAt80 = 4;
+constexpr int kCUs = 256;
+template <bool FIRST>
+static void launch_thomas(pcf_state* st, const double* src, double* out, const SolverScalars* sc) {
+  const Geo& g = st->g;
+  const dim3 tgrd(blocks_for(g.nx, 64), g.ny), tblk(64);
+  if ((long long)tgrd.x * tgrd.y <= (long long)kThomasBlocksPerCuAt80 * kCUs)
+    hipLaunchKernelGGL((k_pc_thomas<FIRST, 80>), tgrd, tblk, 0, st->stream, st->pc_P, st->pc_E, st->pc_Q, src, out,
+                       g.nx, g.ny, g.nz, st->deta, sc);
+  else
+    hipLaunchKernelGGL((k_pc_thomas<FIRST, 64>), tgrd, tblk, 0, st->stream, st->pc_P, st->pc_E, st->pc_Q, src, out,
+                       g.nx, g.ny, g.nz, st->deta, sc);
+}
+
 // M^-1 sin -> sout with preconditioner_iterations pseudo-time steps.  sout
 // must not alias sin.  `sc` (may be null) lets a converged solve skip it.
 static int precondition(pcf_state* st, const double* sin, double* sout, const SolverScalars* sc) {
   const Geo& g = st->g;
   int rc = pc_factors(st);
   if (rc) return rc;
-  dim3 tgrd(blocks_for(g.nx, 64), g.ny), tblk(64);
   const int iters = st->cfg.preconditioner_iterations;
   // pseudo-time step 1 from s = 0
   double* cur = iters == 1 ? sout : st->pc_tmp;
   {
     ProfScope prof(st, PCF_K_PC_THOMAS);
-    hipLaunchKernelGGL(k_pc_thomas<true>, tgrd, tblk, 0, st->stream, st->pc_P, st->pc_E, st->pc_Q, sin, cur, g.nx,
-                       g.ny, g.nz, st->deta, sc);
+    launch_thomas<true>(st, sin, cur, sc);
   }
   for (int it = 2; it <= iters; ++it) {
     {
@@ -1180,8 +1209,7 @@ static int precondition(pcf_state* st, const double* sin, double* sout, const So
     double* next = it == iters ? sout : (cur == st->pc_tmp ? sout : st->pc_tmp);
     {
       ProfScope prof(st, PCF_K_PC_THOMAS);
-      hipLaunchKernelGGL(k_pc_thomas<false>, tgrd, tblk, 0, st->stream, st->pc_P, st->pc_E, st->pc_Q, st->pc_rhs, next,
-                         g.nx, g.ny, g.nz, st->deta, sc);
+      launch_thomas<false>(st, st->pc_rhs, next, sc);
     }
     cur = next;
   }
@@ -1258,7 +1286,6 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
     return PCF_OK;
   }
   const int aparts = apply_parts(g);
-  if (aparts > kPartialStride) return PCF_E_ARG;
   const int maxit = n.poisson_iterations;
   // y = M^-1 p and z = M^-1 s are both kept for the textbook update
   double* ybuf = st->bvpc;

@@ -8,8 +8,6 @@
 namespace pcf {
 namespace {
 
-constexpr int kPartialStride = 1 << 17;  // partial slots per dot product
-
 // ---------------------------------------------------------------------------
 // block reduction of up to 3 values (blockDim.x * blockDim.y == 256)
 // ---------------------------------------------------------------------------

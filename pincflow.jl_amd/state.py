@@ -124,7 +124,13 @@ class State:
     def set_boundaries(self, which: int):
         self._check(self.lib.pcf_set_boundaries(self.h, which))
 
+    def reconstruct(self):
+        """reconstruct!(state): the reconstruction inputs of every variable
+        (the limited face values are evaluated inside compute_fluxes)."""
+        self._check(self.lib.pcf_reconstruct(self.h))
+
     def compute_fluxes(self):
+        """compute_fluxes!(state, p0) and set_boundaries!(state, BoundaryFluxes())."""
         self._check(self.lib.pcf_compute_fluxes(self.h))
 
     def compute_theta_fluxes(self):

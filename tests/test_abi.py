@@ -51,3 +51,39 @@ def test_invalid_options_raise(pcf):
     nl.atmosphere.background = "Bogus"
     with pytest.raises(ValueError):
         pcf.to_config(nl)
+
+
+def _header_config_fields():
+    """(name, Julia type) of every pcf_config member, in order, from the header."""
+    import re
+    src = (ROOT / "include" / "pincflow_hip.h").read_text()
+    body = re.search(r"typedef struct pcf_config \{(.*?)\} pcf_config;", src, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    out = []
+    for stmt in body.split(";"):
+        stmt = stmt.strip()
+        if not stmt:
+            continue
+        m = re.match(r"(int32_t|double|pcf_hill)\s+(.*)", stmt, re.S)
+        assert m, stmt
+        for d in m.group(2).split(","):
+            d = d.strip()
+            arr = re.match(r"(\w+)\[PCF_MAX_HILLS\]", d)
+            if arr:
+                out.append((arr.group(1), "NTuple{32,PcfHill}"))
+            else:
+                out.append((d, {"int32_t": "Int32", "double": "Float64"}[m.group(1)]))
+    return out
+
+
+def test_julia_config_mirror_matches_the_header(abi):
+    """The Julia stub's PcfConfig in INTEGRATION.md has the header's fields in
+    the header's order with the matching types, and the ctypes mirror too."""
+    import re
+    text = (ROOT / "INTEGRATION.md").read_text()
+    block = re.search(r"\nstruct PcfConfig\n(.*?)\nend\n", text, re.S).group(1)
+    julia = [tuple(x.strip() for x in line.split("::")) for line in block.split("\n")
+             if line.strip() and not line.strip().startswith("#")]
+    header = _header_config_fields()
+    assert julia == header
+    assert [n for n, _ in abi.PcfConfig._fields_] == [n for n, _ in header]

@@ -36,6 +36,15 @@ def test_bubble_members_run_on_the_oracle(pcf, abi, oracle_mod, seed):
     assert np.all(np.isfinite(recs[-1]["thetap"]))
 
 
+def test_member_seeds_of_the_oracle_comparison(pcf):
+    """the seeds test_gpu_configs.py compares with the oracle give members of
+    the real ensemble sizes and both models"""
+    shapes = [(m.domain.x_size, m.domain.y_size, m.domain.z_size, m.atmosphere.model)
+              for m in (pcf.examples.bubble_member(seed=s) for s in (11, 34, 23))]
+    assert shapes == [(64, 64, 64, "Boussinesq"), (64, 64, 64, "PseudoIncompressible"),
+                      (64, 64, 96, "PseudoIncompressible")]
+
+
 @pytest.mark.gpu
 def test_members_sharing_the_gpu_match_their_standalone_runs(pcf, tmp_path):
     seeds = (0, 1, 2, 3, 4)

@@ -7,13 +7,14 @@ reduction are expected to agree to the last bit.  Tolerances below:
 * per-kernel outputs (operator coefficients and applies, preconditioner,
   fluxes, updates, corrector): relative 1e-13 of the field's max-norm;
 * anything downstream of a global reduction: the reduction order differs,
-  so BiCGSTAB iterates differ in the last bits and a solve may stop one
-  iteration earlier or later.  A solve must meet the reference criterion
+  so BiCGSTAB iterates differ in the last bits and a solve may stop a few
+  iterations earlier or later.  A solve must meet the reference criterion
   ||b - A x|| / ||b|| <= tol, checked with the oracle's operator, and the two
   solutions may differ only by what both criteria allow:
   ||A (x_gpu - x_oracle)|| <= 2 tol ||b||.  Stages and whole steps are
   compared with the Poisson tolerance tightened to 1e-12 (relative), where
-  the fields agree to 1e-9 relative.
+  the fields agree to 1e-9 relative.  Iteration counts may differ by at most
+  iter_slack(n), the spread that reordering the oracle's own sums gives.
 """
 import hipmem
 import numpy as np
@@ -60,13 +61,31 @@ def configs(pcf):
     mwc.tracer = pcf.TracerNamelist(tracer_setup="TracerOn", initial_tracer=pcf.TracerDipole(5000.0, 5000.0))
     # columns taller than the 80 levels the FP64 line solve keeps in LDS
     mwz = pcf.examples.mountain_wave_3d(x_size=16, y_size=14, z_size=136, iterations=2)
+    # the two sponge options off in every example: the LHS sponge relaxing
+    # the winds to their horizontal layer means (apply_lhs_sponge!.jl:365-530)
+    # and the RHS sponge's Rayleigh damping of the horizontal winds in the
+    # implicit updates and the operator (update!.jl:846-881,
+    # compute_operator!.jl:283-378)
+    mwr = pcf.examples.mountain_wave(x_size=12, y_size=10, z_size=9)
+    mwr.sponge.relax_to_mean = True
+    phd = pcf.examples.periodic_hill(x_size=24, z_size=20)
+    phd.sponge.damp_horizontal_wind_on_rhs = True
     return {"periodic_hill": ph, "mountain_wave": mw, "mountain_wave_3d": mw3, "cold_bubble": cb,
             "mountain_wave_odd": mwo, "vortex": vx, "mountain_wave_tracer": mwt, "hot_bubble": hb,
-            "mountain_wave_compressible": mwc, "mountain_wave_tall": mwz}
+            "mountain_wave_compressible": mwc, "mountain_wave_tall": mwz, "mountain_wave_relax": mwr,
+            "periodic_hill_damp": phd}
 
 
 CASES = ["periodic_hill", "mountain_wave", "mountain_wave_3d", "cold_bubble", "mountain_wave_odd", "vortex",
-         "mountain_wave_tracer", "hot_bubble", "mountain_wave_compressible"]
+         "mountain_wave_tracer", "hot_bubble", "mountain_wave_compressible", "mountain_wave_relax",
+         "periodic_hill_damp"]
+
+# BiCGSTAB iterates of the device and the oracle differ only through the
+# order of the dot products' sums (1e-15 relative); near the tolerance that
+# moves the count at which a solve stops, by as much as reordering the
+# oracle's own sums moves it (test_oracle_reduction_order.py measures the
+# spread and sets the bound)
+from test_oracle_reduction_order import iter_slack  # noqa: E402
 
 
 def compressible(nl):
@@ -207,6 +226,26 @@ def test_preconditioner_tall_columns(pcf, abi, oracle_mod):
         assert err <= KERNEL_RTOL, (half, err)
 
 
+def test_preconditioner_many_columns(pcf, abi, oracle_mod):
+    """1032 blocks of 64 columns, more than four per CU: the line solve keeps
+    64 levels in LDS (five blocks per CU) instead of 80; 72 levels, so both
+    parts of a column are checked against the oracle."""
+    nl = pcf.examples.mountain_wave_3d(x_size=64, y_size=1032, z_size=72, iterations=1)
+    gpu = pcf.State(nl)
+    orc = oracle_mod.Oracle(pcf.to_config(nl), abi)
+    for s in (gpu, orc):
+        s.compute_sponges(0.5, 3.0)
+        s.compute_operator(0.8, 1.0)
+    x = np.random.default_rng(12).standard_normal((gpu.nx, gpu.ny, gpu.nz))
+    dx = hipmem.DeviceArray(x.shape).upload(x)
+    dy = hipmem.DeviceArray(x.shape)
+    gpu.apply_preconditioner_device(dx.ptr, dy.ptr)
+    got, want = dy.download(), orc.apply_preconditioner(x)
+    for half in (slice(0, 64), slice(64, None)):
+        err = rel(got[:, :, half], want[:, :, half])
+        assert err <= KERNEL_RTOL, (half, err)
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_bicgstab(pcf, abi, oracle_mod, name):
     _, gpu, orc = pair(pcf, abi, oracle_mod, name)
@@ -221,7 +260,7 @@ def test_bicgstab(pcf, abi, oracle_mod, name):
     n1 = gpu.apply_bicgstab(tolref)
     rc, n2 = orc.apply_bicgstab(tolref)
     assert rc == 0
-    assert abs(n1 - n2) <= 2 + n2 // 10, (n1, n2)
+    assert abs(n1 - n2) <= iter_slack(n2), (n1, n2)
     tol = 1e-8 / tolref
     b = orc.get_vector("RHS")
     xg, xo = gpu.get_vector("SOLUTION"), orc.get_vector("SOLUTION")
@@ -248,6 +287,11 @@ def test_fluxes(pcf, abi, oracle_mod, name):
     for s in (gpu, orc):
         s.set_boundaries(abi.BOUNDARY_PREDICTANDS)
         s.compute_theta_fluxes()  # before compute_fluxes, which zeroes its wall faces
+    # the reference's sequence (explicit_integration!.jl:53-58) on the device,
+    # the oracle's compute_fluxes runs all three of its steps
+    gpu.reconstruct()
+    gpu.set_boundaries(abi.BOUNDARY_RECONSTRUCTIONS)
+    for s in (gpu, orc):
         s.compute_fluxes()
     nb = 3
     i0, j0, k0 = nb + 1, nb + 1, nb + 1
@@ -312,7 +356,7 @@ def test_implicit_stage(pcf, abi, oracle_mod, name):
         s.compute_sponges(0.3, 2.0)
     n1 = gpu.implicit_integration(0.3, 2.0, 2.0)
     n2 = orc.implicit_integration(0.3, 2.0, 2.0)
-    assert abs(n1 - n2) <= 2 + n2 // 10, (n1, n2)
+    assert abs(n1 - n2) <= iter_slack(n2), (n1, n2)
     for f in names:
         err = rel(gpu.get(f), orc.get(f))
         assert err <= SOLVE_RTOL, (f, err)
@@ -323,12 +367,12 @@ def test_steps(pcf, abi, oracle_mod, name):
     """initial cleaning + three full time steps of integrate."""
     nl, gpu, orc = pair(pcf, abi, oracle_mod, name, tight=True)
     m1, m2 = gpu.initial_cleaning(), orc.initial_cleaning()
-    assert abs(m1 - m2) <= 2 + m2 // 10
+    assert abs(m1 - m2) <= iter_slack(m2), (m1, m2)
     for _ in range(3):
         a, b = gpu.step(), orc.step()
         assert a.dt == pytest.approx(b.dt, rel=1e-10)
         for q in range(2):
-            assert abs(a.niter[q] - b.niter[q]) <= 2 + b.niter[q] // 10
+            assert abs(a.niter[q] - b.niter[q]) <= iter_slack(b.niter[q]), (a.niter[q], b.niter[q])
     names = predictands(nl)
     for f in names:
         err = rel(gpu.get(f), orc.get(f))

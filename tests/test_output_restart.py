@@ -99,3 +99,57 @@ def test_restart_past_tmax_is_an_error(pcf):
     with pytest.raises(pcf.PinCFlowError):
         st.restart(nl.output.tmax * 2, {k: st.get("U")[3:-3, 3:-3, 3:-3] * 0 for k in
                                          ("rhop", "us", "vs", "wts", "pip")})
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("writer", ["oracle", "gpu"])
+@pytest.mark.parametrize("example", ["periodic_hill", "mountain_wave", "vortex", "hot_bubble"])
+def test_restart_matches_the_oracle(pcf, oracle_mod, tmp_path, example, writer):
+    """One side (the oracle or the device) runs three steps and writes its
+    records; the device and the oracle both restart from the last record
+    (output.read_input, then read_input! and integrate.jl:168-186 on each
+    side) and continue for two steps.  With the Poisson tolerance tightened
+    to 1e-12 (relative) the two continued runs agree to STEP_RTOL, and their
+    iteration counts within the spread of reordered sums (iter_slack)."""
+    from test_oracle_reduction_order import iter_slack
+    STEP_RTOL = 1e-9
+    make = {"periodic_hill": lambda: pcf.examples.periodic_hill(x_size=24, z_size=20),
+            "mountain_wave": lambda: pcf.examples.mountain_wave(x_size=12, y_size=10, z_size=9),
+            "vortex": lambda: pcf.examples.vortex(x_size=12, y_size=10),
+            "hot_bubble": lambda: pcf.examples.hot_bubble(x_size=12, z_size=10)}[example]
+
+    def namelists(iterations):
+        nl = make()
+        nl.output.output_steps = True
+        nl.output.iterations = iterations
+        nl.output.prepare_restart = True
+        nl.poisson.tolerance = 1e-12
+        nl.poisson.tolerance_is_relative = True
+        return nl
+
+    path = str(tmp_path / "first.npz")
+    nl = namelists(3)
+    if writer == "gpu":
+        _, recs, _ = pcf.integrate(nl)
+    else:
+        _, recs, _ = oracle_mod.integrate(nl, pcf._abi, pcf.output)
+    pcf.output.write_file(path, recs)
+    time_s, fields = pcf.output.read_input(path, -1)
+    nl = namelists(2)
+    nl.poisson.initial_cleaning = False
+    gpu = pcf.State(nl)
+    orc = oracle_mod.Oracle(pcf.to_config(nl), pcf._abi)
+    gpu.restart(time_s, fields)
+    orc.restart(nl, pcf.output, time_s, fields)
+    for _ in range(2):
+        a, b = gpu.step(), orc.step()
+        assert a.time == pytest.approx(b.time, rel=1e-12)
+        assert a.dt == pytest.approx(b.dt, rel=1e-10)
+        for q in range(2):
+            assert abs(a.niter[q] - b.niter[q]) <= iter_slack(b.niter[q]), (a.niter[q], b.niter[q])
+    names = ["RHOP", "U", "V", "W", "PIP"] + (["RHO"] if nl.atmosphere.model != "Boussinesq" else [])
+    names += (["CHI"] if example == "vortex" else []) + (["P", "PBAR"] if example == "hot_bubble" else [])
+    for f in names:
+        x, y = gpu.get(f), orc.get(f)
+        scale = max(float(np.max(np.abs(y))), 1e-30)
+        assert float(np.max(np.abs(x - y))) / scale <= STEP_RTOL, (f, float(np.max(np.abs(x - y))) / scale)
