@@ -76,6 +76,10 @@ def parse():
                    help="learned guess: residual |A x0 - b|^2 (online, --train-inner Adam steps per solve) or "
                         "the squared error of the solution change")
     p.add_argument("--train-inner", type=int, default=6, help="learned guess: Adam steps per solve (residual loss)")
+    p.add_argument("--train-objective", default="energy", choices=["residual", "energy"],
+                   help="learned networks: the residual |A x - v|^2 or the energy norm of the error under -A")
+    p.add_argument("--stage-iterations", type=int, default=1,
+                   help="learned stage: leading BiCGSTAB iterations whose preconditioner applications it follows")
     p.add_argument("--train-lr", type=float, default=1e-6,
                    help="learned guess: Adam learning rate of the residual loss (1e-5 and above diverge at 256^2 x 128)")
     p.add_argument("--no-variants", action="store_true",
@@ -98,7 +102,7 @@ def workload(pcf, args, iterations, precision):
 
 
 def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False, storage=None, mg_coarse=None,
-               relative=False):
+               relative=False, stage_net=None):
     """Setup, initial cleaning, warmup, then the timed steps of one method,
     with HIP events around the launches of the operator apply only (its
     average duration gives the roofline figure; events around every launch
@@ -114,11 +118,13 @@ def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False
         nl.poisson.mg_smoothing_steps = args.mg_smoothing_steps
         nl.poisson.mg_coarse_steps = mg_coarse or args.mg_coarse_steps
         nl.poisson.preconditioner_storage = storage or args.storage
-    tag = f"{precision}/{preconditioner}/{nl.poisson.preconditioner_storage}/{guess}"
+    tag = f"{precision}/{preconditioner}/{nl.poisson.preconditioner_storage}/{guess}" + ("+stage" if stage_net else "")
     t0 = time.perf_counter()
     st = pcf.State(nl)
     if net is not None:
         pcf.learned.attach(st, net)
+    if stage_net is not None:
+        pcf.learned.attach_stage(st, stage_net, args.stage_iterations)
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     n_clean = st.initial_cleaning()
@@ -184,7 +190,8 @@ def main():
     net = None
     if args.initial_guess == "learned":
         training = train_guess(pcf, args)
-        net = training.pop("net")
+        net = training.pop("net", None) or training.pop("guess")
+        training.pop("stage", None)
     main_run = run_method(pcf, args, args.precision, args.preconditioner, args.initial_guess, net=net,
                           keep=True)
     st = main_run["state"]
@@ -213,20 +220,28 @@ def main():
         # (preconditioner, storage of its finest-level coefficients, multigrid
         # coarsest-level steps, initial guess); the FP64 BiCGSTAB and its 1e-8
         # criterion are the same in all
-        for pc, storage, coarse, guess in (("line_relaxation", "fp32", 20, "zero"),
-                                           ("line_relaxation", "bf16", 20, "zero"),
-                                           ("multigrid", "fp32", 20, "zero"),
-                                           ("multigrid", "bf16", 2, "zero"),
-                                           ("multigrid", "bf16", 2, "previous"),
-                                           ("multigrid", "bf16", 2, "learned")):
-            if guess == "learned":
+        vstage = None
+        for pc, storage, coarse, guess, staged in (("line_relaxation", "fp32", 20, "zero", False),
+                                                   ("line_relaxation", "bf16", 20, "zero", False),
+                                                   ("multigrid", "fp32", 20, "zero", False),
+                                                   ("multigrid", "bf16", 2, "zero", False),
+                                                   ("multigrid", "bf16", 2, "previous", False),
+                                                   ("multigrid", "bf16", 2, "learned", False),
+                                                   ("multigrid", "bf16", 2, "previous", True)):
+            if (guess == "learned" or staged) and vtrain is None:
                 vtrain = train_guess(pcf, args, preconditioner=pc)
-                vnet = vtrain.pop("net")
+                vnet = vtrain.pop("net", None) or vtrain.pop("guess")
+                vstage = vtrain.pop("stage", None)
+            if staged and vstage is None:
+                continue
             r = run_method(pcf, args, "mixed", pc, guess, net=vnet if guess == "learned" else None, keep=True,
-                           storage=storage, mg_coarse=coarse)
+                           storage=storage, mg_coarse=coarse, stage_net=vstage if staged else None)
             desc = pc.replace("_", " ") + " preconditioner"
             if pc == "multigrid":
                 desc += f" ({args.mg_smoothing_steps} smoothing steps, {coarse} coarsest-level steps)"
+            if staged:
+                desc += (f" followed by the learned flexible-BiCGSTAB stage in the first {args.stage_iterations} "
+                         "iteration(s)")
             entry = {"method": f"FP64 BiCGSTAB, {desc} in FP32 with {storage} coefficients, {guess} initial guess",
                      "value": round(r["value"], 4), "ms_per_step": round(r["ms_per_step"], 2),
                      "speedup_vs_plain": round(r["value"] / main_run["value"], 3),
@@ -236,7 +251,9 @@ def main():
             entry["setup_s"], entry["initial_cleaning_s"] = r["setup_s"], r["initial_cleaning_s"]
             if guess in ("previous", "learned"):
                 entry["guess_fallbacks"] = r["state"].guess_fallbacks()
-            if guess == "learned":
+            if staged:
+                entry["stage_fallbacks"] = r["state"].stage_fallbacks()
+            if guess == "learned" or staged:
                 entry["learned_training"] = vtrain
                 net = r["breakdown"].get("net")
                 if net:
@@ -244,8 +261,8 @@ def main():
                     flop = 2 * 27 * (3 * 32 + 4 * 32 * 32 + 32) * n_cells
                     t = net[1] / net[0] * 1e-3
                     entry["network_roofline"] = {
-                        "bound": "mfma", "kernel": "network forward (32-channel layers on bf16 MFMA, first and "
-                                                   "last layers on the vector units)",
+                        "bound": "mfma", "kernel": "network forward (k_brick: every layer an implicit GEMM on "
+                                                   "bf16 MFMA over an LDS-staged input brick)",
                         "achieved": round(flop / t / 1e12, 1), "peak": 2500.0, "unit": "TFLOP/s",
                         "frac": round(flop / t / 1e12 / 2500.0, 4), "flop_per_forward": flop,
                         "avg_forward_ms": round(net[1] / net[0], 3)}
@@ -393,39 +410,56 @@ def train_guess(pcf, args, preconditioner=None):
 
 
 def train_guess_residual(pcf, args, preconditioner=None):
-    """Online training on the residual of the guess (learned.train_residual)
-    over the solves of seeded random-hill runs, then the residual of the
-    guess relative to the warm start's on the benchmark's own terrain."""
+    """Online training on the device (learned.train_online) over the solves
+    of seeded random-hill runs: the initial-guess network and the network of
+    the flexible-BiCGSTAB preconditioning stage, then, on the benchmark's own
+    terrain (which the training runs do not contain), the residual of each
+    relative to what it corrects (the warm start, M^-1 of the first
+    residual).  Returns both inference networks and a summary."""
     t0 = time.perf_counter()
-    net = pcf.learned.GuessNetwork(args.nx, args.nx, args.nz, training=True, seed=0)
+    g = pcf.learned.GuessNetwork(args.nx, args.nx, args.nz, training=True, seed=0)
+    s = pcf.learned.GuessNetwork(args.nx, args.nx, args.nz, training=True, seed=1)
     hist = []
+    log = (lambda m: print("# " + m, file=sys.stderr, flush=True))
     for r in range(args.train_runs):
         nl = pcf.examples.gaussian_hills_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz,
                                             iterations=args.train_steps, seed=100 + r, n_hills=8, precision="mixed")
         nl.poisson.preconditioner_type = preconditioner or args.preconditioner
+        nl.poisson.preconditioner_storage = args.storage
+        nl.poisson.mg_coarse_steps = 2
         nl.atmosphere.model = args.model
         st = pcf.State(nl)
         st.initial_cleaning()
-        hist += pcf.learned.train_residual(net, [st], args.train_steps, inner=args.train_inner, lr=args.train_lr,
-                                           log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+        hist += pcf.learned.train_online([st], args.train_steps, guess=g, stage=s, inner=args.train_inner,
+                                         lr=args.train_lr, objective=args.train_objective, log=log)
         st.close()
-    params = net.params()
-    net.close()
+    gp, sp = g.params(), s.params()
+    g.close()
+    s.close()
     t_train = time.perf_counter() - t0
-    infer = pcf.learned.GuessNetwork(args.nx, args.nx, args.nz, training=False, params=params)
+    guess = pcf.learned.GuessNetwork(args.nx, args.nx, args.nz, training=False, params=gp)
+    stage = pcf.learned.GuessNetwork(args.nx, args.nx, args.nz, training=False, params=sp)
+    # held out: the benchmark's terrain, evaluated without training (inner = 0)
     nl = pcf.examples.mountain_wave_3d(x_size=args.nx, y_size=args.nx, z_size=args.nz, iterations=5,
                                        precision="mixed")
     nl.poisson.preconditioner_type = preconditioner or args.preconditioner
+    nl.poisson.preconditioner_storage = args.storage
+    nl.poisson.mg_coarse_steps = 2
     nl.atmosphere.model = args.model
     st = pcf.State(nl)
     st.initial_cleaning()
-    rl = pcf.learned.residual_loss(infer, st, 5)
+    held = pcf.learned.train_online([st], 5, guess=guess, stage=stage, inner=0)
     st.close()
-    return {"net": infer, "loss": "residual", "solves": len(hist), "adam_steps_per_solve": args.train_inner,
-            "learning_rate": args.train_lr,
-            "train_s": round(t_train, 1),
-            "training_residual_vs_warm_start_last": round(hist[-1][1] / hist[-1][0], 4),
-            "held_out_residual_vs_warm_start": [round(b / a, 4) for a, b in rl]}
+
+    def ratios(recs, key):
+        return [round(r[key][1] / r[key][0], 4) for r in recs]
+
+    return {"guess": guess, "stage": stage, "objective": args.train_objective, "solves": len(hist),
+            "adam_steps_per_solve": args.train_inner, "learning_rate": args.train_lr, "train_s": round(t_train, 1),
+            "training_guess_residual_vs_warm_start_last5": ratios(hist[-5:], "guess"),
+            "training_stage_residual_vs_multigrid_last5": ratios(hist[-5:], "stage"),
+            "held_out_guess_residual_vs_warm_start": ratios(held, "guess"),
+            "held_out_stage_residual_vs_multigrid": ratios(held, "stage")}
 
 
 def fp64_difference(pcf, args, st_mixed):

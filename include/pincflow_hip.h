@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCF_ABI_VERSION 5
+#define PCF_ABI_VERSION 6
 
 /* ---- return codes ------------------------------------------------------ */
 #define PCF_OK 0
@@ -467,6 +467,39 @@ int pcf_set_guess_network(pcf_state* st, pcf_net* net);
 /* Number of solves whose initial guess was replaced by the zero start
  * because it left a residual no smaller than |b|. */
 int pcf_guess_fallbacks(const pcf_state* st, int32_t* count);
+/* Attach the network of the learned flexible-BiCGSTAB preconditioning stage
+ * (no reference counterpart; not owned, grid of the state; null detaches):
+ * in the first `iterations` BiCGSTAB iterations of each mixed solve, every
+ * preconditioner application y = M^-1 v is followed by
+ * y += |v| net(v / |v|, y / |v|, J - 1) (|v| the RMS norm).  The solution is
+ * accumulated in the flexible (textbook) form; a solve that does not
+ * converge with the stage is rerun without it from x0 = 0 (the plain
+ * preconditioned BiCGSTAB is the fallback).  Needs precision = mixed and a
+ * preconditioner (PCF_E_CONFIG otherwise). */
+int pcf_set_stage_network(pcf_state* st, pcf_net* net, int32_t iterations);
+/* Number of solves rerun without the learned stage. */
+int pcf_stage_fallbacks(const pcf_state* st, int32_t* count);
+/* Objectives of pcf_net_train_residual. */
+enum pcf_objective { PCF_OBJECTIVE_RESIDUAL = 0, PCF_OBJECTIVE_ENERGY = 1 };
+/* Train a network on what it proposes for A x = v (no reference
+ * counterpart): for the interior-sized FP64 device vectors v and base (base
+ * may be null = 0), x = base + |v| net(v / |v|, base / |v|, J - 1), A the
+ * state's current total operator.  Takes `inner` Adam steps (learning rate
+ * lr) on the objective:
+ *   PCF_OBJECTIVE_RESIDUAL  L = |A x - v|^2 / |v|^2, output gradient
+ *                           2 |v| A^T (A x - v) / |v|^2;
+ *   PCF_OBJECTIVE_ENERGY    the energy norm of the error under -A (whose
+ *                           symmetric part is positive semidefinite), output
+ *                           gradient -2 |v| (A x - v) / |v|^2: weights the
+ *                           smooth error components that set BiCGSTAB's
+ *                           iteration count more than the residual does.
+ * inner = 0 only evaluates.  *loss_base = L of base alone, *loss_net = L with
+ * the network after the steps (the residual loss for either objective;
+ * either pointer may be null).  v = b, base = the warm start trains the
+ * initial guess; v = a vector being preconditioned, base = M^-1 v trains the
+ * preconditioning stage. */
+int pcf_net_train_residual(pcf_state* st, pcf_net* net, const double* d_v, const double* d_base, int32_t objective,
+                           int32_t inner, float lr, double* loss_base, double* loss_net);
 
 #ifdef __cplusplus
 }

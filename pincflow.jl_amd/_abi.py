@@ -13,7 +13,7 @@ from pathlib import Path
 PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "lib" / "libpincflow_hip.so"
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 MAX_HILLS = 32
 
 # return codes
@@ -216,6 +216,9 @@ SIGNATURES = [
     ("pcf_guess_input", _I, [_P, _P, _P, _P, _PD]),
     ("pcf_set_guess_network", _I, [_P, _P]),
     ("pcf_guess_fallbacks", _I, [_P, _PI32]),
+    ("pcf_set_stage_network", _I, [_P, _P, ctypes.c_int32]),
+    ("pcf_stage_fallbacks", _I, [_P, _PI32]),
+    ("pcf_net_train_residual", _I, [_P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, _PD, _PD]),
     ("pcf_profile_groups", _I, [_P, ctypes.c_uint32]),
     ("pcf_profile_enable", _I, [_P, _I]),
     ("pcf_profile_query", _I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _PD]),

@@ -1098,6 +1098,17 @@ int apply_bicgstab_mixed(pcf_state* st, double tolref, double dt, int* niter_out
     scale = 1.0;
   }
   rc = bicgstab_core(st, tolref, true, guess, scale, niter_out);
+  if (rc == PCF_E_NOCONV && st->stage_net && st->stage_iterations > 0) {
+    // the flexible solve with the learned stage failed: the plain
+    // preconditioned BiCGSTAB from zero is the fallback
+    const int failed = *niter_out;
+    pcf_net* keep = st->stage_net;
+    st->stage_net = nullptr;
+    rc = bicgstab_core(st, tolref, true, nullptr, 0.0, niter_out);
+    st->stage_net = keep;
+    st->stage_fallbacks += 1;
+    *niter_out += failed;
+  }
   if (rc) return rc;
   if (use_prev) {  // keep solution / dt for the same stage of the next step
     const size_t N = st->g.nv();

@@ -2,15 +2,16 @@
 //
 // Six 3 x 3 x 3 convolutions, 3 -> 32 -> 32 -> 32 -> 32 -> 32 -> 1 channels,
 // GELU after the first five, bias on every layer; fields are periodic in x
-// and y and zero beyond the walls in z, like the pressure operator.  The
-// four 32 -> 32 layers run as implicit GEMMs on the bf16 matrix cores
-// (v_mfma_f32_32x32x16_bf16, FP32 accumulation): for each tap, a 32 (output
-// channels) x 32 (voxels) tile accumulates W[tap] (32 x 32) times the input
-// at the tap's neighbours (32 x 32).  The backward pass reuses that kernel
+// and y and zero beyond the walls in z, like the pressure operator.  Every
+// layer's forward runs as an implicit GEMM on the bf16 matrix cores
+// (v_mfma_f32_32x32x16_bf16, FP32 accumulation) over an input brick staged
+// in LDS (k_brick); weights and activations are bf16, the network's input is
+// rounded to bf16 on staging.  The backward pass reuses the 32 -> 32 kernel
 // for the input gradient (transposed, tap-flipped weights) and has its own
 // MFMA kernel for the weight gradient (a GEMM over the voxels, split into
-// chunks whose partial sums are added in a fixed order).  The 3 -> 32 and
-// 32 -> 1 layers are small and run on the vector units.
+// chunks whose partial sums are added in a fixed order); the weight
+// gradients of the first and last layers and the last layer's input
+// gradient run on the vector units from the FP32 master weights.
 //
 // Layout: activations are channels-last [voxel][channel] with voxel =
 // i + nx (j + ny k); parameters are one FP32 vector, layer by layer W[27][cout][cin]
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "pcf_internal.hpp"
+#include "solver_common.hpp"
 
 namespace pcf {
 
@@ -72,121 +74,226 @@ __device__ __forceinline__ float gelu_grad(float x) {
   return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * expf(-0.5f * x * x) * 0.39894228040143268f;
 }
 
-// ---- layer 1: 3 -> 32 (vector units); z (pre-activation, may be null), a = gelu(z) in bf16
-__global__ void k_first(Dims d, const float* __restrict__ x, const float* __restrict__ W, float* z, bf16* a) {
-  const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (p >= d.n) return;
-  int i, j, k;
-  coords(d, p, i, j, k);
-  const float* b = W + 27 * kC * 3;
-  float acc[kC];
-#pragma unroll
-  for (int c = 0; c < kC; ++c) acc[c] = b[c];
-  for (int t = 0; t < 27; ++t) {
-    const long long q = neighbour(d, i, j, k, t);
-    if (q < 0) continue;
-    const float x0 = x[3 * q], x1 = x[3 * q + 1], x2 = x[3 * q + 2];
-    const float* w = W + t * kC * 3;
-#pragma unroll
-    for (int c = 0; c < kC; ++c) acc[c] += w[3 * c] * x0 + w[3 * c + 1] * x1 + w[3 * c + 2] * x2;
-  }
-#pragma unroll
-  for (int c = 0; c < kC; ++c) {
-    if (z) z[p * kC + c] = acc[c];
-    a[p * kC + c] = (bf16)gelu(acc[c]);
-  }
+// ---- the convolutions on the matrix cores (v_mfma_f32_32x32x16_bf16, FP32
+// accumulation).  Operand layout: lane l holds A[l % 32][8 (l / 32) + e] and
+// B[8 (l / 32) + e][l % 32], e = 0..7, and D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32],
+// q = 0..15.  A row is the output channel, a column one of 32 voxels
+// consecutive in x (nx is a multiple of 32), K runs over input channels (and,
+// for the first layer, taps).
+//
+// A workgroup of 4 wavefronts computes a block of 32 (x) x 4 (y) x 4 (z)
+// voxels.  Its input brick, the block with a one-voxel halo (34 x 6 x 6
+// voxels, periodic in x and y, zero beyond the walls in z), is staged in LDS
+// once as bf16; each wavefront computes 2 (y) x 2 (z) rows of 32 voxels, so
+// one B operand read from LDS (the input row at one (y, z), shifted by dx)
+// feeds the MFMAs of every output row of the wavefront whose 3 x 3 x 3
+// neighbourhood holds that row (up to four).  The A operands of one (dx,
+// channel half) -- nine taps -- stay in registers over the 16 input rows.
+// Blocks are numbered so that each XCD (blocks are dealt to the 8 XCDs in
+// turn) gets a contiguous range of neighbouring blocks, whose halos its L2
+// then shares.
+//
+// KIND_FIRST: 3 -> 32, K = 27 taps x 3 channels padded to 96; the brick holds
+//   the FP32 input rounded to bf16, 4 channels (8 B) per voxel.
+// KIND_MID:   32 -> 32 (FWD: W x + bias -> z (FP32, may be null), gelu -> a
+//   (bf16); DATA: W x -> g (FP32), the input gradient with transposed,
+//   tap-flipped W).
+// KIND_LAST:  32 -> 1, the 32 x 32 MFMA with only row 0 of A nonzero; out (FP32).
+enum { KIND_FIRST = 0, KIND_MID = 1, KIND_LAST = 2 };
+enum { MID_FWD = 0, MID_DATA = 1 };
+constexpr int kBX = 34, kBY = 6, kBZ = 6, kBrickVox = kBX * kBY * kBZ;  // 1224 voxels
+constexpr int kK1 = 96;  // padded K of the first layer
+
+struct BrickGrid {
+  int tiles_x, tiles_y, tiles_z;
+};
+
+// 16-byte chunk c (0..3) of brick voxel v, XOR-swizzled over neighbouring
+// voxels so that 8 lanes reading one chunk of consecutive voxels hit distinct banks
+__device__ __forceinline__ int brick_chunk(int v, int c) { return 4 * v + (c ^ ((v >> 1) & 3)); }
+
+__device__ __forceinline__ int wrap(int q, int n) {
+  q %= n;
+  return q < 0 ? q + n : q;
 }
 
-// ---- layers 2..5: 32 -> 32 implicit GEMM on the matrix cores.  One
-// wavefront computes 32 output channels x kMidTiles tiles of 32 voxels
-// consecutive in x (nx is a multiple of 32).  Operand layout of v_mfma_f32_32x32x16_bf16: lane l
-// holds A[l % 32][8 (l / 32) + e] and B[8 (l / 32) + e][l % 32], e = 0..7,
-// and D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32], q = 0..15.
-// FWD: out = W x + bias -> z (FP32, may be null) and gelu -> a (bf16).
-// DATA: out = W x (the input gradient, with transposed tap-flipped W) -> g (FP32).
-enum { MID_FWD = 0, MID_DATA = 1 };
-constexpr int kMidTiles = 4;  // 32-voxel tiles per wavefront: each weight load feeds 4 MFMAs
+__device__ __forceinline__ bf8 as_bf8(uint4 u) {
+  bf8 r;
+  __builtin_memcpy(&r, &u, sizeof(r));
+  return r;
+}
 
-template <int MODE>
-__global__ void __launch_bounds__(256) k_mid(Dims d, const bf16* __restrict__ in, const bf16* __restrict__ W,
-                                             const float* __restrict__ bias, float* z, bf16* a, float* g) {
-  const int lane = threadIdx.x & 63;
-  const size_t wave = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
-  const size_t ntiles = d.n / 32;
-  const size_t tile0 = wave * kMidTiles;
-  if (tile0 >= ntiles) return;  // uniform over the wavefront
-  const int col = lane & 31, half = lane >> 5;
-  int ci[kMidTiles], cj[kMidTiles], ck[kMidTiles];
-  bool valid[kMidTiles];
+__device__ __forceinline__ uint2 pack4(float a, float b, float c, float d) {
+  bf16 h[4] = {(bf16)a, (bf16)b, (bf16)c, (bf16)d};
+  uint2 r;
+  __builtin_memcpy(&r, h, sizeof(r));
+  return r;
+}
+
+template <int KIND, int MODE>
+__global__ void __launch_bounds__(256, 2) k_brick(Dims d, BrickGrid bg, const void* __restrict__ in_,
+                                                  const bf16* __restrict__ W, const float* __restrict__ bias,
+                                                  float* zout, bf16* aout, float* gout) {
+  __shared__ uint4 brick[KIND == KIND_FIRST ? kBrickVox / 2 : kBrickVox * 4];
+  const int nb = (int)gridDim.x;
+  int b = (int)blockIdx.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  const int tx = b % bg.tiles_x, tyz = b / bg.tiles_x, ty = tyz % bg.tiles_y, tz = tyz / bg.tiles_y;
+  const int x0 = 32 * tx, y0 = 4 * ty, z0 = 4 * tz;
+
+  // ---- stage the input brick
+  if (KIND == KIND_FIRST) {
+    const float* in = static_cast<const float*>(in_);
+    uint2* br = reinterpret_cast<uint2*>(brick);
+    for (int v = threadIdx.x; v < kBrickVox; v += 256) {
+      const int bx = v % kBX, byz = v / kBX, by = byz % kBY, bz = byz / kBY;
+      const int gz = z0 + bz - 1;
+      float x0v = 0.0f, x1v = 0.0f, x2v = 0.0f;
+      if (gz >= 0 && gz < d.nz) {
+        const size_t q = (size_t)wrap(x0 + bx - 1, d.nx) + (size_t)d.nx * (wrap(y0 + by - 1, d.ny) + (size_t)d.ny * gz);
+        x0v = in[3 * q];
+        x1v = in[3 * q + 1];
+        x2v = in[3 * q + 2];
+      }
+      br[v] = pack4(x0v, x1v, x2v, 0.0f);
+    }
+  } else {
+    const bf16* in = static_cast<const bf16*>(in_);
+    constexpr int kChunks = kBrickVox * 4, kPer = (kChunks + 255) / 256;
+    uint4 buf[kPer];
 #pragma unroll
-  for (int u = 0; u < kMidTiles; ++u) {
-    valid[u] = tile0 + u < ntiles;
-    coords(d, (tile0 + u) * 32 + col, ci[u], cj[u], ck[u]);
-  }
-  f16v acc[kMidTiles];
-#pragma unroll
-  for (int u = 0; u < kMidTiles; ++u) acc[u] = f16v{};
-  for (int t = 0; t < 27; ++t) {
-    long long q[kMidTiles];
-#pragma unroll
-    for (int u = 0; u < kMidTiles; ++u) q[u] = valid[u] ? neighbour(d, ci[u], cj[u], ck[u], t) : -1;
-#pragma unroll
-    for (int kc = 0; kc < 2; ++kc) {
-      const int c0 = 16 * kc + 8 * half;
-      const bf8 av = *(const bf8*)(W + ((size_t)t * kC + col) * kC + c0);
-      bf8 bv[kMidTiles];
-#pragma unroll
-      for (int u = 0; u < kMidTiles; ++u) {
-        if (q[u] >= 0) {
-          bv[u] = *(const bf8*)(in + (size_t)q[u] * kC + c0);
-        } else {
-#pragma unroll
-          for (int e = 0; e < 8; ++e) bv[u][e] = (bf16)0.0f;
+    for (int u = 0; u < kPer; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      buf[u] = uint4{0u, 0u, 0u, 0u};
+      if (e < kChunks) {
+        const int v = e >> 2, c = e & 3;
+        const int bx = v % kBX, byz = v / kBX, by = byz % kBY, bz = byz / kBY;
+        const int gz = z0 + bz - 1;
+        if (gz >= 0 && gz < d.nz) {
+          const size_t q =
+              (size_t)wrap(x0 + bx - 1, d.nx) + (size_t)d.nx * (wrap(y0 + by - 1, d.ny) + (size_t)d.ny * gz);
+          buf[u] = *reinterpret_cast<const uint4*>(in + q * kC + 8 * c);
         }
       }
+    }
 #pragma unroll
-      for (int u = 0; u < kMidTiles; ++u) acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv[u], acc[u], 0, 0, 0);
+    for (int u = 0; u < kPer; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      if (e < kChunks) brick[brick_chunk(e >> 2, e & 3)] = buf[u];
     }
   }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 31, half = lane >> 5;
+  const int wy = 2 * (wv & 1), wz = 2 * (wv >> 1);  // the wavefront's first output row in the block
+  f16v acc[2][2];
 #pragma unroll
-  for (int u = 0; u < kMidTiles; ++u) {
-    if (!valid[u]) continue;
-    const size_t p = (tile0 + u) * 32 + col;
+  for (int oy = 0; oy < 2; ++oy)
 #pragma unroll
-    for (int qi = 0; qi < 16; ++qi) {
-      const int co = 8 * (qi / 4) + 4 * half + (qi % 4);
-      float v = acc[u][qi];
-      if (MODE == MID_FWD) {
-        v += bias[co];
-        if (z) z[p * kC + co] = v;
-        a[p * kC + co] = (bf16)gelu(v);
-      } else {
-        g[p * kC + co] = v;
+    for (int oz = 0; oz < 2; ++oz) acc[oy][oz] = f16v{};
+
+  if (KIND == KIND_FIRST) {
+    // K index k = 3 t + ci; lane (col, half) of k-block kb holds k = 16 kb + 8 half + e
+    const uint16_t* br = reinterpret_cast<const uint16_t*>(brick);
+    bf8 wa[6];
+#pragma unroll
+    for (int kb = 0; kb < 6; ++kb) wa[kb] = *reinterpret_cast<const bf8*>(W + (size_t)col * kK1 + 16 * kb + 8 * half);
+#pragma unroll
+    for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+      for (int oz = 0; oz < 2; ++oz) {
+        // brick voxel of the output voxel (col, wy + oy, wz + oz) at tap (0, 0, 0)
+        const int vc = (col + 1) + kBX * ((wy + oy + 1) + kBY * (wz + oz + 1));
+#pragma unroll
+        for (int kb = 0; kb < 6; ++kb) {
+          bf8 bv;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int k0 = 16 * kb + e, k1 = k0 + 8;  // half 0 / half 1
+            const int t0 = k0 / 3, c0 = k0 % 3, t1 = k1 / 3, c1 = k1 % 3;
+            const int o0 = (t0 % 3 - 1) + kBX * ((t0 / 3) % 3 - 1) + kBX * kBY * (t0 / 9 - 1);
+            const int o1 = (t1 % 3 - 1) + kBX * ((t1 / 3) % 3 - 1) + kBX * kBY * (t1 / 9 - 1);
+            const bool z0k = k0 >= 81, z1k = k1 >= 81;
+            const int off = half ? 4 * (vc + o1) + c1 : 4 * (vc + o0) + c0;
+            const bool zero = half ? z1k : z0k;
+            const uint16_t raw = br[zero ? 0 : off];
+            bv[e] = zero ? (bf16)0.0f : __builtin_bit_cast(bf16, raw);
+          }
+          acc[oy][oz] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[kb], bv, acc[oy][oz], 0, 0, 0);
+        }
+      }
+  } else {
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) {
+#pragma unroll
+      for (int kc = 0; kc < 2; ++kc) {
+        const int c = 2 * kc + half;  // the lane's 8 channels 8c..8c+7
+        bf8 wa[3][3];                 // [dy + 1][dz + 1]
+#pragma unroll
+        for (int ty2 = 0; ty2 < 3; ++ty2)
+#pragma unroll
+          for (int tz2 = 0; tz2 < 3; ++tz2) {
+            const int t = dx + 3 * ty2 + 9 * tz2;
+            if (KIND == KIND_LAST) {
+              if (col == 0) {
+                wa[ty2][tz2] = *reinterpret_cast<const bf8*>(W + (size_t)t * kC + 8 * c);
+              } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) wa[ty2][tz2][e] = (bf16)0.0f;
+              }
+            } else {
+              wa[ty2][tz2] = *reinterpret_cast<const bf8*>(W + ((size_t)t * kC + col) * kC + 8 * c);
+            }
+          }
+#pragma unroll
+        for (int iy = 0; iy < 4; ++iy)
+#pragma unroll
+          for (int iz = 0; iz < 4; ++iz) {
+            const int v = (col + dx) + kBX * ((wy + iy) + kBY * (wz + iz));
+            const bf8 bv = as_bf8(brick[brick_chunk(v, c)]);
+#pragma unroll
+            for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+              for (int oz = 0; oz < 2; ++oz) {
+                const int ty2 = iy - oy, tz2 = iz - oz;
+                if (ty2 >= 0 && ty2 <= 2 && tz2 >= 0 && tz2 <= 2)
+                  acc[oy][oz] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[ty2][tz2], bv, acc[oy][oz], 0, 0, 0);
+              }
+          }
       }
     }
   }
-}
 
-// ---- layer 6: 32 -> 1 (vector units)
-__global__ void k_last(Dims d, const bf16* __restrict__ in, const float* __restrict__ W, float* out) {
-  const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (p >= d.n) return;
-  int i, j, k;
-  coords(d, p, i, j, k);
-  float s = W[27 * kC];
-  for (int t = 0; t < 27; ++t) {
-    const long long q = neighbour(d, i, j, k, t);
-    if (q < 0) continue;
-    const bf8* v = (const bf8*)(in + (size_t)q * kC);
-    const float* w = W + t * kC;
+  // ---- epilogue
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      const bf8 x = v[h];
+  for (int oy = 0; oy < 2; ++oy)
 #pragma unroll
-      for (int e = 0; e < 8; ++e) s += w[8 * h + e] * (float)x[e];
+    for (int oz = 0; oz < 2; ++oz) {
+      const int y = y0 + wy + oy, z = z0 + wz + oz;
+      if (y >= d.ny || z >= d.nz) continue;
+      const size_t p = (size_t)(x0 + col) + (size_t)d.nx * (y + (size_t)d.ny * z);
+      if (KIND == KIND_LAST) {
+        if (half == 0) zout[p] = acc[oy][oz][0] + bias[0];
+        continue;
+      }
+#pragma unroll
+      for (int gq = 0; gq < 4; ++gq) {
+        const int co = 8 * gq + 4 * half;
+        float v0 = acc[oy][oz][4 * gq], v1 = acc[oy][oz][4 * gq + 1], v2 = acc[oy][oz][4 * gq + 2],
+              v3 = acc[oy][oz][4 * gq + 3];
+        if (KIND == KIND_MID && MODE == MID_DATA) {
+          *reinterpret_cast<float4*>(gout + p * kC + co) = float4{v0, v1, v2, v3};
+          continue;
+        }
+        v0 += bias[co];
+        v1 += bias[co + 1];
+        v2 += bias[co + 2];
+        v3 += bias[co + 3];
+        if (zout) *reinterpret_cast<float4*>(zout + p * kC + co) = float4{v0, v1, v2, v3};
+        *reinterpret_cast<uint2*>(aout + p * kC + co) = pack4(gelu(v0), gelu(v1), gelu(v2), gelu(v3));
+      }
     }
-  }
-  out[p] = s;
 }
 
 // ---- loss: dout = 2 (out - target) / n; partial sums of (out - target)^2 per block
@@ -427,6 +534,19 @@ __global__ void k_pack(const float* params, bf16* w16, bf16* wt16) {
   wt16[base + ((size_t)(26 - t) * kC + ci) * kC + co] = (bf16)W[s];
 }
 
+// bf16 copies of the first layer's weights as the A operand wants them,
+// wf16[co][3 t + ci] padded with zeros to kK1 columns, and of the last
+// layer's, wl16[t][ci]
+__global__ void k_pack_ends(const float* params, bf16* wf16, bf16* wl16) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < kC * kK1) {
+    const int co = s / kK1, k = s % kK1;
+    const float* W = params + layer_offset(0);
+    wf16[s] = k < 81 ? (bf16)W[((k / 3) * kC + co) * 3 + k % 3] : (bf16)0.0f;
+  }
+  if (s < 27 * kC) wl16[s] = (bf16)params[layer_offset(5) + s];
+}
+
 // input channels of the learned guess: b / |b|, c_prev prev / |b| (prev may
 // be null: 0) and jac - 1, channels-last FP32
 __global__ void k_guess_input(pcf::Geo g, const double* b, const double* prev, const double* jac, float* in,
@@ -444,6 +564,61 @@ __global__ void k_guess_input(pcf::Geo g, const double* b, const double* prev, c
 __global__ void k_guess_output(const float* out, double* x, size_t n, double bnorm, const double* prev, double c) {
   for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
     x[q] = (prev ? c * prev[q] : 0.0) + bnorm * (double)out[q];
+}
+
+// ---- the learned preconditioning stage and the residual training
+constexpr int kStageGrid = 2048;  // blocks of the sum-of-squares partials
+
+__global__ void k_sumsq(const double* v, size_t n, double* partials) {
+  double val[1] = {0.0};
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    val[0] += v[q] * v[q];
+  block_partials<1>(val, partials, blockIdx.x);
+}
+
+// *out = sqrt(sum of the partials / n), the RMS norm
+__global__ void k_rms_final(const double* partials, int nparts, double* out, double n) {
+  const double s = finalize_sum(partials, 0, nparts);
+  if (threadIdx.x == 0) *out = sqrt(s / n);
+}
+
+// the stage's input channels v / |v|, y / |v| and J - 1 with |v| on the device
+__global__ void k_stage_input(pcf::Geo g, const double* v, const double* y, const double* jac, float* in,
+                              const double* nrm, const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (p >= g.nv()) return;
+  const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), k = (int)(p / ((size_t)g.nx * g.ny));
+  const double inv = *nrm > 0.0 ? 1.0 / *nrm : 0.0;
+  in[3 * p] = (float)(v[p] * inv);
+  in[3 * p + 1] = (float)(y[p] * inv);
+  in[3 * p + 2] = (float)(jac[g.h(g.i0 + i, g.j0 + j, g.k0 + k)] - 1.0);
+}
+
+// y += |v| out
+__global__ void k_stage_output(const float* out, double* y, size_t n, const double* nrm, const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  const double s = *nrm;
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    y[q] += s * (double)out[q];
+}
+
+// x = base + c out (base may be null: 0)
+__global__ void k_compose(const float* out, const double* base, double* x, size_t n, double c) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    x[q] = (base ? base[q] : 0.0) + c * (double)out[q];
+}
+
+// r = a - b
+__global__ void k_sub(const double* a, const double* b, double* r, size_t n) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    r[q] = a[q] - b[q];
+}
+
+// the output gradient of the residual loss, c A^T r, in FP32
+__global__ void k_dout(const double* atr, float* dout, size_t n, double c) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    dout[q] = (float)(c * atr[q]);
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 parameters
@@ -472,6 +647,7 @@ struct pcf_net {
   int device = 0;
   float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
   bf16 *w16 = nullptr, *wt16 = nullptr;
+  bf16 *w16_first = nullptr, *w16_last = nullptr;  // the first and last layers' weights in bf16
   float* z[5] = {};        // pre-activations of layers 1..5 (training)
   bf16* a[5] = {};         // activations of layers 1..5 (inference: a[0], a[1] alternate)
   float* out = nullptr;    // network output
@@ -496,12 +672,11 @@ int net_alloc(pcf_net* net, void** p, size_t bytes) {
 
 int nchunks(const pcf_net* net) { return (int)((net->d.n + kChunk - 1) / kChunk); }
 
-// forward pass over the FP32 input (n x 3); output in net->out
-// blocks of 4 wavefronts for the k_mid launches
-unsigned mid_blocks(const Dims& d) {
-  const size_t waves = (d.n / 32 + kMidTiles - 1) / kMidTiles;
-  return (unsigned)((waves + 3) / 4);
+// blocks of the k_brick launches
+BrickGrid brick_grid(const Dims& d) {
+  return BrickGrid{d.nx / 32, (d.ny + 3) / 4, (d.nz + 3) / 4};
 }
+unsigned brick_blocks(const BrickGrid& bg) { return (unsigned)(bg.tiles_x * bg.tiles_y * bg.tiles_z); }
 
 // out[0..size) = sum over the nc chunk partials (fixed order, two levels)
 void sum_chunks(pcf_net* net, int nc, int size, float* out) {
@@ -511,22 +686,25 @@ void sum_chunks(pcf_net* net, int nc, int size, float* out) {
   hipLaunchKernelGGL(k_sum_groups, dim3(blocks), dim3(256), 0, net->stream, net->groups, (int)ng, size, out);
 }
 
+// forward pass over the FP32 input (n x 3); output in net->out
 int net_forward(pcf_net* net, const float* input, hipStream_t s) {
   const Dims& d = net->d;
-  const unsigned vb = 256, vg = (unsigned)((d.n + vb - 1) / vb);
-  const unsigned mg = mid_blocks(d);
+  const BrickGrid bg = brick_grid(d);
+  const unsigned nb = brick_blocks(bg);
   const float* P = net->params;
   bf16* cur = net->a[0];
-  hipLaunchKernelGGL(k_first, dim3(vg), dim3(vb), 0, s, d, input, P + layer_offset(0),
-                     net->training ? net->z[0] : nullptr, cur);
+  hipLaunchKernelGGL((k_brick<KIND_FIRST, MID_FWD>), dim3(nb), dim3(256), 0, s, d, bg, (const void*)input,
+                     net->w16_first, P + layer_offset(0) + 27 * kC * 3, net->training ? net->z[0] : nullptr, cur,
+                     nullptr);
   for (int l = 1; l <= 4; ++l) {
     bf16* nxt = net->training ? net->a[l] : net->a[l & 1];
-    hipLaunchKernelGGL(k_mid<MID_FWD>, dim3(mg), dim3(256), 0, s, d, cur,
+    hipLaunchKernelGGL((k_brick<KIND_MID, MID_FWD>), dim3(nb), dim3(256), 0, s, d, bg, (const void*)cur,
                        net->w16 + (size_t)(l - 1) * 27 * kC * kC, P + layer_offset(l) + 27 * kC * kC,
                        net->training ? net->z[l] : nullptr, nxt, nullptr);
     cur = nxt;
   }
-  hipLaunchKernelGGL(k_last, dim3(vg), dim3(vb), 0, s, d, cur, P + layer_offset(5), net->out);
+  hipLaunchKernelGGL((k_brick<KIND_LAST, MID_FWD>), dim3(nb), dim3(256), 0, s, d, bg, (const void*)cur,
+                     net->w16_last, P + layer_offset(5) + 27 * kC, net->out, nullptr, nullptr);
   PCF_HIP(hipGetLastError());
   return PCF_OK;
 }
@@ -536,7 +714,7 @@ int net_backward_dout(pcf_net* net, const float* input) {
   const Dims& d = net->d;
   hipStream_t s = net->stream;
   const unsigned vb = 256, vg = (unsigned)((d.n + vb - 1) / vb);
-  const unsigned mg = mid_blocks(d);
+  const BrickGrid bg = brick_grid(d);
   const int nc = nchunks(net);
   const float* P = net->params;
   float* G = net->grads;
@@ -550,8 +728,9 @@ int net_backward_dout(pcf_net* net, const float* input) {
     sum_chunks(net, nc, kC, G + layer_offset(l) + 27 * kC * kC);
     hipLaunchKernelGGL(k_mid_weights, dim3(nc, 27), dim3(64), 0, s, d, net->dzb, net->a[l - 1], net->partials);
     sum_chunks(net, nc, 27 * kC * kC, G + layer_offset(l));
-    hipLaunchKernelGGL(k_mid<MID_DATA>, dim3(mg), dim3(256), 0, s, d, net->dzb,
-                       net->wt16 + (size_t)(l - 1) * 27 * kC * kC, nullptr, nullptr, nullptr, net->g);
+    hipLaunchKernelGGL((k_brick<KIND_MID, MID_DATA>), dim3(brick_blocks(bg)), dim3(256), 0, s, d, bg,
+                       (const void*)net->dzb, net->wt16 + (size_t)(l - 1) * 27 * kC * kC, nullptr, nullptr, nullptr,
+                       net->g);
   }
   // layer 1
   hipLaunchKernelGGL(k_gelu_back, dim3(nc), dim3(256), 0, s, d, net->g, net->z[0], net->dzb, net->partials);
@@ -597,6 +776,8 @@ int guess_input(pcf_state* st, const double* b, const double* prev, double prev_
 int net_pack(pcf_net* net) {
   hipLaunchKernelGGL(k_pack, dim3((27 * kC * kC + 255) / 256, 4), dim3(256), 0, net->stream, net->params, net->w16,
                      net->wt16);
+  hipLaunchKernelGGL(k_pack_ends, dim3((kC * kK1 + 255) / 256), dim3(256), 0, net->stream, net->params,
+                     net->w16_first, net->w16_last);
   PCF_HIP(hipGetLastError());
   return PCF_OK;
 }
@@ -622,6 +803,25 @@ int learned_guess(pcf_state* st, double dt, const double* prev, const double** x
                      prev, dt);
   PCF_HIP(hipGetLastError());
   *x0 = st->guess_x;
+  return PCF_OK;
+}
+
+int learned_stage(pcf_state* st, const double* v, double* y, const SolverScalars* sc) {
+  pcf_net* net = st->stage_net;
+  const Geo& g = st->g;
+  const size_t N = g.nv();
+  ProfScope prof(st, PCF_K_NET);
+  // the fourth dot slot of the partials: the solver's own dot products use
+  // slots 0-2 between a kernel that writes them and the one that reads them
+  double* part = st->partials + (size_t)3 * kPartialStride;
+  hipLaunchKernelGGL(k_sumsq, dim3(kStageGrid), dim3(kReduceBlock), 0, st->stream, v, N, part);
+  hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(1024), 0, st->stream, part, kStageGrid, st->stage_norm, (double)N);
+  hipLaunchKernelGGL(k_stage_input, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st->stream, g, v, y, st->jac,
+                     st->stage_in, st->stage_norm, sc);
+  int rc = net_forward(net, st->stage_in, st->stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_stage_output, dim3(2048), dim3(256), 0, st->stream, net->out, y, N, st->stage_norm, sc);
+  PCF_HIP(hipGetLastError());
   return PCF_OK;
 }
 
@@ -663,6 +863,34 @@ int pcf_set_guess_network(pcf_state* st, pcf_net* net) {
 }
 
 
+int pcf_set_stage_network(pcf_state* st, pcf_net* net, int32_t iterations) {
+  using namespace pcf;
+  if (!st || iterations < 0) return PCF_E_ARG;
+  if (net && ((size_t)net->d.nx != (size_t)st->g.nx || net->d.ny != st->g.ny || net->d.nz != st->g.nz)) {
+    set_error("pcf_set_stage_network: the network's grid differs from the state's");
+    return PCF_E_ARG;
+  }
+  if (net && (st->cfg.precision != PCF_PRECISION_MIXED || !st->cfg.preconditioner)) {
+    set_error("pcf_set_stage_network: the learned stage needs precision = mixed and a preconditioner "
+              "(a nonlinear stage needs the flexible form of the solution update)");
+    return PCF_E_CONFIG;
+  }
+  if (net && !st->stage_in) {
+    int rc = device_alloc(st, &st->stage_in, 3 * st->g.nv());
+    if (rc == PCF_OK) rc = device_alloc(st, &st->stage_norm, 1);
+    if (rc) return rc;
+  }
+  st->stage_net = net;
+  st->stage_iterations = net ? iterations : 0;
+  return PCF_OK;
+}
+
+int pcf_stage_fallbacks(const pcf_state* st, int32_t* count) {
+  if (!st || !count) return PCF_E_ARG;
+  *count = st->stage_fallbacks;
+  return PCF_OK;
+}
+
 int pcf_net_param_count(void) { return pcf::kParams; }
 
 int pcf_net_create(int32_t nx, int32_t ny, int32_t nz, int32_t training, pcf_net** out) {
@@ -687,6 +915,8 @@ int pcf_net_create(int32_t nx, int32_t ny, int32_t nz, int32_t training, pcf_net
   if ((rc = net_alloc(net, (void**)&net->params, kParams * sizeof(float)))) return fail(rc);
   if ((rc = net_alloc(net, (void**)&net->w16, 4 * 27 * kC * kC * sizeof(bf16)))) return fail(rc);
   if ((rc = net_alloc(net, (void**)&net->wt16, 4 * 27 * kC * kC * sizeof(bf16)))) return fail(rc);
+  if ((rc = net_alloc(net, (void**)&net->w16_first, kC * kK1 * sizeof(bf16)))) return fail(rc);
+  if ((rc = net_alloc(net, (void**)&net->w16_last, 27 * kC * sizeof(bf16)))) return fail(rc);
   if ((rc = net_alloc(net, (void**)&net->out, N * sizeof(float)))) return fail(rc);
   const int na = net->training ? 5 : 2;
   for (int l = 0; l < na; ++l)
@@ -783,6 +1013,84 @@ int pcf_net_train_step(pcf_net* net, const float* d_input, const float* d_target
   int rc = pcf_net_gradients(net, d_input, d_target, nullptr, loss);
   if (rc) return rc;
   return adam_step(net, lr);
+}
+
+// L = |A x - v|^2 / |v|^2 for x = base + |v| out, out the network's output
+// (x in st->train_buf[0]); the residual stays in train_buf[1]
+static int residual_loss(pcf_state* st, pcf_net* net, const double* v, const double* base, double vnorm, double vv,
+                         double* loss) {
+  using namespace pcf;
+  const size_t N = st->g.nv();
+  double *x = st->train_buf[0], *r = st->train_buf[1];
+  int rc = net_forward(net, st->guess_in, st->stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_compose, dim3(2048), dim3(256), 0, st->stream, net->out, base, x, N, vnorm);
+  PCF_HIP(hipGetLastError());
+  if ((rc = apply_operator(st, x, r, true))) return rc;
+  hipLaunchKernelGGL(k_sub, dim3(2048), dim3(256), 0, st->stream, r, v, r, N);
+  PCF_HIP(hipGetLastError());
+  double rr = 0.0;
+  if ((rc = dot(st, r, r, &rr))) return rc;
+  *loss = rr / vv;
+  return PCF_OK;
+}
+
+int pcf_net_train_residual(pcf_state* st, pcf_net* net, const double* d_v, const double* d_base, int32_t objective,
+                           int32_t inner, float lr, double* loss_base, double* loss_net) {
+  using namespace pcf;
+  if (!st || !net || !d_v || inner < 0 || (inner > 0 && !net->training)) return PCF_E_ARG;
+  if (objective != PCF_OBJECTIVE_RESIDUAL && objective != PCF_OBJECTIVE_ENERGY) {
+    set_error("pcf_net_train_residual: unknown objective");
+    return PCF_E_OPTION;
+  }
+  if ((size_t)net->d.nx != (size_t)st->g.nx || net->d.ny != st->g.ny || net->d.nz != st->g.nz) {
+    set_error("pcf_net_train_residual: the network's grid differs from the state's");
+    return PCF_E_ARG;
+  }
+  const size_t N = st->g.nv();
+  int rc;
+  if (!st->guess_in && (rc = device_alloc(st, &st->guess_in, 3 * N))) return rc;
+  for (double*& b : st->train_buf)
+    if (!b && (rc = device_alloc(st, &b, N))) return rc;
+  double vnorm = 0.0;
+  if ((rc = guess_input(st, d_v, d_base, 1.0, st->guess_in, &vnorm))) return rc;
+  const double vv = vnorm * vnorm * (double)N;
+  if (!(vv > 0.0)) {
+    set_error("pcf_net_train_residual: v is zero");
+    return PCF_E_ARG;
+  }
+  double* r = st->train_buf[1];
+  double* atr = st->train_buf[2];
+  if (loss_base) {  // the residual of base alone
+    double rr = vv;
+    if (d_base) {
+      if ((rc = apply_operator(st, d_base, r, true))) return rc;
+      hipLaunchKernelGGL(k_sub, dim3(2048), dim3(256), 0, st->stream, r, d_v, r, N);
+      PCF_HIP(hipGetLastError());
+      if ((rc = dot(st, r, r, &rr))) return rc;
+    }
+    *loss_base = rr / vv;
+  }
+  double loss = 0.0;
+  for (int it = 0; it < inner; ++it) {
+    if ((rc = residual_loss(st, net, d_v, d_base, vnorm, vv, &loss))) return rc;
+    if (objective == PCF_OBJECTIVE_RESIDUAL) {
+      if ((rc = apply_operator_transpose(st, r, atr))) return rc;
+      hipLaunchKernelGGL(k_dout, dim3(2048), dim3(256), 0, st->stream, atr, net->dout, N, 2.0 * vnorm / vv);
+    } else {  // the energy norm of the error under -A: its gradient in x is -(A x - v)
+      hipLaunchKernelGGL(k_dout, dim3(2048), dim3(256), 0, st->stream, r, net->dout, N, -2.0 * vnorm / vv);
+    }
+    PCF_HIP(hipGetLastError());
+    PCF_HIP(hipStreamSynchronize(st->stream));  // the backward pass runs on the network's stream
+    if ((rc = net_backward_dout(net, st->guess_in))) return rc;
+    if ((rc = adam_step(net, lr))) return rc;
+  }
+  if (loss_net) {
+    if ((rc = residual_loss(st, net, d_v, d_base, vnorm, vv, &loss))) return rc;
+    *loss_net = loss;
+  }
+  PCF_HIP(hipStreamSynchronize(st->stream));
+  return PCF_OK;
 }
 
 int pcf_net_train_step_dout(pcf_net* net, const float* d_input, const float* d_dout, float lr) {

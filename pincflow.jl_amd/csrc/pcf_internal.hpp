@@ -190,6 +190,16 @@ struct pcf_state {
   float *guess_in = nullptr, *guess_out = nullptr;
   double* guess_x = nullptr;
   int guess_fallbacks = 0;            // solves whose guess was dropped for a zero start
+  // learned flexible preconditioning stage of the mixed solve: the attached
+  // network, the number of leading BiCGSTAB iterations whose preconditioner
+  // applications it follows, its FP32 input and the device RMS norm of the
+  // vector being preconditioned
+  pcf_net* stage_net = nullptr;
+  int stage_iterations = 0;
+  float* stage_in = nullptr;
+  double* stage_norm = nullptr;
+  int stage_fallbacks = 0;            // solves rerun without the stage after it failed to converge
+  double* train_buf[3] = {};          // network training work: x, A x - v, A^T (A x - v)
   double* partials;                   // reduction partials
   pcf::SolverScalars* sc;             // device scalars
   pcf::SolverScalars* sc_host;        // pinned copy
@@ -287,6 +297,9 @@ int device_alloc(pcf_state* st, T** p, size_t count) {
 
 // ---- net.hip -----------------------------------------------------------------
 int learned_guess(pcf_state* st, double dt, const double* prev, const double** x0);
+// y += |v| net(v / |v|, y / |v|, J - 1) on the state's stream, |v| the RMS
+// norm computed on the device; kernels after convergence (sc->done) leave y.
+int learned_stage(pcf_state* st, const double* v, double* y, const SolverScalars* sc);
 
 // ---- explicit.hip --------------------------------------------------------
 int reconstruct(pcf_state* st);

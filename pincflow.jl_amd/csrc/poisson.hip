@@ -1253,8 +1253,14 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
   const double tol = n.tolerance_is_relative ? n.tolerance : n.tolerance / tolref;
   SolverScalars* sc = st->sc;
   const bool pc = n.preconditioner != 0;
-  auto prec = [&](const double* in, double* out, const SolverScalars* s) -> int {
-    return mixed ? precondition_mixed(st, in, out, s) : precondition(st, in, out, s);
+  // the learned stage follows the preconditioner in the first
+  // stage_iterations iterations (flexible BiCGSTAB: the textbook update below
+  // keeps x consistent with the residual recurrence whatever M^-1 does)
+  const bool stage = mixed && pc && st->stage_net && st->stage_iterations > 0;
+  auto prec = [&](const double* in, double* out, const SolverScalars* s, int iter) -> int {
+    int rc0 = mixed ? precondition_mixed(st, in, out, s) : precondition(st, in, out, s);
+    if (rc0 == PCF_OK && stage && iter <= st->stage_iterations) rc0 = learned_stage(st, in, out, s);
+    return rc0;
   };
   if (guess) {
     hipLaunchKernelGGL(k_bicg_guess, dim3(kVecGrid), dim3(256), 0, st->stream, guess, st->solution, N, guess_scale);
@@ -1300,7 +1306,7 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
   int rc = PCF_OK;
   for (int jb = 1; jb <= maxit; ++jb) {
     // v = A M^-1 p, <v, r0>
-    if ((rc = pc ? prec(st->bp, ybuf, sc) : PCF_OK)) break;
+    if ((rc = pc ? prec(st->bp, ybuf, sc, jb) : PCF_OK)) break;
     const double* vin = pc ? ybuf : st->bp;
     {
       ProfScope prof(st, PCF_K_APPLY_TOTAL_DOT);
@@ -1312,7 +1318,7 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
       hipLaunchKernelGGL(k_bicg_s, dim3(kVecGrid), dim3(256), 0, st->stream, st->br, st->bv, st->bs, N, sc);
     }
     // t = A M^-1 s, <t, s>, <t, t>
-    if ((rc = pc ? prec(st->bs, zbuf, sc) : PCF_OK)) break;
+    if ((rc = pc ? prec(st->bs, zbuf, sc, jb) : PCF_OK)) break;
     const double* sin2 = pc ? zbuf : st->bs;
     {
       ProfScope prof(st, PCF_K_APPLY_TOTAL_DOT);
@@ -1436,3 +1442,4 @@ int correct(pcf_state* st, double dt, double rf) {
 }
 
 }  // namespace pcf
+

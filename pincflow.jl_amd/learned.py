@@ -273,6 +273,84 @@ def train_residual(net: GuessNetwork, states, steps: int, inner: int = 6, lr: fl
     return out
 
 
+OBJECTIVES = {"residual": 0, "energy": 1}
+
+
+def residual_step(net: GuessNetwork, st: State, d_v: DeviceBuffer, d_base: DeviceBuffer | None, inner: int,
+                  lr: float, objective: str = "residual") -> Tuple[float, float]:
+    """`inner` Adam steps on the device (pcf_net_train_residual) for
+    x = base + |v| net(v / |v|, base / |v|, J - 1), A the state's current
+    operator, on the residual |A x - v|^2 / |v|^2 or the energy norm of the
+    error (objective).  Returns (residual loss of base, residual loss with
+    the network after the steps); inner = 0 only evaluates."""
+    lb, ln = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    rc = st.lib.pcf_net_train_residual(st.h, net.h, ctypes.c_void_p(d_v.ptr),
+                                       ctypes.c_void_p(d_base.ptr) if d_base is not None else None,
+                                       OBJECTIVES[objective], int(inner), ctypes.c_float(lr), ctypes.byref(lb),
+                                       ctypes.byref(ln))
+    if rc:
+        raise PinCFlowError(rc, st.lib.pcf_last_error().decode())
+    return lb.value, ln.value
+
+
+def train_online(states, steps: int, guess: GuessNetwork | None = None, stage: GuessNetwork | None = None,
+                 inner: int = 4, lr: float = 1e-4, stage_lr: float | None = None, objective: str = "residual",
+                 log=None):
+    """Online training on the solver's own solves, all on the device.  Every
+    time step of each state after the first gives the solve of the step's
+    second implicit stage (its operator A and right-hand side b stay in the
+    state) and `inner` Adam steps for each network given:
+
+    * the initial guess: v = b, base = the warm start (the previous step's
+      solution rescaled to this dt), so x0 = warm + |b| net(b/|b|, warm/|b|, J-1);
+    * the preconditioning stage: v = b - A warm, the residual the solve's first
+      preconditioner application sees, base = M^-1 v (the state's configured
+      preconditioner), so the stage learns to correct what M^-1 leaves.
+
+    `objective` is "residual" or "energy" (pcf_net_train_residual).  Returns
+    per solve {"guess": (L warm, L learned), "stage": (L M^-1, L M^-1 + net)},
+    the residual losses."""
+    hist = []
+    stage_lr = lr if stage_lr is None else stage_lr
+    for st in states:
+        n = st.nx * st.ny * st.nz
+        fcs = _interior_fcscal(st).ravel(order="F")
+        d_b, d_w, d_r, d_y = (DeviceBuffer(8 * n) for _ in range(4))
+        prev, prev_dt = None, None
+        for _ in range(steps):
+            info = st.step()
+            b = np.asfortranarray(st.get_vector("RHS")).ravel(order="F")
+            x = np.asfortranarray(st.get_vector("SOLUTION")).ravel(order="F") * fcs
+            if prev is not None:
+                warm = prev * (info.dt / prev_dt)
+                rec = {}
+                d_b.upload(b)
+                d_w.upload(warm)
+                if guess is not None:
+                    rec["guess"] = residual_step(guess, st, d_b, d_w, inner, lr, objective)
+                if stage is not None:
+                    st.apply_operator_device(d_w.ptr, d_r.ptr, True)
+                    d_r.upload(b - d_r.download(np.float64, (n,)))
+                    rc = st.lib.pcf_apply_preconditioner(st.h, ctypes.c_void_p(d_r.ptr), ctypes.c_void_p(d_y.ptr))
+                    if rc:
+                        raise PinCFlowError(rc, st.lib.pcf_last_error().decode())
+                    rec["stage"] = residual_step(stage, st, d_r, d_y, inner, stage_lr, objective)
+                hist.append(rec)
+                if log:
+                    log(f"solve {len(hist)}: " + ", ".join(f"{k} {a:.4e} -> {c:.4e}" for k, (a, c) in rec.items()))
+            prev, prev_dt = x, info.dt
+    return hist
+
+
+def attach_stage(st: State, net: GuessNetwork, iterations: int):
+    """Use net as the learned flexible-BiCGSTAB preconditioning stage of this
+    state's mixed solves, in the first `iterations` iterations of each."""
+    rc = st.lib.pcf_set_stage_network(st.h, net.h, int(iterations))
+    if rc:
+        raise PinCFlowError(rc, st.lib.pcf_last_error().decode())
+    st._stage_net = net
+
+
 def residual_loss(net: GuessNetwork, st: State, steps: int) -> List[Tuple[float, float]]:
     """Per solve of `steps` time steps of st (after the first): the residual
     loss |A x0 - b|^2 / |b|^2 of the warm start and of the network's guess."""

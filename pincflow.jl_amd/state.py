@@ -218,6 +218,13 @@ class State:
         self._check(self.lib.pcf_guess_fallbacks(self.h, ctypes.byref(n)))
         return n.value
 
+    def stage_fallbacks(self) -> int:
+        """Solves rerun without the learned preconditioning stage after they
+        failed to converge with it."""
+        n = ctypes.c_int32(0)
+        self._check(self.lib.pcf_stage_fallbacks(self.h, ctypes.byref(n)))
+        return n.value
+
     # ---- time and restart ------------------------------------------------------
     def time(self):
         """(model time in s, steps taken)."""

@@ -1,10 +1,10 @@
 """The initial-guess network on the device against its numpy restatement
 (tests/cnn_reference.py, itself pinned by finite differences).
 
-The device keeps the 32 -> 32 weights, the activations and the backward
+The device keeps the weights, the input, the activations and the backward
 signal of the matrix-core layers in bf16 and accumulates in FP32.  The
-forward check emulates the bf16 rounding of weights and activations in
-numpy; FP32 summation order then still flips the odd activation by one bf16
+forward check emulates the bf16 rounding of input, weights and activations
+in numpy; FP32 summation order then still flips the odd activation by one bf16
 step (0.4 %), which FORWARD_RTOL allows for.  The
 gradient check compares against the exact FP64 gradient: bf16 operands in
 the backward GEMMs leave a relative error of a few 1e-3 (GRAD_RTOL, in the
@@ -56,11 +56,11 @@ def from_device_layout(v, shape):
 
 
 def forward_bf16(params, x):
-    """cnn_reference.forward with the device's bf16 storage emulated."""
-    a = x
+    """cnn_reference.forward with the device's bf16 storage emulated (the
+    input, every layer's weights and the activations; biases stay FP32)."""
+    a = bf16(x).astype(np.float64)
     for n, (w, b) in enumerate(params):
-        if 1 <= n <= 4:
-            w = bf16(w).astype(np.float64)
+        w = bf16(w).astype(np.float64)
         z = cnn.conv(a, w, b)
         a = z if n == len(params) - 1 else bf16(cnn.gelu(z)).astype(np.float64)
     return a
