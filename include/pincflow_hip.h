@@ -396,6 +396,16 @@ int pcf_apply_preconditioner(pcf_state* st, const double* d_in, double* d_out);
 int pcf_apply_bicgstab(pcf_state* st, double tolref, int32_t* niter);
 /* compute_global_dot_product (compute_global_dot_product.jl:20-36), device pointers. */
 int pcf_dot(pcf_state* st, const double* d_a, const double* d_b, double* result);
+/* pcf_dot over n elements without a state, allocating nothing: no reference
+ * counterpart, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  The result (the bits pcf_dot gives
+ * for the same n) is written to the device pointer d_result on `stream` (a
+ * hipStream_t; null = the default stream); nothing is synchronized.  The
+ * block partials live in a 16 KiB array the library holds on each device
+ * until the process ends; calls that overlap on one device race on it.
+ * 0 <= n <= PCF_DOT_NOALLOC_MAX_N, else PCF_E_ARG with a message. */
+#define PCF_DOT_NOALLOC_MAX_N ((int64_t)1 << 34)
+int pcf_dot_noalloc(const double* d_a, const double* d_b, int64_t n, double* d_result, void* stream);
 
 /* ---- host <-> device copies (test and output support) ------------------- */
 int pcf_get_field(pcf_state* st, int field, double* host_out);

@@ -199,6 +199,7 @@ SIGNATURES = [
     ("pcf_apply_preconditioner", _I, [_P, _P, _P]),
     ("pcf_apply_bicgstab", _I, [_P, _D, _PI32]),
     ("pcf_dot", _I, [_P, _P, _P, _PD]),
+    ("pcf_dot_noalloc", _I, [_P, _P, ctypes.c_int64, _P, _P]),
     ("pcf_get_field", _I, [_P, _I, _PD]),
     ("pcf_set_field", _I, [_P, _I, _PD]),
     ("pcf_get_vector", _I, [_P, _I, _PD]),

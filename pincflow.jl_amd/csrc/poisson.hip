@@ -1443,3 +1443,63 @@ int correct(pcf_state* st, double dt, double rf) {
 
 }  // namespace pcf
 
+// ---------------------------------------------------------------------------
+// pcf_dot_noalloc: pcf_dot's kernels with their block partials in a
+// file-scope device array instead of the state's buffer, for callers that
+// cannot allocate device memory (graph capture, serving loops).  The array is
+// addressed through the pointer hipGetSymbolAddress gives on each device; the
+// array's host-side name is not a device address.
+// ---------------------------------------------------------------------------
+#include <mutex>
+
+static __device__ __align__(256) double scratch[pcf::kVecGrid];  // 16 KiB per device
+
+namespace {
+
+constexpr int kNoallocMaxDevices = 64;
+
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kNoallocMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call on that device.
+int scratch_on_current_device(double** out) {
+  int dev = -1;
+  PCF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoallocMaxDevices) {
+    pcf::set_error("pcf_dot_noalloc: device index out of range");
+    return PCF_E_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    PCF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PCF_OK;
+}
+
+}  // namespace
+
+extern "C" int pcf_dot_noalloc(const double* d_a, const double* d_b, int64_t n, double* d_result, void* stream) {
+  using namespace pcf;
+  if (!d_a || !d_b || !d_result) {
+    set_error("pcf_dot_noalloc: null pointer argument");
+    return PCF_E_ARG;
+  }
+  if (n < 0 || n > PCF_DOT_NOALLOC_MAX_N) {
+    set_error("pcf_dot_noalloc: n = " + std::to_string((long long)n) + " outside [0, " +
+              std::to_string((long long)PCF_DOT_NOALLOC_MAX_N) + "]");
+    return PCF_E_ARG;
+  }
+  double* partials = nullptr;
+  int rc = scratch_on_current_device(&partials);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // the kernels and launch shape of pcf_dot, so the result has its bits
+  hipLaunchKernelGGL(k_dot, dim3(kVecGrid), dim3(kReduceBlock), 0, s, d_a, d_b, (size_t)n, partials);
+  hipLaunchKernelGGL(k_finalize_one, dim3(1), dim3(1024), 0, s, partials, kVecGrid, d_result);
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
