@@ -19,6 +19,7 @@ from __future__ import annotations
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -54,6 +55,9 @@ def parse():
     p.add_argument("--nx", type=int, default=256)
     p.add_argument("--nz", type=int, default=128)
     p.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    p.add_argument("--flux-precision", default=None, choices=["fp64", "mixed"],
+                   help="arithmetic of the RK stages' fluxes (default: fp64 with --precision fp64, mixed (FP32) "
+                        "with --precision mixed)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--cpu-sample-nx", type=int, default=128)
@@ -69,19 +73,19 @@ def parse():
     p.add_argument("--mg-coarse-steps", type=int, default=20, help="multigrid steps on the coarsest level")
     p.add_argument("--storage", default="fp32", choices=["fp32", "bf16"],
                    help="mixed precision: storage of the finest level's coefficients in the preconditioner")
-    p.add_argument("--train-runs", type=int, default=2, help="learned guess: training runs (random hills)")
-    p.add_argument("--train-steps", type=int, default=8, help="learned guess: time steps per training run")
+    p.add_argument("--train-runs", type=int, default=3, help="learned networks: training runs (random hills)")
+    p.add_argument("--train-steps", type=int, default=12, help="learned networks: time steps per training run")
     p.add_argument("--train-epochs", type=int, default=10, help="learned guess: passes over the samples (error loss)")
     p.add_argument("--train-loss", default="residual", choices=["residual", "error"],
                    help="learned guess: residual |A x0 - b|^2 (online, --train-inner Adam steps per solve) or "
                         "the squared error of the solution change")
-    p.add_argument("--train-inner", type=int, default=6, help="learned guess: Adam steps per solve (residual loss)")
+    p.add_argument("--train-inner", type=int, default=4, help="learned networks: Adam steps per solve")
     p.add_argument("--train-objective", default="energy", choices=["residual", "energy"],
                    help="learned networks: the residual |A x - v|^2 or the energy norm of the error under -A")
     p.add_argument("--stage-iterations", type=int, default=1,
                    help="learned stage: leading BiCGSTAB iterations whose preconditioner applications it follows")
-    p.add_argument("--train-lr", type=float, default=1e-6,
-                   help="learned guess: Adam learning rate of the residual loss (1e-5 and above diverge at 256^2 x 128)")
+    p.add_argument("--train-lr", type=float, default=1e-4,
+                   help="learned networks: Adam learning rate")
     p.add_argument("--no-variants", action="store_true",
                    help="plain FP64 run: skip the mixed-precision, multigrid and learned lines")
     p.add_argument("--no-fp64-check", action="store_true",
@@ -102,7 +106,7 @@ def workload(pcf, args, iterations, precision):
 
 
 def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False, storage=None, mg_coarse=None,
-               relative=False, stage_net=None):
+               relative=False, stage_net=None, fluxes="fp64"):
     """Setup, initial cleaning, warmup, then the timed steps of one method,
     with HIP events around the launches of the operator apply only (its
     average duration gives the roofline figure; events around every launch
@@ -111,6 +115,7 @@ def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False
     nl = workload(pcf, args, args.steps + args.warmup + 1, precision)
     nl.poisson.initial_guess = guess
     nl.poisson.preconditioner_type = preconditioner
+    nl.discretization.flux_precision = fluxes
     nl.poisson.tolerance_is_relative = relative
     nl.poisson.preconditioner_iterations = args.pc_iterations
     nl.poisson.dtau = args.dtau
@@ -118,7 +123,8 @@ def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False
         nl.poisson.mg_smoothing_steps = args.mg_smoothing_steps
         nl.poisson.mg_coarse_steps = mg_coarse or args.mg_coarse_steps
         nl.poisson.preconditioner_storage = storage or args.storage
-    tag = f"{precision}/{preconditioner}/{nl.poisson.preconditioner_storage}/{guess}" + ("+stage" if stage_net else "")
+    tag = (f"{precision}/{preconditioner}/{nl.poisson.preconditioner_storage}/{guess}" + ("+stage" if stage_net else "")
+           + ("/fp32-fluxes" if fluxes == "mixed" else ""))
     t0 = time.perf_counter()
     st = pcf.State(nl)
     if net is not None:
@@ -192,8 +198,9 @@ def main():
         training = train_guess(pcf, args)
         net = training.pop("net", None) or training.pop("guess")
         training.pop("stage", None)
+    main_fluxes = args.flux_precision or ("fp64" if args.precision == "fp64" else "mixed")
     main_run = run_method(pcf, args, args.precision, args.preconditioner, args.initial_guess, net=net,
-                          keep=True)
+                          keep=True, fluxes=main_fluxes)
     st = main_run["state"]
     n_cells = args.nx * args.nx * args.nz
     prof = main_run["prof"]
@@ -221,28 +228,36 @@ def main():
         # coarsest-level steps, initial guess); the FP64 BiCGSTAB and its 1e-8
         # criterion are the same in all
         vstage = None
-        for pc, storage, coarse, guess, staged in (("line_relaxation", "fp32", 20, "zero", False),
-                                                   ("line_relaxation", "bf16", 20, "zero", False),
-                                                   ("multigrid", "fp32", 20, "zero", False),
-                                                   ("multigrid", "bf16", 2, "zero", False),
-                                                   ("multigrid", "bf16", 2, "previous", False),
-                                                   ("multigrid", "bf16", 2, "learned", False),
-                                                   ("multigrid", "bf16", 2, "previous", True)):
+        # (..., whether the learned stage runs, the RK stages' flux arithmetic)
+        for pc, storage, coarse, guess, staged, fluxes in (("line_relaxation", "fp32", 20, "zero", False, "fp64"),
+                                                           ("line_relaxation", "bf16", 20, "zero", False, "fp64"),
+                                                           ("multigrid", "fp32", 20, "zero", False, "fp64"),
+                                                           ("multigrid", "bf16", 2, "zero", False, "fp64"),
+                                                           ("multigrid", "bf16", 2, "previous", False, "fp64"),
+                                                           ("multigrid", "bf16", 2, "previous", False, "mixed"),
+                                                           ("multigrid", "bf16", 2, "learned", False, "mixed"),
+                                                           ("multigrid", "bf16", 2, "previous", True, "mixed")):
             if (guess == "learned" or staged) and vtrain is None:
                 vtrain = train_guess(pcf, args, preconditioner=pc)
                 vnet = vtrain.pop("net", None) or vtrain.pop("guess")
                 vstage = vtrain.pop("stage", None)
-            if staged and vstage is None:
+            if staged and (vstage is None or not stage_usable(vtrain)):
+                # a stage whose held-out output is not finite or blows the
+                # residual up would only send every solve to the fallback
+                variants.append({"method": "learned stage skipped: its held-out residual ratios "
+                                           f"{vtrain.get('held_out_stage_residual_vs_multigrid')}"})
                 continue
             r = run_method(pcf, args, "mixed", pc, guess, net=vnet if guess == "learned" else None, keep=True,
-                           storage=storage, mg_coarse=coarse, stage_net=vstage if staged else None)
+                           storage=storage, mg_coarse=coarse, stage_net=vstage if staged else None, fluxes=fluxes)
             desc = pc.replace("_", " ") + " preconditioner"
             if pc == "multigrid":
                 desc += f" ({args.mg_smoothing_steps} smoothing steps, {coarse} coarsest-level steps)"
             if staged:
                 desc += (f" followed by the learned flexible-BiCGSTAB stage in the first {args.stage_iterations} "
                          "iteration(s)")
-            entry = {"method": f"FP64 BiCGSTAB, {desc} in FP32 with {storage} coefficients, {guess} initial guess",
+            flux_desc = "; RK-stage fluxes in FP32" if fluxes == "mixed" else ""
+            entry = {"method": f"FP64 BiCGSTAB, {desc} in FP32 with {storage} coefficients, {guess} initial guess"
+                               + flux_desc,
                      "value": round(r["value"], 4), "ms_per_step": round(r["ms_per_step"], 2),
                      "speedup_vs_plain": round(r["value"] / main_run["value"], 3),
                      "bicgstab_iterations_per_solve": round(r["iterations_per_solve"], 1),
@@ -301,14 +316,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if plain else "f64 (preconditioner f32)",
+        "dtype": "f64" if plain else ("f64 (preconditioner f32, RK-stage fluxes f32)" if main_fluxes == "mixed" else "f64 (preconditioner f32)"),
         "data": "synthetic",
         "config": {"workload": (("compressible " if args.model == "Compressible" else "pseudo-incompressible ") +
                                 ("flow over 16 seeded Gaussian hills " if args.workload == "gaussian_hills"
                                  else "mountain wave ") +
                                 f"{args.nx}x{args.nx}x{args.nz}, " +
                                 (f"FP64 BiCGSTAB with the {args.preconditioner.replace('_', ' ')} preconditioner "
-                                 f"evaluated in FP32, initial guess {args.initial_guess}" if not plain
+                                 f"evaluated in FP32, initial guess {args.initial_guess}, RK-stage fluxes in "
+                                 f"{'FP32' if main_fluxes == 'mixed' else 'FP64'}" if not plain
                                  else "plain preconditioned BiCGSTAB (FP64)")),
                    "cells": n_cells, "poisson_tolerance": 1e-8,
                    "preconditioner_iterations": args.pc_iterations, "dtau": args.dtau,
@@ -407,6 +423,11 @@ def train_guess(pcf, args, preconditioner=None):
     return {"net": infer, "samples": len(samples), "epochs": args.train_epochs, "loss_first": losses[0],
             "loss_last": losses[-1], "held_out_error_vs_warm_start": held_out, "data_s": round(t_data, 1),
             "train_s": round(t_train, 1)}
+
+
+def stage_usable(summary) -> bool:
+    r = summary.get("held_out_stage_residual_vs_multigrid") or []
+    return bool(r) and all(math.isfinite(x) and x < 10.0 for x in r)
 
 
 def train_guess_residual(pcf, args, preconditioner=None):

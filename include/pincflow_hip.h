@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCF_ABI_VERSION 6
+#define PCF_ABI_VERSION 7
 
 /* ---- return codes ------------------------------------------------------ */
 #define PCF_OK 0
@@ -227,6 +227,14 @@ typedef struct pcf_config {
   /* AtmosphereNamelist.thermal_conductivity (AtmosphereNamelist.jl:113),
    * m^2/s at the surface; heat conduction of the compressible model */
   double thermal_conductivity;
+  /* Addition of this library: arithmetic of the explicit stage's fluxes.
+   * PCF_PRECISION_FP64 (reference) or PCF_PRECISION_MIXED: the
+   * reconstruction inputs and stress components stored in FP32 and the MUSCL
+   * reconstruction, fluxes and flux divergences of pcf_step's RK stages
+   * evaluated in FP32; the updates, the other fields and pcf_compute_fluxes
+   * stay FP64. */
+  int32_t flux_precision;                 /* enum pcf_precision           */
+  int32_t pad2;
 } pcf_config;
 
 /* Fields reachable through pcf_get_field / pcf_set_field (haloed, FP64). */

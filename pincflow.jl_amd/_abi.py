@@ -13,7 +13,7 @@ from pathlib import Path
 PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "lib" / "libpincflow_hip.so"
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 MAX_HILLS = 32
 
 # return codes
@@ -149,6 +149,8 @@ class PcfConfig(ctypes.Structure):
         ("tracer_rx", ctypes.c_double),
         ("tracer_ry", ctypes.c_double),
         ("thermal_conductivity", ctypes.c_double),
+        ("flux_precision", ctypes.c_int32),
+        ("pad2", ctypes.c_int32),
     ]
 
 

@@ -46,6 +46,13 @@ int alloc(pcf_state* st, double** p, size_t n) {
   return PCF_OK;
 }
 
+int alloc32(pcf_state* st, float** p, size_t n) {
+  int rc = device_alloc(st, p, n);
+  if (rc) return rc;
+  PCF_HIP(hipMemsetAsync(*p, 0, n * sizeof(float), st->stream));
+  return PCF_OK;
+}
+
 double* field_ptr(pcf_state* st, int f) {
   switch (f) {
     case PCF_F_RHO: return st->rho;
@@ -147,21 +154,19 @@ int explicit_lhs(pcf_state* st, double dt, double time) {
     TRY(stage_inputs(st));
     st->lhs_div_valid[0] = st->lhs_div_valid[1] = false;  // the stress arrays were rewritten
     TRY(save_backups(st, true, false, true, true, false));
-    TRY(stage_update(st, dt, m, PCF_VAR_RHO));
-    TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_RHO));
-    TRY(stage_update(st, dt, m, PCF_VAR_RHOP));
-    TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_RHOP));
-    TRY(stage_update(st, dt, m, PCF_VAR_P));  // no-ops outside the compressible model
-    TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_P));
-    TRY(stage_update(st, dt, m, PCF_VAR_CHI));  // no-ops without a tracer
-    TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_CHI));
+    // the sponges that the fused kernels do not apply themselves follow
+    // their variable's update (the wind's after all three updates), in the
+    // reference's order
+    for (int var : {PCF_VAR_RHO, PCF_VAR_RHOP, PCF_VAR_P, PCF_VAR_CHI}) {  // P: compressible; CHI: tracer
+      TRY(stage_update(st, dt, m, var, sdt));
+      if (!sponge_fused(st, var)) TRY(apply_lhs_sponge(st, sdt, time, var));
+    }
     TRY(set_boundaries_predictands(st));
-    TRY(stage_update(st, dt, m, PCF_VAR_U));
-    TRY(stage_update(st, dt, m, PCF_VAR_V));
-    TRY(stage_update(st, dt, m, PCF_VAR_W));
-    TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_U));
-    TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_V));
-    TRY(apply_lhs_sponge(st, sdt, time, PCF_VAR_W));
+    TRY(stage_update(st, dt, m, PCF_VAR_U, sdt));
+    TRY(stage_update(st, dt, m, PCF_VAR_V, sdt));
+    TRY(stage_update(st, dt, m, PCF_VAR_W, sdt));
+    for (int var : {PCF_VAR_U, PCF_VAR_V, PCF_VAR_W})
+      if (!sponge_fused(st, var)) TRY(apply_lhs_sponge(st, sdt, time, var));
     TRY(set_boundaries_predictands(st));
   }
   TRY(synchronize_compressible_atmosphere(st, st->p));
@@ -333,6 +338,14 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
       if ((rc = alloc(st, p, NH))) return fail(rc);
   for (int q = 0; q < 6; ++q)
     if ((rc = alloc(st, &st->stress[q], NH))) return fail(rc);
+  if (cfg->flux_precision == PCF_PRECISION_MIXED)
+    // zeroed like the FP64 copies: the flux stencils read entries outside
+    // the range the input kernels write (phi two levels below the ground),
+    // which hold zeros there as in the reference's freshly allocated arrays
+    for (int q = 0; q < 6; ++q) {
+      if ((q < 5 || st->tracer) && (rc = alloc32(st, &st->phi32[q], NH))) return fail(rc);
+      if ((rc = alloc32(st, &st->stress32[q], NH))) return fail(rc);
+    }
   for (int q = 0; q < T_COUNT; ++q)
     if ((rc = alloc(st, &st->T.a[q], NV))) return fail(rc);
   double** vecs[] = {&st->rhs, &st->solution, &st->bp, &st->br0, &st->br, &st->bs, &st->bt, &st->bv,

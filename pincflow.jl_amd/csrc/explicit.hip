@@ -22,76 +22,80 @@ namespace {
 
 enum { V_RHO = 0, V_RHOP, V_U, V_V, V_W, V_CHI, V_P };
 
-struct FluxArgs {
+// Flux inputs in the arithmetic type T of the flux evaluation: double (the
+// reference's arithmetic), or float for flux_precision = MIXED, where the
+// reconstruction inputs and stress components are stored in FP32 and the
+// MUSCL reconstruction, the fluxes and their divergence run in FP32 (the
+// other fields are read in FP64 and rounded on load).
+template <class T>
+struct FluxArgsT {
   Fields F;
-  const double* phi[6];
+  const T* phi[6];
   const double* u0;
   const double* v0;
   const double* w0;
-  const double* st[6];  // stress 11, 12, 13, 22, 23, 33
+  const T* st[6];  // stress 11, 12, 13, 22, 23, 33
   double* flux[7][3];
   Geo g;
-  double inv_re;        // 1 / re
+  T inv_re;        // 1 / re
   int viscous;
 };
-
-__device__ __forceinline__ Fields fields_at(const FluxArgs& A) { return A.F; }
+typedef FluxArgsT<double> FluxArgs;
 
 // ---------------------------------------------------------------------------
-// reconstruction inputs (reconstruct!.jl:126-286) on layers k0-1..k1+1
+// reconstruction inputs (reconstruct!.jl:126-286): phi of Rho, RhoP, U, V
+// (reconstruct!.jl:154-240) and the tracer (reconstruct!.jl:292-315), the
+// Cartesian vertical wind compute_vertical_wind (compute_vertical_wind.jl:
+// 27-63) and phi of W, Cartesian w times the edge density over the edge P
+// (reconstruct!.jl:242-286)
 // ---------------------------------------------------------------------------
-__global__ void k_phi(Fields F, Geo g, double* phr, double* phrp, double* phu, double* phv, int pi,
-                      const double* chi, double* phchi) {
+// k_phi, k_wcart and k_phi_w in one launch over i, j >= 1, k in 1..nzz-1:
+// the reconstruction inputs on layers k0-1..k1+1, the Cartesian vertical
+// wind where k_wcart computes it (i, j >= 2) and phi of W from it (the same
+// expressions; the fields they share are read once)
+template <class S>
+__global__ void k_inputs(Fields F, Geo g, S* phr, S* phrp, S* phu, S* phv, int pi, const double* chi, S* phchi,
+                         double* wc, S* phw) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y;
-  const int k = g.k0 - 1 + blockIdx.z;
-  if (i > g.nxx) return;
-  const size_t c = g.h(i, j, k);
-  const double pb = F.pbar[c];
-  if (pi) phr[c] = F.rho[c] / pb;
-  phrp[c] = F.rhop[c] / pb;  // reconstruct!.jl:154-176
-  if (phchi) phchi[c] = chi[c] / pb;  // reconstruct!.jl:292-315
-  if (i < g.nxx) {           // reconstruct!.jl:178-208
-    const size_t r = g.h(i + 1, j, k);
-    double rhoedge = 0.5 * (F.rho[c] + F.rho[r] + F.rhobar[c] + F.rhobar[r]);
-    double pedge = 0.5 * (pb + F.pbar[r]);
-    phu[c] = F.u[c] * rhoedge / pedge;
-  }
-  if (j < g.nyy) {  // reconstruct!.jl:210-240
-    const size_t f = g.h(i, j + 1, k);
-    double rhoedge = 0.5 * (F.rho[c] + F.rho[f] + F.rhobar[c] + F.rhobar[f]);
-    double pedge = 0.5 * (pb + F.pbar[f]);
-    phv[c] = F.v[c] * rhoedge / pedge;
-  }
-}
-
-// Cartesian vertical wind compute_vertical_wind(i, j, k) on i >= 2, j >= 2,
-// k <= nzz - 1 (compute_vertical_wind.jl:27-63).
-__global__ void k_wcart(Fields F, Geo g, double* wc) {
-  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y;
   const int k = 1 + blockIdx.z;
   if (i > g.nxx) return;
-  wc[g.h(i, j, k)] = vertical_wind(F, g, i, j, k);
-}
-
-// phi for W (reconstruct!.jl:242-286): Cartesian w times the edge density
-// over the edge P, on layers k0-1..k1+1 and i, j >= 2.
-__global__ void k_phi_w(Fields F, Geo g, const double* wc, double* phw) {
-  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y;
-  const int k = g.k0 - 1 + blockIdx.z;
-  if (i > g.nxx) return;
-  const size_t c = g.h(i, j, k), up = g.h(i, j, k + 1);
-  const double jc = F.jac[c], ju = F.jac[up];
-  double rhoedgeu = (ju * (F.rho[c] + F.rhobar[c]) + jc * (F.rho[up] + F.rhobar[up])) / (jc + ju);
-  double pedgeu = (ju * F.pbar[c] + jc * F.pbar[up]) / (jc + ju);
-  phw[c] = wc[c] * (rhoedgeu / pedgeu);
+  const size_t c = g.h(i, j, k);
+  const bool layer = k >= g.k0 - 1 && k <= g.k1 + 1;
+  if (layer) {
+    const double pb = F.pbar[c];
+    if (pi) phr[c] = (S)(F.rho[c] / pb);
+    phrp[c] = (S)(F.rhop[c] / pb);
+    if (phchi) phchi[c] = (S)(chi[c] / pb);
+    if (i < g.nxx) {
+      const size_t r = g.h(i + 1, j, k);
+      double rhoedge = 0.5 * (F.rho[c] + F.rho[r] + F.rhobar[c] + F.rhobar[r]);
+      double pedge = 0.5 * (pb + F.pbar[r]);
+      phu[c] = (S)(F.u[c] * rhoedge / pedge);
+    }
+    if (j < g.nyy) {
+      const size_t f = g.h(i, j + 1, k);
+      double rhoedge = 0.5 * (F.rho[c] + F.rho[f] + F.rhobar[c] + F.rhobar[f]);
+      double pedge = 0.5 * (pb + F.pbar[f]);
+      phv[c] = (S)(F.v[c] * rhoedge / pedge);
+    }
+  }
+  if (i < 2 || j < 2) return;
+  const double w = vertical_wind(F, g, i, j, k);
+  wc[c] = w;
+  if (layer) {
+    const size_t up = g.h(i, j, k + 1);
+    const double jc = F.jac[c], ju = F.jac[up];
+    double rhoedgeu = (ju * (F.rho[c] + F.rhobar[c]) + jc * (F.rho[up] + F.rhobar[up])) / (jc + ju);
+    double pedgeu = (ju * F.pbar[c] + jc * F.pbar[up]) / (jc + ju);
+    phw[c] = (S)(w * (rhoedgeu / pedgeu));
+  }
 }
 
 // compute_stress_tensor.jl:55-164: the six distinct components at one cell.
-__global__ void k_stress(Fields F, Geo g, const double* wc, double* s11, double* s12, double* s13, double* s22,
-                         double* s23, double* s33, Range3 R) {
+template <class S>
+__global__ void k_stress(Fields F, Geo g, const double* wc, S* s11, S* s12, S* s13, S* s22, S* s23, S* s33,
+                         Range3 R) {
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
   const double *u = F.u, *v = F.v, *w = F.w, *jac = F.jac;
@@ -122,191 +126,234 @@ __global__ void k_stress(Fields F, Geo g, const double* wc, double* s11, double*
                 (jacedgeu * w[c] - jacedged * w[g.h(i, j, k - 1)]) / dz) /
                jac[c];
   double dwc = wc[c] - wc[g.h(i, j, k - 1)];
-  s11[c] = 2.0 * (u[c] - u[g.h(i - 1, j, k)]) / dx + F.m13[c] * (uu - ud) / dz - div;
-  s12[c] = 0.5 * (uf - ub) / dy + 0.5 * F.m23[c] * (uu - ud) / dz + 0.5 * (vr - vl) / dx +
-           0.5 * F.m13[c] * (vu - vd) / dz;
-  s13[c] = 0.5 * (uu - ud) / dz / jac[c] + 0.5 * (wr - wl) / dx + F.m13[c] * dwc / dz;
-  s22[c] = 2.0 * (v[c] - v[g.h(i, j - 1, k)]) / dy + F.m23[c] * (vu - vd) / dz - div;
-  s23[c] = 0.5 * (vu - vd) / dz / jac[c] + 0.5 * (wf - wb) / dy + F.m23[c] * dwc / dz;
-  s33[c] = 2.0 * dwc / dz / jac[c] - div;
+  s11[c] = (S)(2.0 * (u[c] - u[g.h(i - 1, j, k)]) / dx + F.m13[c] * (uu - ud) / dz - div);
+  s12[c] = (S)(0.5 * (uf - ub) / dy + 0.5 * F.m23[c] * (uu - ud) / dz + 0.5 * (vr - vl) / dx +
+               0.5 * F.m13[c] * (vu - vd) / dz);
+  s13[c] = (S)(0.5 * (uu - ud) / dz / jac[c] + 0.5 * (wr - wl) / dx + F.m13[c] * dwc / dz);
+  s22[c] = (S)(2.0 * (v[c] - v[g.h(i, j - 1, k)]) / dy + F.m23[c] * (vu - vd) / dz - div);
+  s23[c] = (S)(0.5 * (vu - vd) / dz / jac[c] + 0.5 * (wf - wb) / dy + F.m23[c] * dwc / dz);
+  s33[c] = (S)(2.0 * dwc / dz / jac[c] - div);
 }
 
 // MUSCL face values from phi along a stride: right face of cell c and left
 // face of cell c + 1.
-__device__ __forceinline__ void faces(const double* phi, size_t c, size_t stride, double& right_of_c,
-                                      double& left_of_next) {
-  double pm = phi[c - stride], p0 = phi[c], p1 = phi[c + stride], p2 = phi[c + 2 * stride];
-  double t1, t2;
-  muscl(pm, p0, p1, t1, t2);
+template <class T>
+__device__ __forceinline__ T tmin(T a, T b) { return b < a ? b : a; }
+template <class T>
+__device__ __forceinline__ T tmax(T a, T b) { return a < b ? b : a; }
+
+// apply_1d_muscl!.jl:51-93 at one interior point of a line: given
+// phi(q - 1), phi(q), phi(q + 1) return tilde(q, 1) (left face value) and
+// tilde(q, 2) (right face value), in the arithmetic type T.
+template <class T>
+__device__ __forceinline__ void muscl_t(T pm, T p, T pp, T& t1, T& t2) {
+  T deltal = p - pm;
+  T deltar = pp - p;
+  if (deltar == T(0)) {
+    t2 = p;
+    t1 = p;
+  } else if (deltal == T(0)) {
+    T theta = deltal / deltar;
+    T s = (T(2) + theta) / T(3);
+    T sigmal = tmax(T(0), tmin(tmin(T(2) * theta, s), T(2)));
+    t2 = p + T(0.5) * sigmal * deltar;
+    t1 = p;
+  } else {
+    T theta = deltal / deltar;
+    T s = (T(2) + theta) / T(3);
+    T sigmal = tmax(T(0), tmin(tmin(T(2) * theta, s), T(2)));
+    s = (T(2) + T(1) / theta) / T(3);
+    T sigmar = tmax(T(0), tmin(tmin(T(2) / theta, s), T(2)));
+    t2 = p + T(0.5) * sigmal * deltar;
+    t1 = p - T(0.5) * sigmar * deltal;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T upwind_t(T usurf, T phiup, T phidown) {  // compute_flux.jl:22-32
+  return usurf > T(0) ? usurf * phiup : usurf * phidown;
+}
+
+template <class T>
+__device__ __forceinline__ void faces(const T* phi, size_t c, size_t stride, T& right_of_c, T& left_of_next) {
+  T pm = phi[c - stride], p0 = phi[c], p1 = phi[c + stride], p2 = phi[c + 2 * stride];
+  T t1, t2;
+  muscl_t<T>(pm, p0, p1, t1, t2);
   right_of_c = t2;
-  muscl(p0, p1, p2, t1, t2);
+  muscl_t<T>(p0, p1, p2, t1, t2);
   left_of_next = t1;
 }
 
-__device__ __forceinline__ double pe_x(const Fields& F, const Geo& g, int i, int j, int k) {
+// a field value in the arithmetic type T
+#define LT(arr, idx) ((T)(arr)[idx])
+
+template <class T>
+__device__ __forceinline__ T pe_x(const Fields& F, const Geo& g, int i, int j, int k) {
   size_t c = g.h(i, j, k), r = g.h(i + 1, j, k);
-  return 0.5 * (F.jac[c] * F.pbar[c] + F.jac[r] * F.pbar[r]);
+  return T(0.5) * (LT(F.jac, c) * LT(F.pbar, c) + LT(F.jac, r) * LT(F.pbar, r));
 }
-__device__ __forceinline__ double pe_y(const Fields& F, const Geo& g, int i, int j, int k) {
+template <class T>
+__device__ __forceinline__ T pe_y(const Fields& F, const Geo& g, int i, int j, int k) {
   size_t c = g.h(i, j, k), f = g.h(i, j + 1, k);
-  return 0.5 * (F.jac[c] * F.pbar[c] + F.jac[f] * F.pbar[f]);
+  return T(0.5) * (LT(F.jac, c) * LT(F.pbar, c) + LT(F.jac, f) * LT(F.pbar, f));
 }
-__device__ __forceinline__ double pe_z(const Fields& F, const Geo& g, int i, int j, int k) {
+template <class T>
+__device__ __forceinline__ T pe_z(const Fields& F, const Geo& g, int i, int j, int k) {
   size_t c = g.h(i, j, k), u = g.h(i, j, k + 1);
-  return F.jac[c] * F.jac[u] * (F.pbar[c] + F.pbar[u]) / (F.jac[c] + F.jac[u]);
+  return LT(F.jac, c) * LT(F.jac, u) * (LT(F.pbar, c) + LT(F.pbar, u)) / (LT(F.jac, c) + LT(F.jac, u));
 }
 
 // One flux component of one variable at face (i, j, k)
 // (compute_fluxes!.jl:359-520, 587-1503): a pure function of the
 // reconstruction inputs, the p0 winds, the stress components and the static
 // fields, so every kernel that evaluates it at a face gets the same bits.
-template <int VAR, int DIR>
-__device__ __forceinline__ double flux_value(const FluxArgs& A, int i, int j, int k) {
+template <int VAR, int DIR, class T = double>
+__device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int k) {
   const Geo& g = A.g;
   const Fields& F = A.F;
   const size_t c = g.h(i, j, k);
   const size_t sx = 1, sy = (size_t)g.nxx, sz = (size_t)g.nxx * g.nyy;
   const size_t stride = DIR == 0 ? sx : DIR == 1 ? sy : sz;
-  double right_of_c, left_of_next;
+  T right_of_c, left_of_next;
   faces(A.phi[VAR], c, stride, right_of_c, left_of_next);
-  double val;
+  T val;
   if (VAR == V_RHO || VAR == V_RHOP || VAR == V_CHI) {  // the tracer: RhoP's carrier (compute_fluxes!.jl:1513-1570)
-    double hi = left_of_next, lo = right_of_c;  // value at c + e, value at c
+    T hi = left_of_next, lo = right_of_c;  // value at c + e, value at c
     if (DIR == 0) {
       if (VAR == V_RHO) {
         const size_t r = g.h(i + 1, j, k);
-        double rhobaredger = 0.5 * (F.rhobar[c] + F.rhobar[r]);
-        double pe = 0.5 * (F.pbar[c] + F.pbar[r]);
+        T rhobaredger = T(0.5) * (LT(F.rhobar, c) + LT(F.rhobar, r));
+        T pe = T(0.5) * (LT(F.pbar, c) + LT(F.pbar, r));
         hi = hi + rhobaredger / pe;
         lo = lo + rhobaredger / pe;
       }
-      val = upwind(pe_x(F, g, i, j, k) * A.u0[c], lo, hi);
+      val = upwind_t<T>(pe_x<T>(F, g, i, j, k) * LT(A.u0, c), lo, hi);
     } else if (DIR == 1) {
       if (VAR == V_RHO) {
         const size_t f = g.h(i, j + 1, k);
-        double rhobaredgef = 0.5 * (F.rhobar[c] + F.rhobar[f]);
-        double pe = 0.5 * (F.pbar[c] + F.pbar[f]);
+        T rhobaredgef = T(0.5) * (LT(F.rhobar, c) + LT(F.rhobar, f));
+        T pe = T(0.5) * (LT(F.pbar, c) + LT(F.pbar, f));
         hi = hi + rhobaredgef / pe;
         lo = lo + rhobaredgef / pe;
       }
-      val = upwind(pe_y(F, g, i, j, k) * A.v0[c], lo, hi);
+      val = upwind_t<T>(pe_y<T>(F, g, i, j, k) * LT(A.v0, c), lo, hi);
     } else {
       const size_t u = g.h(i, j, k + 1);
-      const double jc = F.jac[c], ju = F.jac[u];
+      const T jc = LT(F.jac, c), ju = LT(F.jac, u);
       if (VAR == V_RHO) {
-        double rhobaredgeu = (ju * F.rhobar[c] + jc * F.rhobar[u]) / (jc + ju);
-        double pe = (ju * F.pbar[c] + jc * F.pbar[u]) / (jc + ju);
+        T rhobaredgeu = (ju * LT(F.rhobar, c) + jc * LT(F.rhobar, u)) / (jc + ju);
+        T pe = (ju * LT(F.pbar, c) + jc * LT(F.pbar, u)) / (jc + ju);
         hi = hi + rhobaredgeu / pe;
         lo = lo + rhobaredgeu / pe;
       }
-      val = upwind(pe_z(F, g, i, j, k) * A.w0[c], lo, hi);
+      val = upwind_t<T>(pe_z<T>(F, g, i, j, k) * LT(A.w0, c), lo, hi);
     }
   } else if (VAR == V_U) {
-    const double* S11 = A.st[0];
-    const double* S12 = A.st[1];
-    const double* S13 = A.st[2];
+    const T* S11 = A.st[0];
+    const T* S12 = A.st[1];
+    const T* S13 = A.st[2];
     if (DIR == 0) {
-      double usurf = 0.5 * (pe_x(F, g, i, j, k) * A.u0[c] + pe_x(F, g, i + 1, j, k) * A.u0[g.h(i + 1, j, k)]);
-      val = upwind(usurf, right_of_c, left_of_next);
+      T usurf = T(0.5) * (pe_x<T>(F, g, i, j, k) * LT(A.u0, c) + pe_x<T>(F, g, i + 1, j, k) * LT(A.u0, g.h(i + 1, j, k)));
+      val = upwind_t<T>(usurf, right_of_c, left_of_next);
       if (A.viscous) {
-        double coef_v = A.inv_re * F.rhobar[g.h(i + 1, j, g.k0)];
-        val -= coef_v * F.jac[g.h(i + 1, j, k)] * S11[g.h(i + 1, j, k)];
+        T coef_v = A.inv_re * LT(F.rhobar, g.h(i + 1, j, g.k0));
+        val -= coef_v * LT(F.jac, g.h(i + 1, j, k)) * S11[g.h(i + 1, j, k)];
       }
     } else if (DIR == 1) {
-      double vsurf = 0.5 * (pe_y(F, g, i, j, k) * A.v0[c] + pe_y(F, g, i + 1, j, k) * A.v0[g.h(i + 1, j, k)]);
-      val = upwind(vsurf, right_of_c, left_of_next);
+      T vsurf = T(0.5) * (pe_y<T>(F, g, i, j, k) * LT(A.v0, c) + pe_y<T>(F, g, i + 1, j, k) * LT(A.v0, g.h(i + 1, j, k)));
+      val = upwind_t<T>(vsurf, right_of_c, left_of_next);
       if (A.viscous) {
         const size_t r = g.h(i + 1, j, k), f = g.h(i, j + 1, k), rf = g.h(i + 1, j + 1, k);
-        double coef_v = A.inv_re * 0.25 *
-                        (F.rhobar[g.h(i, j, g.k0)] + F.rhobar[g.h(i + 1, j, g.k0)] + F.rhobar[g.h(i, j + 1, g.k0)] +
-                         F.rhobar[g.h(i + 1, j + 1, g.k0)]);
-        val -= coef_v * 0.25 * (F.jac[c] * S12[c] + F.jac[r] * S12[r] + F.jac[f] * S12[f] + F.jac[rf] * S12[rf]);
+        T coef_v = A.inv_re * T(0.25) *
+                        (LT(F.rhobar, g.h(i, j, g.k0)) + LT(F.rhobar, g.h(i + 1, j, g.k0)) + LT(F.rhobar, g.h(i, j + 1, g.k0)) +
+                         LT(F.rhobar, g.h(i + 1, j + 1, g.k0)));
+        val -= coef_v * T(0.25) * (LT(F.jac, c) * S12[c] + LT(F.jac, r) * S12[r] + LT(F.jac, f) * S12[f] + LT(F.jac, rf) * S12[rf]);
       }
     } else {
-      double wsurf = 0.5 * (pe_z(F, g, i, j, k) * A.w0[c] + pe_z(F, g, i + 1, j, k) * A.w0[g.h(i + 1, j, k)]);
-      val = upwind(wsurf, right_of_c, left_of_next);
+      T wsurf = T(0.5) * (pe_z<T>(F, g, i, j, k) * LT(A.w0, c) + pe_z<T>(F, g, i + 1, j, k) * LT(A.w0, g.h(i + 1, j, k)));
+      val = upwind_t<T>(wsurf, right_of_c, left_of_next);
       if (A.viscous) {
-        double coef_v = A.inv_re * 0.5 * (F.rhobar[g.h(i, j, g.k0)] + F.rhobar[g.h(i + 1, j, g.k0)]);
-        auto s13 = [&](size_t q) { return F.m13[q] * S11[q] + F.m23[q] * S12[q] + S13[q] / F.jac[q]; };
+        T coef_v = A.inv_re * T(0.5) * (LT(F.rhobar, g.h(i, j, g.k0)) + LT(F.rhobar, g.h(i + 1, j, g.k0)));
+        auto s13 = [&](size_t q) { return LT(F.m13, q) * S11[q] + LT(F.m23, q) * S12[q] + S13[q] / LT(F.jac, q); };
         const size_t r = g.h(i + 1, j, k), u = g.h(i, j, k + 1), ru = g.h(i + 1, j, k + 1);
-        double t0 = s13(c), tr = s13(r), tu = s13(u), tru = s13(ru);
-        val -= coef_v * 0.5 *
-               (F.jac[c] * F.jac[u] * (t0 + tu) / (F.jac[c] + F.jac[u]) +
-                F.jac[r] * F.jac[ru] * (tr + tru) / (F.jac[r] + F.jac[ru]));
+        T t0 = s13(c), tr = s13(r), tu = s13(u), tru = s13(ru);
+        val -= coef_v * T(0.5) *
+               (LT(F.jac, c) * LT(F.jac, u) * (t0 + tu) / (LT(F.jac, c) + LT(F.jac, u)) +
+                LT(F.jac, r) * LT(F.jac, ru) * (tr + tru) / (LT(F.jac, r) + LT(F.jac, ru)));
       }
     }
   } else if (VAR == V_V) {
-    const double* S12 = A.st[1];
-    const double* S22 = A.st[3];
-    const double* S23 = A.st[4];
+    const T* S12 = A.st[1];
+    const T* S22 = A.st[3];
+    const T* S23 = A.st[4];
     if (DIR == 0) {
-      double usurf = 0.5 * (pe_x(F, g, i, j, k) * A.u0[c] + pe_x(F, g, i, j + 1, k) * A.u0[g.h(i, j + 1, k)]);
-      val = upwind(usurf, right_of_c, left_of_next);
+      T usurf = T(0.5) * (pe_x<T>(F, g, i, j, k) * LT(A.u0, c) + pe_x<T>(F, g, i, j + 1, k) * LT(A.u0, g.h(i, j + 1, k)));
+      val = upwind_t<T>(usurf, right_of_c, left_of_next);
       if (A.viscous) {
         const size_t r = g.h(i + 1, j, k), f = g.h(i, j + 1, k), rf = g.h(i + 1, j + 1, k);
-        double coef_v = A.inv_re * 0.25 *
-                        (F.rhobar[g.h(i, j, g.k0)] + F.rhobar[g.h(i + 1, j, g.k0)] + F.rhobar[g.h(i, j + 1, g.k0)] +
-                         F.rhobar[g.h(i + 1, j + 1, g.k0)]);
-        val -= coef_v * 0.25 * (F.jac[c] * S12[c] + F.jac[r] * S12[r] + F.jac[f] * S12[f] + F.jac[rf] * S12[rf]);
+        T coef_v = A.inv_re * T(0.25) *
+                        (LT(F.rhobar, g.h(i, j, g.k0)) + LT(F.rhobar, g.h(i + 1, j, g.k0)) + LT(F.rhobar, g.h(i, j + 1, g.k0)) +
+                         LT(F.rhobar, g.h(i + 1, j + 1, g.k0)));
+        val -= coef_v * T(0.25) * (LT(F.jac, c) * S12[c] + LT(F.jac, r) * S12[r] + LT(F.jac, f) * S12[f] + LT(F.jac, rf) * S12[rf]);
       }
     } else if (DIR == 1) {
-      double vsurf = 0.5 * (pe_y(F, g, i, j, k) * A.v0[c] + pe_y(F, g, i, j + 1, k) * A.v0[g.h(i, j + 1, k)]);
-      val = upwind(vsurf, right_of_c, left_of_next);
+      T vsurf = T(0.5) * (pe_y<T>(F, g, i, j, k) * LT(A.v0, c) + pe_y<T>(F, g, i, j + 1, k) * LT(A.v0, g.h(i, j + 1, k)));
+      val = upwind_t<T>(vsurf, right_of_c, left_of_next);
       if (A.viscous) {
-        double coef_v = A.inv_re * F.rhobar[g.h(i, j + 1, g.k0)];
-        val -= coef_v * F.jac[g.h(i, j + 1, k)] * S22[g.h(i, j + 1, k)];
+        T coef_v = A.inv_re * LT(F.rhobar, g.h(i, j + 1, g.k0));
+        val -= coef_v * LT(F.jac, g.h(i, j + 1, k)) * S22[g.h(i, j + 1, k)];
       }
     } else {
-      double wsurf = 0.5 * (pe_z(F, g, i, j, k) * A.w0[c] + pe_z(F, g, i, j + 1, k) * A.w0[g.h(i, j + 1, k)]);
-      val = upwind(wsurf, right_of_c, left_of_next);
+      T wsurf = T(0.5) * (pe_z<T>(F, g, i, j, k) * LT(A.w0, c) + pe_z<T>(F, g, i, j + 1, k) * LT(A.w0, g.h(i, j + 1, k)));
+      val = upwind_t<T>(wsurf, right_of_c, left_of_next);
       if (A.viscous) {
-        double coef_v = A.inv_re * 0.5 * (F.rhobar[g.h(i, j, g.k0)] + F.rhobar[g.h(i, j + 1, g.k0)]);
-        auto s23 = [&](size_t q) { return F.m13[q] * S12[q] + F.m23[q] * S22[q] + S23[q] / F.jac[q]; };
+        T coef_v = A.inv_re * T(0.5) * (LT(F.rhobar, g.h(i, j, g.k0)) + LT(F.rhobar, g.h(i, j + 1, g.k0)));
+        auto s23 = [&](size_t q) { return LT(F.m13, q) * S12[q] + LT(F.m23, q) * S22[q] + S23[q] / LT(F.jac, q); };
         const size_t f = g.h(i, j + 1, k), u = g.h(i, j, k + 1), fu = g.h(i, j + 1, k + 1);
-        double t0 = s23(c), tf = s23(f), tu = s23(u), tfu = s23(fu);
-        val -= coef_v * 0.5 *
-               (F.jac[c] * F.jac[u] * (t0 + tu) / (F.jac[c] + F.jac[u]) +
-                F.jac[f] * F.jac[fu] * (tf + tfu) / (F.jac[f] + F.jac[fu]));
+        T t0 = s23(c), tf = s23(f), tu = s23(u), tfu = s23(fu);
+        val -= coef_v * T(0.5) *
+               (LT(F.jac, c) * LT(F.jac, u) * (t0 + tu) / (LT(F.jac, c) + LT(F.jac, u)) +
+                LT(F.jac, f) * LT(F.jac, fu) * (tf + tfu) / (LT(F.jac, f) + LT(F.jac, fu)));
       }
     }
   } else {  // W
-    const double* S13 = A.st[2];
-    const double* S23 = A.st[4];
-    const double* S33 = A.st[5];
+    const T* S13 = A.st[2];
+    const T* S23 = A.st[4];
+    const T* S33 = A.st[5];
     if (DIR == 0) {
       const size_t r = g.h(i + 1, j, k), u = g.h(i, j, k + 1), ru = g.h(i + 1, j, k + 1);
-      double pedger = 0.5 * (F.jac[c] * F.pbar[c] + F.jac[r] * F.pbar[r]);
-      double puedger = 0.5 * (F.jac[u] * F.pbar[u] + F.jac[ru] * F.pbar[ru]);
-      double usurf = ((F.jac[u] + F.jac[ru]) * pedger * A.u0[c] + (F.jac[c] + F.jac[r]) * puedger * A.u0[u]) /
-                     (F.jac[c] + F.jac[r] + F.jac[u] + F.jac[ru]);
-      val = upwind(usurf, right_of_c, left_of_next);
+      T pedger = T(0.5) * (LT(F.jac, c) * LT(F.pbar, c) + LT(F.jac, r) * LT(F.pbar, r));
+      T puedger = T(0.5) * (LT(F.jac, u) * LT(F.pbar, u) + LT(F.jac, ru) * LT(F.pbar, ru));
+      T usurf = ((LT(F.jac, u) + LT(F.jac, ru)) * pedger * LT(A.u0, c) + (LT(F.jac, c) + LT(F.jac, r)) * puedger * LT(A.u0, u)) /
+                     (LT(F.jac, c) + LT(F.jac, r) + LT(F.jac, u) + LT(F.jac, ru));
+      val = upwind_t<T>(usurf, right_of_c, left_of_next);
       if (A.viscous) {
-        double coef_v = A.inv_re * 0.5 * (F.rhobar[g.h(i, j, g.k0)] + F.rhobar[g.h(i + 1, j, g.k0)]);
-        val -= coef_v * 0.5 *
-               (F.jac[c] * F.jac[u] * (S13[c] + S13[u]) / (F.jac[c] + F.jac[u]) +
-                F.jac[r] * F.jac[ru] * (S13[r] + S13[ru]) / (F.jac[r] + F.jac[ru]));
+        T coef_v = A.inv_re * T(0.5) * (LT(F.rhobar, g.h(i, j, g.k0)) + LT(F.rhobar, g.h(i + 1, j, g.k0)));
+        val -= coef_v * T(0.5) *
+               (LT(F.jac, c) * LT(F.jac, u) * (S13[c] + S13[u]) / (LT(F.jac, c) + LT(F.jac, u)) +
+                LT(F.jac, r) * LT(F.jac, ru) * (S13[r] + S13[ru]) / (LT(F.jac, r) + LT(F.jac, ru)));
       }
     } else if (DIR == 1) {
       const size_t f = g.h(i, j + 1, k), u = g.h(i, j, k + 1), fu = g.h(i, j + 1, k + 1);
-      double pedgef = 0.5 * (F.jac[c] * F.pbar[c] + F.jac[f] * F.pbar[f]);
-      double puedgef = 0.5 * (F.jac[u] * F.pbar[u] + F.jac[fu] * F.pbar[fu]);
-      double vsurf = ((F.jac[u] + F.jac[fu]) * pedgef * A.v0[c] + (F.jac[c] + F.jac[f]) * puedgef * A.v0[u]) /
-                     (F.jac[c] + F.jac[f] + F.jac[u] + F.jac[fu]);
-      val = upwind(vsurf, right_of_c, left_of_next);
+      T pedgef = T(0.5) * (LT(F.jac, c) * LT(F.pbar, c) + LT(F.jac, f) * LT(F.pbar, f));
+      T puedgef = T(0.5) * (LT(F.jac, u) * LT(F.pbar, u) + LT(F.jac, fu) * LT(F.pbar, fu));
+      T vsurf = ((LT(F.jac, u) + LT(F.jac, fu)) * pedgef * LT(A.v0, c) + (LT(F.jac, c) + LT(F.jac, f)) * puedgef * LT(A.v0, u)) /
+                     (LT(F.jac, c) + LT(F.jac, f) + LT(F.jac, u) + LT(F.jac, fu));
+      val = upwind_t<T>(vsurf, right_of_c, left_of_next);
       if (A.viscous) {
         // compute_fluxes!.jl:1351-1367 takes the (3, 1) component on this face
-        double coef_v = A.inv_re * 0.5 * (F.rhobar[g.h(i, j, g.k0)] + F.rhobar[g.h(i, j + 1, g.k0)]);
-        val -= coef_v * 0.5 *
-               (F.jac[c] * F.jac[u] * (S13[c] + S13[u]) / (F.jac[c] + F.jac[u]) +
-                F.jac[f] * F.jac[fu] * (S13[f] + S13[fu]) / (F.jac[f] + F.jac[fu]));
+        T coef_v = A.inv_re * T(0.5) * (LT(F.rhobar, g.h(i, j, g.k0)) + LT(F.rhobar, g.h(i, j + 1, g.k0)));
+        val -= coef_v * T(0.5) *
+               (LT(F.jac, c) * LT(F.jac, u) * (S13[c] + S13[u]) / (LT(F.jac, c) + LT(F.jac, u)) +
+                LT(F.jac, f) * LT(F.jac, fu) * (S13[f] + S13[fu]) / (LT(F.jac, f) + LT(F.jac, fu)));
       }
     } else {
-      double wsurf = 0.5 * (pe_z(F, g, i, j, k) * A.w0[c] + pe_z(F, g, i, j, k + 1) * A.w0[g.h(i, j, k + 1)]);
-      val = upwind(wsurf, right_of_c, left_of_next);
+      T wsurf = T(0.5) * (pe_z<T>(F, g, i, j, k) * LT(A.w0, c) + pe_z<T>(F, g, i, j, k + 1) * LT(A.w0, g.h(i, j, k + 1)));
+      val = upwind_t<T>(wsurf, right_of_c, left_of_next);
       if (A.viscous) {
         const size_t u = g.h(i, j, k + 1);
-        double coef_v = A.inv_re * F.rhobar[g.h(i, j, g.k0)];
-        val -= coef_v * (F.jac[u] * F.m13[u] * S13[u] + F.jac[u] * F.m23[u] * S23[u] + S33[u]);
+        T coef_v = A.inv_re * LT(F.rhobar, g.h(i, j, g.k0));
+        val -= coef_v * (LT(F.jac, u) * LT(F.m13, u) * S13[u] + LT(F.jac, u) * LT(F.m23, u) * S23[u] + S33[u]);
       }
     }
   }
@@ -329,9 +376,9 @@ __global__ void k_flux_p(FluxArgs A, Range3 R) {
   const Geo& g = A.g;
   const size_t c = g.h(i, j, k);
   double val;
-  if (DIR == 0) val = pe_x(A.F, g, i, j, k) * A.u0[c];
-  else if (DIR == 1) val = pe_y(A.F, g, i, j, k) * A.v0[c];
-  else val = pe_z(A.F, g, i, j, k) * A.w0[c];
+  if (DIR == 0) val = pe_x<double>(A.F, g, i, j, k) * A.u0[c];
+  else if (DIR == 1) val = pe_y<double>(A.F, g, i, j, k) * A.v0[c];
+  else val = pe_z<double>(A.F, g, i, j, k) * A.w0[c];
   A.flux[V_P][DIR][c] = val;
 }
 
@@ -429,7 +476,44 @@ struct UpdArgs {
   const double* div_u;    // W update: those of the U and V updates of this stage (or null: recompute)
   const double* div_v;
   double* fdiv;           // fused stage, W: its flux divergence, read by k_update_w_fdiv
+  // fused stage: apply_lhs_sponge! (apply_lhs_sponge!.jl:278-530) on the
+  // interior cells right after their update (sponge = 0: none), with the
+  // sponge's time step and relaxation targets of k_lhs_sponge
+  int sponge;
+  const double* alphar;
+  double sdt, bgu, bgv, bgw_u, bgw_v, bgw_w, uref;
 };
+
+// apply_lhs_sponge! at one cell of the interior: the new value of q there,
+// given its updated value (the arithmetic of k_lhs_sponge)
+template <int VAR>
+__device__ __forceinline__ double sponged(const UpdArgs& A, int i, int j, int k, double q) {
+  const Geo& g = A.g;
+  if (!A.sponge || i < g.i0 || i > g.i1 || j < g.j0 || j > g.j1 || k < g.k0 || k > g.k1) return q;
+  const size_t c = g.h(i, j, k);
+  const double* alphar = A.alphar;
+  double bg, alpha;
+  if (VAR == 0 || VAR == 1) {  // V_RHO, V_RHOP
+    bg = 0.0;
+    alpha = alphar[c];
+  } else if (VAR == 2) {  // V_U
+    bg = A.bgu;
+    alpha = 0.5 * (alphar[c] + alphar[g.h(i + 1, j, k)]);
+  } else if (VAR == 3) {  // V_V
+    bg = A.bgv;
+    alpha = 0.5 * (alphar[c] + alphar[g.h(i, j + 1, k)]);
+  } else {
+    const size_t up = g.h(i, j, k + 1);
+    const double *jac = A.F.jac, *m13 = A.F.m13, *m23 = A.F.m23;
+    const double jc = jac[c], ju = jac[up];
+    bg = (jc * (m13[up] * A.bgw_u + m23[up] * A.bgw_v + A.bgw_w / ju) +
+          ju * (m13[c] * A.bgw_u + m23[c] * A.bgw_v + A.bgw_w / jc)) /
+         (jc + ju) / A.uref;
+    alpha = (ju * alphar[c] + jc * alphar[up]) / (jc + ju);
+  }
+  double beta = 1.0 / (1.0 + alpha * A.sdt);
+  return (1.0 - beta) * bg + beta * q;
+}
 
 __device__ __forceinline__ double divergence(const double* const* f, const Geo& g, int i, int j, int k) {
   const size_t c = g.h(i, j, k);
@@ -440,7 +524,7 @@ __device__ __forceinline__ double divergence(const double* const* f, const Geo& 
 // The updates at one cell from the flux divergence `fluxdiff` (the three
 // differences over dx, dy, dz, summed in that order), shared by the update
 // kernels that read stored fluxes and by the fused stage kernels.
-__device__ __forceinline__ void update_scalar_at(const UpdArgs& A, int i, int j, int k, double fluxdiff) {
+__device__ __forceinline__ double update_scalar_at(const UpdArgs& A, int i, int j, int k, double fluxdiff) {
   // update!.jl:521-597
   const size_t c = A.g.h(i, j, k);
   fluxdiff /= A.F.jac[c];
@@ -451,16 +535,16 @@ __device__ __forceinline__ void update_scalar_at(const UpdArgs& A, int i, int j,
   double dqv = A.first ? 0.0 : A.dq[c];
   dqv = A.dt * fr + A.am * dqv;
   A.dq[c] = dqv;
-  A.q[c] += A.bm * dqv;
+  return A.q[c] + A.bm * dqv;
 }
 
 __global__ void k_update_scalar(UpdArgs A, Range3 R) {
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
-  update_scalar_at(A, i, j, k, divergence(A.f, A.g, i, j, k));
+  A.q[A.g.h(i, j, k)] = update_scalar_at(A, i, j, k, divergence(A.f, A.g, i, j, k));
 }
 
-__device__ __forceinline__ void update_u_at(const UpdArgs& A, int i, int j, int k, double fluxdiff) {
+__device__ __forceinline__ double update_u_at(const UpdArgs& A, int i, int j, int k, double fluxdiff) {
   // update!.jl:730-806
   const Geo& g = A.g;
   const Fields& F = A.F;
@@ -483,16 +567,16 @@ __device__ __forceinline__ void update_u_at(const UpdArgs& A, int i, int j, int 
   d = A.dt * force + A.am * d;
   A.dq[c] = d;
   double momm = momm_1 + A.bm * d;
-  A.q[c] = momm / rhom;
+  return momm / rhom;
 }
 
 __global__ void k_update_u(UpdArgs A, Range3 R) {
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
-  update_u_at(A, i, j, k, divergence(A.f, A.g, i, j, k));
+  A.q[A.g.h(i, j, k)] = update_u_at(A, i, j, k, divergence(A.f, A.g, i, j, k));
 }
 
-__device__ __forceinline__ void update_v_at(const UpdArgs& A, int i, int j, int k, double fluxdiff) {
+__device__ __forceinline__ double update_v_at(const UpdArgs& A, int i, int j, int k, double fluxdiff) {
   // update!.jl:884-957
   const Geo& g = A.g;
   const Fields& F = A.F;
@@ -515,20 +599,20 @@ __device__ __forceinline__ void update_v_at(const UpdArgs& A, int i, int j, int 
   d = A.dt * force + A.am * d;
   A.dq[c] = d;
   double momm = momm_1 + A.bm * d;
-  A.q[c] = momm / rhom;
+  return momm / rhom;
 }
 
 __global__ void k_update_v(UpdArgs A, Range3 R) {
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
-  update_v_at(A, i, j, k, divergence(A.f, A.g, i, j, k));
+  A.q[A.g.h(i, j, k)] = update_v_at(A, i, j, k, divergence(A.f, A.g, i, j, k));
 }
 
 // REUSE: the divergences of this stage's U and V updates are known to be in
 // A.div_u / A.div_v (the fused stage); else they are read from there when
 // given and recomputed from the stored fluxes otherwise.
 template <bool REUSE>
-__device__ __forceinline__ void update_w_at(const UpdArgs& A, int i, int j, int k, double fluxdiff) {
+__device__ __forceinline__ double update_w_at(const UpdArgs& A, int i, int j, int k, double fluxdiff) {
   // update!.jl:1035-1192
   const Geo& g = A.g;
   const Fields& F = A.F;
@@ -580,66 +664,90 @@ __device__ __forceinline__ void update_w_at(const UpdArgs& A, int i, int j, int 
   d = A.dt * force + A.am * d;
   A.dq[c] = d;
   double momm = momm_1 + A.bm * d;
-  A.q[c] = momm / rhom;
+  return momm / rhom;
 }
 
 __global__ void k_update_w(UpdArgs A, Range3 R) {
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
-  update_w_at<false>(A, i, j, k, divergence(A.f, A.g, i, j, k));
+  A.q[A.g.h(i, j, k)] = update_w_at<false>(A, i, j, k, divergence(A.f, A.g, i, j, k));
 }
 
-// the W update of the fused stage from its stored flux divergence
+// the W update of the fused stage from its stored flux divergence, then
+// the sponge; the top interior level, which the W update leaves, is in R
+// for the sponge alone
 __global__ void k_update_w_fdiv(UpdArgs A, Range3 R) {
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
-  update_w_at<true>(A, i, j, k, A.fdiv[A.g.h(i, j, k)]);
+  const size_t c = A.g.h(i, j, k);
+  const double q = k < A.g.k1 ? update_w_at<true>(A, i, j, k, A.fdiv[c]) : A.q[c];
+  A.q[c] = sponged<V_W>(A, i, j, k, q);
 }
 
 // ---------------------------------------------------------------------------
 // Fused LHS stage kernels (W: the W fluxes and their divergence, stored in
 // A.fdiv, then k_update_w_fdiv, because the W update alone takes about as
-// many registers as the rest of the fused kernel): the fluxes of one variable at the faces of a cell,
-// their divergence and the RK3 update in one pass, for the stage loop of
-// explicit_integration! (explicit_integration!.jl:42-101).  No flux array is
-// stored: a thread evaluates the x and y faces on both sides of its cell
-// (flux_value) and marches up a chunk of kc levels, carrying the z face
-// between levels; the vertical wall faces, which set_boundaries!(state,
-// BoundaryFluxes()) zeroes, are zero here too.  The flux inputs (phi, the
-// Cartesian w, the stress components) are computed at the start of the stage
-// and the p0 winds are fixed, so evaluating a variable's fluxes just before
-// its update gives the bits the stored fluxes of compute_fluxes! had.  The
-// divergence is summed as divergence() sums it.  Threads map to the cells of
-// one layer of the update range R with x fastest.
+// many registers as the rest of the fused kernel): the fluxes of one
+// variable at the faces of a cell, their divergence, the RK3 update and the
+// LHS sponge in one pass, for the stage loop of explicit_integration!
+// (explicit_integration!.jl:42-101).  No flux array is stored: a thread
+// evaluates the x faces on both sides of its cells (flux_value), walks
+// kStageJ rows in y carrying the y face between them, and marches up a
+// chunk of kc levels carrying the z face of each row; the vertical wall
+// faces, which set_boundaries!(state, BoundaryFluxes()) zeroes, are zero
+// here too.  The flux inputs (phi, the Cartesian w, the stress components)
+// are computed at the start of the stage and the p0 winds are fixed, so
+// evaluating a variable's fluxes just before its update gives the bits the
+// stored fluxes of compute_fluxes! had.  The divergence is summed as
+// divergence() sums it.  Threads map to the i of the update range R, x
+// fastest.
 // ---------------------------------------------------------------------------
-template <int VAR>
-__device__ __forceinline__ double zface(const FluxArgs& FA, int i, int j, int k) {
-  if (VAR != V_W && (k == FA.g.k0 - 1 || k == FA.g.k1)) return 0.0;
-  return flux_value<VAR, 2>(FA, i, j, k);
+template <int VAR, class T>
+__device__ __forceinline__ T zface(const FluxArgsT<T>& FA, int i, int j, int k) {
+  if (VAR != V_W && (k == FA.g.k0 - 1 || k == FA.g.k1)) return T(0);
+  return flux_value<VAR, 2, T>(FA, i, j, k);
 }
 
-template <int VAR>
-__global__ void __launch_bounds__(256) k_stage_update(FluxArgs FA, UpdArgs A, Range3 R, int kc) {
+constexpr int kStageJ = 4;  // rows of a fused stage kernel's thread
+
+template <int VAR, class T, int JC>
+__global__ void __launch_bounds__(256) k_stage_update(FluxArgsT<T> FA, UpdArgs A, Range3 R, int kc) {
+  // a thread: one i of the range, JC consecutive j (the y face between two
+  // of them is evaluated once and carried) and a chunk of kc levels
+  const int njc = (R.nj + JC - 1) / JC;
   const long long t = blockIdx.x * 256LL + threadIdx.x;
-  if (t >= (long long)R.ni * R.nj) return;
-  const int jj = (int)(t / R.ni);
-  const int i = R.i0 + (int)(t - (long long)jj * R.ni);
-  const int j = R.j0 + jj;
+  if (t >= (long long)R.ni * njc) return;
+  const int jc = (int)(t / R.ni);
+  const int i = R.i0 + (int)(t - (long long)jc * R.ni);
+  const int jb = R.j0 + jc * JC;
+  const int nj = min(JC, R.j0 + R.nj - jb);
   const int kb = R.k0 + (int)blockIdx.y * kc;
   const int ke = min(kb + kc, R.k0 + R.nk);
   const Geo& g = A.g;
-  double fzd = zface<VAR>(FA, i, j, kb - 1);
+  T fzd[JC];
+#pragma unroll
+  for (int jj = 0; jj < JC; ++jj)
+    if (jj < nj) fzd[jj] = zface<VAR, T>(FA, i, jb + jj, kb - 1);
 #pragma unroll 1
   for (int k = kb; k < ke; ++k) {
-    const double fzu = zface<VAR>(FA, i, j, k);
-    const double fxr = flux_value<VAR, 0>(FA, i, j, k), fxl = flux_value<VAR, 0>(FA, i - 1, j, k);
-    const double fyf = flux_value<VAR, 1>(FA, i, j, k), fyb = flux_value<VAR, 1>(FA, i, j - 1, k);
-    const double fluxdiff = (fxr - fxl) / g.dx + (fyf - fyb) / g.dy + (fzu - fzd) / g.dz;
-    if (VAR == V_U) update_u_at(A, i, j, k, fluxdiff);
-    else if (VAR == V_V) update_v_at(A, i, j, k, fluxdiff);
-    else if (VAR == V_W) A.fdiv[g.h(i, j, k)] = fluxdiff;  // the update follows in k_update_w_fdiv
-    else update_scalar_at(A, i, j, k, fluxdiff);
-    fzd = fzu;
+    T fyb = flux_value<VAR, 1, T>(FA, i, jb - 1, k);
+#pragma unroll
+    for (int jj = 0; jj < JC; ++jj) {
+      if (jj >= nj) break;
+      const int j = jb + jj;
+      const T fzu = zface<VAR, T>(FA, i, j, k);
+      const T fxr = flux_value<VAR, 0, T>(FA, i, j, k), fxl = flux_value<VAR, 0, T>(FA, i - 1, j, k);
+      const T fyf = flux_value<VAR, 1, T>(FA, i, j, k);
+      const double fluxdiff =
+          (double)((fxr - fxl) / T(g.dx) + (fyf - fyb) / T(g.dy) + (fzu - fzd[jj]) / T(g.dz));
+      const size_t c = g.h(i, j, k);
+      if (VAR == V_U) A.q[c] = sponged<V_U>(A, i, j, k, update_u_at(A, i, j, k, fluxdiff));
+      else if (VAR == V_V) A.q[c] = sponged<V_V>(A, i, j, k, update_v_at(A, i, j, k, fluxdiff));
+      else if (VAR == V_W) A.fdiv[c] = fluxdiff;  // the update follows in k_update_w_fdiv
+      else A.q[c] = sponged<VAR == V_CHI ? V_RHO : VAR>(A, i, j, k, update_scalar_at(A, i, j, k, fluxdiff));
+      fyb = fyf;
+      fzd[jj] = fzu;
+    }
   }
 }
 
@@ -1032,13 +1140,9 @@ int reconstruct(pcf_state* st) {
   const Geo& g = st->g;
   Fields F = fields_of(st);
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
-  const int nlev = g.nz + 2;  // k0-1 .. k1+1
-  hipLaunchKernelGGL(k_phi, dim3(blocks_for(g.nxx, 64), g.nyy, nlev), dim3(64), 0, st->stream, F, g, st->phi[0],
-                     st->phi[1], st->phi[2], st->phi[3], pi ? 1 : 0, st->chi, st->tracer ? st->phi[V_CHI] : nullptr);
-  hipLaunchKernelGGL(k_wcart, dim3(blocks_for(g.nxx - 1, 64), g.nyy - 1, g.nzz - 1), dim3(64), 0, st->stream, F, g,
-                     st->wcart);
-  hipLaunchKernelGGL(k_phi_w, dim3(blocks_for(g.nxx - 1, 64), g.nyy - 1, nlev), dim3(64), 0, st->stream, F, g,
-                     st->wcart, st->phi[4]);
+  hipLaunchKernelGGL(k_inputs<double>, dim3(blocks_for(g.nxx, 64), g.nyy, g.nzz - 1), dim3(64), 0, st->stream, F,
+                     g, st->phi[0], st->phi[1], st->phi[2], st->phi[3], pi ? 1 : 0, st->chi,
+                     st->tracer ? st->phi[V_CHI] : nullptr, st->wcart, st->phi[4]);
   PCF_HIP(hipGetLastError());
   return PCF_OK;
 }
@@ -1051,15 +1155,40 @@ namespace {
 
 // the six stress components compute_stress_tensor gives the viscous fluxes
 // (compute_stress_tensor.jl:55-164), from the current winds
-int stress_tensor(pcf_state* st) {
+// (FP32 copies with flux_precision MIXED and f32)
+int stress_tensor(pcf_state* st, bool f32 = false) {
   if (!st->ph.viscous) return PCF_OK;
   const Geo& g = st->g;
   Range3 R{g.i0 - 1, g.nx + 2, g.j0 - 1, g.ny + 2, g.k0, g.nz};
-  hipLaunchKernelGGL(k_stress, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, fields_of(st), g,
-                     st->wcart, st->stress[0], st->stress[1], st->stress[2], st->stress[3], st->stress[4],
-                     st->stress[5], R);
+  if (f32)
+    hipLaunchKernelGGL(k_stress<float>, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, fields_of(st),
+                       g, st->wcart, st->stress32[0], st->stress32[1], st->stress32[2], st->stress32[3],
+                       st->stress32[4], st->stress32[5], R);
+  else
+    hipLaunchKernelGGL(k_stress<double>, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, fields_of(st),
+                       g, st->wcart, st->stress[0], st->stress[1], st->stress[2], st->stress[3], st->stress[4],
+                       st->stress[5], R);
   PCF_HIP(hipGetLastError());
   return PCF_OK;
+}
+
+bool flux32(const pcf_state* st) { return st->cfg.flux_precision == PCF_PRECISION_MIXED; }
+
+// the FP32 flux inputs of the RK stages (flux_precision MIXED)
+FluxArgsT<float> flux_args32(pcf_state* st) {
+  FluxArgsT<float> A;
+  A.F = fields_of(st);
+  for (int q = 0; q < 6; ++q) A.phi[q] = st->phi32[q];
+  A.u0 = st->p0[2];
+  A.v0 = st->p0[3];
+  A.w0 = st->p0[4];
+  for (int q = 0; q < 6; ++q) A.st[q] = st->stress32[q];
+  for (int q = 0; q < 7; ++q)
+    for (int d = 0; d < 3; ++d) A.flux[q][d] = nullptr;
+  A.g = st->g;
+  A.inv_re = (float)(1 / st->ph.re);
+  A.viscous = st->ph.viscous;
+  return A;
 }
 
 FluxArgs flux_args(pcf_state* st) {
@@ -1079,6 +1208,38 @@ FluxArgs flux_args(pcf_state* st) {
 }
 
 // arguments of update!(state, dt, m, var, LHS()) common to every variable
+UpdArgs upd_args(pcf_state* st, double dt, int m);
+
+}  // namespace
+
+// Whether the fused stage kernel of var applies its LHS sponge itself: the
+// sponges of k_lhs_sponge (relaxation to fixed values; the compressible
+// model's, the tracer's and relax_to_mean's, which needs the layer mean of
+// the updated field, stay separate launches)
+bool sponge_fused(const pcf_state* st, int var) {
+  if (var == PCF_VAR_RHO) return st->cfg.model != PCF_BOUSSINESQ && !st->compressible;
+  if (var == PCF_VAR_RHOP) return !st->compressible;
+  if (var == PCF_VAR_U || var == PCF_VAR_V || var == PCF_VAR_W) return !st->cfg.relax_to_mean;
+  return false;
+}
+
+namespace {
+
+// the sponge arguments of the fused stage kernels (those of k_lhs_sponge)
+void sponge_args(const pcf_state* st, int var, double sdt, UpdArgs& A) {
+  const pcf_config& n = st->cfg;
+  A.sponge = sponge_fused(st, var) ? 1 : 0;
+  A.alphar = st->alphar;
+  A.sdt = sdt;
+  const double uref = st->uref;
+  A.bgu = (n.relaxed_u + n.relaxed_u) / uref / 2;
+  A.bgv = (n.relaxed_v + n.relaxed_v) / uref / 2;
+  A.bgw_u = n.relaxed_u;
+  A.bgw_v = n.relaxed_v;
+  A.bgw_w = n.relaxed_w;
+  A.uref = uref;
+}
+
 UpdArgs upd_args(pcf_state* st, double dt, int m) {
   UpdArgs A;
   A.F = fields_of(st);
@@ -1102,6 +1263,9 @@ UpdArgs upd_args(pcf_state* st, double dt, int m) {
     A.f[d] = nullptr;
   }
   A.q = A.dq = nullptr;
+  A.sponge = 0;
+  A.alphar = nullptr;
+  A.sdt = A.bgu = A.bgv = A.bgw_u = A.bgw_v = A.bgw_w = A.uref = 0.0;
   return A;
 }
 
@@ -1292,10 +1456,21 @@ int update_lhs(pcf_state* st, double dt, int m, int var) {
 // The inputs of a stage's fluxes: reconstruct! and the stress tensor from
 // the current predictands (compute_fluxes without its flux kernels).
 int stage_inputs(pcf_state* st) {
-  int rc = reconstruct(st);
-  if (rc) return rc;
+  if (!flux32(st)) {
+    int rc = reconstruct(st);
+    if (rc) return rc;
+    ProfScope prof(st, PCF_K_FLUXES);
+    return stress_tensor(st);
+  }
+  // flux_precision MIXED: the same inputs, computed in FP64 and stored in FP32
   ProfScope prof(st, PCF_K_FLUXES);
-  return stress_tensor(st);
+  const Geo& g = st->g;
+  hipLaunchKernelGGL(k_inputs<float>, dim3(blocks_for(g.nxx, 64), g.nyy, g.nzz - 1), dim3(64), 0, st->stream,
+                     fields_of(st), g, st->phi32[0], st->phi32[1], st->phi32[2], st->phi32[3],
+                     st->cfg.model != PCF_BOUSSINESQ ? 1 : 0, st->chi, st->tracer ? st->phi32[V_CHI] : nullptr,
+                     st->wcart, st->phi32[4]);
+  PCF_HIP(hipGetLastError());
+  return stress_tensor(st, true);
 }
 
 // update!(state, dt, m, var, LHS()) of one RK stage with the variable's
@@ -1303,16 +1478,20 @@ int stage_inputs(pcf_state* st) {
 // after stage_inputs: the sequence of explicit_integration!.jl:53-99 without
 // storing the fluxes.  The U and V updates leave their flux divergences in
 // div_u / div_v for the W update of the same stage.
-int stage_update(pcf_state* st, double dt, int m, int var) {
+int stage_update(pcf_state* st, double dt, int m, int var, double sdt) {
   const Geo& g = st->g;
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
-  constexpr int kc = 8;  // levels per thread
+  constexpr int kc = 8;  // levels per thread (kStageJ rows each)
   UpdArgs A = upd_args(st, dt, m);
+  sponge_args(st, var, sdt, A);
   Range3 R{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
-  auto launch = [&](auto kernel) {
+  const bool f32 = flux32(st);
+  // k_stage_update<VAR, double> or, with flux_precision MIXED, <VAR, float>
+  auto launch = [&](auto k64, auto k32) {
     if (R.ni <= 0 || R.nj <= 0 || R.nk <= 0) return;
-    dim3 grd(blocks_for((long long)R.ni * R.nj, 256), (unsigned)((R.nk + kc - 1) / kc));
-    hipLaunchKernelGGL(kernel, grd, dim3(256), 0, st->stream, flux_args(st), A, R, kc);
+    dim3 grd(blocks_for((long long)R.ni * ((R.nj + kStageJ - 1) / kStageJ), 256), (unsigned)((R.nk + kc - 1) / kc));
+    if (f32) hipLaunchKernelGGL(k32, grd, dim3(256), 0, st->stream, flux_args32(st), A, R, kc);
+    else hipLaunchKernelGGL(k64, grd, dim3(256), 0, st->stream, flux_args(st), A, R, kc);
   };
   if (var == PCF_VAR_P) {  // the carrier flux of P has no reconstruction: stored, then the update
     if (!st->compressible) return PCF_OK;
@@ -1337,34 +1516,34 @@ int stage_update(pcf_state* st, double dt, int m, int var) {
       if (!pi) return PCF_OK;
       A.q = st->rho;
       A.dq = st->drho;
-      launch(k_stage_update<V_RHO>);
+      launch(k_stage_update<V_RHO, double, kStageJ>, k_stage_update<V_RHO, float, kStageJ>);
       break;
     case PCF_VAR_RHOP:
       A.q = st->rhop;
       A.dq = st->drhop;
       A.heat = st->compressible ? 1 : 0;
-      launch(k_stage_update<V_RHOP>);
+      launch(k_stage_update<V_RHOP, double, kStageJ>, k_stage_update<V_RHOP, float, kStageJ>);
       break;
     case PCF_VAR_CHI:
       if (!st->tracer) return PCF_OK;
       A.q = st->chi;
       A.dq = st->dchi;
       A.force0 = 1;
-      launch(k_stage_update<V_CHI>);
+      launch(k_stage_update<V_CHI, double, kStageJ>, k_stage_update<V_CHI, float, kStageJ>);
       break;
     case PCF_VAR_U:
       A.q = st->u;
       A.dq = st->du;
       A.div_out = st->div_u;
       R = Range3{g.i0 - 1, g.nx + 1, g.j0, g.ny, g.k0, g.nz};
-      launch(k_stage_update<V_U>);
+      launch(k_stage_update<V_U, double, kStageJ>, k_stage_update<V_U, float, kStageJ>);
       break;
     case PCF_VAR_V:
       A.q = st->v;
       A.dq = st->dv;
       A.div_out = st->div_v;
       R = Range3{g.i0, g.nx, g.j0 - 1, g.ny + 1, g.k0, g.nz};
-      launch(k_stage_update<V_V>);
+      launch(k_stage_update<V_V, double, kStageJ>, k_stage_update<V_V, float, kStageJ>);
       break;
     case PCF_VAR_W:
       A.q = st->w;
@@ -1373,9 +1552,13 @@ int stage_update(pcf_state* st, double dt, int m, int var) {
       A.div_v = st->div_v;
       A.fdiv = st->fdiv_w;
       R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz - 1};
-      launch(k_stage_update<V_W>);
-      if (R.nk > 0)
-        hipLaunchKernelGGL(k_update_w_fdiv, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, A, R);
+      launch(k_stage_update<V_W, double, kStageJ>, k_stage_update<V_W, float, kStageJ>);
+      {
+        // the update, and with a fused sponge the top interior level too
+        Range3 rw{g.i0, g.nx, g.j0, g.ny, g.k0, A.sponge ? g.nz : g.nz - 1};
+        if (rw.nk > 0)
+          hipLaunchKernelGGL(k_update_w_fdiv, dim3(blocks_for(rw.count(), 256)), dim3(256), 0, st->stream, A, rw);
+      }
       break;
     default:
       return PCF_E_ARG;

@@ -182,37 +182,6 @@ __device__ __forceinline__ double transform_transformed(const Fields& F, const G
          wedgeu / jacedgeu;
 }
 
-// apply_1d_muscl!.jl:51-93 at one interior point of a line: given
-// phi(q - 1), phi(q), phi(q + 1) return tilde(q, 1) (left face value) and
-// tilde(q, 2) (right face value).
-__device__ __forceinline__ void muscl(double pm, double p, double pp, double& t1, double& t2) {
-  double deltal = p - pm;
-  double deltar = pp - p;
-  if (deltar == 0.0) {
-    t2 = p;
-    t1 = p;
-  } else if (deltal == 0.0) {
-    double theta = deltal / deltar;
-    double s = (2.0 + theta) / 3.0;
-    double sigmal = dmax(0.0, dmin(dmin(2 * theta, s), 2.0));
-    t2 = p + 0.5 * sigmal * deltar;
-    t1 = p;
-  } else {
-    double theta = deltal / deltar;
-    double s = (2.0 + theta) / 3.0;
-    double sigmal = dmax(0.0, dmin(dmin(2 * theta, s), 2.0));
-    s = (2.0 + 1.0 / theta) / 3.0;
-    double sigmar = dmax(0.0, dmin(dmin(2 / theta, s), 2.0));
-    t2 = p + 0.5 * sigmal * deltar;
-    t1 = p - 0.5 * sigmar * deltal;
-  }
-}
-
-// compute_flux.jl:22-32
-__device__ __forceinline__ double upwind(double usurf, double phiup, double phidown) {
-  return usurf > 0.0 ? usurf * phiup : usurf * phidown;
-}
-
 // Flat thread index -> (i, j, k) of a range.
 __device__ __forceinline__ bool range_ijk(const Range3& R, long long t, int& i, int& j, int& k) {
   if (t >= R.count()) return false;

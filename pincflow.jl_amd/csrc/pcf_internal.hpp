@@ -154,6 +154,10 @@ struct pcf_state {
   double* phi[6] = {};                // reconstruction inputs
   double* wcart;                      // Cartesian vertical wind
   double* stress[6];                  // 11, 12, 13, 22, 23, 33
+  // flux_precision MIXED: FP32 copies of phi and the stress components that
+  // the RK stages' flux evaluation reads (null otherwise)
+  float* phi32[6] = {};
+  float* stress32[6] = {};
   // between flux computations stress[0] and stress[1] hold the flux
   // divergences of the stage's U and V updates, which the W update reuses
   bool lhs_div_valid[2] = {false, false};
@@ -305,7 +309,10 @@ int learned_stage(pcf_state* st, const double* v, double* y, const SolverScalars
 int reconstruct(pcf_state* st);
 int compute_fluxes(pcf_state* st);
 int stage_inputs(pcf_state* st);
-int stage_update(pcf_state* st, double dt, int m, int var);
+// the fused fluxes + update of one variable in RK stage m, with the LHS
+// sponge of time step sdt applied in the same kernel when sponge_fused
+int stage_update(pcf_state* st, double dt, int m, int var, double sdt);
+bool sponge_fused(const pcf_state* st, int var);
 int update_lhs(pcf_state* st, double dt, int m, int var);
 int update_rhs(pcf_state* st, double dt, int var, bool implicit, double rf);
 int apply_lhs_sponge(pcf_state* st, double dt, double time, int var);

@@ -177,6 +177,10 @@ int host_setup(pcf_state* st, std::string& err) {
   if (!(n.thermal_conductivity >= 0.0)) { err = "thermal_conductivity < 0"; return PCF_E_CONFIG; }
   if (n.preconditioner_iterations < 1 && n.preconditioner) { err = "preconditioner_iterations < 1"; return PCF_E_CONFIG; }
   if (n.precision != PCF_PRECISION_FP64 && n.precision != PCF_PRECISION_MIXED) { err = "Invalid precision"; return PCF_E_OPTION; }
+  if (n.flux_precision != PCF_PRECISION_FP64 && n.flux_precision != PCF_PRECISION_MIXED) {
+    err = "Invalid flux_precision";
+    return PCF_E_OPTION;
+  }
   if (n.initial_guess != PCF_GUESS_ZERO && n.initial_guess != PCF_GUESS_PREVIOUS && n.initial_guess != PCF_GUESS_LEARNED) {
     err = "Invalid initial_guess";
     return PCF_E_OPTION;

@@ -138,6 +138,10 @@ class DiscretizationNamelist:
     wkb_cfl_number: float = 3.0e-1
     dtmin: float = 1.0e-6
     dtmax: float = 1.0e3
+    # addition of this build: arithmetic of the RK stages' fluxes, "fp64"
+    # (reference) or "mixed" (reconstruction, fluxes and flux divergences in
+    # FP32, updates in FP64)
+    flux_precision: str = "fp64"
     adaptive_time_step: bool = True
     limiter_type: str = "MCVariant"
 
@@ -285,6 +289,7 @@ def to_config(nl: Namelists) -> _abi.PcfConfig:
     c.initial_cleaning = int(po.initial_cleaning)
     c.tolerance_is_relative = int(po.tolerance_is_relative)
     c.precision = _option(_PRECISIONS, po.precision, "precision")
+    c.flux_precision = _option(_PRECISIONS, di.flux_precision, "flux_precision")
     c.model = _option(_MODELS, a.model, "model")
     c.background = _option(_BACKGROUNDS, a.background, "background")
     c.specify_reynolds_number = int(a.specify_reynolds_number)

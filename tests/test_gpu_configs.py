@@ -29,6 +29,7 @@ from test_oracle_reduction_order import iter_slack
 pytestmark = pytest.mark.gpu
 
 FIELD_RTOL = 1e-5
+FLUX32_RTOL = 2e-4
 STEP_RTOL = 1e-9
 
 
@@ -83,21 +84,30 @@ def test_sixteen_hills_mixed_solve_meets_the_criterion(pcf, abi, oracle_mod, see
     assert rms(orc.apply_operator(x_m - x_o)) <= 2 * tol * rms(b)
 
 
+@pytest.mark.parametrize("fluxes", ["fp64", "mixed"])
 @pytest.mark.parametrize("seed", [101, 202, 303])
-def test_sixteen_hills_mixed_steps_track_the_oracle(pcf, abi, oracle_mod, seed):
+def test_sixteen_hills_mixed_steps_track_the_oracle(pcf, abi, oracle_mod, seed, fluxes):
+    """The mixed path (multigrid, bf16 coefficients, warm start), with the
+    RK stages' fluxes in FP64 or in FP32 (flux_precision = "mixed": the
+    configuration's FP32 fluxes and preconditioner), against three FP64
+    oracle steps.  FP32 fluxes round the background density the density
+    flux carries (rhobar / pbar) to FP32; relative to the density
+    fluctuations that leaves errors of order 1e-5 after three steps, which
+    FLUX32_RTOL allows; every solve still meets the FP64 criterion."""
     nl = hills(pcf, seed, "mixed")
     nl.poisson.preconditioner_type = "multigrid"
     nl.poisson.preconditioner_storage = "bf16"
     nl.poisson.mg_coarse_steps = 2
     nl.poisson.initial_guess = "previous"
+    nl.discretization.flux_precision = fluxes
     gpu, _, _ = pcf.integrate(nl, records=False)
     orc, _, _ = oracle_mod.integrate(hills(pcf, seed, "fp64"), abi, pcf.output, records=False)
     assert gpu.time()[0] == pytest.approx(orc.time(), rel=1e-6)
     errs = {f: field_error(f, gpu.get(f), orc.get(f)) for f in ("RHO", "RHOP", "U", "V", "W", "PIP")}
-    print("seed", seed, "error of the mixed path against the FP64 oracle after 3 steps:",
+    print("seed", seed, f"error of the mixed path ({fluxes} fluxes) against the FP64 oracle after 3 steps:",
           {k: f"{v:.2e}" for k, v in errs.items()})
     for f, e in errs.items():
-        assert e <= FIELD_RTOL, (f, e)
+        assert e <= (FIELD_RTOL if fluxes == "fp64" else FLUX32_RTOL), (f, e)
 
 
 # 64^2 x 64 Boussinesq + hill, 64^2 x 64 PI flat, 64^2 x 96 PI + hill
