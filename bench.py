@@ -157,6 +157,7 @@ def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False
     st.profile(False)
     print(f"# [{tag}] breakdown step: iterations {list(info.niter)}", file=sys.stderr, flush=True)
     res = {"nl": nl, "iters": iters, "prof": prof, "breakdown": breakdown, "elapsed": elapsed,
+           "stage_tuning": st.stage_tuning(),
            "setup_s": round(t_setup, 3), "initial_cleaning_s": round(t_clean, 3),
            "value": args.steps / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
            "iterations_per_solve": sum(sum(i) for i in iters) / (2 * len(iters))}
@@ -262,7 +263,8 @@ def main():
                      "speedup_vs_plain": round(r["value"] / main_run["value"], 3),
                      "bicgstab_iterations_per_solve": round(r["iterations_per_solve"], 1),
                      "error_vs_fp64": field_difference(r["state"], st),
-                     "kernel_ms_one_step": {k: [v[0], round(v[1], 3)] for k, v in sorted(r["breakdown"].items())}}
+                     "kernel_ms_one_step": {k: [v[0], round(v[1], 3)] for k, v in sorted(r["breakdown"].items())},
+                     "stage_kernel_shapes": r["stage_tuning"]}
             entry["setup_s"], entry["initial_cleaning_s"] = r["setup_s"], r["initial_cleaning_s"]
             if guess in ("previous", "learned"):
                 entry["guess_fallbacks"] = r["state"].guess_fallbacks()
@@ -343,6 +345,7 @@ def main():
         "fp64_reference_run": fp64_run,
         "learned_training": training,
         "kernel_ms_one_step": {k: [v[0], round(v[1], 3)] for k, v in sorted(main_run["breakdown"].items())},
+        "stage_kernel_shapes": main_run["stage_tuning"],
     }
     print(json.dumps(out), flush=True)
 

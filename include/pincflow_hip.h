@@ -446,6 +446,16 @@ int pcf_profile_enable(pcf_state* st, int on);
 int pcf_profile_groups(pcf_state* st, uint32_t mask);
 /* Launch count and summed device time of one group since it was enabled. */
 int pcf_profile_query(pcf_state* st, int group, int64_t* launches, double* total_ms);
+/* Launch shapes the autotuner chose for the fused RK-stage kernels (no
+ * reference counterpart): shape[2 v + p] for variable v (0 rho, 1 rho', 2 u,
+ * 3 v, 4 w, 5 chi, 6 unused) and flux precision p (0 FP64, 1 FP32), -1 where
+ * not tuned yet; shape s means 1, 2 or 4 rows (s / 3) and 4, 8 or 16 levels
+ * (s % 3) per thread.  ms (may be null) gets the 9 timed shapes per slot in
+ * milliseconds per launch (14 x 9 floats). */
+int pcf_stage_tuning(const pcf_state* st, int32_t* shape, float* ms);
+/* Set the launch shape of one slot of pcf_stage_tuning (0..8), or -1 to
+ * have it tuned again on its next launch. */
+int pcf_set_stage_shape(pcf_state* st, int32_t slot, int32_t shape);
 
 /* ---- initial-guess network of the learned solve (no reference
  * counterpart) -------------------------------------------------------------

@@ -225,6 +225,8 @@ SIGNATURES = [
     ("pcf_profile_groups", _I, [_P, ctypes.c_uint32]),
     ("pcf_profile_enable", _I, [_P, _I]),
     ("pcf_profile_query", _I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _PD]),
+    ("pcf_stage_tuning", _I, [_P, _PI32, ctypes.POINTER(ctypes.c_float)]),
+    ("pcf_set_stage_shape", _I, [_P, ctypes.c_int32, ctypes.c_int32]),
 ]
 
 KERNEL_GROUPS = ["apply_total_dot", "pc_thomas", "pc_rhs", "compute_operator", "bicg_vector", "fluxes", "updates",

@@ -314,9 +314,10 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
   const size_t NH = g.nh(), NV = g.nv();
   double** haloed[] = {&st->drho, &st->drhop, &st->du, &st->dv, &st->dw, &st->rhoold, &st->rhopold, &st->uold,
                        &st->vold, &st->wold, &st->betar, &st->dpip, &st->corx, &st->cory, &st->wcart,
-                       &st->div_u, &st->div_v, &st->fdiv_w};
+                       &st->div_u, &st->div_v, &st->fdiv_w, &st->tune_q, &st->tune_dq, &st->tune_div};
   for (double** p : haloed)
     if ((rc = alloc(st, p, NH))) return fail(rc);
+  for (int& s : st->stage_shape) s = -1;
   for (int q = 0; q < (st->compressible ? 7 : 6); ++q) {
     if ((rc = alloc(st, &st->p0[q], NH))) return fail(rc);
     if ((rc = alloc(st, &st->p1[q], NH))) return fail(rc);
@@ -338,7 +339,9 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
       if ((rc = alloc(st, p, NH))) return fail(rc);
   for (int q = 0; q < 6; ++q)
     if ((rc = alloc(st, &st->stress[q], NH))) return fail(rc);
-  if (cfg->flux_precision == PCF_PRECISION_MIXED)
+  for (int d = 0; d < 3; ++d)
+    if ((rc = alloc(st, &st->carrier[d], NH))) return fail(rc);
+  if (cfg->flux_precision == PCF_PRECISION_MIXED) {
     // zeroed like the FP64 copies: the flux stencils read entries outside
     // the range the input kernels write (phi two levels below the ground),
     // which hold zeros there as in the reference's freshly allocated arrays
@@ -346,6 +349,9 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
       if ((q < 5 || st->tracer) && (rc = alloc32(st, &st->phi32[q], NH))) return fail(rc);
       if ((rc = alloc32(st, &st->stress32[q], NH))) return fail(rc);
     }
+    for (int d = 0; d < 3; ++d)
+      if ((rc = alloc32(st, &st->carrier32[d], NH))) return fail(rc);
+  }
   for (int q = 0; q < T_COUNT; ++q)
     if ((rc = alloc(st, &st->T.a[q], NV))) return fail(rc);
   double** vecs[] = {&st->rhs, &st->solution, &st->bp, &st->br0, &st->br, &st->bs, &st->bt, &st->bv,
@@ -631,6 +637,22 @@ int pcf_apply_bicgstab(pcf_state* st, double tolref, int32_t* niter) {
   int rc = solve(st, tolref, 0.0, &it);
   if (niter) *niter = it;
   return rc;
+}
+
+int pcf_stage_tuning(const pcf_state* st, int32_t* shape, float* ms) {
+  if (!st || !shape) return PCF_E_ARG;
+  for (int q = 0; q < 14; ++q) {
+    shape[q] = st->stage_shape[q];
+    if (ms)
+      for (int c = 0; c < 9; ++c) ms[9 * q + c] = st->stage_shape_ms[q][c];
+  }
+  return PCF_OK;
+}
+
+int pcf_set_stage_shape(pcf_state* st, int32_t slot, int32_t shape) {
+  if (!st || slot < 0 || slot >= 14 || shape < -1 || shape >= 9) return PCF_E_ARG;
+  st->stage_shape[slot] = shape;
+  return PCF_OK;
 }
 
 int pcf_dot(pcf_state* st, const double* a, const double* b, double* result) {

@@ -13,6 +13,7 @@
 // are written as zeros.
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 #include "pcf_device.hpp"
 
@@ -31,6 +32,11 @@ template <class T>
 struct FluxArgsT {
   Fields F;
   const T* phi[6];
+  // the carriers of the p0 winds at the faces, pe_x u0, pe_y v0 and pe_z w0
+  // (J P at the face times the wind, k_carriers), computed once per stage
+  const T* cx;
+  const T* cy;
+  const T* cz;
   const double* u0;
   const double* v0;
   const double* w0;
@@ -203,6 +209,22 @@ __device__ __forceinline__ T pe_z(const Fields& F, const Geo& g, int i, int j, i
   return LT(F.jac, c) * LT(F.jac, u) * (LT(F.pbar, c) + LT(F.pbar, u)) / (LT(F.jac, c) + LT(F.jac, u));
 }
 
+// The carrier of each face direction, pe u0 (compute_fluxes!.jl:359-520),
+// where the face's far cell exists: x faces for i < nxx, y faces for
+// j < nyy, z faces for k < nzz.
+template <class T>
+__global__ void k_carriers(Fields F, Geo g, const double* u0, const double* v0, const double* w0, T* cx, T* cy,
+                           T* cz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.nxx) return;
+  const size_t c = g.h(i, j, k);
+  if (i < g.nxx) cx[c] = pe_x<T>(F, g, i, j, k) * (T)u0[c];
+  if (j < g.nyy) cy[c] = pe_y<T>(F, g, i, j, k) * (T)v0[c];
+  if (k < g.nzz) cz[c] = pe_z<T>(F, g, i, j, k) * (T)w0[c];
+}
+
 // One flux component of one variable at face (i, j, k)
 // (compute_fluxes!.jl:359-520, 587-1503): a pure function of the
 // reconstruction inputs, the p0 winds, the stress components and the static
@@ -227,7 +249,7 @@ __device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int
         hi = hi + rhobaredger / pe;
         lo = lo + rhobaredger / pe;
       }
-      val = upwind_t<T>(pe_x<T>(F, g, i, j, k) * LT(A.u0, c), lo, hi);
+      val = upwind_t<T>(A.cx[c], lo, hi);
     } else if (DIR == 1) {
       if (VAR == V_RHO) {
         const size_t f = g.h(i, j + 1, k);
@@ -236,7 +258,7 @@ __device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int
         hi = hi + rhobaredgef / pe;
         lo = lo + rhobaredgef / pe;
       }
-      val = upwind_t<T>(pe_y<T>(F, g, i, j, k) * LT(A.v0, c), lo, hi);
+      val = upwind_t<T>(A.cy[c], lo, hi);
     } else {
       const size_t u = g.h(i, j, k + 1);
       const T jc = LT(F.jac, c), ju = LT(F.jac, u);
@@ -246,21 +268,21 @@ __device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int
         hi = hi + rhobaredgeu / pe;
         lo = lo + rhobaredgeu / pe;
       }
-      val = upwind_t<T>(pe_z<T>(F, g, i, j, k) * LT(A.w0, c), lo, hi);
+      val = upwind_t<T>(A.cz[c], lo, hi);
     }
   } else if (VAR == V_U) {
     const T* S11 = A.st[0];
     const T* S12 = A.st[1];
     const T* S13 = A.st[2];
     if (DIR == 0) {
-      T usurf = T(0.5) * (pe_x<T>(F, g, i, j, k) * LT(A.u0, c) + pe_x<T>(F, g, i + 1, j, k) * LT(A.u0, g.h(i + 1, j, k)));
+      T usurf = T(0.5) * (A.cx[c] + A.cx[g.h(i + 1, j, k)]);
       val = upwind_t<T>(usurf, right_of_c, left_of_next);
       if (A.viscous) {
         T coef_v = A.inv_re * LT(F.rhobar, g.h(i + 1, j, g.k0));
         val -= coef_v * LT(F.jac, g.h(i + 1, j, k)) * S11[g.h(i + 1, j, k)];
       }
     } else if (DIR == 1) {
-      T vsurf = T(0.5) * (pe_y<T>(F, g, i, j, k) * LT(A.v0, c) + pe_y<T>(F, g, i + 1, j, k) * LT(A.v0, g.h(i + 1, j, k)));
+      T vsurf = T(0.5) * (A.cy[c] + A.cy[g.h(i + 1, j, k)]);
       val = upwind_t<T>(vsurf, right_of_c, left_of_next);
       if (A.viscous) {
         const size_t r = g.h(i + 1, j, k), f = g.h(i, j + 1, k), rf = g.h(i + 1, j + 1, k);
@@ -270,7 +292,7 @@ __device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int
         val -= coef_v * T(0.25) * (LT(F.jac, c) * S12[c] + LT(F.jac, r) * S12[r] + LT(F.jac, f) * S12[f] + LT(F.jac, rf) * S12[rf]);
       }
     } else {
-      T wsurf = T(0.5) * (pe_z<T>(F, g, i, j, k) * LT(A.w0, c) + pe_z<T>(F, g, i + 1, j, k) * LT(A.w0, g.h(i + 1, j, k)));
+      T wsurf = T(0.5) * (A.cz[c] + A.cz[g.h(i + 1, j, k)]);
       val = upwind_t<T>(wsurf, right_of_c, left_of_next);
       if (A.viscous) {
         T coef_v = A.inv_re * T(0.5) * (LT(F.rhobar, g.h(i, j, g.k0)) + LT(F.rhobar, g.h(i + 1, j, g.k0)));
@@ -287,7 +309,7 @@ __device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int
     const T* S22 = A.st[3];
     const T* S23 = A.st[4];
     if (DIR == 0) {
-      T usurf = T(0.5) * (pe_x<T>(F, g, i, j, k) * LT(A.u0, c) + pe_x<T>(F, g, i, j + 1, k) * LT(A.u0, g.h(i, j + 1, k)));
+      T usurf = T(0.5) * (A.cx[c] + A.cx[g.h(i, j + 1, k)]);
       val = upwind_t<T>(usurf, right_of_c, left_of_next);
       if (A.viscous) {
         const size_t r = g.h(i + 1, j, k), f = g.h(i, j + 1, k), rf = g.h(i + 1, j + 1, k);
@@ -297,14 +319,14 @@ __device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int
         val -= coef_v * T(0.25) * (LT(F.jac, c) * S12[c] + LT(F.jac, r) * S12[r] + LT(F.jac, f) * S12[f] + LT(F.jac, rf) * S12[rf]);
       }
     } else if (DIR == 1) {
-      T vsurf = T(0.5) * (pe_y<T>(F, g, i, j, k) * LT(A.v0, c) + pe_y<T>(F, g, i, j + 1, k) * LT(A.v0, g.h(i, j + 1, k)));
+      T vsurf = T(0.5) * (A.cy[c] + A.cy[g.h(i, j + 1, k)]);
       val = upwind_t<T>(vsurf, right_of_c, left_of_next);
       if (A.viscous) {
         T coef_v = A.inv_re * LT(F.rhobar, g.h(i, j + 1, g.k0));
         val -= coef_v * LT(F.jac, g.h(i, j + 1, k)) * S22[g.h(i, j + 1, k)];
       }
     } else {
-      T wsurf = T(0.5) * (pe_z<T>(F, g, i, j, k) * LT(A.w0, c) + pe_z<T>(F, g, i, j + 1, k) * LT(A.w0, g.h(i, j + 1, k)));
+      T wsurf = T(0.5) * (A.cz[c] + A.cz[g.h(i, j + 1, k)]);
       val = upwind_t<T>(wsurf, right_of_c, left_of_next);
       if (A.viscous) {
         T coef_v = A.inv_re * T(0.5) * (LT(F.rhobar, g.h(i, j, g.k0)) + LT(F.rhobar, g.h(i, j + 1, g.k0)));
@@ -348,7 +370,7 @@ __device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int
                 LT(F.jac, f) * LT(F.jac, fu) * (S13[f] + S13[fu]) / (LT(F.jac, f) + LT(F.jac, fu)));
       }
     } else {
-      T wsurf = T(0.5) * (pe_z<T>(F, g, i, j, k) * LT(A.w0, c) + pe_z<T>(F, g, i, j, k + 1) * LT(A.w0, g.h(i, j, k + 1)));
+      T wsurf = T(0.5) * (A.cz[c] + A.cz[g.h(i, j, k + 1)]);
       val = upwind_t<T>(wsurf, right_of_c, left_of_next);
       if (A.viscous) {
         const size_t u = g.h(i, j, k + 1);
@@ -691,9 +713,10 @@ __global__ void k_update_w_fdiv(UpdArgs A, Range3 R) {
 // variable at the faces of a cell, their divergence, the RK3 update and the
 // LHS sponge in one pass, for the stage loop of explicit_integration!
 // (explicit_integration!.jl:42-101).  No flux array is stored: a thread
-// evaluates the x faces on both sides of its cells (flux_value), walks
-// kStageJ rows in y carrying the y face between them, and marches up a
-// chunk of kc levels carrying the z face of each row; the vertical wall
+// evaluates the x faces on both sides of its cells (flux_value), walks JC
+// rows in y carrying the y face between them, and marches up a chunk of kc
+// levels carrying the z face of each row (JC and kc: launch_stage's
+// autotuner); the vertical wall
 // faces, which set_boundaries!(state, BoundaryFluxes()) zeroes, are zero
 // here too.  The flux inputs (phi, the Cartesian w, the stress components)
 // are computed at the start of the stage and the p0 winds are fixed, so
@@ -707,8 +730,6 @@ __device__ __forceinline__ T zface(const FluxArgsT<T>& FA, int i, int j, int k) 
   if (VAR != V_W && (k == FA.g.k0 - 1 || k == FA.g.k1)) return T(0);
   return flux_value<VAR, 2, T>(FA, i, j, k);
 }
-
-constexpr int kStageJ = 4;  // rows of a fused stage kernel's thread
 
 template <int VAR, class T, int JC>
 __global__ void __launch_bounds__(256) k_stage_update(FluxArgsT<T> FA, UpdArgs A, Range3 R, int kc) {
@@ -1174,11 +1195,28 @@ int stress_tensor(pcf_state* st, bool f32 = false) {
 
 bool flux32(const pcf_state* st) { return st->cfg.flux_precision == PCF_PRECISION_MIXED; }
 
+// the face carriers of the p0 winds for the flux evaluation (FP32 copies with f32)
+int carriers(pcf_state* st, bool f32) {
+  const Geo& g = st->g;
+  const dim3 grd(blocks_for(g.nxx, 64), g.nyy, g.nzz), blk(64);
+  if (f32)
+    hipLaunchKernelGGL(k_carriers<float>, grd, blk, 0, st->stream, fields_of(st), g, st->p0[2], st->p0[3], st->p0[4],
+                       st->carrier32[0], st->carrier32[1], st->carrier32[2]);
+  else
+    hipLaunchKernelGGL(k_carriers<double>, grd, blk, 0, st->stream, fields_of(st), g, st->p0[2], st->p0[3],
+                       st->p0[4], st->carrier[0], st->carrier[1], st->carrier[2]);
+  PCF_HIP(hipGetLastError());
+  return PCF_OK;
+}
+
 // the FP32 flux inputs of the RK stages (flux_precision MIXED)
 FluxArgsT<float> flux_args32(pcf_state* st) {
   FluxArgsT<float> A;
   A.F = fields_of(st);
   for (int q = 0; q < 6; ++q) A.phi[q] = st->phi32[q];
+  A.cx = st->carrier32[0];
+  A.cy = st->carrier32[1];
+  A.cz = st->carrier32[2];
   A.u0 = st->p0[2];
   A.v0 = st->p0[3];
   A.w0 = st->p0[4];
@@ -1195,6 +1233,9 @@ FluxArgs flux_args(pcf_state* st) {
   FluxArgs A;
   A.F = fields_of(st);
   for (int q = 0; q < 6; ++q) A.phi[q] = st->phi[q];
+  A.cx = st->carrier[0];
+  A.cy = st->carrier[1];
+  A.cz = st->carrier[2];
   A.u0 = st->p0[2];
   A.v0 = st->p0[3];
   A.w0 = st->p0[4];
@@ -1279,6 +1320,7 @@ int compute_fluxes(pcf_state* st) {
   st->lhs_div_valid[0] = st->lhs_div_valid[1] = false;  // the stress arrays are rewritten below
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
   if ((rc = stress_tensor(st))) return rc;
+  if ((rc = carriers(st, false))) return rc;
   FluxArgs A = flux_args(st);
   const int nx = g.nx, ny = g.ny, nz = g.nz, i0 = g.i0, j0 = g.j0, k0 = g.k0;
   // faces the updates consume (update!.jl loop ranges); vertical wall faces
@@ -1460,6 +1502,7 @@ int stage_inputs(pcf_state* st) {
     int rc = reconstruct(st);
     if (rc) return rc;
     ProfScope prof(st, PCF_K_FLUXES);
+    if ((rc = carriers(st, false))) return rc;
     return stress_tensor(st);
   }
   // flux_precision MIXED: the same inputs, computed in FP64 and stored in FP32
@@ -1470,6 +1513,8 @@ int stage_inputs(pcf_state* st) {
                      st->cfg.model != PCF_BOUSSINESQ ? 1 : 0, st->chi, st->tracer ? st->phi32[V_CHI] : nullptr,
                      st->wcart, st->phi32[4]);
   PCF_HIP(hipGetLastError());
+  int rc = carriers(st, true);
+  if (rc) return rc;
   return stress_tensor(st, true);
 }
 
@@ -1478,20 +1523,79 @@ int stage_inputs(pcf_state* st) {
 // after stage_inputs: the sequence of explicit_integration!.jl:53-99 without
 // storing the fluxes.  The U and V updates leave their flux divergences in
 // div_u / div_v for the W update of the same stage.
+namespace {
+
+// Launch shapes of the fused stage kernels: rows per thread (the template
+// argument JC) x levels per thread (kc).  Every shape evaluates the same
+// faces with the same arithmetic, so the choice never changes a bit.
+constexpr int kTuneRows[3] = {1, 2, 4};
+constexpr int kTuneLevels[3] = {4, 8, 16};
+
+template <int VAR, class T>
+void launch_stage_shape(pcf_state* st, const FluxArgsT<T>& FA, const UpdArgs& A, const Range3& R, int shape) {
+  const int jc = kTuneRows[shape / 3], kc = kTuneLevels[shape % 3];
+  dim3 grd(blocks_for((long long)R.ni * ((R.nj + jc - 1) / jc), 256), (unsigned)((R.nk + kc - 1) / kc));
+  if (jc == 1) hipLaunchKernelGGL((k_stage_update<VAR, T, 1>), grd, dim3(256), 0, st->stream, FA, A, R, kc);
+  else if (jc == 2) hipLaunchKernelGGL((k_stage_update<VAR, T, 2>), grd, dim3(256), 0, st->stream, FA, A, R, kc);
+  else hipLaunchKernelGGL((k_stage_update<VAR, T, 4>), grd, dim3(256), 0, st->stream, FA, A, R, kc);
+}
+
+// The project's autotuner for the fused stage kernels: the first time a
+// variable's kernel runs on a state, every launch shape is timed on copies
+// of the variable and its increment (the state itself is not touched) and
+// the fastest is kept for the rest of the run.
+template <int VAR, class T>
+int launch_stage(pcf_state* st, const FluxArgsT<T>& FA, const UpdArgs& A, const Range3& R) {
+  if (R.ni <= 0 || R.nj <= 0 || R.nk <= 0) return PCF_OK;
+  const int slot = VAR * 2 + (sizeof(T) == 4 ? 1 : 0);
+  int& best = st->stage_shape[slot];
+  if (best < 0) {
+    UpdArgs T_args = A;
+    const size_t bytes = st->g.nh() * sizeof(double);
+    PCF_HIP(hipMemcpyAsync(st->tune_q, A.q, bytes, hipMemcpyDeviceToDevice, st->stream));
+    PCF_HIP(hipMemcpyAsync(st->tune_dq, A.dq, bytes, hipMemcpyDeviceToDevice, st->stream));
+    T_args.q = st->tune_q;
+    T_args.dq = st->tune_dq;
+    if (A.div_out) T_args.div_out = st->tune_div;
+    hipEvent_t e0, e1;
+    PCF_HIP(hipEventCreate(&e0));
+    PCF_HIP(hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    for (int shape = 0; shape < 9; ++shape) {
+      launch_stage_shape<VAR, T>(st, FA, T_args, R, shape);  // warm-up
+      PCF_HIP(hipEventRecord(e0, st->stream));
+      for (int rep = 0; rep < 2; ++rep) launch_stage_shape<VAR, T>(st, FA, T_args, R, shape);
+      PCF_HIP(hipEventRecord(e1, st->stream));
+      PCF_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      PCF_HIP(hipEventElapsedTime(&ms, e0, e1));
+      st->stage_shape_ms[slot][shape] = 0.5f * ms;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = shape;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  launch_stage_shape<VAR, T>(st, FA, A, R, best);
+  return PCF_OK;
+}
+
+}  // namespace
+
 int stage_update(pcf_state* st, double dt, int m, int var, double sdt) {
   const Geo& g = st->g;
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
-  constexpr int kc = 8;  // levels per thread (kStageJ rows each)
   UpdArgs A = upd_args(st, dt, m);
   sponge_args(st, var, sdt, A);
   Range3 R{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz};
   const bool f32 = flux32(st);
-  // k_stage_update<VAR, double> or, with flux_precision MIXED, <VAR, float>
-  auto launch = [&](auto k64, auto k32) {
-    if (R.ni <= 0 || R.nj <= 0 || R.nk <= 0) return;
-    dim3 grd(blocks_for((long long)R.ni * ((R.nj + kStageJ - 1) / kStageJ), 256), (unsigned)((R.nk + kc - 1) / kc));
-    if (f32) hipLaunchKernelGGL(k32, grd, dim3(256), 0, st->stream, flux_args32(st), A, R, kc);
-    else hipLaunchKernelGGL(k64, grd, dim3(256), 0, st->stream, flux_args(st), A, R, kc);
+  // k_stage_update<VAR, double, JC> or, with flux_precision MIXED, <VAR, float, JC>
+  int trc = PCF_OK;
+  auto launch = [&](auto tag) {
+    constexpr int VAR = decltype(tag)::value;
+    trc = f32 ? launch_stage<VAR, float>(st, flux_args32(st), A, R) : launch_stage<VAR, double>(st, flux_args(st), A, R);
   };
   if (var == PCF_VAR_P) {  // the carrier flux of P has no reconstruction: stored, then the update
     if (!st->compressible) return PCF_OK;
@@ -1516,34 +1620,39 @@ int stage_update(pcf_state* st, double dt, int m, int var, double sdt) {
       if (!pi) return PCF_OK;
       A.q = st->rho;
       A.dq = st->drho;
-      launch(k_stage_update<V_RHO, double, kStageJ>, k_stage_update<V_RHO, float, kStageJ>);
+      launch(std::integral_constant<int, V_RHO>());
+      if (trc) return trc;
       break;
     case PCF_VAR_RHOP:
       A.q = st->rhop;
       A.dq = st->drhop;
       A.heat = st->compressible ? 1 : 0;
-      launch(k_stage_update<V_RHOP, double, kStageJ>, k_stage_update<V_RHOP, float, kStageJ>);
+      launch(std::integral_constant<int, V_RHOP>());
+      if (trc) return trc;
       break;
     case PCF_VAR_CHI:
       if (!st->tracer) return PCF_OK;
       A.q = st->chi;
       A.dq = st->dchi;
       A.force0 = 1;
-      launch(k_stage_update<V_CHI, double, kStageJ>, k_stage_update<V_CHI, float, kStageJ>);
+      launch(std::integral_constant<int, V_CHI>());
+      if (trc) return trc;
       break;
     case PCF_VAR_U:
       A.q = st->u;
       A.dq = st->du;
       A.div_out = st->div_u;
       R = Range3{g.i0 - 1, g.nx + 1, g.j0, g.ny, g.k0, g.nz};
-      launch(k_stage_update<V_U, double, kStageJ>, k_stage_update<V_U, float, kStageJ>);
+      launch(std::integral_constant<int, V_U>());
+      if (trc) return trc;
       break;
     case PCF_VAR_V:
       A.q = st->v;
       A.dq = st->dv;
       A.div_out = st->div_v;
       R = Range3{g.i0, g.nx, g.j0 - 1, g.ny + 1, g.k0, g.nz};
-      launch(k_stage_update<V_V, double, kStageJ>, k_stage_update<V_V, float, kStageJ>);
+      launch(std::integral_constant<int, V_V>());
+      if (trc) return trc;
       break;
     case PCF_VAR_W:
       A.q = st->w;
@@ -1552,7 +1661,8 @@ int stage_update(pcf_state* st, double dt, int m, int var, double sdt) {
       A.div_v = st->div_v;
       A.fdiv = st->fdiv_w;
       R = Range3{g.i0, g.nx, g.j0, g.ny, g.k0, g.nz - 1};
-      launch(k_stage_update<V_W, double, kStageJ>, k_stage_update<V_W, float, kStageJ>);
+      launch(std::integral_constant<int, V_W>());
+      if (trc) return trc;
       {
         // the update, and with a fused sponge the top interior level too
         Range3 rw{g.i0, g.nx, g.j0, g.ny, g.k0, A.sponge ? g.nz : g.nz - 1};

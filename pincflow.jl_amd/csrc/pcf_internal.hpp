@@ -158,6 +158,10 @@ struct pcf_state {
   // the RK stages' flux evaluation reads (null otherwise)
   float* phi32[6] = {};
   float* stress32[6] = {};
+  // the face carriers of the p0 winds (k_carriers), FP64 and, with
+  // flux_precision MIXED, FP32
+  double* carrier[3] = {};
+  float* carrier32[3] = {};
   // between flux computations stress[0] and stress[1] hold the flux
   // divergences of the stage's U and V updates, which the W update reuses
   bool lhs_div_valid[2] = {false, false};
@@ -165,6 +169,12 @@ struct pcf_state {
   // read the stress arrays while they write these
   double *div_u = nullptr, *div_v = nullptr;
   double* fdiv_w = nullptr;  // the W flux divergence of a fused stage
+  // launch-shape autotuner of the fused stage kernels: the chosen shape per
+  // (variable, FP64/FP32) (-1: not tuned yet), the timings, and copies of a
+  // variable, its increment and its flux divergence the timing runs write
+  int stage_shape[14];
+  float stage_shape_ms[14][9] = {};
+  double *tune_q = nullptr, *tune_dq = nullptr, *tune_div = nullptr;
   // corrector
   double *dpip, *corx, *cory;
   // Poisson (interior)

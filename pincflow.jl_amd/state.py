@@ -218,6 +218,21 @@ class State:
         self._check(self.lib.pcf_guess_fallbacks(self.h, ctypes.byref(n)))
         return n.value
 
+    def stage_tuning(self) -> dict:
+        """The launch shapes the autotuner chose for the fused RK-stage
+        kernels: {"u/fp64": "2 rows x 8 levels (0.51 ms)", ...}."""
+        shape = (ctypes.c_int32 * 14)()
+        ms = (ctypes.c_float * (14 * 9))()
+        self._check(self.lib.pcf_stage_tuning(self.h, shape, ms))
+        names = ["rho", "rhop", "u", "v", "w", "chi", "p"]
+        out = {}
+        for q in range(14):
+            s = shape[q]
+            if s >= 0:
+                out[f"{names[q // 2]}/{'fp32' if q % 2 else 'fp64'}"] = (
+                    f"{(1, 2, 4)[s // 3]} rows x {(4, 8, 16)[s % 3]} levels ({ms[9 * q + s]:.3f} ms)")
+        return out
+
     def stage_fallbacks(self) -> int:
         """Solves rerun without the learned preconditioning stage after they
         failed to converge with it."""

@@ -383,6 +383,40 @@ def test_explicit_stage_fp32_fluxes(pcf, abi, oracle_mod, name):
         assert derr <= FLUX32_INCR_RTOL, (INCREMENTS[f], derr)
 
 
+@pytest.mark.parametrize("name", ["mountain_wave_3d", "periodic_hill"])
+@pytest.mark.parametrize("fluxes", ["fp64", "mixed"])
+def test_stage_launch_shapes_give_the_same_bits(pcf, abi, oracle_mod, name, fluxes):
+    """Every launch shape the autotuner of the fused stage kernels can pick
+    (rows and levels per thread) gives the same fields, bit for bit, and the
+    tuner records a choice for every variable it ran."""
+    import ctypes
+    results = []
+    for shape in range(9):
+        nl, gpu, orc = pair(pcf, abi, oracle_mod, name, flux_precision=fluxes)
+        names = predictands(nl)
+        perturb(gpu, orc, names, seed=7, amp=0.03)
+        for slot in range(14):
+            assert gpu.lib.pcf_set_stage_shape(gpu.h, slot, shape) == 0
+        gpu.set_boundaries(abi.BOUNDARY_PREDICTANDS)
+        gpu.compute_sponges(0.3, 2.0)
+        gpu.backup_predictands()
+        gpu.explicit_integration_lhs(0.3, 2.0)
+        results.append({f: gpu.get(f) for f in names})
+        gpu.close()
+    for r in results[1:]:
+        for f in r:
+            assert np.array_equal(r[f], results[0][f]), f
+    nl, gpu, orc = pair(pcf, abi, oracle_mod, name, flux_precision=fluxes)
+    gpu.set_boundaries(abi.BOUNDARY_PREDICTANDS)
+    gpu.compute_sponges(0.3, 2.0)
+    gpu.backup_predictands()
+    gpu.explicit_integration_lhs(0.3, 2.0)
+    tuned = gpu.stage_tuning()
+    print(name, fluxes, tuned)
+    want = {"rhop", "u", "v", "w"} | ({"rho"} if "RHO" in predictands(nl) else set())
+    assert {k.split("/")[0] for k in tuned} == want
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_implicit_stage(pcf, abi, oracle_mod, name):
     nl, gpu, orc = pair(pcf, abi, oracle_mod, name, tight=True)
