@@ -341,6 +341,8 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
     if ((rc = alloc(st, &st->stress[q], NH))) return fail(rc);
   for (int d = 0; d < 3; ++d)
     if ((rc = alloc(st, &st->carrier[d], NH))) return fail(rc);
+  for (int d = 0; d < 2; ++d)
+    if ((rc = alloc(st, &st->tstress[d], NH))) return fail(rc);
   if (cfg->flux_precision == PCF_PRECISION_MIXED) {
     // zeroed like the FP64 copies: the flux stencils read entries outside
     // the range the input kernels write (phi two levels below the ground),
@@ -351,6 +353,8 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
     }
     for (int d = 0; d < 3; ++d)
       if ((rc = alloc32(st, &st->carrier32[d], NH))) return fail(rc);
+    for (int d = 0; d < 2; ++d)
+      if ((rc = alloc32(st, &st->tstress32[d], NH))) return fail(rc);
   }
   for (int q = 0; q < T_COUNT; ++q)
     if ((rc = alloc(st, &st->T.a[q], NV))) return fail(rc);

@@ -41,6 +41,8 @@ struct FluxArgsT {
   const double* v0;
   const double* w0;
   const T* st[6];  // stress 11, 12, 13, 22, 23, 33
+  const T* t13;    // met13 s11 + met23 s12 + s13 / J (k_stress)
+  const T* t23;    // met13 s12 + met23 s22 + s23 / J
   double* flux[7][3];
   Geo g;
   T inv_re;        // 1 / re
@@ -99,9 +101,13 @@ __global__ void k_inputs(Fields F, Geo g, S* phr, S* phrp, S* phu, S* phv, int p
 }
 
 // compute_stress_tensor.jl:55-164: the six distinct components at one cell.
+// Besides the six components: the Cartesian-to-transformed combinations the
+// U and V vertical viscous fluxes take at each cell, t13 = met13 s11 +
+// met23 s12 + s13 / J and t23 = met13 s12 + met23 s22 + s23 / J
+// (compute_fluxes!.jl:743-765, 1047-1069), from the FP64 components.
 template <class S>
 __global__ void k_stress(Fields F, Geo g, const double* wc, S* s11, S* s12, S* s13, S* s22, S* s23, S* s33,
-                         Range3 R) {
+                         S* t13, S* t23, Range3 R) {
   int i, j, k;
   if (!range_ijk(R, blockIdx.x * (long long)blockDim.x + threadIdx.x, i, j, k)) return;
   const double *u = F.u, *v = F.v, *w = F.w, *jac = F.jac;
@@ -132,13 +138,20 @@ __global__ void k_stress(Fields F, Geo g, const double* wc, S* s11, S* s12, S* s
                 (jacedgeu * w[c] - jacedged * w[g.h(i, j, k - 1)]) / dz) /
                jac[c];
   double dwc = wc[c] - wc[g.h(i, j, k - 1)];
-  s11[c] = (S)(2.0 * (u[c] - u[g.h(i - 1, j, k)]) / dx + F.m13[c] * (uu - ud) / dz - div);
-  s12[c] = (S)(0.5 * (uf - ub) / dy + 0.5 * F.m23[c] * (uu - ud) / dz + 0.5 * (vr - vl) / dx +
-               0.5 * F.m13[c] * (vu - vd) / dz);
-  s13[c] = (S)(0.5 * (uu - ud) / dz / jac[c] + 0.5 * (wr - wl) / dx + F.m13[c] * dwc / dz);
-  s22[c] = (S)(2.0 * (v[c] - v[g.h(i, j - 1, k)]) / dy + F.m23[c] * (vu - vd) / dz - div);
-  s23[c] = (S)(0.5 * (vu - vd) / dz / jac[c] + 0.5 * (wf - wb) / dy + F.m23[c] * dwc / dz);
+  const double v11 = 2.0 * (u[c] - u[g.h(i - 1, j, k)]) / dx + F.m13[c] * (uu - ud) / dz - div;
+  const double v12 = 0.5 * (uf - ub) / dy + 0.5 * F.m23[c] * (uu - ud) / dz + 0.5 * (vr - vl) / dx +
+                     0.5 * F.m13[c] * (vu - vd) / dz;
+  const double v13 = 0.5 * (uu - ud) / dz / jac[c] + 0.5 * (wr - wl) / dx + F.m13[c] * dwc / dz;
+  const double v22 = 2.0 * (v[c] - v[g.h(i, j - 1, k)]) / dy + F.m23[c] * (vu - vd) / dz - div;
+  const double v23 = 0.5 * (vu - vd) / dz / jac[c] + 0.5 * (wf - wb) / dy + F.m23[c] * dwc / dz;
+  s11[c] = (S)v11;
+  s12[c] = (S)v12;
+  s13[c] = (S)v13;
+  s22[c] = (S)v22;
+  s23[c] = (S)v23;
   s33[c] = (S)(2.0 * dwc / dz / jac[c] - div);
+  t13[c] = (S)(F.m13[c] * v11 + F.m23[c] * v12 + v13 / jac[c]);
+  t23[c] = (S)(F.m13[c] * v12 + F.m23[c] * v22 + v23 / jac[c]);
 }
 
 // MUSCL face values from phi along a stride: right face of cell c and left
@@ -296,9 +309,9 @@ __device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int
       val = upwind_t<T>(wsurf, right_of_c, left_of_next);
       if (A.viscous) {
         T coef_v = A.inv_re * T(0.5) * (LT(F.rhobar, g.h(i, j, g.k0)) + LT(F.rhobar, g.h(i + 1, j, g.k0)));
-        auto s13 = [&](size_t q) { return LT(F.m13, q) * S11[q] + LT(F.m23, q) * S12[q] + S13[q] / LT(F.jac, q); };
+        const T* T13 = A.t13;
         const size_t r = g.h(i + 1, j, k), u = g.h(i, j, k + 1), ru = g.h(i + 1, j, k + 1);
-        T t0 = s13(c), tr = s13(r), tu = s13(u), tru = s13(ru);
+        T t0 = T13[c], tr = T13[r], tu = T13[u], tru = T13[ru];
         val -= coef_v * T(0.5) *
                (LT(F.jac, c) * LT(F.jac, u) * (t0 + tu) / (LT(F.jac, c) + LT(F.jac, u)) +
                 LT(F.jac, r) * LT(F.jac, ru) * (tr + tru) / (LT(F.jac, r) + LT(F.jac, ru)));
@@ -330,9 +343,9 @@ __device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int
       val = upwind_t<T>(wsurf, right_of_c, left_of_next);
       if (A.viscous) {
         T coef_v = A.inv_re * T(0.5) * (LT(F.rhobar, g.h(i, j, g.k0)) + LT(F.rhobar, g.h(i, j + 1, g.k0)));
-        auto s23 = [&](size_t q) { return LT(F.m13, q) * S12[q] + LT(F.m23, q) * S22[q] + S23[q] / LT(F.jac, q); };
+        const T* T23 = A.t23;
         const size_t f = g.h(i, j + 1, k), u = g.h(i, j, k + 1), fu = g.h(i, j + 1, k + 1);
-        T t0 = s23(c), tf = s23(f), tu = s23(u), tfu = s23(fu);
+        T t0 = T23[c], tf = T23[f], tu = T23[u], tfu = T23[fu];
         val -= coef_v * T(0.5) *
                (LT(F.jac, c) * LT(F.jac, u) * (t0 + tu) / (LT(F.jac, c) + LT(F.jac, u)) +
                 LT(F.jac, f) * LT(F.jac, fu) * (tf + tfu) / (LT(F.jac, f) + LT(F.jac, fu)));
@@ -1184,11 +1197,11 @@ int stress_tensor(pcf_state* st, bool f32 = false) {
   if (f32)
     hipLaunchKernelGGL(k_stress<float>, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, fields_of(st),
                        g, st->wcart, st->stress32[0], st->stress32[1], st->stress32[2], st->stress32[3],
-                       st->stress32[4], st->stress32[5], R);
+                       st->stress32[4], st->stress32[5], st->tstress32[0], st->tstress32[1], R);
   else
     hipLaunchKernelGGL(k_stress<double>, dim3(blocks_for(R.count(), 256)), dim3(256), 0, st->stream, fields_of(st),
                        g, st->wcart, st->stress[0], st->stress[1], st->stress[2], st->stress[3], st->stress[4],
-                       st->stress[5], R);
+                       st->stress[5], st->tstress[0], st->tstress[1], R);
   PCF_HIP(hipGetLastError());
   return PCF_OK;
 }
@@ -1221,6 +1234,8 @@ FluxArgsT<float> flux_args32(pcf_state* st) {
   A.v0 = st->p0[3];
   A.w0 = st->p0[4];
   for (int q = 0; q < 6; ++q) A.st[q] = st->stress32[q];
+  A.t13 = st->tstress32[0];
+  A.t23 = st->tstress32[1];
   for (int q = 0; q < 7; ++q)
     for (int d = 0; d < 3; ++d) A.flux[q][d] = nullptr;
   A.g = st->g;
@@ -1240,6 +1255,8 @@ FluxArgs flux_args(pcf_state* st) {
   A.v0 = st->p0[3];
   A.w0 = st->p0[4];
   for (int q = 0; q < 6; ++q) A.st[q] = st->stress[q];
+  A.t13 = st->tstress[0];
+  A.t23 = st->tstress[1];
   for (int q = 0; q < 7; ++q)
     for (int d = 0; d < 3; ++d) A.flux[q][d] = st->flux[q][d];
   A.g = st->g;

@@ -162,6 +162,10 @@ struct pcf_state {
   // flux_precision MIXED, FP32
   double* carrier[3] = {};
   float* carrier32[3] = {};
+  // t13, t23 of the stress kernel (the U and V vertical viscous fluxes' cell
+  // values), FP64 and FP32
+  double* tstress[2] = {};
+  float* tstress32[2] = {};
   // between flux computations stress[0] and stress[1] hold the flux
   // divergences of the stage's U and V updates, which the W update reuses
   bool lhs_div_valid[2] = {false, false};
