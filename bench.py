@@ -50,6 +50,9 @@ def parse():
                         "physics over 16 seeded hills (the mixed-precision configuration, run at --nx 512); "
                         "ensemble: --members seeded bubble members sharing the GPU, against one at a time")
     p.add_argument("--members", type=int, default=16, help="ensemble workload: number of members")
+    p.add_argument("--ensemble-modes", default="alone,streams,batched",
+                   help="ensemble workload: comma-separated runs among alone (one member at a time), streams "
+                        "(all members at once, each solve on its own) and batched (all at once, solves batched)")
     p.add_argument("--model", default="PseudoIncompressible", choices=["PseudoIncompressible", "Compressible"],
                    help="dynamic equations of the mountain-wave / hill workloads")
     p.add_argument("--nx", type=int, default=256)
@@ -106,7 +109,7 @@ def workload(pcf, args, iterations, precision):
 
 
 def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False, storage=None, mg_coarse=None,
-               relative=False, stage_net=None, fluxes="fp64"):
+               relative=False, stage_net=None, fluxes="fp64", profile_apply=True):
     """Setup, initial cleaning, warmup, then the timed steps of one method,
     with HIP events around the launches of the operator apply only (its
     average duration gives the roofline figure; events around every launch
@@ -141,7 +144,10 @@ def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False
         info = st.step()
         print(f"# [{tag}] warmup step {w + 1}: dt {info.dt:.3f}s iterations {list(info.niter)}", file=sys.stderr,
               flush=True)
-    st.profile(True, groups=["apply_total_dot"])
+    if profile_apply:
+        # events around the operator apply only (the roofline kernel); a
+        # profiled state launches its BiCGSTAB iterations kernel by kernel
+        st.profile(True, groups=["apply_total_dot"])
     iters = []
     t0 = time.perf_counter()
     for s in range(args.steps):
@@ -249,7 +255,8 @@ def main():
                                            f"{vtrain.get('held_out_stage_residual_vs_multigrid')}"})
                 continue
             r = run_method(pcf, args, "mixed", pc, guess, net=vnet if guess == "learned" else None, keep=True,
-                           storage=storage, mg_coarse=coarse, stage_net=vstage if staged else None, fluxes=fluxes)
+                           storage=storage, mg_coarse=coarse, stage_net=vstage if staged else None, fluxes=fluxes,
+                           profile_apply=False)
             desc = pc.replace("_", " ") + " preconditioner"
             if pc == "multigrid":
                 desc += f" ({args.mg_smoothing_steps} smoothing steps, {coarse} coarsest-level steps)"
@@ -361,29 +368,43 @@ def ensemble_bench(pcf, args):
             nl.poisson.preconditioner_type = args.preconditioner
             nl.poisson.preconditioner_storage = args.storage
             nl.poisson.mg_coarse_steps = args.mg_coarse_steps
+            nl.discretization.flux_precision = args.flux_precision or "mixed"
         return nl
     params = {"seed": tuple(range(args.members)),
               "output_file": tuple(f"member{m}.npz" for m in range(args.members))}
     cells = [sim(m).domain for m in range(args.members)]
     cells = sum(d.x_size * d.y_size * d.z_size for d in cells)
     runs = {}
-    for label, conc in (("one at a time", 1), ("sharing the GPU", args.members)):
+    modes = {"alone": ("one at a time", 1, False), "streams": ("sharing the GPU", args.members, False),
+             "batched": ("sharing the GPU, batched solves", args.members, True)}
+    for mode in args.ensemble_modes.split(","):
+        label, conc, batched = modes[mode]
+        batch = pcf.SolveBatch() if batched else None
         t0 = time.perf_counter()
-        res = pcf.ensemble(sim, params, {}, max_concurrent=conc, write=False)
+        def progress(member, itime, info, label=label, conc=conc):
+            if (member == 1 or conc == 1) and itime % 10 == 0:
+                print(f"# [ensemble] {label}: member {member} at step {itime}", file=sys.stderr, flush=True)
+        res = pcf.ensemble(sim, params, {}, max_concurrent=conc, write=False, batched=batched, batch=batch,
+                           on_step=progress)
         elapsed = time.perf_counter() - t0
         its = [sum(s[2] + s[3] for s in r[2]) / (2 * len(r[2])) for r in res]
         for r in res:
             r[0].close()
         runs[label] = {"member_steps_per_s": round(args.members * args.steps / elapsed, 3),
                        "wall_s": round(elapsed, 2), "bicgstab_iterations_per_solve": round(sum(its) / len(its), 1)}
+        if batch is not None:
+            runs[label]["batch"] = batch.stats()
+            batch.close()
         print(f"# [ensemble] {label}: {elapsed:.1f}s", file=sys.stderr, flush=True)
+    base = runs.get("one at a time", runs.get("sharing the GPU"))
     print(json.dumps({"workload": f"seeded bubble ensemble, {args.members} members, {args.steps} steps each "
                                   "(setup and initial cleaning included)",
                       "cells": cells, "precision": args.precision,
-                      **({"preconditioner": args.preconditioner} if args.precision == "mixed" else {}),
+                      **({"preconditioner": args.preconditioner,
+                          "flux_precision": args.flux_precision or "mixed"} if args.precision == "mixed" else {}),
                       "runs": runs,
-                      "speedup": round(runs["sharing the GPU"]["member_steps_per_s"] /
-                                       runs["one at a time"]["member_steps_per_s"], 3)}), flush=True)
+                      "speedup": {k: round(v["member_steps_per_s"] / base["member_steps_per_s"], 3)
+                                  for k, v in runs.items()}}), flush=True)
 
 
 def train_guess(pcf, args, preconditioner=None):

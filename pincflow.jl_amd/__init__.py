@@ -11,11 +11,11 @@ from .namelists import (AgnesiHill, AtmosphereNamelist, CosineBubble, CosineHill
                         Sin2Sponge, Sin2TopSponge, SinhStretching, SpongeNamelist, TracerDipole, TracerNamelist,
                         Vortex, WKBNamelist, to_config)
 from . import examples, learned, output
-from .ensemble import EnsembleError, ensemble
+from .ensemble import EnsembleError, SolveBatch, ensemble
 from .state import PinCFlowError, PoissonNotConverged, State, integrate
 
 __all__ = [
-    "State", "integrate", "ensemble", "EnsembleError", "PinCFlowError", "PoissonNotConverged",
+    "State", "integrate", "ensemble", "EnsembleError", "SolveBatch", "PinCFlowError", "PoissonNotConverged",
     "AgnesiHill", "AtmosphereNamelist", "CosineBubble", "CosineHill", "DiscretizationNamelist", "DomainNamelist",
     "GaussianHills", "GridNamelist", "Namelists", "OutputNamelist", "PoissonNamelist", "Sin2Sponge",
     "Sin2TopSponge", "SinhStretching", "SpongeNamelist", "TracerDipole", "TracerNamelist", "Vortex", "WKBNamelist",

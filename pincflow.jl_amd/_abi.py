@@ -13,7 +13,7 @@ from pathlib import Path
 PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "lib" / "libpincflow_hip.so"
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 MAX_HILLS = 32
 
 # return codes
@@ -227,6 +227,13 @@ SIGNATURES = [
     ("pcf_profile_query", _I, [_P, _I, ctypes.POINTER(ctypes.c_int64), _PD]),
     ("pcf_stage_tuning", _I, [_P, _PI32, ctypes.POINTER(ctypes.c_float)]),
     ("pcf_set_stage_shape", _I, [_P, ctypes.c_int32, ctypes.c_int32]),
+    ("pcf_set_solver_graphs", _I, [_P, ctypes.c_int32]),
+    ("pcf_ensemble_create", _I, [ctypes.POINTER(ctypes.c_void_p)]),
+    ("pcf_ensemble_destroy", _I, [_P]),
+    ("pcf_ensemble_attach", _I, [_P, _P]),
+    ("pcf_ensemble_detach", _I, [_P]),
+    ("pcf_ensemble_stats", _I, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    ("pcf_ensemble_set_graphs", _I, [_P, ctypes.c_int32]),
 ]
 
 KERNEL_GROUPS = ["apply_total_dot", "pc_thomas", "pc_rhs", "compute_operator", "bicg_vector", "fluxes", "updates",

@@ -151,7 +151,7 @@ int save_backups(pcf_state* st, bool rho, bool rhop, bool u, bool v, bool w) {  
 int explicit_lhs(pcf_state* st, double dt, double time) {
   for (int m = 1; m <= 3; ++m) {
     const double sdt = st->stepfrac[m - 1] * dt;
-    TRY(stage_inputs(st));
+    TRY(stage_inputs(st, m));
     st->lhs_div_valid[0] = st->lhs_div_valid[1] = false;  // the stress arrays were rewritten
     TRY(save_backups(st, true, false, true, true, false));
     // the sponges that the fused kernels do not apply themselves follow
@@ -394,8 +394,12 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
 
 int pcf_destroy(pcf_state* st) {
   if (!st) return PCF_OK;
+  if (st->ens) (void)pcf_ensemble_detach(st);
   if (st->stream) (void)hipStreamSynchronize(st->stream);
+  if (st->ens_ready) (void)hipEventDestroy(st->ens_ready);
   prof_release(st);
+  for (hipGraphExec_t& e : st->iter_graph)
+    if (e) (void)hipGraphExecDestroy(e);
   for (void* p : st->allocations) (void)hipFree(p);
   if (st->sc_host) (void)hipHostFree(st->sc_host);
   if (st->cfl_host) (void)hipHostFree(st->cfl_host);
@@ -650,6 +654,12 @@ int pcf_stage_tuning(const pcf_state* st, int32_t* shape, float* ms) {
     if (ms)
       for (int c = 0; c < 9; ++c) ms[9 * q + c] = st->stage_shape_ms[q][c];
   }
+  return PCF_OK;
+}
+
+int pcf_set_solver_graphs(pcf_state* st, int32_t on) {
+  if (!st) return PCF_E_ARG;
+  st->solver_graphs = on != 0;
   return PCF_OK;
 }
 

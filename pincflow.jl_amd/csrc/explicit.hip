@@ -64,10 +64,12 @@ typedef FluxArgsT<double> FluxArgs;
 template <class S>
 __global__ void k_inputs(Fields F, Geo g, S* phr, S* phrp, S* phu, S* phv, int pi, const double* chi, S* phchi,
                          double* wc, S* phw) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > g.nxx) return;
+  // 256 threads over the (i, j) of one level, x fastest
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nxx * g.nyy) return;
+  const int j = 1 + t / g.nxx;
+  const int i = 1 + t - (j - 1) * g.nxx;
+  const int k = 1 + blockIdx.y;
   const size_t c = g.h(i, j, k);
   const bool layer = k >= g.k0 - 1 && k <= g.k1 + 1;
   if (layer) {
@@ -1174,8 +1176,8 @@ int reconstruct(pcf_state* st) {
   const Geo& g = st->g;
   Fields F = fields_of(st);
   const bool pi = st->cfg.model != PCF_BOUSSINESQ;
-  hipLaunchKernelGGL(k_inputs<double>, dim3(blocks_for(g.nxx, 64), g.nyy, g.nzz - 1), dim3(64), 0, st->stream, F,
-                     g, st->phi[0], st->phi[1], st->phi[2], st->phi[3], pi ? 1 : 0, st->chi,
+  hipLaunchKernelGGL(k_inputs<double>, dim3(blocks_for((long long)g.nxx * g.nyy, 256), g.nzz - 1), dim3(256), 0,
+                     st->stream, F, g, st->phi[0], st->phi[1], st->phi[2], st->phi[3], pi ? 1 : 0, st->chi,
                      st->tracer ? st->phi[V_CHI] : nullptr, st->wcart, st->phi[4]);
   PCF_HIP(hipGetLastError());
   return PCF_OK;
@@ -1513,24 +1515,26 @@ int update_lhs(pcf_state* st, double dt, int m, int var) {
 }
 
 // The inputs of a stage's fluxes: reconstruct! and the stress tensor from
-// the current predictands (compute_fluxes without its flux kernels).
-int stage_inputs(pcf_state* st) {
+// the current predictands (compute_fluxes without its flux kernels), and on
+// the first stage (m = 1) the face carriers, which depend only on the p0
+// winds, J and pbar, none of which the three RK stages change.
+int stage_inputs(pcf_state* st, int m) {
   if (!flux32(st)) {
     int rc = reconstruct(st);
     if (rc) return rc;
     ProfScope prof(st, PCF_K_FLUXES);
-    if ((rc = carriers(st, false))) return rc;
+    if (m == 1 && (rc = carriers(st, false))) return rc;
     return stress_tensor(st);
   }
   // flux_precision MIXED: the same inputs, computed in FP64 and stored in FP32
   ProfScope prof(st, PCF_K_FLUXES);
   const Geo& g = st->g;
-  hipLaunchKernelGGL(k_inputs<float>, dim3(blocks_for(g.nxx, 64), g.nyy, g.nzz - 1), dim3(64), 0, st->stream,
-                     fields_of(st), g, st->phi32[0], st->phi32[1], st->phi32[2], st->phi32[3],
+  hipLaunchKernelGGL(k_inputs<float>, dim3(blocks_for((long long)g.nxx * g.nyy, 256), g.nzz - 1), dim3(256), 0,
+                     st->stream, fields_of(st), g, st->phi32[0], st->phi32[1], st->phi32[2], st->phi32[3],
                      st->cfg.model != PCF_BOUSSINESQ ? 1 : 0, st->chi, st->tracer ? st->phi32[V_CHI] : nullptr,
                      st->wcart, st->phi32[4]);
   PCF_HIP(hipGetLastError());
-  int rc = carriers(st, true);
+  int rc = m == 1 ? carriers(st, true) : PCF_OK;
   if (rc) return rc;
   return stress_tensor(st, true);
 }

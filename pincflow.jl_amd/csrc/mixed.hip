@@ -142,14 +142,15 @@ __device__ __forceinline__ void ld_coef(const CT* p, float (&o)[W]) {
 // 64 x (4 W) cells of one z-level; CT: the coefficients' storage.  PRO: the
 // iterate is s + omega P e, the multigrid prolongation of the coarse-level
 // correction fused into the first post-smoothing step.
+// The body takes the block's index, so that the ensemble kernel can run it
+// for a block of any member.
 template <int W, typename SinT, typename CT = float, bool PRO = false>
-__global__ void __launch_bounds__(256) k_pc_rhs32(PcArgs<CT> A, const SinT* __restrict__ sin, float deta,
-                                                  const SolverScalars* sc) {
-  if (sc && sc->done) return;
+__device__ __forceinline__ void pc_rhs32(const PcArgs<CT>& A, const SinT* __restrict__ sin, float deta, int bx,
+                                         int by, int bz) {
   constexpr int BX = 64 / W;
-  const int i0 = (blockIdx.x * BX + threadIdx.x) * W;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
+  const int i0 = (bx * BX + threadIdx.x) * W;
+  const int j = by * (4 * W) + threadIdx.y;
+  const int k = bz;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
   if (i0 >= nx || j >= ny) return;
   const size_t sxy = (size_t)nx * ny;
@@ -223,6 +224,12 @@ __global__ void __launch_bounds__(256) k_pc_rhs32(PcArgs<CT> A, const SinT* __re
   }
   st_stream<W>(A.out + o, out);
 }
+template <int W, typename SinT, typename CT = float, bool PRO = false>
+__global__ void __launch_bounds__(256) k_pc_rhs32(PcArgs<CT> A, const SinT* __restrict__ sin, float deta,
+                                                  const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  pc_rhs32<W, SinT, CT, PRO>(A, sin, deta, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 // The preconditioner's line solve in z with FP32 factors (the recurrence of
 // the FP64 k_pc_thomas); SrcT / OutT: FP64 or FP32 input and output vectors.
@@ -237,15 +244,13 @@ constexpr int kThomas32Lds = 128;
 constexpr size_t kThomas32LdsColumns = 32768;
 
 template <bool FIRST, typename SrcT, typename OutT, int NLDS = kThomas32Lds>
-__global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float* E, const float* Q, const SrcT* src,
-                                                    OutT* out, float* col, int nx, int ny, int nz, float deta,
-                                                    const SolverScalars* sc) {
+__device__ __forceinline__ void pc_thomas32(const float* P, const float* E, const float* Q, const SrcT* src, OutT* out,
+                                            float* col, int nx, int ny, int nz, float deta, int bx, int by) {
   __shared__ float up[NLDS > 0 ? NLDS : 1][64];
-  if (sc && sc->done) return;
   constexpr int U = 8;
   const int tid = threadIdx.x;
-  const int i = blockIdx.x * blockDim.x + tid;
-  const int j = blockIdx.y;
+  const int i = bx * 64 + tid;
+  const int j = by;
   if (i >= nx) return;
   const size_t sxy = (size_t)nx * ny;
   const size_t c = (size_t)i + (size_t)nx * j;
@@ -301,6 +306,13 @@ __global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float*
     sk = kept(k) + Q[c + sxy * k] * sk;
     out[c + sxy * k] = (OutT)sk;
   }
+}
+template <bool FIRST, typename SrcT, typename OutT, int NLDS = kThomas32Lds>
+__global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float* E, const float* Q, const SrcT* src,
+                                                    OutT* out, float* col, int nx, int ny, int nz, float deta,
+                                                    const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  pc_thomas32<FIRST, SrcT, OutT, NLDS>(P, E, Q, src, out, col, nx, ny, nz, deta, blockIdx.x, blockIdx.y);
 }
 
 __global__ void k_to_f32(const double* a, float* b, size_t n) {
@@ -1118,6 +1130,94 @@ int apply_bicgstab_mixed(pcf_state* st, double tolref, double dt, int* niter_out
     st->m_prev_valid[slot] = true;
   }
   return PCF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble batch: the FP32 line preconditioner over the members of a table
+// (see the ensemble kernels of poisson.hip).  Each member runs the blocks of
+// its single solve's kernels (precondition_mixed; x_size a multiple of 4).
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ int ens_member32(const int* start, int count, int b) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// FIRST: from the FP64 vector v[in]; else from the member's FP32 rhs.
+// OutT double: to v[out]; float: to the member's FP32 iterate.
+template <bool FIRST, typename OutT>
+__global__ void __launch_bounds__(64) kEm_thomas(const EnsTable* __restrict__ E, int in, int out) {
+  const int m = ens_member32(E->thomas_start, E->count, blockIdx.x);
+  const EnsMember& M = E->m[m];
+  if (M.sc->done) return;
+  const int b = blockIdx.x - E->thomas_start[m];
+  const int gx = (M.nx + 63) / 64;
+  OutT* dst;
+  if constexpr (sizeof(OutT) == 8) dst = M.v[out];
+  else dst = M.cur32;
+  const float deta = (float)M.deta;
+  if constexpr (FIRST)
+    pc_thomas32<true, double, OutT>(M.P32, M.E32, M.Q32, (const double*)M.v[in], dst, M.col32, M.nx, M.ny, M.nz,
+                                    deta, b % gx, b / gx);
+  else
+    pc_thomas32<false, float, OutT>(M.P32, M.E32, M.Q32, (const float*)M.rhs32, dst, M.col32, M.nx, M.ny, M.nz,
+                                    deta, b % gx, b / gx);
+}
+
+// rhs32 = cur32 + deta (H cur32 - v[in]) with the coefficients in storage CT
+template <typename CT>
+__global__ void __launch_bounds__(256) kEm_rhs(const EnsTable* __restrict__ E, int in) {
+  const int m = ens_member32(E->rhs32_start, E->count, blockIdx.x);
+  const EnsMember& M = E->m[m];
+  if (M.sc->done) return;
+  const int b = blockIdx.x - E->rhs32_start[m];
+  const int gx = (M.nx + 63) / 64, gy = (M.ny + 15) / 16;
+  PcArgs<CT> A;
+  for (int t = 0; t < T_COUNT; ++t) {
+    if constexpr (sizeof(CT) == 2) A.a[t] = M.T16[t];
+    else A.a[t] = M.T32[t];
+  }
+  A.s = M.cur32;
+  A.out = M.rhs32;
+  A.nx = M.nx;
+  A.ny = M.ny;
+  A.nz = M.nz;
+  A.e = nullptr;
+  A.omega = 0.0f;
+  A.sx = A.sy = 0;
+  A.cl = Lay{1, 1, 1, false};
+  pc_rhs32<4, double, CT>(A, (const double*)M.v[in], (float)M.deta, b % gx, (b / gx) % gy, b / (gx * gy));
+}
+
+}  // namespace
+
+int ensemble_rhs32_blocks(const pcf_state* st) {
+  if (st->cfg.precision != PCF_PRECISION_MIXED) return 0;
+  const Geo& g = st->g;
+  return (int)(blocks_for(g.nx, 64) * blocks_for(g.ny, 16) * g.nz);
+}
+
+// M^-1 v[in] -> v[out] for every member (precondition_mixed's line branch)
+void ensemble_precondition_mixed(hipStream_t s, const EnsTable* dT, const EnsTable& H, int iters, bool bf16, int in,
+                                 int out) {
+  const int tb = H.thomas_start[H.count], rb = H.rhs32_start[H.count];
+  if (iters == 1) {
+    hipLaunchKernelGGL((kEm_thomas<true, double>), dim3(tb), dim3(64), 0, s, dT, in, out);
+    return;
+  }
+  hipLaunchKernelGGL((kEm_thomas<true, float>), dim3(tb), dim3(64), 0, s, dT, in, out);
+  for (int it = 2; it <= iters; ++it) {
+    if (bf16) hipLaunchKernelGGL(kEm_rhs<uint16_t>, dim3(rb), dim3(16, 16), 0, s, dT, in);
+    else hipLaunchKernelGGL(kEm_rhs<float>, dim3(rb), dim3(16, 16), 0, s, dT, in);
+    if (it == iters) hipLaunchKernelGGL((kEm_thomas<false, double>), dim3(tb), dim3(64), 0, s, dT, in, out);
+    else hipLaunchKernelGGL((kEm_thomas<false, float>), dim3(tb), dim3(64), 0, s, dT, in, out);
+  }
 }
 
 // Allocation of the mixed-precision buffers (pcf_create, precision MIXED).

@@ -10,8 +10,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
 #include <cstddef>
 #include <cstdint>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -98,7 +101,7 @@ struct SolverScalars {
   double tol;
   double tolref;
   double divl2, divl2_norm;
-  int32_t done;    // 1 once res / b_norm <= tol
+  int32_t done;    // 1 once res / b_norm <= tol; 2: stopped at the ensemble solve's maxit
   int32_t niter;
   int32_t pad[2];
 };
@@ -112,6 +115,39 @@ constexpr int kMaxPartials = 4096;    // partial sums per reduction
 constexpr int kMaxDots = 4;
 
 // ---------------------------------------------------------------------------
+// ensemble solve: one launch per BiCGSTAB kernel over the members whose
+// solves wait together (pcf_ensemble); the table lives in device memory
+// ---------------------------------------------------------------------------
+constexpr int kEnsMax = 128;  // members of one batch
+// the solver vectors of a member, by index
+enum { EV_X = 0, EV_R0, EV_R, EV_P, EV_V, EV_S, EV_T, EV_Y, EV_Z, EV_PCTMP, EV_PCRHS, EV_COUNT };
+struct EnsMember {
+  Tensor T;                         // FP64 operator
+  double* v[EV_COUNT];
+  const double *P, *E, *Q;          // FP64 line-solve factors
+  const float *P32, *E32, *Q32;     // mixed: FP32 factors, coefficients and work
+  const float* T32[T_COUNT];
+  const uint16_t* T16[T_COUNT];     // bf16 storage (else null)
+  float *col32, *cur32, *rhs32;
+  double* partials;
+  SolverScalars* sc;
+  size_t n;                         // cells
+  double nxyz;                      // x_size y_size z_size
+  double deta;
+  int nx, ny, nz;
+  int aparts;                       // partials of the dot-fused apply
+  int maxit;                        // poisson_iterations
+  int pad;
+};
+struct EnsTable {
+  int count;
+  int pair_start[kEnsMax + 1];      // first block of each member in the apply grids
+  int thomas_start[kEnsMax + 1];    // ... in the line-solve grids
+  int rhs32_start[kEnsMax + 1];     // ... in the FP32 pc-rhs grid (mixed)
+  EnsMember m[kEnsMax];
+};
+
+// ---------------------------------------------------------------------------
 // host-side state
 // ---------------------------------------------------------------------------
 struct HostSetup;  // setup.cpp
@@ -119,6 +155,35 @@ struct HostSetup;  // setup.cpp
 }  // namespace pcf
 
 struct pcf_net;
+
+// Members of an ensemble whose BiCGSTAB solves run together: a member's
+// solve waits until every attached member waits (or steps out), and the
+// last to arrive runs the batch for all of them (pcf_ensemble_attach).
+struct pcf_ensemble {
+  std::mutex mu;
+  std::condition_variable cv;
+  int attached = 0;                      // members that will arrive at a solve or detach
+  std::vector<pcf_state*> waiting;       // members whose solve waits for the next batch
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // members batch together when they share the solve's kind (FP64 or mixed,
+  // preconditioner iterations, coefficient storage): per kind the device
+  // table, its host copy, the members it was built for and the captured
+  // iteration
+  struct Group {
+    int key = -1;
+    pcf::EnsTable* table = nullptr;
+    std::unique_ptr<pcf::EnsTable> host;
+    std::vector<pcf_state*> members;
+    hipGraphExec_t graph = nullptr;
+  };
+  std::vector<std::unique_ptr<Group>> groups;
+  int32_t* flags_dev = nullptr;          // done flags of the batch's members
+  int32_t* flags_host = nullptr;         // pinned, two slots of kEnsMax
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool graphs = true;
+  int64_t batches = 0, iterations = 0, member_solves = 0, stepped_out = 0;
+};
 
 struct pcf_state {
   pcf_config cfg;
@@ -218,6 +283,16 @@ struct pcf_state {
   double* stage_norm = nullptr;
   int stage_fallbacks = 0;            // solves rerun without the stage after it failed to converge
   double* train_buf[3] = {};          // network training work: x, A x - v, A^T (A x - v)
+  // the BiCGSTAB iteration captured as a hipGraph (FP64 / mixed solve),
+  // instantiated on first use and replayed once per iteration
+  hipGraphExec_t iter_graph[2] = {nullptr, nullptr};
+  bool solver_graphs = true;
+  // ensemble membership (pcf_ensemble_attach): the ensemble, the event a
+  // batch waits for on this state's stream, and the result of the batch
+  pcf_ensemble* ens = nullptr;
+  hipEvent_t ens_ready = nullptr;
+  bool ens_mixed = false;
+  int ens_rc = -1;                    // -1 while the solve waits
   double* partials;                   // reduction partials
   pcf::SolverScalars* sc;             // device scalars
   pcf::SolverScalars* sc_host;        // pinned copy
@@ -292,6 +367,13 @@ int apply_bicgstab(pcf_state* st, double tolref, int* niter);
 int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess, double guess_scale, int* niter);
 int solve(pcf_state* st, double tolref, double dt, int* niter);  // apply_bicgstab, configured precision
 int pc_factors(pcf_state* st);
+// ensemble (poisson.hip): whether this solve can join a batch, the batch
+// over one group of waiting members, and the member's side of it
+bool ensemble_batchable(const pcf_state* st, bool mixed, bool stage);
+int ensemble_key(const pcf_state* st, bool mixed);
+int ensemble_batch(pcf_ensemble* e, pcf_ensemble::Group& grp, const std::vector<pcf_state*>& members);
+int ensemble_solve(pcf_state* st, bool mixed);
+void ensemble_step_out(pcf_state* st, bool out);  // a solve outside the batch leaves (out) and rejoins the count
 int check_reduction_sizes(const Geo& g, std::string& err);
 int apply_corrector(pcf_state* st, double dt, double rf, int* niter);
 int correct(pcf_state* st, double dt, double rf);
@@ -300,6 +382,10 @@ int dot(pcf_state* st, const double* a, const double* b, double* out);
 // ---- mixed.hip -------------------------------------------------------------
 int apply_bicgstab_mixed(pcf_state* st, double tolref, double dt, int* niter);
 int precondition_mixed(pcf_state* st, const double* sin, double* sout, const SolverScalars* sc);
+// the ensemble batch's FP32 line preconditioner and its pc-rhs grid per member
+int ensemble_rhs32_blocks(const pcf_state* st);
+void ensemble_precondition_mixed(hipStream_t s, const EnsTable* dT, const EnsTable& H, int iters, bool bf16, int in,
+                                 int out);
 int alloc_mixed(pcf_state* st);
 // Device buffer of `bytes` owned by the state (freed by pcf_destroy).  The
 // start of the n-th buffer is offset by (n mod 28) x 2304 B within its
@@ -322,7 +408,7 @@ int learned_stage(pcf_state* st, const double* v, double* y, const SolverScalars
 // ---- explicit.hip --------------------------------------------------------
 int reconstruct(pcf_state* st);
 int compute_fluxes(pcf_state* st);
-int stage_inputs(pcf_state* st);
+int stage_inputs(pcf_state* st, int m);  // m: the RK stage (1: also the face carriers)
 // the fused fluxes + update of one variable in RK stage m, with the LHS
 // sponge of time step sdt applied in the same kernel when sponge_fused
 int stage_update(pcf_state* st, double dt, int m, int var, double sdt);

@@ -569,17 +569,17 @@ __global__ void __launch_bounds__(64) k_pc_factors(const double* ac, const doubl
 // on larger grids, which have blocks enough to use the fifth slot.
 // FIRST: rhs = 0 + deta (0 - src), the first pseudo-time step from s = 0,
 // whose horizontal apply is zero.
+// The body takes the block's (x, y) index, so that the ensemble kernel can
+// run it for a block of any member.
 template <bool FIRST, int NLDS>
-__global__ void __launch_bounds__(64) k_pc_thomas(const double* P, const double* E, const double* Q, const double* src,
-                                                  double* out, int nx, int ny, int nz, double deta,
-                                                  const SolverScalars* sc) {
+__device__ __forceinline__ void pc_thomas(const double* P, const double* E, const double* Q, const double* src,
+                                          double* out, int nx, int ny, int nz, double deta, int bx, int by) {
   constexpr int kThomasLds = NLDS;
   __shared__ double up[kThomasLds][64];
-  if (sc && sc->done) return;
   constexpr int U = 8;
   const int tid = threadIdx.x;
-  const int i = blockIdx.x * blockDim.x + tid;
-  const int j = blockIdx.y;
+  const int i = bx * 64 + tid;
+  const int j = by;
   if (i >= nx) return;
   const size_t sxy = (size_t)nx * ny;
   const size_t c = (size_t)i + (size_t)nx * j;
@@ -638,6 +638,13 @@ __global__ void __launch_bounds__(64) k_pc_thomas(const double* P, const double*
     out[c + sxy * k] = sk;
   }
 }
+template <bool FIRST, int NLDS>
+__global__ void __launch_bounds__(64) k_pc_thomas(const double* P, const double* E, const double* Q, const double* src,
+                                                  double* out, int nx, int ny, int nz, double deta,
+                                                  const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  pc_thomas<FIRST, NLDS>(P, E, Q, src, out, nx, ny, nz, deta, blockIdx.x, blockIdx.y);
+}
 
 // rhs of a later pseudo-time step: s + deta (L_h s - sin) (apply_preconditioner!.jl:62-63)
 __global__ void __launch_bounds__(256) k_pc_rhs(ApplyArgs A, const double* sin, double deta, const SolverScalars* sc) {
@@ -683,14 +690,15 @@ enum { MODE_TOTAL = 0, MODE_HORIZONTAL = 1, MODE_PC_RHS = 2 };
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 constexpr int kPairBlockY = 8;
 
+// The body takes the block's index (bx, by, bz) and the index of its partial
+// sums, so that the ensemble kernel can run it for a block of any member.
 template <int MODE, int NDOT>
-__global__ void __launch_bounds__(256) k_stencil_pair(ApplyArgs A, const double* __restrict__ d1,
-                                                      const double* __restrict__ sin, double deta, double* partials,
-                                                      const SolverScalars* sc) {
-  if (sc && sc->done) return;
-  const int i0 = (blockIdx.x * 32 + threadIdx.x) * 2;
-  const int j = blockIdx.y * kPairBlockY + threadIdx.y;
-  const int k = blockIdx.z;
+__device__ __forceinline__ void stencil_pair(const ApplyArgs& A, const double* __restrict__ d1,
+                                             const double* __restrict__ sin, double deta, double* partials, int bx,
+                                             int by, int bz, int block) {
+  const int i0 = (bx * 32 + threadIdx.x) * 2;
+  const int j = by * kPairBlockY + threadIdx.y;
+  const int k = bz;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
   double val[NDOT > 0 ? NDOT : 1];
 #pragma unroll
@@ -773,10 +781,15 @@ __global__ void __launch_bounds__(256) k_stencil_pair(ApplyArgs A, const double*
       val[1] += out[1] * out[1];
     }
   }
-  if (NDOT > 0) {
-    const int block = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    block_partials<(NDOT > 0 ? NDOT : 1)>(val, partials, block);
-  }
+  if (NDOT > 0) block_partials<(NDOT > 0 ? NDOT : 1)>(val, partials, block);
+}
+template <int MODE, int NDOT>
+__global__ void __launch_bounds__(256) k_stencil_pair(ApplyArgs A, const double* __restrict__ d1,
+                                                      const double* __restrict__ sin, double deta, double* partials,
+                                                      const SolverScalars* sc) {
+  if (sc && sc->done) return;
+  stencil_pair<MODE, NDOT>(A, d1, sin, deta, partials, blockIdx.x, blockIdx.y, blockIdx.z,
+                           blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
 }
 
 // ---------------------------------------------------------------------------
@@ -821,28 +834,41 @@ __global__ void k_bicg_init_guessed(const double* b, const double* ax, double* r
   block_partials<2>(val, partials, blockIdx.x);
 }
 
+// The vector kernels below are grid-stride loops over kVecGrid blocks of 256
+// threads; their bodies take the block index, so that the ensemble kernels
+// run the same blocks for every member (and so give the same partial sums).
+
 // s = r - alpha v
+__device__ __forceinline__ void bicg_s(const double* r, const double* v, double* s, size_t n,
+                                       const SolverScalars* sc, int b) {
+  const double alpha = sc->alpha;
+  for (size_t q = b * (size_t)kReduceBlock + threadIdx.x; q < n; q += (size_t)kVecGrid * kReduceBlock)
+    s[q] = r[q] - alpha * v[q];
+}
 __global__ void k_bicg_s(const double* r, const double* v, double* s, size_t n, const SolverScalars* sc) {
   if (sc->done) return;
-  const double alpha = sc->alpha;
-  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
-    s[q] = r[q] - alpha * v[q];
+  bicg_s(r, v, s, n, sc, blockIdx.x);
 }
 
 // x += alpha p + omega s; r = s - omega t; partials <r, r>, <r, r0>
-__global__ void k_bicg_xr(double* x, const double* p, const double* s, const double* t, double* r, const double* r0,
-                          size_t n, double* partials, const SolverScalars* sc) {
-  if (sc->done) return;
+__device__ __forceinline__ void bicg_xr(double* x, const double* p, const double* s, const double* t, double* r,
+                                        const double* r0, size_t n, double* partials, const SolverScalars* sc,
+                                        int b) {
   const double alpha = sc->alpha, omega = sc->omega;
   double val[2] = {0.0, 0.0};
-  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+  for (size_t q = b * (size_t)kReduceBlock + threadIdx.x; q < n; q += (size_t)kVecGrid * kReduceBlock) {
     x[q] += alpha * p[q] + omega * s[q];
     double rn = s[q] - omega * t[q];
     r[q] = rn;
     val[0] += rn * rn;
     val[1] += rn * r0[q];
   }
-  block_partials<2>(val, partials, blockIdx.x);
+  block_partials<2>(val, partials, b);
+}
+__global__ void k_bicg_xr(double* x, const double* p, const double* s, const double* t, double* r, const double* r0,
+                          size_t n, double* partials, const SolverScalars* sc) {
+  if (sc->done) return;
+  bicg_xr(x, p, s, t, r, r0, n, partials, sc, blockIdx.x);
 }
 
 // The update of the textbook right-preconditioned form (van der Vorst 1992):
@@ -850,27 +876,36 @@ __global__ void k_bicg_xr(double* x, const double* p, const double* s, const dou
 // partials <r, r>, <r, r0>.  It equals the reference's form (x += alpha p +
 // omega s, then x := M^-1 x at the end) for a linear M, and stays consistent
 // with the residual recurrence when M is evaluated in FP32.
-__global__ void k_bicg_xr_yz(double* x, const double* y, const double* z, const double* s, const double* t, double* r,
-                             const double* r0, size_t n, double* partials, const SolverScalars* sc) {
-  if (sc->done) return;
+__device__ __forceinline__ void bicg_xr_yz(double* x, const double* y, const double* z, const double* s,
+                                           const double* t, double* r, const double* r0, size_t n, double* partials,
+                                           const SolverScalars* sc, int b) {
   const double alpha = sc->alpha, omega = sc->omega;
   double val[2] = {0.0, 0.0};
-  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+  for (size_t q = b * (size_t)kReduceBlock + threadIdx.x; q < n; q += (size_t)kVecGrid * kReduceBlock) {
     x[q] += alpha * y[q] + omega * z[q];
     double rn = s[q] - omega * t[q];
     r[q] = rn;
     val[0] += rn * rn;
     val[1] += rn * r0[q];
   }
-  block_partials<2>(val, partials, blockIdx.x);
+  block_partials<2>(val, partials, b);
+}
+__global__ void k_bicg_xr_yz(double* x, const double* y, const double* z, const double* s, const double* t, double* r,
+                             const double* r0, size_t n, double* partials, const SolverScalars* sc) {
+  if (sc->done) return;
+  bicg_xr_yz(x, y, z, s, t, r, r0, n, partials, sc, blockIdx.x);
 }
 
 // p = r + beta (p - omega v)
+__device__ __forceinline__ void bicg_p(double* p, const double* r, const double* v, size_t n,
+                                       const SolverScalars* sc, int b) {
+  const double beta = sc->beta, omega = sc->omega;
+  for (size_t q = b * (size_t)kReduceBlock + threadIdx.x; q < n; q += (size_t)kVecGrid * kReduceBlock)
+    p[q] = r[q] + beta * (p[q] - omega * v[q]);
+}
 __global__ void k_bicg_p(double* p, const double* r, const double* v, size_t n, const SolverScalars* sc) {
   if (sc->done) return;
-  const double beta = sc->beta, omega = sc->omega;
-  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
-    p[q] = r[q] + beta * (p[q] - omega * v[q]);
+  bicg_p(p, r, v, n, sc, blockIdx.x);
 }
 
 __global__ void k_copy(double* dst, const double* src, size_t n) {
@@ -987,6 +1022,99 @@ __global__ void k_correct_pip(double* pip, const double* dpip, Range3 R, Geo g) 
 
 Fields fields_of(pcf_state* st) { return make_fields(st); }
 
+
+// ---------------------------------------------------------------------------
+// Ensemble kernels: one launch over the members of a batch (EnsTable).  A
+// block finds its member from the table's first-block offsets and runs the
+// single-solve kernel's body for that member's block; the blocks of a member
+// whose solve has stopped (sc->done) return at once, which is the batch's
+// per-member convergence mask.  Every member runs the same blocks as its
+// single solve, so it gets the same partial sums and the same bits.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int ens_member(const int* start, int count, int b) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// line solve of one pseudo-time step: v[out] = (line solve of v[src])
+template <bool FIRST>
+__global__ void __launch_bounds__(64) kE_thomas(const EnsTable* __restrict__ E, int src, int out) {
+  const int m = ens_member(E->thomas_start, E->count, blockIdx.x);
+  const EnsMember& M = E->m[m];
+  if (M.sc->done) return;
+  const int b = blockIdx.x - E->thomas_start[m];
+  const int gx = (M.nx + 63) / 64;
+  pc_thomas<FIRST, 64>(M.P, M.E, M.Q, M.v[src], M.v[out], M.nx, M.ny, M.nz, M.deta, b % gx, b / gx);
+}
+
+// v[out] = A v[in] (MODE_TOTAL, with NDOT dots against v[d1]) or
+// v[in] + deta (H v[in] - v[sin]) (MODE_PC_RHS)
+template <int MODE, int NDOT>
+__global__ void __launch_bounds__(256) kE_pair(const EnsTable* __restrict__ E, int in, int out, int d1, int sin) {
+  const int m = ens_member(E->pair_start, E->count, blockIdx.x);
+  const EnsMember& M = E->m[m];
+  if (M.sc->done) return;
+  const int b = blockIdx.x - E->pair_start[m];
+  const int gx = (M.nx + 63) / 64, gy = (M.ny + kPairBlockY - 1) / kPairBlockY;
+  ApplyArgs A;
+  A.T = M.T;
+  A.s = M.v[in];
+  A.ls = M.v[out];
+  A.nx = M.nx;
+  A.ny = M.ny;
+  A.nz = M.nz;
+  stencil_pair<MODE, NDOT>(A, d1 >= 0 ? M.v[d1] : nullptr, sin >= 0 ? M.v[sin] : nullptr, M.deta, M.partials,
+                           b % gx, (b / gx) % gy, b / (gx * gy), b);
+}
+
+// the scalar steps, one block of 1024 threads per member
+__global__ void kE_alpha(const EnsTable* __restrict__ E) {
+  const EnsMember& M = E->m[blockIdx.x];
+  bicg_alpha(M.partials, M.aparts, M.sc);
+}
+__global__ void kE_omega(const EnsTable* __restrict__ E) {
+  const EnsMember& M = E->m[blockIdx.x];
+  bicg_omega(M.partials, M.aparts, M.sc);
+}
+__global__ void kE_check(const EnsTable* __restrict__ E) {
+  const EnsMember& M = E->m[blockIdx.x];
+  bicg_check(M.partials, kVecGrid, M.sc, M.nxyz, M.maxit);
+}
+
+// the vector steps, kVecGrid blocks per member
+__global__ void __launch_bounds__(256) kE_s(const EnsTable* __restrict__ E) {
+  const EnsMember& M = E->m[blockIdx.x / kVecGrid];
+  if (M.sc->done) return;
+  bicg_s(M.v[EV_R], M.v[EV_V], M.v[EV_S], M.n, M.sc, blockIdx.x % kVecGrid);
+}
+// YZ: the textbook update x += alpha y + omega z (mixed); else x += alpha p + omega s
+template <bool YZ>
+__global__ void __launch_bounds__(256) kE_xr(const EnsTable* __restrict__ E) {
+  const EnsMember& M = E->m[blockIdx.x / kVecGrid];
+  if (M.sc->done) return;
+  const int b = blockIdx.x % kVecGrid;
+  if (YZ)
+    bicg_xr_yz(M.v[EV_X], M.v[EV_Y], M.v[EV_Z], M.v[EV_S], M.v[EV_T], M.v[EV_R], M.v[EV_R0], M.n, M.partials, M.sc,
+               b);
+  else
+    bicg_xr(M.v[EV_X], M.v[EV_P], M.v[EV_S], M.v[EV_T], M.v[EV_R], M.v[EV_R0], M.n, M.partials, M.sc, b);
+}
+__global__ void __launch_bounds__(256) kE_p(const EnsTable* __restrict__ E) {
+  const EnsMember& M = E->m[blockIdx.x / kVecGrid];
+  if (M.sc->done) return;
+  bicg_p(M.v[EV_P], M.v[EV_R], M.v[EV_V], M.n, M.sc, blockIdx.x % kVecGrid);
+}
+
+// the members' done flags, for the host's look at the batch
+__global__ void kE_flags(const EnsTable* __restrict__ E, int32_t* flags) {
+  const int m = threadIdx.x;
+  if (m < E->count) flags[m] = E->m[m].sc->done;
+}
 
 }  // namespace
 
@@ -1293,10 +1421,49 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
   }
   const int aparts = apply_parts(g);
   const int maxit = n.poisson_iterations;
+  // the line-solve factors of this operator, before any capture of the
+  // iteration (precondition() computes them on first use)
+  if (pc && !mixed) {
+    int rcf = pc_factors(st);
+    if (rcf) return rcf;
+  }
   // y = M^-1 p and z = M^-1 s are both kept for the textbook update
   double* ybuf = st->bvpc;
   double* zbuf = mixed ? st->bzpc : st->bvpc;
   const bool yz = mixed && pc;
+  // final x := M^-1 x (apply_bicgstab!.jl:132-135); the textbook form has
+  // accumulated M^-1 already
+  auto finish = [&]() -> int {
+    if (pc && !yz) {
+      hipLaunchKernelGGL(k_copy, dim3(kVecGrid), dim3(256), 0, st->stream, st->bs, st->solution, N);
+      int rc1 = precondition(st, st->bs, st->solution, nullptr);
+      if (rc1) return rc1;
+    }
+    PCF_HIP(hipGetLastError());
+    return PCF_OK;
+  };
+  if (st->ens && ensemble_batchable(st, mixed, stage)) {
+    // the iterations run in the ensemble's batch, which leaves the final
+    // scalars in sc_host[0]
+    int rce = ensemble_solve(st, mixed);
+    if (rce) return rce;
+    const SolverScalars fin = st->sc_host[0];
+    *niter_out = fin.niter;
+    st->last_residual = fin.res / fin.b_norm;
+    if (fin.done != 1) return PCF_E_NOCONV;
+    return finish();
+  }
+  // a member of an ensemble whose solve runs on its own leaves the batch's
+  // count meanwhile
+  struct StepOut {
+    pcf_state* st;
+    StepOut(pcf_state* s) : st(s) {
+      if (st) ensemble_step_out(st, true);
+    }
+    ~StepOut() {
+      if (st) ensemble_step_out(st, false);
+    }
+  } step_out(st->ens ? st : nullptr);
   SolverScalars* hs = st->sc_host;  // pinned, two slots (no copy is pending here)
   std::memset(hs, 0, 2 * sizeof(SolverScalars));
   hipEvent_t ev[2];
@@ -1304,9 +1471,14 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
   PCF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   int converged_at = -1, last_seen = 0;
   int rc = PCF_OK;
-  for (int jb = 1; jb <= maxit; ++jb) {
+  // One BiCGSTAB iteration's launches.  Every argument is fixed for the
+  // state (the scalars live in device memory), so the sequence is captured
+  // once into a hipGraph and replayed as one launch per iteration; the
+  // learned stage's iterations and profiled runs launch it directly.
+  auto iteration = [&](int jb) -> int {
+    int rc1 = PCF_OK;
     // v = A M^-1 p, <v, r0>
-    if ((rc = pc ? prec(st->bp, ybuf, sc, jb) : PCF_OK)) break;
+    if ((rc1 = pc ? prec(st->bp, ybuf, sc, jb) : PCF_OK)) return rc1;
     const double* vin = pc ? ybuf : st->bp;
     {
       ProfScope prof(st, PCF_K_APPLY_TOTAL_DOT);
@@ -1318,7 +1490,7 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
       hipLaunchKernelGGL(k_bicg_s, dim3(kVecGrid), dim3(256), 0, st->stream, st->br, st->bv, st->bs, N, sc);
     }
     // t = A M^-1 s, <t, s>, <t, t>
-    if ((rc = pc ? prec(st->bs, zbuf, sc, jb) : PCF_OK)) break;
+    if ((rc1 = pc ? prec(st->bs, zbuf, sc, jb) : PCF_OK)) return rc1;
     const double* sin2 = pc ? zbuf : st->bs;
     {
       ProfScope prof(st, PCF_K_APPLY_TOTAL_DOT);
@@ -1337,6 +1509,30 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
       hipLaunchKernelGGL(k_bicg_p, dim3(kVecGrid), dim3(256), 0, st->stream, st->bp, st->br, st->bv, N, sc);
     }
     PCF_HIP(hipGetLastError());
+    return PCF_OK;
+  };
+  for (int jb = 1; jb <= maxit; ++jb) {
+    const bool graphed = st->solver_graphs && !st->prof_on && !(stage && jb <= st->stage_iterations);
+    if (graphed) {
+      hipGraphExec_t& exec = st->iter_graph[mixed ? 1 : 0];
+      if (!exec) {
+        hipGraph_t graph = nullptr;
+        PCF_HIP(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
+        rc = iteration(jb);
+        hipError_t e = hipStreamEndCapture(st->stream, &graph);
+        if (rc) {
+          if (graph) (void)hipGraphDestroy(graph);
+          break;
+        }
+        PCF_HIP(e);
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        PCF_HIP(e);
+      }
+      PCF_HIP(hipGraphLaunch(exec, st->stream));
+    } else if ((rc = iteration(jb))) {
+      break;
+    }
     PCF_HIP(hipMemcpyAsync(hs + (jb & 1), sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, st->stream));
     PCF_HIP(hipEventRecord(ev[jb & 1], st->stream));
     if (jb >= 2) {
@@ -1371,13 +1567,184 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
   SolverScalars fin = hs[0].niter >= hs[1].niter ? hs[0] : hs[1];
   st->last_residual = fin.res / fin.b_norm;
   *niter_out = converged_at;
-  // final x := M^-1 x (apply_bicgstab!.jl:132-135); the textbook form has
-  // accumulated M^-1 already
-  if (pc && !yz) {
-    hipLaunchKernelGGL(k_copy, dim3(kVecGrid), dim3(256), 0, st->stream, st->bs, st->solution, N);
-    if ((rc = precondition(st, st->bs, st->solution, nullptr))) return rc;
+  return finish();
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble batch (pcf_ensemble): the BiCGSTAB iterations of every waiting
+// member of one solve kind, one launch per kernel over all of them.
+// ---------------------------------------------------------------------------
+
+// A solve joins the ensemble's batch when the member table covers it: the
+// line preconditioner (FP64, or FP32 in the mixed solve), an even x_size
+// (the two-cell apply kernels; a multiple of 4 for the FP32 pc-rhs kernel),
+// no learned stage and no profiling.
+bool ensemble_batchable(const pcf_state* st, bool mixed, bool stage) {
+  const pcf_config& n = st->cfg;
+  if (st->prof_on || stage || n.preconditioner == 0 || !use_pairs(st->g)) return false;
+  if (mixed && (n.preconditioner_type == PCF_PC_MULTIGRID || st->g.nx % 4 != 0)) return false;
+  return true;
+}
+
+// members with the same key run in one batch
+int ensemble_key(const pcf_state* st, bool mixed) {
+  const pcf_config& n = st->cfg;
+  const int bf16 = mixed && n.preconditioner_storage == PCF_STORAGE_BF16 ? 1 : 0;
+  return (mixed ? 1 : 0) | bf16 << 1 | n.preconditioner_iterations << 2;
+}
+
+// M^-1 v[in] -> v[out] for every member of the table (FP64 line
+// preconditioner; the pseudo-time steps and buffers of precondition())
+static void ensemble_precondition(hipStream_t s, const EnsTable* dT, const EnsTable& H, int iters, int in,
+                                  int out) {
+  const int tb = H.thomas_start[H.count], pb = H.pair_start[H.count];
+  int cur = iters == 1 ? out : EV_PCTMP;
+  hipLaunchKernelGGL(kE_thomas<true>, dim3(tb), dim3(64), 0, s, dT, in, cur);
+  for (int it = 2; it <= iters; ++it) {
+    hipLaunchKernelGGL((kE_pair<MODE_PC_RHS, 0>), dim3(pb), pair_block(), 0, s, dT, cur, EV_PCRHS, -1, in);
+    const int next = it == iters ? out : (cur == EV_PCTMP ? out : EV_PCTMP);
+    hipLaunchKernelGGL(kE_thomas<false>, dim3(tb), dim3(64), 0, s, dT, EV_PCRHS, next);
+    cur = next;
   }
-  PCF_HIP(hipGetLastError());
+}
+
+static void ensemble_fill(EnsTable& H, const std::vector<pcf_state*>& mem) {
+  H.count = (int)mem.size();
+  int pb = 0, tb = 0, rb = 0;
+  for (int q = 0; q < H.count; ++q) {
+    pcf_state* st = mem[q];
+    const Geo& g = st->g;
+    EnsMember& M = H.m[q];
+    std::memset(&M, 0, sizeof(M));
+    M.T = st->T;
+    double* v[EV_COUNT] = {st->solution, st->br0, st->br, st->bp, st->bv, st->bs, st->bt, st->bvpc,
+                           st->bzpc ? st->bzpc : st->bvpc, st->pc_tmp, st->pc_rhs};
+    for (int e = 0; e < EV_COUNT; ++e) M.v[e] = v[e];
+    M.P = st->pc_P;
+    M.E = st->pc_E;
+    M.Q = st->pc_Q;
+    M.P32 = st->pc32_P;
+    M.E32 = st->pc32_E;
+    M.Q32 = st->pc32_Q;
+    for (int t = 0; t < T_COUNT; ++t) {
+      M.T32[t] = st->T32[t];
+      M.T16[t] = st->T16[t];
+    }
+    M.col32 = st->m_pc_tmp;
+    M.cur32 = st->m_pc_cur;
+    M.rhs32 = st->m_pc_rhs;
+    M.partials = st->partials;
+    M.sc = st->sc;
+    M.n = g.nv();
+    M.nxyz = (double)st->cfg.x_size * st->cfg.y_size * st->cfg.z_size;
+    M.deta = st->deta;
+    M.nx = g.nx;
+    M.ny = g.ny;
+    M.nz = g.nz;
+    M.aparts = apply_parts(g);
+    M.maxit = st->cfg.poisson_iterations;
+    H.pair_start[q] = pb;
+    H.thomas_start[q] = tb;
+    H.rhs32_start[q] = rb;
+    const dim3 pg = pair_grid(g);
+    pb += (int)(pg.x * pg.y * pg.z);
+    tb += (int)(blocks_for(g.nx, 64) * g.ny);
+    rb += ensemble_rhs32_blocks(st);
+  }
+  H.pair_start[H.count] = pb;
+  H.thomas_start[H.count] = tb;
+  H.rhs32_start[H.count] = rb;
+}
+
+int ensemble_batch(pcf_ensemble* e, pcf_ensemble::Group& grp, const std::vector<pcf_state*>& mem) {
+  const int cnt = (int)mem.size();
+  if (cnt == 0) return PCF_OK;
+  pcf_state* first = mem[0];
+  const bool mixed = (grp.key & 1) != 0;
+  const bool bf16 = (grp.key & 2) != 0;
+  const int iters = first->cfg.preconditioner_iterations;
+  hipStream_t s = e->stream;
+  if (!grp.table) PCF_HIP(hipMalloc((void**)&grp.table, sizeof(EnsTable)));
+  if (!grp.host) grp.host.reset(new EnsTable());
+  EnsTable& H = *grp.host;
+  if (grp.members != mem) {
+    ensemble_fill(H, mem);
+    PCF_HIP(hipMemcpyAsync(grp.table, &H, sizeof(EnsTable), hipMemcpyHostToDevice, s));
+    PCF_HIP(hipStreamSynchronize(s));
+    grp.members = mem;
+    if (grp.graph) {
+      (void)hipGraphExecDestroy(grp.graph);
+      grp.graph = nullptr;
+    }
+  }
+  const EnsTable* dT = grp.table;
+  // the batch starts after each member's own work up to its solve
+  for (pcf_state* m : mem) PCF_HIP(hipStreamWaitEvent(s, m->ens_ready, 0));
+  const int pb = H.pair_start[cnt];
+  const int vb = cnt * kVecGrid;
+  auto prec = [&](int in, int out) {
+    if (mixed) ensemble_precondition_mixed(s, dT, H, iters, bf16, in, out);
+    else ensemble_precondition(s, dT, H, iters, in, out);
+  };
+  // one iteration of bicgstab_core's loop for every member
+  auto iteration = [&]() -> int {
+    prec(EV_P, EV_Y);
+    hipLaunchKernelGGL((kE_pair<MODE_TOTAL, 1>), dim3(pb), pair_block(), 0, s, dT, EV_Y, EV_V, EV_R0, -1);
+    hipLaunchKernelGGL(kE_alpha, dim3(cnt), dim3(1024), 0, s, dT);
+    hipLaunchKernelGGL(kE_s, dim3(vb), dim3(256), 0, s, dT);
+    const int z = mixed ? EV_Z : EV_Y;
+    prec(EV_S, z);
+    hipLaunchKernelGGL((kE_pair<MODE_TOTAL, 2>), dim3(pb), pair_block(), 0, s, dT, z, EV_T, EV_S, -1);
+    hipLaunchKernelGGL(kE_omega, dim3(cnt), dim3(1024), 0, s, dT);
+    if (mixed) hipLaunchKernelGGL(kE_xr<true>, dim3(vb), dim3(kReduceBlock), 0, s, dT);
+    else hipLaunchKernelGGL(kE_xr<false>, dim3(vb), dim3(kReduceBlock), 0, s, dT);
+    hipLaunchKernelGGL(kE_check, dim3(cnt), dim3(1024), 0, s, dT);
+    hipLaunchKernelGGL(kE_p, dim3(vb), dim3(256), 0, s, dT);
+    hipLaunchKernelGGL(kE_flags, dim3(1), dim3(kEnsMax), 0, s, dT, e->flags_dev);
+    PCF_HIP(hipGetLastError());
+    return PCF_OK;
+  };
+  // every member stops on the device, converged or at its own maxit
+  int maxit = 0;
+  for (pcf_state* m : mem) maxit = m->cfg.poisson_iterations > maxit ? m->cfg.poisson_iterations : maxit;
+  int rc = PCF_OK;
+  for (int jb = 1; jb <= maxit + 1; ++jb) {
+    if (e->graphs) {
+      if (!grp.graph) {
+        hipGraph_t graph = nullptr;
+        PCF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rc = iteration();
+        hipError_t err = hipStreamEndCapture(s, &graph);
+        if (rc) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return rc;
+        }
+        PCF_HIP(err);
+        err = hipGraphInstantiate(&grp.graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        PCF_HIP(err);
+      }
+      PCF_HIP(hipGraphLaunch(grp.graph, s));
+    } else if ((rc = iteration())) {
+      return rc;
+    }
+    e->iterations += 1;
+    int32_t* fl = e->flags_host + (jb & 1) * kEnsMax;
+    PCF_HIP(hipMemcpyAsync(fl, e->flags_dev, cnt * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    PCF_HIP(hipEventRecord(e->ev[jb & 1], s));
+    if (jb >= 2) {
+      PCF_HIP(hipEventSynchronize(e->ev[(jb - 1) & 1]));
+      const int32_t* seen = e->flags_host + ((jb - 1) & 1) * kEnsMax;
+      bool all = true;
+      for (int q = 0; q < cnt && all; ++q) all = seen[q] != 0;
+      if (all) break;
+    }
+  }
+  for (pcf_state* m : mem)
+    PCF_HIP(hipMemcpyAsync(m->sc_host, m->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, s));
+  PCF_HIP(hipStreamSynchronize(s));
+  e->batches += 1;
+  e->member_solves += cnt;
   return PCF_OK;
 }
 

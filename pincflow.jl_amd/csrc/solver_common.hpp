@@ -80,8 +80,12 @@ __global__ void k_bicg_start(const double* partials, int nparts, SolverScalars* 
   }
 }
 
+// The scalar steps below are one block of 1024 threads per solve: the
+// single-solve kernels run them on their state's scalars, the ensemble
+// kernels (poisson.hip) on each member's, one block per member.
+
 // alpha = <r, r0> / <v, r0>, from the partials of the A p kernel
-__global__ void k_bicg_alpha(const double* partials, int nparts, SolverScalars* sc) {
+__device__ __forceinline__ void bicg_alpha(const double* partials, int nparts, SolverScalars* sc) {
   if (sc->done) return;
   double vr0 = finalize_sum(partials, 0, nparts);
   if (threadIdx.x == 0) {
@@ -89,9 +93,12 @@ __global__ void k_bicg_alpha(const double* partials, int nparts, SolverScalars* 
     sc->alpha = sc->rr0 / vr0;
   }
 }
+__global__ void k_bicg_alpha(const double* partials, int nparts, SolverScalars* sc) {
+  bicg_alpha(partials, nparts, sc);
+}
 
 // omega = <t, s> / <t, t>
-__global__ void k_bicg_omega(const double* partials, int nparts, SolverScalars* sc) {
+__device__ __forceinline__ void bicg_omega(const double* partials, int nparts, SolverScalars* sc) {
   if (sc->done) return;
   double ts = finalize_sum(partials, 0, nparts);
   double tt = finalize_sum(partials, 1, nparts);
@@ -101,9 +108,15 @@ __global__ void k_bicg_omega(const double* partials, int nparts, SolverScalars* 
     sc->omega = ts / tt;
   }
 }
+__global__ void k_bicg_omega(const double* partials, int nparts, SolverScalars* sc) {
+  bicg_omega(partials, nparts, sc);
+}
 
-// residual, convergence and beta (apply_bicgstab!.jl:116-142)
-__global__ void k_bicg_check(const double* partials, int nparts, SolverScalars* sc, double nxyz) {
+// residual, convergence and beta (apply_bicgstab!.jl:116-142).  maxit > 0
+// (the ensemble solve) stops a solve that reaches maxit iterations without
+// converging with done = 2; the single solve stops by its host loop.
+__device__ __forceinline__ void bicg_check(const double* partials, int nparts, SolverScalars* sc, double nxyz,
+                                           int maxit) {
   if (sc->done) return;
   double rr = finalize_sum(partials, 0, nparts);
   double rr0n = finalize_sum(partials, 1, nparts);
@@ -117,9 +130,13 @@ __global__ void k_bicg_check(const double* partials, int nparts, SolverScalars* 
     } else {
       sc->beta = sc->alpha / sc->omega * rr0n / sc->rr0;
       sc->rr0 = rr0n;
+      if (maxit > 0 && sc->niter >= maxit) sc->done = 2;
     }
     sc->rr0_new = rr0n;
   }
+}
+__global__ void k_bicg_check(const double* partials, int nparts, SolverScalars* sc, double nxyz) {
+  bicg_check(partials, nparts, sc, nxyz, 0);
 }
 
 }  // namespace

@@ -295,19 +295,34 @@ class State:
         return r.value
 
 
-def integrate(namelists: Namelists, records: bool = True, max_steps: int = 1 << 30, write: bool = False
-              ) -> Tuple[State, List[dict], List[tuple]]:
+def integrate(namelists: Namelists, records: bool = True, max_steps: int = 1 << 30, write: bool = False,
+              batch=None, on_step=None) -> Tuple[State, List[dict], List[tuple]]:
     """integrate(namelists) (integrate.jl:68-415) on the device.
 
     Returns the state, the output records (datasets of write_output at the
     start time and at every output time) and per-step (time, dt, niter1,
     niter2).  With output.restart the run continues from record iin of
     output.input_file; with write=True the records are stored in
-    output.output_file (output.write_file)."""
-    from . import output
-
+    output.output_file (output.write_file).  ``batch`` (an
+    ``ensemble.SolveBatch``): the state's solves join the batch's, from its
+    creation to the end (or error) of the run.  ``on_step(itime, info)``
+    is called after every step."""
     nl = namelists
     st = State(nl)
+    if batch is None:
+        return _integrate(st, records, max_steps, write, on_step)
+    batch.attach(st)
+    try:
+        return _integrate(st, records, max_steps, write, on_step)
+    finally:
+        batch.detach(st)
+
+
+def _integrate(st: State, records: bool, max_steps: int, write: bool, on_step
+               ) -> Tuple[State, List[dict], List[tuple]]:
+    from . import output
+
+    nl = st.namelists
     recs: List[dict] = []
     steps: List[tuple] = []
     if nl.poisson.initial_cleaning:
@@ -321,6 +336,8 @@ def integrate(namelists: Namelists, records: bool = True, max_steps: int = 1 << 
     for itime in range(1, n + 1):
         info = st.step()
         steps.append((info.time, info.dt, info.niter[0], info.niter[1]))
+        if on_step is not None:
+            on_step(itime, info)
         if records and ((nl.output.output_steps and itime % nl.output.nout == 0) or
                         (not nl.output.output_steps and info.output)):
             recs.append(output.record_datasets(nl, st.get, info.time, False))

@@ -59,3 +59,85 @@ def test_members_sharing_the_gpu_match_their_standalone_runs(pcf, tmp_path):
             assert np.array_equal(st.get(f), alone.get(f)), (seed, f)
         with np.load(files[c], allow_pickle=False) as data:
             assert data["thetap"].shape[-1] == len(recs)
+
+
+def _members(pcf, precision, seeds, sizes, iterations, pc=None, poisson_iterations=None):
+    def sim(seed, x_size, iterations, pc=None, poisson_iterations=None):
+        nl = pcf.examples.bubble_member(seed=seed, x_size=x_size, z_size=12, iterations=iterations,
+                                        precision=precision)
+        if pc:
+            nl.poisson.preconditioner_type = pc
+        if poisson_iterations:
+            nl.poisson.poisson_iterations = poisson_iterations
+        return nl
+    params = {"seed": seeds, "x_size": sizes, "iterations": iterations}
+    if pc:
+        params["pc"] = pc
+    if poisson_iterations:
+        params["poisson_iterations"] = poisson_iterations
+    return sim, params
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_solves_give_each_member_its_standalone_bits(pcf, tmp_path, precision):
+    """Members of uneven grids (16-32 cells square, 12 levels) and uneven run
+    lengths whose solves run as one launch per kernel over all of them
+    (SolveBatch): every member's iteration counts and fields equal its own
+    standalone run's bit for bit.  One member uses the multigrid
+    preconditioner, which the batch does not cover: its solves run on
+    their own and leave the batch's count meanwhile; the shorter runs
+    detach while the others go on."""
+    seeds = (0, 1, 2, 3, 4, 5)
+    sizes = (16, 24, 16, 32, 24, 32)
+    iterations = (3, 3, 1, 3, 2, 3)
+    pcs = ("line_relaxation",) * 5 + ("multigrid" if precision == "mixed" else "line_relaxation",)
+    sim, params = _members(pcf, precision, seeds, sizes, iterations, pc=pcs)
+    params["output_file"] = tuple(str(tmp_path / f"m{s}.npz") for s in seeds)
+    batch = pcf.SolveBatch()
+    res = pcf.ensemble(sim, params, {}, write=False, batch=batch)
+    stats = batch.stats()
+    batch.close()
+    print(precision, "batch statistics:", stats)
+    assert stats["batches"] > 0 and stats["member_solves"] > stats["batches"]
+    if precision == "mixed":
+        assert stats["solves_alone"] > 0
+    for c, seed in enumerate(seeds):
+        alone, _, steps = pcf.integrate(sim(seed, sizes[c], iterations[c], pcs[c]))
+        st, _, steps_e = res[c]
+        assert [s[2:] for s in steps_e] == [s[2:] for s in steps], (seed, steps_e, steps)
+        for f in ("RHOP", "U", "V", "W", "PIP", "RHO"):
+            assert np.array_equal(st.get(f), alone.get(f)), (seed, f)
+        alone.close()
+
+
+@pytest.mark.gpu
+def test_batched_solve_that_fails_stops_at_its_own_limit(pcf, abi, tmp_path):
+    """A member whose poisson_iterations no solve can meet fails with the
+    reference's non-convergence error after exactly that many iterations,
+    as on its own, while the batch goes on with the others; the ensemble
+    then raises the failure with the member named (reduce_exceptions)."""
+    seeds = (0, 1, 2)
+    sim, params = _members(pcf, "fp64", seeds, (16, 24, 16), (2, 2, 2), poisson_iterations=(1000, 3, 1000))
+    params["output_file"] = tuple(str(tmp_path / f"m{s}.npz") for s in seeds)
+    nl = sim(1, 24, 2, poisson_iterations=3)
+    nl.poisson.initial_cleaning = False
+    with pytest.raises(pcf.PoissonNotConverged) as alone:
+        pcf.integrate(nl)
+    keywords = {}
+    params_nc = dict(params)
+    batch = pcf.SolveBatch()
+    with pytest.raises(pcf.EnsembleError) as e:
+        pcf.ensemble(lambda **kw: _no_cleaning(sim(**kw)), params_nc, keywords, write=False, batch=batch)
+    stats = batch.stats()
+    batch.close()
+    assert e.value.member == 2
+    assert isinstance(e.value.error, pcf.PoissonNotConverged)
+    assert e.value.error.rc == alone.value.rc == abi.E_NOCONV
+    assert str(e.value.error) == str(alone.value)
+    assert stats["batches"] > 0
+
+
+def _no_cleaning(nl):
+    nl.poisson.initial_cleaning = False
+    return nl
