@@ -50,6 +50,10 @@ def parse():
                         "physics over 16 seeded hills (the mixed-precision configuration, run at --nx 512); "
                         "ensemble: --members seeded bubble members sharing the GPU, against one at a time")
     p.add_argument("--members", type=int, default=16, help="ensemble workload: number of members")
+    p.add_argument("--member-x", type=int, default=0,
+                   help="ensemble workload: x_size = y_size of every member (default: the member's own)")
+    p.add_argument("--member-z", type=int, default=0,
+                   help="ensemble workload: z_size of every member (default: the member's own)")
     p.add_argument("--ensemble-modes", default="alone,streams,batched",
                    help="ensemble workload: comma-separated runs among alone (one member at a time), streams "
                         "(all members at once, each solve on its own) and batched (all at once, solves batched)")
@@ -130,6 +134,7 @@ def run_method(pcf, args, precision, preconditioner, guess, net=None, keep=False
            + ("/fp32-fluxes" if fluxes == "mixed" else ""))
     t0 = time.perf_counter()
     st = pcf.State(nl)
+    st.solver_graphs(True)  # the bench steps one state at a time from this thread
     if net is not None:
         pcf.learned.attach(st, net)
     if stage_net is not None:
@@ -363,7 +368,8 @@ def ensemble_bench(pcf, args):
     by ensemble() with all members sharing the GPU, against the same members
     one at a time; the members' records are kept in memory, not written."""
     def sim(seed):
-        nl = pcf.examples.bubble_member(seed=seed, iterations=args.steps, precision=args.precision)
+        nl = pcf.examples.bubble_member(seed=seed, iterations=args.steps, precision=args.precision,
+                                        x_size=args.member_x, z_size=args.member_z)
         if args.precision == "mixed":
             nl.poisson.preconditioner_type = args.preconditioner
             nl.poisson.preconditioner_storage = args.storage
@@ -382,7 +388,7 @@ def ensemble_bench(pcf, args):
         batch = pcf.SolveBatch() if batched else None
         t0 = time.perf_counter()
         def progress(member, itime, info, label=label, conc=conc):
-            if (member == 1 or conc == 1) and itime % 10 == 0:
+            if (member == 1 and conc > 1) or (conc == 1 and itime % 10 == 0):
                 print(f"# [ensemble] {label}: member {member} at step {itime}", file=sys.stderr, flush=True)
         res = pcf.ensemble(sim, params, {}, max_concurrent=conc, write=False, batched=batched, batch=batch,
                            on_step=progress)
@@ -397,8 +403,10 @@ def ensemble_bench(pcf, args):
             batch.close()
         print(f"# [ensemble] {label}: {elapsed:.1f}s", file=sys.stderr, flush=True)
     base = runs.get("one at a time", runs.get("sharing the GPU"))
+    sizes = (f", every member {args.member_x}x{args.member_x}x{args.member_z}"
+             if args.member_x and args.member_z else "")
     print(json.dumps({"workload": f"seeded bubble ensemble, {args.members} members, {args.steps} steps each "
-                                  "(setup and initial cleaning included)",
+                                  f"(setup and initial cleaning included){sizes}",
                       "cells": cells, "precision": args.precision,
                       **({"preconditioner": args.preconditioner,
                           "flux_precision": args.flux_precision or "mixed"} if args.precision == "mixed" else {}),

@@ -457,9 +457,13 @@ int pcf_stage_tuning(const pcf_state* st, int32_t* shape, float* ms);
  * have it tuned again on its next launch. */
 int pcf_set_stage_shape(pcf_state* st, int32_t slot, int32_t shape);
 /* BiCGSTAB iterations replayed from a hipGraph captured on the first
- * iteration (on = 1, the default) or launched kernel by kernel (on = 0);
- * the kernels, and so the results, are the same either way.  No reference
- * counterpart: the reference has no launches to batch. */
+ * iteration (on = 1) or launched kernel by kernel (on = 0, the default);
+ * the kernels, and so the results, are the same either way.  While a
+ * capture is in progress, HIP refuses stream synchronisations and
+ * allocations of other host threads, so turn it on only for a state whose
+ * thread is the only one using HIP (one state per process, or several
+ * stepped from one thread).  No reference counterpart: the reference has no
+ * launches to batch. */
 int pcf_set_solver_graphs(pcf_state* st, int32_t on);
 
 /* ---- ensembles (src/ensemble.jl:20-74) ------------------------------------
@@ -467,30 +471,31 @@ int pcf_set_solver_graphs(pcf_state* st, int32_t on);
  * keywords) in an MPI communicator of its own.  Here the members share the
  * GPU, each a state stepped by a host thread of its own (one thread per
  * state, as for every state).  Attached to a pcf_ensemble, a member's
- * BiCGSTAB solve waits until every attached member's solve waits (or the
- * member has detached), and the iterations of all of them then run as one
- * launch per kernel over a table of the members, replayed from a hipGraph,
- * with each member's convergence flag masking its blocks once it has
- * converged or reached its poisson_iterations.  Every member gets the bits
- * of its own solve.  The batch covers the line preconditioner (FP64, or
- * FP32 in the mixed solve) on grids with an even x_size (a multiple of 4 for
- * the mixed solve); other solves of a member run on their own and leave the
- * batch's count while they do.  A member thread must detach when it stops
- * stepping (its run ends or fails), or the other members wait for it. */
+ * BiCGSTAB solves run in the ensemble's batch: a service thread admits a
+ * solve at the next iteration boundary, runs one iteration of every solve
+ * in progress as one launch per kernel whose grid covers these solves (a
+ * list of their members in a device table of the members' vectors and
+ * operators; a solve's convergence flag masks its blocks in the iteration
+ * after its last), and releases a solve as soon as it has converged or
+ * reached its poisson_iterations.  Members step independently; every member
+ * gets the bits of its own solve.  The batch
+ * covers the line preconditioner (FP64, or FP32 in the mixed solve) on grids
+ * with an even x_size (a multiple of 4 for the mixed solve); other solves of
+ * a member run on their own.  A member's thread detaches when it stops
+ * stepping. */
 typedef struct pcf_ensemble pcf_ensemble;
+/* no device work until the first attach */
 int pcf_ensemble_create(pcf_ensemble** out);
 /* PCF_E_ARG while members are attached */
 int pcf_ensemble_destroy(pcf_ensemble* e);
-/* at most 128 members; PCF_E_ARG for a state attached already */
+/* at most 128 members and 8 solve kinds, one device; PCF_E_ARG for a state
+ * attached already.  Call from the thread that steps st, between its calls. */
 int pcf_ensemble_attach(pcf_ensemble* e, pcf_state* st);
 /* also done by pcf_destroy */
 int pcf_ensemble_detach(pcf_state* st);
-/* out: batches run, batched iterations (launches of the captured
- * iteration), member solves in batches, solves run outside a batch */
+/* out: batched iterations (passes of the service), member solves run in the
+ * batch, solves run on their own, member-table builds */
 int pcf_ensemble_stats(pcf_ensemble* e, int64_t out[4]);
-/* batched iterations replayed from a hipGraph (1, the default) or launched
- * kernel by kernel (0) */
-int pcf_ensemble_set_graphs(pcf_ensemble* e, int32_t on);
 
 /* ---- initial-guess network of the learned solve (no reference
  * counterpart) -------------------------------------------------------------

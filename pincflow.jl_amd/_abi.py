@@ -233,7 +233,6 @@ SIGNATURES = [
     ("pcf_ensemble_attach", _I, [_P, _P]),
     ("pcf_ensemble_detach", _I, [_P]),
     ("pcf_ensemble_stats", _I, [_P, ctypes.POINTER(ctypes.c_int64)]),
-    ("pcf_ensemble_set_graphs", _I, [_P, ctypes.c_int32]),
 ]
 
 KERNEL_GROUPS = ["apply_total_dot", "pc_thomas", "pc_rhs", "compute_operator", "bicg_vector", "fluxes", "updates",

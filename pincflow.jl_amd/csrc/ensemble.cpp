@@ -1,15 +1,20 @@
 // Ensembles on one GPU (src/ensemble.jl:20-74 runs each member in an MPI
 // communicator of its own).  Each member is a state stepped by a host thread
-// of its own; attached to a pcf_ensemble, the BiCGSTAB iterations of its
-// solves run in batches with the other members': a member whose solve is
-// ready waits until every attached member waits, and the last to arrive
-// runs one launch per kernel of the iteration over all waiting members
-// (ensemble_batch), replayed from a hipGraph, until each has converged or
-// reached its poisson_iterations.  A member's results are those of its own
-// solve (the same blocks, the same partial sums, the same bits); only the
-// launches are shared.  Solves that the batch does not cover (the multigrid
-// or learned-stage preconditioners, profiled runs) step out of the count
-// while they run on their own.
+// of its own; attached to a pcf_ensemble, its BiCGSTAB solves run in the
+// ensemble's batch.  A service thread owns the batch: it admits a member's
+// solve at the next iteration boundary (the solve's vectors and scalars are
+// set up on the member's stream by bicgstab_core, and the service's stream
+// waits for them), launches one iteration of every solve in progress as one
+// launch per kernel whose grid covers these solves (a list of their slots in
+// the member table of each solve kind), and releases a solve as soon as its
+// convergence flag (or its own poisson_iterations) is seen, one iteration
+// later.  Members step
+// independently: a member whose solve ends goes on with its explicit work
+// while the others' solves continue, and the batch is never held for the
+// slowest member.  A member's results are those of its own solve (the same
+// blocks, the same partial sums, the same bits); only the launches are
+// shared.  Solves that the table does not cover (the multigrid or
+// learned-stage preconditioners, profiled runs) run on their own.
 #include <algorithm>
 
 #include "pcf_internal.hpp"
@@ -18,85 +23,164 @@ namespace pcf {
 
 namespace {
 
-// Runs the batches of everything waiting (lock held), sets each member's
-// result and wakes the waiting threads.
-void run_batches(pcf_ensemble* e) {
-  std::vector<pcf_state*> waiting;
-  waiting.swap(e->waiting);
-  std::vector<int> keys;
-  for (pcf_state* m : waiting) {
-    const int k = ensemble_key(m, m->ens_mixed);
-    if (std::find(keys.begin(), keys.end(), k) == keys.end()) keys.push_back(k);
-  }
-  for (int k : keys) {
-    std::vector<pcf_state*> mem;
-    for (pcf_state* m : waiting)
-      if (ensemble_key(m, m->ens_mixed) == k) mem.push_back(m);
-    // one order whatever the order of arrival, so that the table (and its
-    // captured iteration) is rebuilt only when the members change
-    std::sort(mem.begin(), mem.end());
-    pcf_ensemble::Group* grp = nullptr;
-    for (auto& g : e->groups)
-      if (g->key == k) grp = g.get();
-    if (!grp) {
-      e->groups.emplace_back(new pcf_ensemble::Group());
-      grp = e->groups.back().get();
-      grp->key = k;
+// Records the service's first error and fails every solve in the batch and
+// every waiting one (lock held).
+void fail_all(pcf_ensemble* e, int rc) {
+  if (!e->err) e->err = rc;
+  for (auto& g : e->groups) {
+    for (pcf_state* m : g->active) {
+      m->ens_active = false;
+      m->ens_rc = rc;
     }
-    int rc = hipSetDevice(e->device) == hipSuccess ? ensemble_batch(e, *grp, mem) : PCF_E_HIP;
-    for (pcf_state* m : mem) m->ens_rc = rc;
+    g->active.clear();
   }
-  e->cv.notify_all();
+  for (pcf_state* m : e->pending) m->ens_rc = rc;
+  e->pending.clear();
+  e->done_cv.notify_all();
+}
+
+bool any_active(const pcf_ensemble* e) {
+  for (auto& g : e->groups)
+    if (!g->active.empty()) return true;
+  return false;
+}
+
+void service_loop(pcf_ensemble* e) {
+  std::unique_lock<std::mutex> lk(e->mu);
+  if (hipSetDevice(e->device) != hipSuccess) e->err = PCF_E_HIP;
+  long long pass = 0;
+  for (;;) {
+    while (!e->stop && e->pending.empty() && !any_active(e)) e->wake.wait(lk);
+    if (e->pending.empty() && !any_active(e)) break;  // stop
+    if (e->err) {
+      fail_all(e, e->err);
+      continue;
+    }
+    int rc = PCF_OK;
+    // tables of groups whose members changed (nothing in flight after the
+    // upload's synchronisation); slots may move, so the previous pass's
+    // flags are not read in this one
+    bool rebuilt = false;
+    for (auto& g : e->groups)
+      if (g->stale && !g->slots.empty()) {
+        if ((rc = ensemble_upload(e, *g))) break;
+        rebuilt = true;
+      }
+    // admission: the batch's stream waits for the member's set-up
+    for (size_t q = 0; q < e->pending.size() && rc == PCF_OK; ++q) {
+      pcf_state* m = e->pending[q];
+      if (hipStreamWaitEvent(e->stream, m->ens_ready, 0) != hipSuccess) rc = PCF_E_HIP;
+      m->ens_active = true;
+      m->ens_since = pass + 1;
+      e->groups[m->ens_group]->active.push_back(m);
+    }
+    e->pending.clear();
+    if (rc) {
+      fail_all(e, rc);
+      continue;
+    }
+    ++pass;
+    // one iteration of every solve in the batch (the lock is held: member
+    // threads only queue solves or attach meanwhile), then the done flags
+    int32_t* fl = e->flags_host + (pass & 1) * kEnsGroups * kEnsMax;
+    for (size_t gi = 0; gi < e->groups.size() && rc == PCF_OK; ++gi)
+      rc = ensemble_iterate(e, *e->groups[gi], (int)(pass & 1), e->flags_dev + gi * kEnsMax);
+    if (rc == PCF_OK &&
+        (hipMemcpyAsync(fl, e->flags_dev, e->groups.size() * kEnsMax * sizeof(int32_t), hipMemcpyDeviceToHost,
+                        e->stream) != hipSuccess ||
+         hipEventRecord(e->ev[pass & 1], e->stream) != hipSuccess))
+      rc = PCF_E_HIP;
+    if (rc) {
+      fail_all(e, rc);
+      continue;
+    }
+    e->iterations += 1;
+    if (pass < 2) continue;
+    // the previous iteration's flags, for the solves it ran
+    lk.unlock();
+    const bool waited = hipEventSynchronize(e->ev[(pass - 1) & 1]) == hipSuccess;
+    lk.lock();
+    if (!waited) {
+      fail_all(e, PCF_E_HIP);
+      continue;
+    }
+    if (rebuilt) continue;
+    const int32_t* seen = e->flags_host + ((pass - 1) & 1) * kEnsGroups * kEnsMax;
+    bool released = false;
+    for (auto& g : e->groups) {
+      for (size_t q = 0; q < g->active.size();) {
+        pcf_state* m = g->active[q];
+        if (m->ens_since > pass - 1 || seen[m->ens_group * kEnsMax + m->ens_slot] == 0) {
+          ++q;
+          continue;
+        }
+        // converged or stopped (its blocks in this pass's iteration returned
+        // at once): out of the next iterations, its final scalars to the
+        // host, and its stream may go on after this point
+        if (hipMemcpyAsync(m->sc_host, m->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, e->stream) !=
+                hipSuccess ||
+            hipEventRecord(m->ens_done, e->stream) != hipSuccess) {
+          rc = PCF_E_HIP;
+          break;
+        }
+        m->ens_active = false;
+        m->ens_rc = PCF_OK;
+        e->member_solves += 1;
+        g->active.erase(g->active.begin() + q);
+        released = true;
+      }
+      if (rc) break;
+    }
+    if (rc) fail_all(e, rc);
+    else if (released) e->done_cv.notify_all();
+  }
 }
 
 }  // namespace
 
-// The member's side: wait for (or run) the batch that takes this solve.
-int ensemble_solve(pcf_state* st, bool mixed) {
+// The member's side: hand the solve (set up on this state's stream) to the
+// service and wait for its release.
+int ensemble_solve(pcf_state* st) {
   pcf_ensemble* e = st->ens;
   PCF_HIP(hipEventRecord(st->ens_ready, st->stream));
   std::unique_lock<std::mutex> lk(e->mu);
-  st->ens_mixed = mixed;
+  if (e->err) return e->err;
   st->ens_rc = -1;
-  e->waiting.push_back(st);
-  while (st->ens_rc < 0) {
-    if ((int)e->waiting.size() >= e->attached) run_batches(e);
-    else e->cv.wait(lk);
-  }
-  return st->ens_rc;
-}
-
-// A solve outside the batch leaves the count while it runs.
-void ensemble_step_out(pcf_state* st, bool out) {
-  pcf_ensemble* e = st->ens;
-  std::unique_lock<std::mutex> lk(e->mu);
-  if (out) {
-    e->attached -= 1;
-    e->stepped_out += 1;
-    if (!e->waiting.empty() && (int)e->waiting.size() >= e->attached) run_batches(e);
-  } else {
-    e->attached += 1;
-  }
+  e->pending.push_back(st);
+  e->wake.notify_one();
+  e->done_cv.wait(lk, [&] { return st->ens_rc >= 0; });
+  const int rc = st->ens_rc;
+  lk.unlock();
+  if (rc) return rc;
+  // the batch's last work on this state (and the copy of its scalars)
+  PCF_HIP(hipEventSynchronize(st->ens_done));
+  return PCF_OK;
 }
 
 }  // namespace pcf
 
 using namespace pcf;
 
-extern "C" {
+namespace {
 
-// The ensemble's device resources, made by the first attach (on the
-// device of the first member; lock held).
-static int ensemble_device_setup(pcf_ensemble* e, const pcf_state* st) {
+// The ensemble's device resources and service thread, made by the first
+// attach (on the device of the first member; lock held).
+int ensemble_start(pcf_ensemble* e, const pcf_state* st) {
   if (e->stream) return PCF_OK;
   e->device = st->device;
   PCF_HIP(hipSetDevice(e->device));
   PCF_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-  PCF_HIP(hipMalloc((void**)&e->flags_dev, kEnsMax * sizeof(int32_t)));
-  PCF_HIP(hipHostMalloc((void**)&e->flags_host, 2 * kEnsMax * sizeof(int32_t), hipHostMallocDefault));
+  PCF_HIP(hipMalloc((void**)&e->flags_dev, kEnsGroups * kEnsMax * sizeof(int32_t)));
+  PCF_HIP(hipMemset(e->flags_dev, 0, kEnsGroups * kEnsMax * sizeof(int32_t)));
+  PCF_HIP(hipHostMalloc((void**)&e->flags_host, 2 * kEnsGroups * kEnsMax * sizeof(int32_t), hipHostMallocDefault));
   for (int q = 0; q < 2; ++q) PCF_HIP(hipEventCreateWithFlags(&e->ev[q], hipEventDisableTiming));
+  e->service = std::thread(service_loop, e);
   return PCF_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int pcf_ensemble_create(pcf_ensemble** out) {
   if (!out) return PCF_E_ARG;
@@ -112,11 +196,15 @@ int pcf_ensemble_destroy(pcf_ensemble* e) {
       set_error("pcf_ensemble_destroy: members are still attached");
       return PCF_E_ARG;
     }
+    e->stop = true;
+    e->wake.notify_all();
   }
+  if (e->service.joinable()) e->service.join();
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   for (auto& g : e->groups) {
-    if (g->graph) (void)hipGraphExecDestroy(g->graph);
     if (g->table) (void)hipFree(g->table);
+    if (g->list) (void)hipFree(g->list);
+    if (g->list_host) (void)hipHostFree(g->list_host);
   }
   for (int q = 0; q < 2; ++q)
     if (e->ev[q]) (void)hipEventDestroy(e->ev[q]);
@@ -133,10 +221,11 @@ int pcf_ensemble_attach(pcf_ensemble* e, pcf_state* st) {
     set_error("pcf_ensemble_attach: the state is attached to an ensemble already");
     return PCF_E_ARG;
   }
-  if (!st->ens_ready && hipEventCreateWithFlags(&st->ens_ready, hipEventDisableTiming) != hipSuccess) {
-    set_error("pcf_ensemble_attach: hipEventCreateWithFlags failed");
-    return PCF_E_HIP;
-  }
+  for (hipEvent_t* ev : {&st->ens_ready, &st->ens_done})
+    if (!*ev && hipEventCreateWithFlags(ev, hipEventDisableTiming) != hipSuccess) {
+      set_error("pcf_ensemble_attach: hipEventCreateWithFlags failed");
+      return PCF_E_HIP;
+    }
   std::unique_lock<std::mutex> lk(e->mu);
   if (e->attached >= kEnsMax) {
     set_error("pcf_ensemble_attach: at most " + std::to_string(kEnsMax) + " members");
@@ -146,10 +235,33 @@ int pcf_ensemble_attach(pcf_ensemble* e, pcf_state* st) {
     set_error("pcf_ensemble_attach: the members of an ensemble share one device");
     return PCF_E_ARG;
   }
-  int rc = ensemble_device_setup(e, st);
+  const int key = ensemble_key(st);
+  int gi = -1;
+  if (key >= 0) {
+    for (size_t q = 0; q < e->groups.size(); ++q)
+      if (e->groups[q]->key == key) gi = (int)q;
+    if (gi < 0 && (int)e->groups.size() >= kEnsGroups) {
+      set_error("pcf_ensemble_attach: at most " + std::to_string(kEnsGroups) + " solve kinds in one ensemble");
+      return PCF_E_ARG;
+    }
+  }
+  int rc = ensemble_start(e, st);
   if (rc) return rc;
-  e->attached += 1;
+  if (key >= 0) {
+    if (gi < 0) {
+      e->groups.emplace_back(new pcf_ensemble::Group());
+      gi = (int)e->groups.size() - 1;
+      e->groups[gi]->key = key;
+    }
+    pcf_ensemble::Group& g = *e->groups[gi];
+    st->ens_slot = (int)g.slots.size();
+    g.slots.push_back(st);
+    g.stale = true;
+  }
+  st->ens_group = gi;
+  st->ens_active = false;
   st->ens = e;
+  e->attached += 1;
   return PCF_OK;
 }
 
@@ -158,27 +270,26 @@ int pcf_ensemble_detach(pcf_state* st) {
   pcf_ensemble* e = st->ens;
   if (!e) return PCF_OK;
   std::unique_lock<std::mutex> lk(e->mu);
-  e->attached -= 1;
+  // a member's thread detaches between its calls, so its solve is not in
+  // the batch
+  if (st->ens_group >= 0) {
+    pcf_ensemble::Group& g = *e->groups[st->ens_group];
+    g.slots.erase(std::find(g.slots.begin(), g.slots.end(), st));
+    g.stale = true;
+  }
+  st->ens_group = st->ens_slot = -1;
   st->ens = nullptr;
-  // the members waiting for this one's solve need not wait any longer
-  if (!e->waiting.empty() && (int)e->waiting.size() >= e->attached) run_batches(e);
+  e->attached -= 1;
   return PCF_OK;
 }
 
 int pcf_ensemble_stats(pcf_ensemble* e, int64_t out[4]) {
   if (!e || !out) return PCF_E_ARG;
   std::unique_lock<std::mutex> lk(e->mu);
-  out[0] = e->batches;
-  out[1] = e->iterations;
-  out[2] = e->member_solves;
-  out[3] = e->stepped_out;
-  return PCF_OK;
-}
-
-int pcf_ensemble_set_graphs(pcf_ensemble* e, int32_t on) {
-  if (!e) return PCF_E_ARG;
-  std::unique_lock<std::mutex> lk(e->mu);
-  e->graphs = on != 0;
+  out[0] = e->iterations;
+  out[1] = e->member_solves;
+  out[2] = e->solves_alone;
+  out[3] = e->table_builds;
   return PCF_OK;
 }
 

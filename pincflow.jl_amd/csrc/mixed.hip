@@ -1133,9 +1133,10 @@ int apply_bicgstab_mixed(pcf_state* st, double tolref, double dt, int* niter_out
 }
 
 // ---------------------------------------------------------------------------
-// Ensemble batch: the FP32 line preconditioner over the members of a table
-// (see the ensemble kernels of poisson.hip).  Each member runs the blocks of
-// its single solve's kernels (precondition_mixed; x_size a multiple of 4).
+// Ensemble batch: the FP32 line preconditioner over the solves of a batched
+// iteration (see the ensemble kernels of poisson.hip).  Each member runs the
+// blocks of its single solve's kernels (precondition_mixed; x_size a
+// multiple of 4).
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -1152,11 +1153,12 @@ __device__ __forceinline__ int ens_member32(const int* start, int count, int b) 
 // FIRST: from the FP64 vector v[in]; else from the member's FP32 rhs.
 // OutT double: to v[out]; float: to the member's FP32 iterate.
 template <bool FIRST, typename OutT>
-__global__ void __launch_bounds__(64) kEm_thomas(const EnsTable* __restrict__ E, int in, int out) {
-  const int m = ens_member32(E->thomas_start, E->count, blockIdx.x);
-  const EnsMember& M = E->m[m];
+__global__ void __launch_bounds__(64) kEm_thomas(const EnsTable* __restrict__ E, const EnsList* __restrict__ L, int in,
+                                                 int out) {
+  const int a = ens_member32(L->thomas_start, L->count, blockIdx.x);
+  const EnsMember& M = E->m[L->member[a]];
   if (M.sc->done) return;
-  const int b = blockIdx.x - E->thomas_start[m];
+  const int b = blockIdx.x - L->thomas_start[a];
   const int gx = (M.nx + 63) / 64;
   OutT* dst;
   if constexpr (sizeof(OutT) == 8) dst = M.v[out];
@@ -1172,11 +1174,11 @@ __global__ void __launch_bounds__(64) kEm_thomas(const EnsTable* __restrict__ E,
 
 // rhs32 = cur32 + deta (H cur32 - v[in]) with the coefficients in storage CT
 template <typename CT>
-__global__ void __launch_bounds__(256) kEm_rhs(const EnsTable* __restrict__ E, int in) {
-  const int m = ens_member32(E->rhs32_start, E->count, blockIdx.x);
-  const EnsMember& M = E->m[m];
+__global__ void __launch_bounds__(256) kEm_rhs(const EnsTable* __restrict__ E, const EnsList* __restrict__ L, int in) {
+  const int a = ens_member32(L->rhs32_start, L->count, blockIdx.x);
+  const EnsMember& M = E->m[L->member[a]];
   if (M.sc->done) return;
-  const int b = blockIdx.x - E->rhs32_start[m];
+  const int b = blockIdx.x - L->rhs32_start[a];
   const int gx = (M.nx + 63) / 64, gy = (M.ny + 15) / 16;
   PcArgs<CT> A;
   for (int t = 0; t < T_COUNT; ++t) {
@@ -1203,20 +1205,21 @@ int ensemble_rhs32_blocks(const pcf_state* st) {
   return (int)(blocks_for(g.nx, 64) * blocks_for(g.ny, 16) * g.nz);
 }
 
-// M^-1 v[in] -> v[out] for every member (precondition_mixed's line branch)
-void ensemble_precondition_mixed(hipStream_t s, const EnsTable* dT, const EnsTable& H, int iters, bool bf16, int in,
-                                 int out) {
-  const int tb = H.thomas_start[H.count], rb = H.rhs32_start[H.count];
+// M^-1 v[in] -> v[out] for every solve of the list (precondition_mixed's
+// line branch)
+void ensemble_precondition_mixed(hipStream_t s, const EnsTable* dT, const EnsList* dL, const EnsList& L, int iters,
+                                 bool bf16, int in, int out) {
+  const int tb = L.thomas_start[L.count], rb = L.rhs32_start[L.count];
   if (iters == 1) {
-    hipLaunchKernelGGL((kEm_thomas<true, double>), dim3(tb), dim3(64), 0, s, dT, in, out);
+    hipLaunchKernelGGL((kEm_thomas<true, double>), dim3(tb), dim3(64), 0, s, dT, dL, in, out);
     return;
   }
-  hipLaunchKernelGGL((kEm_thomas<true, float>), dim3(tb), dim3(64), 0, s, dT, in, out);
+  hipLaunchKernelGGL((kEm_thomas<true, float>), dim3(tb), dim3(64), 0, s, dT, dL, in, out);
   for (int it = 2; it <= iters; ++it) {
-    if (bf16) hipLaunchKernelGGL(kEm_rhs<uint16_t>, dim3(rb), dim3(16, 16), 0, s, dT, in);
-    else hipLaunchKernelGGL(kEm_rhs<float>, dim3(rb), dim3(16, 16), 0, s, dT, in);
-    if (it == iters) hipLaunchKernelGGL((kEm_thomas<false, double>), dim3(tb), dim3(64), 0, s, dT, in, out);
-    else hipLaunchKernelGGL((kEm_thomas<false, float>), dim3(tb), dim3(64), 0, s, dT, in, out);
+    if (bf16) hipLaunchKernelGGL(kEm_rhs<uint16_t>, dim3(rb), dim3(16, 16), 0, s, dT, dL, in);
+    else hipLaunchKernelGGL(kEm_rhs<float>, dim3(rb), dim3(16, 16), 0, s, dT, dL, in);
+    if (it == iters) hipLaunchKernelGGL((kEm_thomas<false, double>), dim3(tb), dim3(64), 0, s, dT, dL, in, out);
+    else hipLaunchKernelGGL((kEm_thomas<false, float>), dim3(tb), dim3(64), 0, s, dT, dL, in, out);
   }
 }
 

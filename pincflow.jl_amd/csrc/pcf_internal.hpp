@@ -16,6 +16,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pincflow_hip.h"
@@ -115,10 +116,12 @@ constexpr int kMaxPartials = 4096;    // partial sums per reduction
 constexpr int kMaxDots = 4;
 
 // ---------------------------------------------------------------------------
-// ensemble solve: one launch per BiCGSTAB kernel over the members whose
-// solves wait together (pcf_ensemble); the table lives in device memory
+// ensemble solve: one launch per BiCGSTAB kernel over the members of an
+// ensemble whose solves are in progress (pcf_ensemble); the member table
+// lives in device memory
 // ---------------------------------------------------------------------------
-constexpr int kEnsMax = 128;  // members of one batch
+constexpr int kEnsMax = 128;   // members of an ensemble
+constexpr int kEnsGroups = 8;  // solve kinds of an ensemble
 // the solver vectors of a member, by index
 enum { EV_X = 0, EV_R0, EV_R, EV_P, EV_V, EV_S, EV_T, EV_Y, EV_Z, EV_PCTMP, EV_PCRHS, EV_COUNT };
 struct EnsMember {
@@ -141,10 +144,17 @@ struct EnsMember {
 };
 struct EnsTable {
   int count;
-  int pair_start[kEnsMax + 1];      // first block of each member in the apply grids
-  int thomas_start[kEnsMax + 1];    // ... in the line-solve grids
-  int rhs32_start[kEnsMax + 1];     // ... in the FP32 pc-rhs grid (mixed)
   EnsMember m[kEnsMax];
+};
+// The solves in one batched iteration: their members' slots in the table
+// and the first block of each in the launch grids of the iteration's
+// kernels, which cover these members only.
+struct EnsList {
+  int count;
+  int member[kEnsMax];
+  int pair_start[kEnsMax + 1];      // apply and pc-rhs grids
+  int thomas_start[kEnsMax + 1];    // line-solve grids
+  int rhs32_start[kEnsMax + 1];     // FP32 pc-rhs grid (mixed)
 };
 
 // ---------------------------------------------------------------------------
@@ -156,33 +166,44 @@ struct HostSetup;  // setup.cpp
 
 struct pcf_net;
 
-// Members of an ensemble whose BiCGSTAB solves run together: a member's
-// solve waits until every attached member waits (or steps out), and the
-// last to arrive runs the batch for all of them (pcf_ensemble_attach).
+// The members of an ensemble (pcf_ensemble_attach), each stepped by a host
+// thread of its own, and the service thread that runs their solves: a
+// member's solve is admitted at the next iteration boundary, runs with the
+// solves already in progress as one launch per kernel over the solves of
+// its kind, and is released as soon as the service sees its convergence
+// flag.
 struct pcf_ensemble {
   std::mutex mu;
-  std::condition_variable cv;
-  int attached = 0;                      // members that will arrive at a solve or detach
-  std::vector<pcf_state*> waiting;       // members whose solve waits for the next batch
+  std::condition_variable done_cv;       // member threads wait here for their solve
+  std::condition_variable wake;          // the service waits here for work
+  std::thread service;
+  bool stop = false;
+  int err = 0;                           // the service's first error (PCF_E_*), sticky
+  std::vector<pcf_state*> pending;       // solves waiting for admission
   hipStream_t stream = nullptr;
   int device = 0;
-  // members batch together when they share the solve's kind (FP64 or mixed,
-  // preconditioner iterations, coefficient storage): per kind the device
-  // table, its host copy, the members it was built for and the captured
-  // iteration
+  int attached = 0;
+  // per solve kind (FP64 or mixed, preconditioner iterations, coefficient
+  // storage): the members, the device table, its host copy and the
+  // captured iteration
+  // the member table, its host copy, each slot's blocks in the grids, the
+  // solves of the current iteration (host, two pinned slots, device)
   struct Group {
     int key = -1;
+    std::vector<pcf_state*> slots;
+    bool stale = true;                   // the table does not match slots yet
+    std::vector<pcf_state*> active;      // solves in the batch
     pcf::EnsTable* table = nullptr;
     std::unique_ptr<pcf::EnsTable> host;
-    std::vector<pcf_state*> members;
-    hipGraphExec_t graph = nullptr;
+    std::vector<int> pair_blocks, thomas_blocks, rhs32_blocks;
+    pcf::EnsList* list_host = nullptr;   // pinned, two slots
+    pcf::EnsList* list = nullptr;
   };
   std::vector<std::unique_ptr<Group>> groups;
-  int32_t* flags_dev = nullptr;          // done flags of the batch's members
-  int32_t* flags_host = nullptr;         // pinned, two slots of kEnsMax
+  int32_t* flags_dev = nullptr;          // done flags, kEnsMax per group
+  int32_t* flags_host = nullptr;         // pinned, two slots of kEnsGroups x kEnsMax
   hipEvent_t ev[2] = {nullptr, nullptr};
-  bool graphs = true;
-  int64_t batches = 0, iterations = 0, member_solves = 0, stepped_out = 0;
+  int64_t iterations = 0, member_solves = 0, solves_alone = 0, table_builds = 0;
 };
 
 struct pcf_state {
@@ -284,15 +305,20 @@ struct pcf_state {
   int stage_fallbacks = 0;            // solves rerun without the stage after it failed to converge
   double* train_buf[3] = {};          // network training work: x, A x - v, A^T (A x - v)
   // the BiCGSTAB iteration captured as a hipGraph (FP64 / mixed solve),
-  // instantiated on first use and replayed once per iteration
+  // instantiated on first use and replayed once per iteration (when
+  // solver_graphs is set: pcf_set_solver_graphs)
   hipGraphExec_t iter_graph[2] = {nullptr, nullptr};
-  bool solver_graphs = true;
-  // ensemble membership (pcf_ensemble_attach): the ensemble, the event a
-  // batch waits for on this state's stream, and the result of the batch
+  bool solver_graphs = false;
+  // ensemble membership (pcf_ensemble_attach): the ensemble, the member's
+  // group (-1: its solves run on their own) and slot in the group's table,
+  // the events that order the service's work after this state's stream and
+  // this state's after the service's, and the state of a solve in the batch
   pcf_ensemble* ens = nullptr;
-  hipEvent_t ens_ready = nullptr;
-  bool ens_mixed = false;
-  int ens_rc = -1;                    // -1 while the solve waits
+  int ens_group = -1, ens_slot = -1;
+  hipEvent_t ens_ready = nullptr, ens_done = nullptr;
+  bool ens_active = false;
+  long long ens_since = 0;            // the service pass that admitted the solve
+  int ens_rc = -1;                    // -1 while the solve is in the batch
   double* partials;                   // reduction partials
   pcf::SolverScalars* sc;             // device scalars
   pcf::SolverScalars* sc_host;        // pinned copy
@@ -367,13 +393,15 @@ int apply_bicgstab(pcf_state* st, double tolref, int* niter);
 int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess, double guess_scale, int* niter);
 int solve(pcf_state* st, double tolref, double dt, int* niter);  // apply_bicgstab, configured precision
 int pc_factors(pcf_state* st);
-// ensemble (poisson.hip): whether this solve can join a batch, the batch
-// over one group of waiting members, and the member's side of it
+// ensemble (poisson.hip): the solve kind of a member (-1: the member table
+// does not cover it), whether this solve joins the batch, the table upload,
+// one batched iteration of a group's active solves (list slot `parity`,
+// done flags to `flags`); ensemble.cpp: the member's side
+int ensemble_key(const pcf_state* st);
 bool ensemble_batchable(const pcf_state* st, bool mixed, bool stage);
-int ensemble_key(const pcf_state* st, bool mixed);
-int ensemble_batch(pcf_ensemble* e, pcf_ensemble::Group& grp, const std::vector<pcf_state*>& members);
-int ensemble_solve(pcf_state* st, bool mixed);
-void ensemble_step_out(pcf_state* st, bool out);  // a solve outside the batch leaves (out) and rejoins the count
+int ensemble_upload(pcf_ensemble* e, pcf_ensemble::Group& grp);
+int ensemble_iterate(pcf_ensemble* e, pcf_ensemble::Group& grp, int parity, int32_t* flags);
+int ensemble_solve(pcf_state* st);
 int check_reduction_sizes(const Geo& g, std::string& err);
 int apply_corrector(pcf_state* st, double dt, double rf, int* niter);
 int correct(pcf_state* st, double dt, double rf);
@@ -384,8 +412,8 @@ int apply_bicgstab_mixed(pcf_state* st, double tolref, double dt, int* niter);
 int precondition_mixed(pcf_state* st, const double* sin, double* sout, const SolverScalars* sc);
 // the ensemble batch's FP32 line preconditioner and its pc-rhs grid per member
 int ensemble_rhs32_blocks(const pcf_state* st);
-void ensemble_precondition_mixed(hipStream_t s, const EnsTable* dT, const EnsTable& H, int iters, bool bf16, int in,
-                                 int out);
+void ensemble_precondition_mixed(hipStream_t s, const EnsTable* dT, const EnsList* dL, const EnsList& L, int iters,
+                                 bool bf16, int in, int out);
 int alloc_mixed(pcf_state* st);
 // Device buffer of `bytes` owned by the state (freed by pcf_destroy).  The
 // start of the n-th buffer is offset by (n mod 28) x 2304 B within its

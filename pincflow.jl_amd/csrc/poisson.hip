@@ -1024,14 +1024,15 @@ Fields fields_of(pcf_state* st) { return make_fields(st); }
 
 
 // ---------------------------------------------------------------------------
-// Ensemble kernels: one launch over the members of a batch (EnsTable).  A
-// block finds its member from the table's first-block offsets and runs the
-// single-solve kernel's body for that member's block; the blocks of a member
-// whose solve has stopped (sc->done) return at once, which is the batch's
-// per-member convergence mask.  Every member runs the same blocks as its
-// single solve, so it gets the same partial sums and the same bits.
+// Ensemble kernels: one launch over the solves of a batched iteration
+// (EnsList, members of an EnsTable).  A block finds its solve from the
+// list's first-block offsets and runs the single-solve kernel's body for
+// that member's block; the blocks of a solve that has stopped (sc->done, in
+// the iteration after its last) return at once.  Every member runs the
+// same blocks as its single solve, so it gets the same partial sums and the
+// same bits.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int ens_member(const int* start, int count, int b) {
+__device__ __forceinline__ int ens_find(const int* start, int count, int b) {
   int lo = 0, hi = count - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -1043,11 +1044,12 @@ __device__ __forceinline__ int ens_member(const int* start, int count, int b) {
 
 // line solve of one pseudo-time step: v[out] = (line solve of v[src])
 template <bool FIRST>
-__global__ void __launch_bounds__(64) kE_thomas(const EnsTable* __restrict__ E, int src, int out) {
-  const int m = ens_member(E->thomas_start, E->count, blockIdx.x);
-  const EnsMember& M = E->m[m];
+__global__ void __launch_bounds__(64) kE_thomas(const EnsTable* __restrict__ E, const EnsList* __restrict__ L,
+                                                int src, int out) {
+  const int a = ens_find(L->thomas_start, L->count, blockIdx.x);
+  const EnsMember& M = E->m[L->member[a]];
   if (M.sc->done) return;
-  const int b = blockIdx.x - E->thomas_start[m];
+  const int b = blockIdx.x - L->thomas_start[a];
   const int gx = (M.nx + 63) / 64;
   pc_thomas<FIRST, 64>(M.P, M.E, M.Q, M.v[src], M.v[out], M.nx, M.ny, M.nz, M.deta, b % gx, b / gx);
 }
@@ -1055,11 +1057,12 @@ __global__ void __launch_bounds__(64) kE_thomas(const EnsTable* __restrict__ E, 
 // v[out] = A v[in] (MODE_TOTAL, with NDOT dots against v[d1]) or
 // v[in] + deta (H v[in] - v[sin]) (MODE_PC_RHS)
 template <int MODE, int NDOT>
-__global__ void __launch_bounds__(256) kE_pair(const EnsTable* __restrict__ E, int in, int out, int d1, int sin) {
-  const int m = ens_member(E->pair_start, E->count, blockIdx.x);
-  const EnsMember& M = E->m[m];
+__global__ void __launch_bounds__(256) kE_pair(const EnsTable* __restrict__ E, const EnsList* __restrict__ L, int in,
+                                               int out, int d1, int sin) {
+  const int a = ens_find(L->pair_start, L->count, blockIdx.x);
+  const EnsMember& M = E->m[L->member[a]];
   if (M.sc->done) return;
-  const int b = blockIdx.x - E->pair_start[m];
+  const int b = blockIdx.x - L->pair_start[a];
   const int gx = (M.nx + 63) / 64, gy = (M.ny + kPairBlockY - 1) / kPairBlockY;
   ApplyArgs A;
   A.T = M.T;
@@ -1072,30 +1075,30 @@ __global__ void __launch_bounds__(256) kE_pair(const EnsTable* __restrict__ E, i
                            b % gx, (b / gx) % gy, b / (gx * gy), b);
 }
 
-// the scalar steps, one block of 1024 threads per member
-__global__ void kE_alpha(const EnsTable* __restrict__ E) {
-  const EnsMember& M = E->m[blockIdx.x];
+// the scalar steps, one block of 1024 threads per solve
+__global__ void kE_alpha(const EnsTable* __restrict__ E, const EnsList* __restrict__ L) {
+  const EnsMember& M = E->m[L->member[blockIdx.x]];
   bicg_alpha(M.partials, M.aparts, M.sc);
 }
-__global__ void kE_omega(const EnsTable* __restrict__ E) {
-  const EnsMember& M = E->m[blockIdx.x];
+__global__ void kE_omega(const EnsTable* __restrict__ E, const EnsList* __restrict__ L) {
+  const EnsMember& M = E->m[L->member[blockIdx.x]];
   bicg_omega(M.partials, M.aparts, M.sc);
 }
-__global__ void kE_check(const EnsTable* __restrict__ E) {
-  const EnsMember& M = E->m[blockIdx.x];
+__global__ void kE_check(const EnsTable* __restrict__ E, const EnsList* __restrict__ L) {
+  const EnsMember& M = E->m[L->member[blockIdx.x]];
   bicg_check(M.partials, kVecGrid, M.sc, M.nxyz, M.maxit);
 }
 
-// the vector steps, kVecGrid blocks per member
-__global__ void __launch_bounds__(256) kE_s(const EnsTable* __restrict__ E) {
-  const EnsMember& M = E->m[blockIdx.x / kVecGrid];
+// the vector steps, kVecGrid blocks per solve
+__global__ void __launch_bounds__(256) kE_s(const EnsTable* __restrict__ E, const EnsList* __restrict__ L) {
+  const EnsMember& M = E->m[L->member[blockIdx.x / kVecGrid]];
   if (M.sc->done) return;
   bicg_s(M.v[EV_R], M.v[EV_V], M.v[EV_S], M.n, M.sc, blockIdx.x % kVecGrid);
 }
 // YZ: the textbook update x += alpha y + omega z (mixed); else x += alpha p + omega s
 template <bool YZ>
-__global__ void __launch_bounds__(256) kE_xr(const EnsTable* __restrict__ E) {
-  const EnsMember& M = E->m[blockIdx.x / kVecGrid];
+__global__ void __launch_bounds__(256) kE_xr(const EnsTable* __restrict__ E, const EnsList* __restrict__ L) {
+  const EnsMember& M = E->m[L->member[blockIdx.x / kVecGrid]];
   if (M.sc->done) return;
   const int b = blockIdx.x % kVecGrid;
   if (YZ)
@@ -1104,16 +1107,16 @@ __global__ void __launch_bounds__(256) kE_xr(const EnsTable* __restrict__ E) {
   else
     bicg_xr(M.v[EV_X], M.v[EV_P], M.v[EV_S], M.v[EV_T], M.v[EV_R], M.v[EV_R0], M.n, M.partials, M.sc, b);
 }
-__global__ void __launch_bounds__(256) kE_p(const EnsTable* __restrict__ E) {
-  const EnsMember& M = E->m[blockIdx.x / kVecGrid];
+__global__ void __launch_bounds__(256) kE_p(const EnsTable* __restrict__ E, const EnsList* __restrict__ L) {
+  const EnsMember& M = E->m[L->member[blockIdx.x / kVecGrid]];
   if (M.sc->done) return;
   bicg_p(M.v[EV_P], M.v[EV_R], M.v[EV_V], M.n, M.sc, blockIdx.x % kVecGrid);
 }
 
-// the members' done flags, for the host's look at the batch
-__global__ void kE_flags(const EnsTable* __restrict__ E, int32_t* flags) {
-  const int m = threadIdx.x;
-  if (m < E->count) flags[m] = E->m[m].sc->done;
+// the solves' done flags, by table slot, for the host's look at the batch
+__global__ void kE_flags(const EnsTable* __restrict__ E, const EnsList* __restrict__ L, int32_t* flags) {
+  const int a = threadIdx.x;
+  if (a < L->count) flags[L->member[a]] = E->m[L->member[a]].sc->done;
 }
 
 }  // namespace
@@ -1445,7 +1448,7 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
   if (st->ens && ensemble_batchable(st, mixed, stage)) {
     // the iterations run in the ensemble's batch, which leaves the final
     // scalars in sc_host[0]
-    int rce = ensemble_solve(st, mixed);
+    int rce = ensemble_solve(st);
     if (rce) return rce;
     const SolverScalars fin = st->sc_host[0];
     *niter_out = fin.niter;
@@ -1453,17 +1456,10 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
     if (fin.done != 1) return PCF_E_NOCONV;
     return finish();
   }
-  // a member of an ensemble whose solve runs on its own leaves the batch's
-  // count meanwhile
-  struct StepOut {
-    pcf_state* st;
-    StepOut(pcf_state* s) : st(s) {
-      if (st) ensemble_step_out(st, true);
-    }
-    ~StepOut() {
-      if (st) ensemble_step_out(st, false);
-    }
-  } step_out(st->ens ? st : nullptr);
+  if (st->ens) {
+    std::unique_lock<std::mutex> lk(st->ens->mu);
+    st->ens->solves_alone += 1;
+  }
   SolverScalars* hs = st->sc_host;  // pinned, two slots (no copy is pending here)
   std::memset(hs, 0, 2 * sizeof(SolverScalars));
   hipEvent_t ev[2];
@@ -1575,42 +1571,47 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
 // member of one solve kind, one launch per kernel over all of them.
 // ---------------------------------------------------------------------------
 
-// A solve joins the ensemble's batch when the member table covers it: the
-// line preconditioner (FP64, or FP32 in the mixed solve), an even x_size
-// (the two-cell apply kernels; a multiple of 4 for the FP32 pc-rhs kernel),
-// no learned stage and no profiling.
-bool ensemble_batchable(const pcf_state* st, bool mixed, bool stage) {
+// The member table covers the line preconditioner (FP64, or FP32 in the
+// mixed solve) on grids with an even x_size (the two-cell apply kernels; a
+// multiple of 4 for the FP32 pc-rhs kernel).  Members with the same key
+// share a table; -1: not covered.
+int ensemble_key(const pcf_state* st) {
   const pcf_config& n = st->cfg;
-  if (st->prof_on || stage || n.preconditioner == 0 || !use_pairs(st->g)) return false;
-  if (mixed && (n.preconditioner_type == PCF_PC_MULTIGRID || st->g.nx % 4 != 0)) return false;
-  return true;
-}
-
-// members with the same key run in one batch
-int ensemble_key(const pcf_state* st, bool mixed) {
-  const pcf_config& n = st->cfg;
+  const bool mixed = n.precision == PCF_PRECISION_MIXED;
+  if (n.preconditioner == 0 || !use_pairs(st->g)) return -1;
+  if (mixed && (n.preconditioner_type == PCF_PC_MULTIGRID || st->g.nx % 4 != 0)) return -1;
   const int bf16 = mixed && n.preconditioner_storage == PCF_STORAGE_BF16 ? 1 : 0;
   return (mixed ? 1 : 0) | bf16 << 1 | n.preconditioner_iterations << 2;
 }
 
-// M^-1 v[in] -> v[out] for every member of the table (FP64 line
+// A solve joins the batch when its member is in a table and the solve is
+// the member's own kind, without the learned stage or profiling.
+bool ensemble_batchable(const pcf_state* st, bool mixed, bool stage) {
+  return st->ens_group >= 0 && mixed == (st->cfg.precision == PCF_PRECISION_MIXED) && !stage && !st->prof_on;
+}
+
+// M^-1 v[in] -> v[out] for every solve of the list (FP64 line
 // preconditioner; the pseudo-time steps and buffers of precondition())
-static void ensemble_precondition(hipStream_t s, const EnsTable* dT, const EnsTable& H, int iters, int in,
-                                  int out) {
-  const int tb = H.thomas_start[H.count], pb = H.pair_start[H.count];
+static void ensemble_precondition(hipStream_t s, const EnsTable* dT, const EnsList* dL, const EnsList& L, int iters,
+                                  int in, int out) {
+  const int tb = L.thomas_start[L.count], pb = L.pair_start[L.count];
   int cur = iters == 1 ? out : EV_PCTMP;
-  hipLaunchKernelGGL(kE_thomas<true>, dim3(tb), dim3(64), 0, s, dT, in, cur);
+  hipLaunchKernelGGL(kE_thomas<true>, dim3(tb), dim3(64), 0, s, dT, dL, in, cur);
   for (int it = 2; it <= iters; ++it) {
-    hipLaunchKernelGGL((kE_pair<MODE_PC_RHS, 0>), dim3(pb), pair_block(), 0, s, dT, cur, EV_PCRHS, -1, in);
+    hipLaunchKernelGGL((kE_pair<MODE_PC_RHS, 0>), dim3(pb), pair_block(), 0, s, dT, dL, cur, EV_PCRHS, -1, in);
     const int next = it == iters ? out : (cur == EV_PCTMP ? out : EV_PCTMP);
-    hipLaunchKernelGGL(kE_thomas<false>, dim3(tb), dim3(64), 0, s, dT, EV_PCRHS, next);
+    hipLaunchKernelGGL(kE_thomas<false>, dim3(tb), dim3(64), 0, s, dT, dL, EV_PCRHS, next);
     cur = next;
   }
 }
 
-static void ensemble_fill(EnsTable& H, const std::vector<pcf_state*>& mem) {
+static void ensemble_fill(pcf_ensemble::Group& grp) {
+  EnsTable& H = *grp.host;
+  const std::vector<pcf_state*>& mem = grp.slots;
   H.count = (int)mem.size();
-  int pb = 0, tb = 0, rb = 0;
+  grp.pair_blocks.assign(H.count, 0);
+  grp.thomas_blocks.assign(H.count, 0);
+  grp.rhs32_blocks.assign(H.count, 0);
   for (int q = 0; q < H.count; ++q) {
     pcf_state* st = mem[q];
     const Geo& g = st->g;
@@ -1643,108 +1644,81 @@ static void ensemble_fill(EnsTable& H, const std::vector<pcf_state*>& mem) {
     M.nz = g.nz;
     M.aparts = apply_parts(g);
     M.maxit = st->cfg.poisson_iterations;
-    H.pair_start[q] = pb;
-    H.thomas_start[q] = tb;
-    H.rhs32_start[q] = rb;
     const dim3 pg = pair_grid(g);
-    pb += (int)(pg.x * pg.y * pg.z);
-    tb += (int)(blocks_for(g.nx, 64) * g.ny);
-    rb += ensemble_rhs32_blocks(st);
+    grp.pair_blocks[q] = (int)(pg.x * pg.y * pg.z);
+    grp.thomas_blocks[q] = (int)(blocks_for(g.nx, 64) * g.ny);
+    grp.rhs32_blocks[q] = ensemble_rhs32_blocks(st);
   }
-  H.pair_start[H.count] = pb;
-  H.thomas_start[H.count] = tb;
-  H.rhs32_start[H.count] = rb;
 }
 
-int ensemble_batch(pcf_ensemble* e, pcf_ensemble::Group& grp, const std::vector<pcf_state*>& mem) {
-  const int cnt = (int)mem.size();
+// The group's table after a change of its members (between iterations:
+// nothing of the ensemble is in flight afterwards).
+int ensemble_upload(pcf_ensemble* e, pcf_ensemble::Group& grp) {
+  hipStream_t s = e->stream;
+  PCF_HIP(hipStreamSynchronize(s));
+  if (!grp.table) PCF_HIP(hipMalloc((void**)&grp.table, sizeof(EnsTable)));
+  if (!grp.list) PCF_HIP(hipMalloc((void**)&grp.list, sizeof(EnsList)));
+  if (!grp.list_host) PCF_HIP(hipHostMalloc((void**)&grp.list_host, 2 * sizeof(EnsList), hipHostMallocDefault));
+  if (!grp.host) grp.host.reset(new EnsTable());
+  for (int q = 0; q < (int)grp.slots.size(); ++q) grp.slots[q]->ens_slot = q;
+  ensemble_fill(grp);
+  PCF_HIP(hipMemcpyAsync(grp.table, grp.host.get(), sizeof(EnsTable), hipMemcpyHostToDevice, s));
+  PCF_HIP(hipStreamSynchronize(s));
+  grp.stale = false;
+  e->table_builds += 1;
+  return PCF_OK;
+}
+
+// One BiCGSTAB iteration (bicgstab_core's loop body) of every solve in the
+// group's batch, each kernel one launch whose grid covers these solves; the
+// list goes through pinned slot `parity` (the copy of the slot's previous
+// use has completed: the host has seen the iteration after it end); the
+// done flags go to `flags` by table slot.
+int ensemble_iterate(pcf_ensemble* e, pcf_ensemble::Group& grp, int parity, int32_t* flags) {
+  const int cnt = (int)grp.active.size();
   if (cnt == 0) return PCF_OK;
-  pcf_state* first = mem[0];
   const bool mixed = (grp.key & 1) != 0;
   const bool bf16 = (grp.key & 2) != 0;
-  const int iters = first->cfg.preconditioner_iterations;
+  const int iters = grp.key >> 2;
   hipStream_t s = e->stream;
-  if (!grp.table) PCF_HIP(hipMalloc((void**)&grp.table, sizeof(EnsTable)));
-  if (!grp.host) grp.host.reset(new EnsTable());
-  EnsTable& H = *grp.host;
-  if (grp.members != mem) {
-    ensemble_fill(H, mem);
-    PCF_HIP(hipMemcpyAsync(grp.table, &H, sizeof(EnsTable), hipMemcpyHostToDevice, s));
-    PCF_HIP(hipStreamSynchronize(s));
-    grp.members = mem;
-    if (grp.graph) {
-      (void)hipGraphExecDestroy(grp.graph);
-      grp.graph = nullptr;
-    }
+  EnsList& L = grp.list_host[parity];
+  L.count = cnt;
+  int pb = 0, tb = 0, rb = 0;
+  for (int a = 0; a < cnt; ++a) {
+    const int m = grp.active[a]->ens_slot;
+    L.member[a] = m;
+    L.pair_start[a] = pb;
+    L.thomas_start[a] = tb;
+    L.rhs32_start[a] = rb;
+    pb += grp.pair_blocks[m];
+    tb += grp.thomas_blocks[m];
+    rb += grp.rhs32_blocks[m];
   }
+  L.pair_start[cnt] = pb;
+  L.thomas_start[cnt] = tb;
+  L.rhs32_start[cnt] = rb;
+  PCF_HIP(hipMemcpyAsync(grp.list, &L, sizeof(EnsList), hipMemcpyHostToDevice, s));
   const EnsTable* dT = grp.table;
-  // the batch starts after each member's own work up to its solve
-  for (pcf_state* m : mem) PCF_HIP(hipStreamWaitEvent(s, m->ens_ready, 0));
-  const int pb = H.pair_start[cnt];
+  const EnsList* dL = grp.list;
   const int vb = cnt * kVecGrid;
   auto prec = [&](int in, int out) {
-    if (mixed) ensemble_precondition_mixed(s, dT, H, iters, bf16, in, out);
-    else ensemble_precondition(s, dT, H, iters, in, out);
+    if (mixed) ensemble_precondition_mixed(s, dT, dL, L, iters, bf16, in, out);
+    else ensemble_precondition(s, dT, dL, L, iters, in, out);
   };
-  // one iteration of bicgstab_core's loop for every member
-  auto iteration = [&]() -> int {
-    prec(EV_P, EV_Y);
-    hipLaunchKernelGGL((kE_pair<MODE_TOTAL, 1>), dim3(pb), pair_block(), 0, s, dT, EV_Y, EV_V, EV_R0, -1);
-    hipLaunchKernelGGL(kE_alpha, dim3(cnt), dim3(1024), 0, s, dT);
-    hipLaunchKernelGGL(kE_s, dim3(vb), dim3(256), 0, s, dT);
-    const int z = mixed ? EV_Z : EV_Y;
-    prec(EV_S, z);
-    hipLaunchKernelGGL((kE_pair<MODE_TOTAL, 2>), dim3(pb), pair_block(), 0, s, dT, z, EV_T, EV_S, -1);
-    hipLaunchKernelGGL(kE_omega, dim3(cnt), dim3(1024), 0, s, dT);
-    if (mixed) hipLaunchKernelGGL(kE_xr<true>, dim3(vb), dim3(kReduceBlock), 0, s, dT);
-    else hipLaunchKernelGGL(kE_xr<false>, dim3(vb), dim3(kReduceBlock), 0, s, dT);
-    hipLaunchKernelGGL(kE_check, dim3(cnt), dim3(1024), 0, s, dT);
-    hipLaunchKernelGGL(kE_p, dim3(vb), dim3(256), 0, s, dT);
-    hipLaunchKernelGGL(kE_flags, dim3(1), dim3(kEnsMax), 0, s, dT, e->flags_dev);
-    PCF_HIP(hipGetLastError());
-    return PCF_OK;
-  };
-  // every member stops on the device, converged or at its own maxit
-  int maxit = 0;
-  for (pcf_state* m : mem) maxit = m->cfg.poisson_iterations > maxit ? m->cfg.poisson_iterations : maxit;
-  int rc = PCF_OK;
-  for (int jb = 1; jb <= maxit + 1; ++jb) {
-    if (e->graphs) {
-      if (!grp.graph) {
-        hipGraph_t graph = nullptr;
-        PCF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        rc = iteration();
-        hipError_t err = hipStreamEndCapture(s, &graph);
-        if (rc) {
-          if (graph) (void)hipGraphDestroy(graph);
-          return rc;
-        }
-        PCF_HIP(err);
-        err = hipGraphInstantiate(&grp.graph, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        PCF_HIP(err);
-      }
-      PCF_HIP(hipGraphLaunch(grp.graph, s));
-    } else if ((rc = iteration())) {
-      return rc;
-    }
-    e->iterations += 1;
-    int32_t* fl = e->flags_host + (jb & 1) * kEnsMax;
-    PCF_HIP(hipMemcpyAsync(fl, e->flags_dev, cnt * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    PCF_HIP(hipEventRecord(e->ev[jb & 1], s));
-    if (jb >= 2) {
-      PCF_HIP(hipEventSynchronize(e->ev[(jb - 1) & 1]));
-      const int32_t* seen = e->flags_host + ((jb - 1) & 1) * kEnsMax;
-      bool all = true;
-      for (int q = 0; q < cnt && all; ++q) all = seen[q] != 0;
-      if (all) break;
-    }
-  }
-  for (pcf_state* m : mem)
-    PCF_HIP(hipMemcpyAsync(m->sc_host, m->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, s));
-  PCF_HIP(hipStreamSynchronize(s));
-  e->batches += 1;
-  e->member_solves += cnt;
+  prec(EV_P, EV_Y);
+  hipLaunchKernelGGL((kE_pair<MODE_TOTAL, 1>), dim3(pb), pair_block(), 0, s, dT, dL, EV_Y, EV_V, EV_R0, -1);
+  hipLaunchKernelGGL(kE_alpha, dim3(cnt), dim3(1024), 0, s, dT, dL);
+  hipLaunchKernelGGL(kE_s, dim3(vb), dim3(256), 0, s, dT, dL);
+  const int z = mixed ? EV_Z : EV_Y;
+  prec(EV_S, z);
+  hipLaunchKernelGGL((kE_pair<MODE_TOTAL, 2>), dim3(pb), pair_block(), 0, s, dT, dL, z, EV_T, EV_S, -1);
+  hipLaunchKernelGGL(kE_omega, dim3(cnt), dim3(1024), 0, s, dT, dL);
+  if (mixed) hipLaunchKernelGGL(kE_xr<true>, dim3(vb), dim3(kReduceBlock), 0, s, dT, dL);
+  else hipLaunchKernelGGL(kE_xr<false>, dim3(vb), dim3(kReduceBlock), 0, s, dT, dL);
+  hipLaunchKernelGGL(kE_check, dim3(cnt), dim3(1024), 0, s, dT, dL);
+  hipLaunchKernelGGL(kE_p, dim3(vb), dim3(256), 0, s, dT, dL);
+  hipLaunchKernelGGL(kE_flags, dim3(1), dim3(kEnsMax), 0, s, dT, dL, flags);
+  PCF_HIP(hipGetLastError());
   return PCF_OK;
 }
 

@@ -7,8 +7,9 @@ state, and ctypes releases the interpreter lock while a library call runs),
 so the kernels of small members run side by side and fill the chip
 together.  With ``batched`` (the default) the members' pressure solves run
 together as well: a ``SolveBatch`` (pcf_ensemble) runs the BiCGSTAB
-iterations of every member whose solve waits as one launch per kernel over
-all of them, each member masked out once it has converged.  Errors follow
+iterations of all solves in progress as one launch per kernel over the
+members, each solve joining at an iteration boundary and leaving as soon as
+it has converged.  Errors follow
 reduce_exceptions: every member runs to its end or to its error, then the
 first error is raised with the member named.
 """
@@ -26,21 +27,20 @@ from .state import PinCFlowError, State, integrate
 
 
 class SolveBatch:
-    """The pcf_ensemble of a set of states: their BiCGSTAB solves run in
-    batches, one launch per kernel of an iteration over all waiting members
+    """The pcf_ensemble of a set of states: their BiCGSTAB solves run in one
+    batch, one launch per kernel of an iteration over all solves in progress
     (the line preconditioner in FP64 or FP32, x_size even, a multiple of 4 in
-    the mixed solve; other solves run on their own).  A state is attached
-    from the thread that steps it and must be detached when that thread
-    stops stepping it."""
+    the mixed solve; other solves run on their own); a solve joins and leaves
+    the batch at iteration boundaries.  A state is attached from the thread
+    that steps it and detached when that thread stops stepping it."""
 
-    def __init__(self, graphs: bool = True):
+    def __init__(self):
         self.lib = _abi.load_library()
         h = ctypes.c_void_p()
         rc = self.lib.pcf_ensemble_create(ctypes.byref(h))
         if rc:
             raise PinCFlowError(rc, self.lib.pcf_last_error().decode())
         self.h = h
-        self.lib.pcf_ensemble_set_graphs(self.h, int(graphs))
 
     def attach(self, st: State):
         rc = self.lib.pcf_ensemble_attach(self.h, st.h)
@@ -53,7 +53,8 @@ class SolveBatch:
     def stats(self) -> Dict[str, int]:
         out = (ctypes.c_int64 * 4)()
         self.lib.pcf_ensemble_stats(self.h, out)
-        return {"batches": out[0], "batched_iterations": out[1], "member_solves": out[2], "solves_alone": out[3]}
+        return {"batched_iterations": out[0], "member_solves": out[1], "solves_alone": out[2],
+                "table_builds": out[3]}
 
     def close(self):
         if getattr(self, "h", None):

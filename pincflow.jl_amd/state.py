@@ -97,6 +97,13 @@ class State:
         self._check(rc, max(info.niter))
         return info
 
+    def solver_graphs(self, on: bool = True):
+        """Replay BiCGSTAB iterations from a captured hipGraph (same kernels,
+        same bits).  Only for a state stepped by the one thread that uses
+        HIP: while a capture runs, HIP refuses other threads' stream
+        synchronisations and allocations."""
+        self._check(self.lib.pcf_set_solver_graphs(self.h, int(on)))
+
     def compute_time_step(self) -> float:
         dt = ctypes.c_double(0.0)
         self._check(self.lib.pcf_compute_time_step(self.h, ctypes.byref(dt)))
