@@ -63,8 +63,10 @@ def test_members_sharing_the_gpu_match_their_standalone_runs(pcf, tmp_path):
 
 def _members(pcf, precision, seeds, sizes, iterations, pc=None, poisson_iterations=None):
     def sim(seed, x_size, iterations, pc=None, poisson_iterations=None):
+        # "both": FP64 and mixed members by seed parity, two solve kinds in one batch
+        prec = ("fp64", "mixed")[seed % 2] if precision == "both" else precision
         nl = pcf.examples.bubble_member(seed=seed, x_size=x_size, z_size=12, iterations=iterations,
-                                        precision=precision)
+                                        precision=prec)
         if pc:
             nl.poisson.preconditioner_type = pc
         if poisson_iterations:
@@ -79,7 +81,7 @@ def _members(pcf, precision, seeds, sizes, iterations, pc=None, poisson_iteratio
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "both"])
 def test_batched_solves_give_each_member_its_standalone_bits(pcf, tmp_path, precision):
     """Members of uneven grids (16-32 cells square, 12 levels) and uneven run
     lengths whose solves run as one launch per kernel over all of them
@@ -87,11 +89,13 @@ def test_batched_solves_give_each_member_its_standalone_bits(pcf, tmp_path, prec
     every member's iteration counts and fields equal its own standalone
     run's bit for bit.  One member uses the multigrid preconditioner, which
     the batch does not cover: its solves run on their own; the shorter runs
-    detach (and the tables are rebuilt) while the others go on."""
+    detach (and the tables are rebuilt) while the others go on.  "both":
+    FP64 and mixed members in one ensemble, two solve kinds batched side by
+    side."""
     seeds = (0, 1, 2, 3, 4, 5)
     sizes = (16, 24, 16, 32, 24, 32)
     iterations = (3, 3, 1, 3, 2, 3)
-    pcs = ("line_relaxation",) * 5 + ("multigrid" if precision == "mixed" else "line_relaxation",)
+    pcs = ("line_relaxation",) * 5 + ("multigrid" if precision in ("mixed", "both") else "line_relaxation",)
     sim, params = _members(pcf, precision, seeds, sizes, iterations, pc=pcs)
     params["output_file"] = tuple(str(tmp_path / f"m{s}.npz") for s in seeds)
     batch = pcf.SolveBatch()
@@ -100,7 +104,7 @@ def test_batched_solves_give_each_member_its_standalone_bits(pcf, tmp_path, prec
     batch.close()
     print(precision, "batch statistics:", stats)
     assert stats["member_solves"] > 0 and stats["batched_iterations"] > 0
-    if precision == "mixed":
+    if precision != "fp64":
         assert stats["solves_alone"] > 0
     for c, seed in enumerate(seeds):
         alone, _, steps = pcf.integrate(sim(seed, sizes[c], iterations[c], pcs[c]))
