@@ -1508,7 +1508,10 @@ int bicgstab_core(pcf_state* st, double tolref, bool mixed, const double* guess,
     return PCF_OK;
   };
   for (int jb = 1; jb <= maxit; ++jb) {
-    const bool graphed = st->solver_graphs && !st->prof_on && !(stage && jb <= st->stage_iterations);
+    // (never for an ensemble member: its service thread synchronises
+    // while this thread would capture)
+    const bool graphed =
+        st->solver_graphs && !st->ens && !st->prof_on && !(stage && jb <= st->stage_iterations);
     if (graphed) {
       hipGraphExec_t& exec = st->iter_graph[mixed ? 1 : 0];
       if (!exec) {
