@@ -145,3 +145,28 @@ def test_batched_solve_that_fails_stops_at_its_own_limit(pcf, abi, tmp_path):
 def _no_cleaning(nl):
     nl.poisson.initial_cleaning = False
     return nl
+
+
+@pytest.mark.gpu
+def test_ensemble_attach_errors(pcf, abi):
+    """The ensemble's argument checks: a state attaches to one ensemble at a
+    time, an ensemble with members attached is not destroyed, and detach is
+    idempotent (pcf_destroy detaches too)."""
+    import ctypes
+    nl = pcf.examples.bubble_member(seed=0, x_size=16, z_size=12, iterations=1)
+    st = pcf.State(nl)
+    a, b = pcf.SolveBatch(), pcf.SolveBatch()
+    a.attach(st)
+    with pytest.raises(pcf.PinCFlowError) as e:
+        b.attach(st)
+    assert e.value.rc == abi.E_ARG
+    assert st.lib.pcf_ensemble_destroy(a.h) == abi.E_ARG
+    a.detach(st)
+    a.detach(st)
+    b.attach(st)
+    st.step()
+    st.close()  # detaches from b
+    a.close()
+    b.close()
+    assert st.lib.pcf_ensemble_attach(None, None) == abi.E_ARG
+    assert st.lib.pcf_ensemble_stats(None, (ctypes.c_int64 * 4)()) == abi.E_ARG
