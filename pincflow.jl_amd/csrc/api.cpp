@@ -355,6 +355,9 @@ int pcf_create(const pcf_config* cfg, pcf_state** out) {
       if ((rc = alloc32(st, &st->carrier32[d], NH))) return fail(rc);
     for (int d = 0; d < 2; ++d)
       if ((rc = alloc32(st, &st->tstress32[d], NH))) return fail(rc);
+    if (cfg->model != PCF_BOUSSINESQ)
+      for (int d = 0; d < 3; ++d)
+        if ((rc = alloc(st, &st->rho_bg[d], NH))) return fail(rc);
   }
   for (int q = 0; q < T_COUNT; ++q)
     if ((rc = alloc(st, &st->T.a[q], NV))) return fail(rc);

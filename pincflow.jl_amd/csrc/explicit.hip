@@ -43,6 +43,10 @@ struct FluxArgsT {
   const T* st[6];  // stress 11, 12, 13, 22, 23, 33
   const T* t13;    // met13 s11 + met23 s12 + s13 / J (k_stress)
   const T* t23;    // met13 s12 + met23 s22 + s23 / J
+  // FP32 fluxes (T = float): the background part of the Rho flux at each
+  // face, carrier x (rhobar / P at the face), in FP64 (k_rho_background);
+  // the FP32 flux is then the fluctuation's alone
+  const double* bg[3];
   double* flux[7][3];
   Geo g;
   T inv_re;        // 1 / re
@@ -240,6 +244,38 @@ __global__ void k_carriers(Fields F, Geo g, const double* u0, const double* v0, 
   if (k < g.nzz) cz[c] = pe_z<T>(F, g, i, j, k) * (T)w0[c];
 }
 
+// With FP32 fluxes, the part of the Rho flux that carries the background
+// density, carrier x rhobar / P at each face (the term the FP64 flux adds
+// to both reconstructed values, compute_fluxes!.jl:359-520), in FP64: the
+// FP32 flux then carries only the small fluctuation rho / P, and the
+// divergence of this large, nearly divergence-free part keeps FP64 rounding.
+// Fixed through an LHS integration (the p0 winds and the background are).
+__global__ void k_rho_background(Fields F, Geo g, const double* u0, const double* v0, const double* w0, double* bx,
+                                 double* by, double* bz) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.nxx) return;
+  const size_t c = g.h(i, j, k);
+  if (i < g.nxx) {
+    const size_t r = g.h(i + 1, j, k);
+    const double rb = 0.5 * (F.rhobar[c] + F.rhobar[r]), pe = 0.5 * (F.pbar[c] + F.pbar[r]);
+    bx[c] = pe_x<double>(F, g, i, j, k) * u0[c] * (rb / pe);
+  }
+  if (j < g.nyy) {
+    const size_t f = g.h(i, j + 1, k);
+    const double rb = 0.5 * (F.rhobar[c] + F.rhobar[f]), pe = 0.5 * (F.pbar[c] + F.pbar[f]);
+    by[c] = pe_y<double>(F, g, i, j, k) * v0[c] * (rb / pe);
+  }
+  if (k < g.nzz) {
+    const size_t u = g.h(i, j, k + 1);
+    const double jc = F.jac[c], ju = F.jac[u];
+    const double rb = (ju * F.rhobar[c] + jc * F.rhobar[u]) / (jc + ju);
+    const double pe = (ju * F.pbar[c] + jc * F.pbar[u]) / (jc + ju);
+    bz[c] = pe_z<double>(F, g, i, j, k) * w0[c] * (rb / pe);
+  }
+}
+
 // One flux component of one variable at face (i, j, k)
 // (compute_fluxes!.jl:359-520, 587-1503): a pure function of the
 // reconstruction inputs, the p0 winds, the stress components and the static
@@ -256,8 +292,11 @@ __device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int
   T val;
   if (VAR == V_RHO || VAR == V_RHOP || VAR == V_CHI) {  // the tracer: RhoP's carrier (compute_fluxes!.jl:1513-1570)
     T hi = left_of_next, lo = right_of_c;  // value at c + e, value at c
+    // the background term rhobar / P enters FP64 fluxes here and FP32
+    // fluxes as the separate FP64 part A.bg
+    constexpr bool bg_here = VAR == V_RHO && sizeof(T) == 8;
     if (DIR == 0) {
-      if (VAR == V_RHO) {
+      if (bg_here) {
         const size_t r = g.h(i + 1, j, k);
         T rhobaredger = T(0.5) * (LT(F.rhobar, c) + LT(F.rhobar, r));
         T pe = T(0.5) * (LT(F.pbar, c) + LT(F.pbar, r));
@@ -266,7 +305,7 @@ __device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int
       }
       val = upwind_t<T>(A.cx[c], lo, hi);
     } else if (DIR == 1) {
-      if (VAR == V_RHO) {
+      if (bg_here) {
         const size_t f = g.h(i, j + 1, k);
         T rhobaredgef = T(0.5) * (LT(F.rhobar, c) + LT(F.rhobar, f));
         T pe = T(0.5) * (LT(F.pbar, c) + LT(F.pbar, f));
@@ -277,7 +316,7 @@ __device__ __forceinline__ T flux_value(const FluxArgsT<T>& A, int i, int j, int
     } else {
       const size_t u = g.h(i, j, k + 1);
       const T jc = LT(F.jac, c), ju = LT(F.jac, u);
-      if (VAR == V_RHO) {
+      if (bg_here) {
         T rhobaredgeu = (ju * LT(F.rhobar, c) + jc * LT(F.rhobar, u)) / (jc + ju);
         T pe = (ju * LT(F.pbar, c) + jc * LT(F.pbar, u)) / (jc + ju);
         hi = hi + rhobaredgeu / pe;
@@ -774,9 +813,14 @@ __global__ void __launch_bounds__(256) k_stage_update(FluxArgsT<T> FA, UpdArgs A
       const T fzu = zface<VAR, T>(FA, i, j, k);
       const T fxr = flux_value<VAR, 0, T>(FA, i, j, k), fxl = flux_value<VAR, 0, T>(FA, i - 1, j, k);
       const T fyf = flux_value<VAR, 1, T>(FA, i, j, k);
-      const double fluxdiff =
-          (double)((fxr - fxl) / T(g.dx) + (fyf - fyb) / T(g.dy) + (fzu - fzd[jj]) / T(g.dz));
+      double fluxdiff = (double)((fxr - fxl) / T(g.dx) + (fyf - fyb) / T(g.dy) + (fzu - fzd[jj]) / T(g.dz));
       const size_t c = g.h(i, j, k);
+      if constexpr (VAR == V_RHO && sizeof(T) == 4) {  // the FP64 background part of the Rho flux
+        const size_t sy = (size_t)g.nxx, sz = sy * g.nyy;
+        const double bzu = k == g.k1 ? 0.0 : FA.bg[2][c], bzd = k - 1 == g.k0 - 1 ? 0.0 : FA.bg[2][c - sz];
+        fluxdiff += (FA.bg[0][c] - FA.bg[0][c - 1]) / g.dx + (FA.bg[1][c] - FA.bg[1][c - sy]) / g.dy +
+                    (bzu - bzd) / g.dz;
+      }
       if (VAR == V_U) A.q[c] = sponged<V_U>(A, i, j, k, update_u_at(A, i, j, k, fluxdiff));
       else if (VAR == V_V) A.q[c] = sponged<V_V>(A, i, j, k, update_v_at(A, i, j, k, fluxdiff));
       else if (VAR == V_W) A.fdiv[c] = fluxdiff;  // the update follows in k_update_w_fdiv
@@ -1238,6 +1282,7 @@ FluxArgsT<float> flux_args32(pcf_state* st) {
   for (int q = 0; q < 6; ++q) A.st[q] = st->stress32[q];
   A.t13 = st->tstress32[0];
   A.t23 = st->tstress32[1];
+  for (int d = 0; d < 3; ++d) A.bg[d] = st->rho_bg[d];
   for (int q = 0; q < 7; ++q)
     for (int d = 0; d < 3; ++d) A.flux[q][d] = nullptr;
   A.g = st->g;
@@ -1259,6 +1304,7 @@ FluxArgs flux_args(pcf_state* st) {
   for (int q = 0; q < 6; ++q) A.st[q] = st->stress[q];
   A.t13 = st->tstress[0];
   A.t23 = st->tstress[1];
+  for (int d = 0; d < 3; ++d) A.bg[d] = nullptr;  // the FP64 Rho flux carries its background itself
   for (int q = 0; q < 7; ++q)
     for (int d = 0; d < 3; ++d) A.flux[q][d] = st->flux[q][d];
   A.g = st->g;
@@ -1536,6 +1582,12 @@ int stage_inputs(pcf_state* st, int m) {
   PCF_HIP(hipGetLastError());
   int rc = m == 1 ? carriers(st, true) : PCF_OK;
   if (rc) return rc;
+  if (m == 1 && st->rho_bg[0]) {
+    const dim3 grd(blocks_for(g.nxx, 64), g.nyy, g.nzz), blk(64);
+    hipLaunchKernelGGL(k_rho_background, grd, blk, 0, st->stream, fields_of(st), g, st->p0[2], st->p0[3], st->p0[4],
+                       st->rho_bg[0], st->rho_bg[1], st->rho_bg[2]);
+    PCF_HIP(hipGetLastError());
+  }
   return stress_tensor(st, true);
 }
 

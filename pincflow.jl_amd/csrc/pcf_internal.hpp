@@ -252,6 +252,9 @@ struct pcf_state {
   // values), FP64 and FP32
   double* tstress[2] = {};
   float* tstress32[2] = {};
+  // with FP32 fluxes, the FP64 background part of the Rho flux at the x, y
+  // and z faces (k_rho_background; pseudo-incompressible and compressible)
+  double* rho_bg[3] = {};
   // between flux computations stress[0] and stress[1] hold the flux
   // divergences of the stage's U and V updates, which the W update reuses
   bool lhs_div_valid[2] = {false, false};

@@ -29,7 +29,7 @@ from test_oracle_reduction_order import iter_slack
 pytestmark = pytest.mark.gpu
 
 FIELD_RTOL = 1e-5
-FLUX32_RTOL = 2e-4
+FLUX32_RTOL = 2e-6
 STEP_RTOL = 1e-9
 
 
@@ -90,10 +90,10 @@ def test_sixteen_hills_mixed_steps_track_the_oracle(pcf, abi, oracle_mod, seed, 
     """The mixed path (multigrid, bf16 coefficients, warm start), with the
     RK stages' fluxes in FP64 or in FP32 (flux_precision = "mixed": the
     configuration's FP32 fluxes and preconditioner), against three FP64
-    oracle steps.  FP32 fluxes round the background density the density
-    flux carries (rhobar / pbar) to FP32; relative to the density
-    fluctuations that leaves errors of order 1e-5 after three steps, which
-    FLUX32_RTOL allows; every solve still meets the FP64 criterion."""
+    oracle steps.  With FP32 fluxes (the density flux's background part in
+    FP64) the fields stay within 4.7e-7 of the oracle's after three steps
+    (measured on these seeds), which FLUX32_RTOL allows; every solve still
+    meets the FP64 criterion."""
     nl = hills(pcf, seed, "mixed")
     nl.poisson.preconditioner_type = "multigrid"
     nl.poisson.preconditioner_storage = "bf16"
