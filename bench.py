@@ -73,7 +73,7 @@ def parse():
     p.add_argument("--dtau", type=float, default=1.0, help="pseudo-time step of the preconditioner (default 1)")
     p.add_argument("--preconditioner", default="line_relaxation", choices=["line_relaxation", "multigrid"],
                    help="mixed precision: the reference's line relaxation or the multigrid V-cycle")
-    p.add_argument("--initial-guess", default="zero", choices=["zero", "previous", "learned"],
+    p.add_argument("--initial-guess", default="zero", choices=["zero", "previous", "extrapolated", "learned"],
                    help="mixed precision: start each solve from zero, from the same stage's previous solution, or "
                         "from the learned network's guess (trained first, on held-out terrain)")
     p.add_argument("--mg-smoothing-steps", type=int, default=2, help="multigrid pre- and post-smoothing steps")

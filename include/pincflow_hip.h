@@ -100,11 +100,14 @@ enum pcf_rhs_sponge {
 /* Initial guess of the mixed-precision solve (the FP64 solve starts from
  * x = 0 as apply_bicgstab!.jl:44 does).  PREVIOUS: the solution of the same
  * implicit stage in the previous time step, rescaled by the ratio of the
- * time steps. */
+ * time steps.  EXTRAPOLATED: the same stage's solutions of the previous two
+ * time steps (per unit time step) extrapolated linearly in time to this
+ * one (PREVIOUS until two exist). */
 enum pcf_initial_guess {
   PCF_GUESS_ZERO = 0,
   PCF_GUESS_PREVIOUS = 1,
-  PCF_GUESS_LEARNED = 2 /* the attached network (pcf_set_guess_network) */
+  PCF_GUESS_LEARNED = 2, /* the attached network (pcf_set_guess_network) */
+  PCF_GUESS_EXTRAPOLATED = 3
 };
 
 /* Preconditioner of the mixed-precision solve.  LINE_RELAXATION: the

@@ -37,7 +37,7 @@ PERT_NONE, PERT_COSINE_BUBBLE, PERT_VORTEX = 0, 1, 2
 LHS_SPONGE_NONE, LHS_SPONGE_SIN2 = 0, 1
 RHS_SPONGE_NONE, RHS_SPONGE_SIN2_TOP = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED = 0, 1, 2
+GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED, GUESS_EXTRAPOLATED = 0, 1, 2, 3
 PC_LINE_RELAXATION, PC_MULTIGRID = 0, 1
 STORAGE_FP32, STORAGE_BF16 = 0, 1
 

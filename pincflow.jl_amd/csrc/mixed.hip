@@ -13,7 +13,8 @@
 // Optionally (initial_guess = PCF_GUESS_PREVIOUS) each implicit stage starts
 // from the same stage's solution of the previous time step, rescaled by the
 // ratio of the time steps (the solution is proportional to dt times the
-// pressure correction); the criterion stays |r| / |b|.
+// pressure correction), or (PCF_GUESS_EXTRAPOLATED) from the linear
+// extrapolation in time of the last two; the criterion stays |r| / |b|.
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -318,6 +319,13 @@ __global__ void __launch_bounds__(64) k_pc_thomas32(const float* P, const float*
 __global__ void k_to_f32(const double* a, float* b, size_t n) {
   for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
     b[q] = (float)a[q];
+}
+
+// g = q1 + r (q1 - q2): the linear extrapolation in time of the last two
+// solutions per unit time step (r: this step over the previous one)
+__global__ void k_extrapolate(const double* q1, const double* q2, double* g, size_t n, double r) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    g[q] = q1[q] + r * (q1[q] - q2[q]);
 }
 
 // g = c x (the saved warm start, c = 1 / dt)
@@ -1101,9 +1109,18 @@ int apply_bicgstab_mixed(pcf_state* st, double tolref, double dt, int* niter_out
   if (rc) return rc;
   const int slot = st->solve_slot;
   const bool stage = slot >= 0 && slot < 2 && dt > 0.0;
-  const bool use_prev = (n.initial_guess == PCF_GUESS_PREVIOUS || n.initial_guess == PCF_GUESS_LEARNED) && stage;
+  const bool extrap = n.initial_guess == PCF_GUESS_EXTRAPOLATED;
+  const bool use_prev =
+      (n.initial_guess == PCF_GUESS_PREVIOUS || n.initial_guess == PCF_GUESS_LEARNED || extrap) && stage;
   const double* prev = use_prev && st->m_prev_valid[slot] ? st->m_prev[slot] : nullptr;
-  const double* guess = n.initial_guess == PCF_GUESS_PREVIOUS ? prev : nullptr;
+  const double* guess = n.initial_guess == PCF_GUESS_PREVIOUS || extrap ? prev : nullptr;
+  const size_t N = st->g.nv();
+  if (extrap && prev && st->m_prev2_valid[slot] && st->m_prev_dt[slot] > 0.0) {
+    hipLaunchKernelGGL(k_extrapolate, dim3(2048), dim3(256), 0, st->stream, prev, st->m_prev2[slot], st->m_guess, N,
+                       dt / st->m_prev_dt[slot]);
+    PCF_HIP(hipGetLastError());
+    guess = st->m_guess;
+  }
   double scale = dt;
   if (n.initial_guess == PCF_GUESS_LEARNED && stage) {
     if ((rc = learned_guess(st, dt, prev, &guess))) return rc;
@@ -1123,11 +1140,15 @@ int apply_bicgstab_mixed(pcf_state* st, double tolref, double dt, int* niter_out
   }
   if (rc) return rc;
   if (use_prev) {  // keep solution / dt for the same stage of the next step
-    const size_t N = st->g.nv();
+    if (extrap) {  // and the one before it
+      std::swap(st->m_prev[slot], st->m_prev2[slot]);
+      st->m_prev2_valid[slot] = st->m_prev_valid[slot];
+    }
     hipLaunchKernelGGL(k_scale_copy, dim3(2048), dim3(256), 0, st->stream, st->solution, st->m_prev[slot], N,
                        1.0 / dt);
     PCF_HIP(hipGetLastError());
     st->m_prev_valid[slot] = true;
+    st->m_prev_dt[slot] = dt;
   }
   return PCF_OK;
 }
@@ -1239,6 +1260,11 @@ int alloc_mixed(pcf_state* st) {
   double** dl[] = {&st->bzpc, &st->m_prev[0], &st->m_prev[1]};
   for (double** p : dl)
     if ((rc = device_alloc(st, p, N))) return rc;
+  if (st->cfg.initial_guess == PCF_GUESS_EXTRAPOLATED) {
+    double** el[] = {&st->m_prev2[0], &st->m_prev2[1], &st->m_guess};
+    for (double** p : el)
+      if ((rc = device_alloc(st, p, N))) return rc;
+  }
   if (st->cfg.preconditioner_type != PCF_PC_MULTIGRID) return PCF_OK;
   // level 0 works on the buffers above; coarser levels get their own
   const Geo& g = st->g;

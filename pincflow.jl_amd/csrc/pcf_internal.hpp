@@ -291,6 +291,12 @@ struct pcf_state {
   // the previous time step (slot 0: first stage, 1: second stage)
   double* m_prev[2] = {nullptr, nullptr};
   bool m_prev_valid[2] = {false, false};
+  // initial_guess EXTRAPOLATED: the same per slot one step older, the time
+  // step of the solve m_prev holds, and the extrapolated guess
+  double* m_prev2[2] = {nullptr, nullptr};
+  bool m_prev2_valid[2] = {false, false};
+  double m_prev_dt[2] = {0.0, 0.0};
+  double* m_guess = nullptr;
   int solve_slot = -1;                // stage of the solve in progress, -1 outside pcf_step
   // learned initial guess: the attached network and its FP32 input / output
   pcf_net* guess_net = nullptr;

@@ -181,7 +181,8 @@ int host_setup(pcf_state* st, std::string& err) {
     err = "Invalid flux_precision";
     return PCF_E_OPTION;
   }
-  if (n.initial_guess != PCF_GUESS_ZERO && n.initial_guess != PCF_GUESS_PREVIOUS && n.initial_guess != PCF_GUESS_LEARNED) {
+  if (n.initial_guess != PCF_GUESS_ZERO && n.initial_guess != PCF_GUESS_PREVIOUS && n.initial_guess != PCF_GUESS_LEARNED &&
+      n.initial_guess != PCF_GUESS_EXTRAPOLATED) {
     err = "Invalid initial_guess";
     return PCF_E_OPTION;
   }

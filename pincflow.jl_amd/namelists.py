@@ -159,7 +159,9 @@ class PoissonNamelist:
     # FP64 BiCGSTAB with the preconditioner evaluated in FP32)
     precision: str = "fp64"
     # "zero" (as the reference), "previous" (the same stage's solution of the
-    # previous time step) or "learned" (the attached network); mixed solve only
+    # previous time step), "extrapolated" (the linear extrapolation in time of
+    # the same stage's last two solutions) or "learned" (the attached
+    # network); mixed solve only
     initial_guess: str = "zero"
     # "line_relaxation" (the reference's) or "multigrid"; mixed solve only
     preconditioner_type: str = "line_relaxation"
@@ -374,7 +376,8 @@ def to_config(nl: Namelists) -> _abi.PcfConfig:
     c.output_interval = o.output_interval
     c.tmax = o.tmax
     c.initial_guess = _option({"zero": _abi.GUESS_ZERO, "previous": _abi.GUESS_PREVIOUS,
-                               "learned": _abi.GUESS_LEARNED}, po.initial_guess, "initial_guess")
+                               "learned": _abi.GUESS_LEARNED, "extrapolated": _abi.GUESS_EXTRAPOLATED},
+                              po.initial_guess, "initial_guess")
     c.preconditioner_type = _option({"line_relaxation": _abi.PC_LINE_RELAXATION, "multigrid": _abi.PC_MULTIGRID},
                                     po.preconditioner_type, "preconditioner_type")
     c.mg_smoothing_steps = int(po.mg_smoothing_steps)

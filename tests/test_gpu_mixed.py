@@ -121,7 +121,7 @@ def solve_case(pcf, abi, oracle_mod, name, pc, storage):
 
 
 @pytest.mark.parametrize("guess,pc", [("zero", "line_relaxation"), ("previous", "line_relaxation"),
-                                      ("zero", "multigrid")])
+                                      ("zero", "multigrid"), ("extrapolated", "multigrid")])
 @pytest.mark.parametrize("name", ["mountain_wave_3d", "periodic_hill", "hot_bubble"])
 def test_mixed_steps_track_fp64(pcf, name, guess, pc):
     nl = configs(pcf)[name]
