@@ -247,6 +247,7 @@ def main():
                                                            ("multigrid", "bf16", 2, "zero", False, "fp64"),
                                                            ("multigrid", "bf16", 2, "previous", False, "fp64"),
                                                            ("multigrid", "bf16", 2, "previous", False, "mixed"),
+                                                           ("multigrid", "bf16", 2, "extrapolated", False, "mixed"),
                                                            ("multigrid", "bf16", 2, "learned", False, "mixed"),
                                                            ("multigrid", "bf16", 2, "previous", True, "mixed")):
             if (guess == "learned" or staged) and vtrain is None:
@@ -278,7 +279,7 @@ def main():
                      "kernel_ms_one_step": {k: [v[0], round(v[1], 3)] for k, v in sorted(r["breakdown"].items())},
                      "stage_kernel_shapes": r["stage_tuning"]}
             entry["setup_s"], entry["initial_cleaning_s"] = r["setup_s"], r["initial_cleaning_s"]
-            if guess in ("previous", "learned"):
+            if guess in ("previous", "extrapolated", "learned"):
                 entry["guess_fallbacks"] = r["state"].guess_fallbacks()
             if staged:
                 entry["stage_fallbacks"] = r["state"].stage_fallbacks()
