@@ -156,3 +156,21 @@ def test_isentropic_background_is_the_hydrostatic_adiabat(pcf, oracle_mod, abi, 
         got = o.get(name)
         assert np.max(np.abs(got - expect) / np.abs(expect)) <= 1e-12, name
     assert np.max(np.abs(interior(o.get("N2")))) == 0.0
+
+
+@pytest.mark.parametrize("example,stable", [("periodic_hill", True), ("vortex", False)])
+def test_boussinesq_background_is_uniform(pcf, oracle_mod, abi, example, stable):
+    """the Boussinesq reference state: rho_ref, theta0, P = rho theta, and N^2
+    the namelist's buoyancy frequency squared in units of t_ref^-2 (zero for
+    the neutral stratification)"""
+    nl = getattr(pcf.examples, example)(x_size=16)
+    cfg = pcf.to_config(nl)
+    o = oracle_mod.Oracle(cfg, abi)
+    th0 = cfg.potential_temperature / THETAREF
+    assert np.array_equal(o.get("RHOBAR"), np.ones_like(o.get("RHOBAR")))
+    assert np.max(np.abs(o.get("THETABAR") / th0 - 1.0)) <= 1e-15
+    assert np.max(np.abs(o.get("PBAR") / th0 - 1.0)) <= 1e-15
+    n2 = (cfg.buoyancy_frequency * TREF) ** 2 if stable else 0.0
+    if stable:
+        assert n2 > 0.0
+    assert np.max(np.abs(o.get("N2") - n2)) <= 1e-14 * max(n2, 1.0)
