@@ -184,10 +184,9 @@ struct pcf_ensemble {
   int device = 0;
   int attached = 0;
   // per solve kind (FP64 or mixed, preconditioner iterations, coefficient
-  // storage): the members, the device table, its host copy and the
-  // captured iteration
-  // the member table, its host copy, each slot's blocks in the grids, the
-  // solves of the current iteration (host, two pinned slots, device)
+  // storage): the members, the device member table and its host copy, each
+  // slot's blocks in the grids, and the solves of the current iteration
+  // (host, two pinned slots, device)
   struct Group {
     int key = -1;
     std::vector<pcf_state*> slots;
