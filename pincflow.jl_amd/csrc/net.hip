@@ -413,44 +413,72 @@ __global__ void k_gelu_back(Dims d, const float* g, const float* z, bf16* dz, fl
 
 // ---- weight gradient of a 32 -> 32 layer on the matrix cores: for tap t and
 // a chunk of voxels, dW[t] (32 co x 32 ci) += sum_p dz(p)[co] in(p + tap_t)[ci],
-// with 16 voxels per MFMA step staged through LDS.  One wavefront per (chunk, tap).
-__global__ void __launch_bounds__(64) k_mid_weights(Dims d, const bf16* __restrict__ dz, const bf16* __restrict__ in,
-                                                    float* partials) {
-  __shared__ bf16 ld[16][kC];
-  __shared__ bf16 la[16][kC];
-  const int lane = threadIdx.x, t = blockIdx.y;
+// with 16 voxels per MFMA step.  A workgroup of four wavefronts takes a chunk
+// and four taps (blockIdx.y = tap group, wavefront w: tap 4 y + w, the last
+// group has three): batches of 64 voxels are staged in LDS, dz once for the
+// four taps and each tap's neighbours apart, and the next batch's global
+// loads are issued before the current batch's MFMAs.  Each tap's MFMAs run
+// over the chunk's 16-voxel steps in order.
+constexpr int kWBatch = 64;  // voxels per staged batch
+
+__global__ void __launch_bounds__(256) k_mid_weights(Dims d, const bf16* __restrict__ dz, const bf16* __restrict__ in,
+                                                     float* partials) {
+  __shared__ bf16 ld[kWBatch][kC];
+  __shared__ bf16 la[4][kWBatch][kC];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int t = 4 * (int)blockIdx.y + w;  // 27: no tap (the wavefront only stages)
   const int col = lane & 31, half = lane >> 5;
   const size_t p0 = (size_t)blockIdx.x * kChunk, p1 = p0 + kChunk < d.n ? p0 + kChunk : d.n;
+  // staging: thread T loads dz of voxel T / 4 (8 channels 8 (T % 4)); lane L
+  // of wavefront w loads its tap's neighbour of voxels L / 4 + 16 u, u < 4
+  const int vd = (int)threadIdx.x >> 2, cd = 8 * (threadIdx.x & 3);
+  const int va = lane >> 2, ca = 8 * (lane & 3);
+  auto load = [&](size_t pb, bf8& dv, bf8 (&av)[4]) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) dv[e] = (bf16)0.0f;
+    const size_t p = pb + vd;
+    if (p < p1) dv = *(const bf8*)(dz + p * kC + cd);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) av[u][e] = (bf16)0.0f;
+      const size_t q = pb + va + 16 * u;
+      if (t < 27 && q < p1) {
+        int i, j, k;
+        coords(d, q, i, j, k);
+        const long long nq = neighbour(d, i, j, k, t);
+        if (nq >= 0) av[u] = *(const bf8*)(in + (size_t)nq * kC + ca);
+      }
+    }
+  };
   f16v acc = {};
-  for (size_t pb = p0; pb < p1; pb += 16) {
-    // each lane stages 8 channels of one voxel of both tiles
-    const int v = lane >> 2, c0 = 8 * (lane & 3);
-    const size_t p = pb + v;
-    bf8 dv, av;
+  bf8 dv, av[4];
+  load(p0, dv, av);
+  for (size_t pb = p0; pb < p1; pb += kWBatch) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) dv[e] = av[e] = (bf16)0.0f;
-    if (p < p1) {
-      dv = *(const bf8*)(dz + p * kC + c0);
-      int i, j, k;
-      coords(d, p, i, j, k);
-      const long long q = neighbour(d, i, j, k, t);
-      if (q >= 0) av = *(const bf8*)(in + (size_t)q * kC + c0);
-    }
+    for (int e = 0; e < 8; ++e) ld[vd][cd + e] = dv[e];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      ld[v][c0 + e] = dv[e];
-      la[v][c0 + e] = av[e];
-    }
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) la[w][va + 16 * u][ca + e] = av[u][e];
     __syncthreads();
-    bf8 A, B;
+    if (pb + kWBatch < p1) load(pb + kWBatch, dv, av);  // in flight during the MFMAs
+    if (t < 27) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      A[e] = ld[8 * half + e][col];  // A[co][k = voxel]
-      B[e] = la[8 * half + e][col];  // B[k = voxel][ci]
+      for (int s = 0; s < kWBatch / 16; ++s) {
+        if (pb + 16 * s >= p1) break;
+        bf8 A, B;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          A[e] = ld[16 * s + 8 * half + e][col];     // A[co][k = voxel]
+          B[e] = la[w][16 * s + 8 * half + e][col];  // B[k = voxel][ci]
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc, 0, 0, 0);
+      }
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc, 0, 0, 0);
     __syncthreads();
   }
+  if (t >= 27) return;
   float* out = partials + ((size_t)blockIdx.x * 27 + t) * kC * kC;
 #pragma unroll
   for (int qi = 0; qi < 16; ++qi) {
@@ -743,7 +771,7 @@ int net_backward_dout(pcf_net* net, const float* input) {
   for (int l = 4; l >= 1; --l) {
     hipLaunchKernelGGL(k_gelu_back, dim3(nc), dim3(256), 0, s, d, net->g, net->z[l], net->dzb, net->partials);
     sum_chunks(net, nc, kC, G + layer_offset(l) + 27 * kC * kC);
-    hipLaunchKernelGGL(k_mid_weights, dim3(nc, 27), dim3(64), 0, s, d, net->dzb, net->a[l - 1], net->partials);
+    hipLaunchKernelGGL(k_mid_weights, dim3(nc, 7), dim3(256), 0, s, d, net->dzb, net->a[l - 1], net->partials);
     sum_chunks(net, nc, 27 * kC * kC, G + layer_offset(l));
     hipLaunchKernelGGL((k_brick<KIND_MID, MID_DATA>), dim3(brick_blocks(bg)), dim3(256), 0, s, d, bg,
                        (const void*)net->dzb, net->wt16 + (size_t)(l - 1) * 27 * kC * kC, nullptr, nullptr, nullptr,
