@@ -507,8 +507,9 @@ int pcf_ensemble_stats(pcf_ensemble* e, int64_t out[4]);
  * z.  Parameters: one FP32 vector, layer by layer W[27][cout][cin] then
  * b[cout] (tap t = (dx+1) + 3 (dy+1) + 9 (dz+1)).  Device vectors are
  * channels-last FP32: input n x 3, output and target n, with voxel
- * i + nx (j + ny k).  A handle serves one grid size (nx a multiple of 32);
- * training = 1 keeps what the backward pass needs. */
+ * i + nx (j + ny k).  A handle serves one grid size (nx a multiple of 32,
+ * at most 2^31 - 1 voxels); training = 1 keeps what the backward pass
+ * needs. */
 typedef struct pcf_net pcf_net;
 int pcf_net_param_count(void);
 int pcf_net_create(int32_t nx, int32_t ny, int32_t nz, int32_t training, pcf_net** out);

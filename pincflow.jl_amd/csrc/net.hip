@@ -63,10 +63,14 @@ __device__ __forceinline__ long long neighbour(const Dims& d, int i, int j, int 
   return (long long)ii + (long long)d.nx * (jj + (long long)d.ny * kk);
 }
 
+// (i, j, k) of voxel p in 32-bit arithmetic: pcf_net_create admits at most
+// 2^31 - 1 voxels
 __device__ __forceinline__ void coords(const Dims& d, size_t p, int& i, int& j, int& k) {
-  i = (int)(p % d.nx);
-  j = (int)((p / d.nx) % d.ny);
-  k = (int)(p / ((size_t)d.nx * d.ny));
+  const unsigned q = (unsigned)p, nx = (unsigned)d.nx, ny = (unsigned)d.ny;
+  const unsigned r = q / nx;
+  i = (int)(q - r * nx);
+  j = (int)(r % ny);
+  k = (int)(r / ny);
 }
 
 // GELU x Phi(x) = x (1 + erf(x / sqrt 2)) / 2 with 1 + erf(z) from the
@@ -942,6 +946,10 @@ int pcf_net_create(int32_t nx, int32_t ny, int32_t nz, int32_t training, pcf_net
   using namespace pcf;
   if (!out || nx < 32 || nx % 32 != 0 || ny < 1 || nz < 1) {
     set_error("pcf_net_create: nx must be a positive multiple of 32");
+    return PCF_E_ARG;
+  }
+  if ((long long)nx * ny * nz > 0x7fffffffLL) {
+    set_error("pcf_net_create: at most 2^31 - 1 voxels");
     return PCF_E_ARG;
   }
   pcf_net* net = new pcf_net();

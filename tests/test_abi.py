@@ -87,3 +87,14 @@ def test_julia_config_mirror_matches_the_header(abi):
     header = _header_config_fields()
     assert julia == header
     assert [n for n, _ in abi.PcfConfig._fields_] == [n for n, _ in header]
+
+
+def test_network_sizes_are_checked_before_any_device_call(abi):
+    """pcf_net_create refuses grids it cannot index (nx not a multiple of 32,
+    more than 2^31 - 1 voxels) with PCF_E_ARG and a message, before it
+    touches a device"""
+    lib = abi.load_library()
+    h = ctypes.c_void_p()
+    for nx, ny, nz, words in ((48, 8, 8, b"multiple of 32"), (2048, 2048, 1024, b"2^31")):
+        assert lib.pcf_net_create(nx, ny, nz, 0, ctypes.byref(h)) == abi.E_ARG
+        assert words in lib.pcf_last_error()
