@@ -558,7 +558,7 @@ __global__ void __launch_bounds__(64) k_pc_factors(const double* ac, const doubl
 }
 
 // The line solve with precomputed factors, one column per thread and 64
-// adjacent columns per wavefront (blockDim.x must be 64); the loads of 8
+// adjacent columns per wavefront (blockDim.x must be 64); the loads of 16
 // levels are issued before their (sequential) recurrence steps, and a
 // compiler barrier keeps them there.  The upward sweep's values of the
 // lowest NLDS levels stay in LDS; those of higher levels go through `out`,
@@ -576,7 +576,7 @@ __device__ __forceinline__ void pc_thomas(const double* P, const double* E, cons
                                           double* out, int nx, int ny, int nz, double deta, int bx, int by) {
   constexpr int kThomasLds = NLDS;
   __shared__ double up[kThomasLds][64];
-  constexpr int U = 8;
+  constexpr int U = 16;
   const int tid = threadIdx.x;
   const int i = bx * 64 + tid;
   const int j = by;
@@ -593,49 +593,50 @@ __device__ __forceinline__ void pc_thomas(const double* P, const double* E, cons
   auto kept = [&](int k) -> double { return k < kThomasLds ? up[k][tid] : out[c + sxy * k]; };
   double sk = rhs_of(src[c]) / P[c];
   keep(0, sk);
-  int k = 1;
-  for (; k + U <= nz; k += U) {
+  // batches of U levels, the last one partial (its loads and steps
+  // predicated on the level)
+  for (int k = 1; k < nz; k += U) {
     double r[U], e[U], p[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const size_t o = c + sxy * (k + u);
-      r[u] = src[o];
-      e[u] = E[o];
-      p[u] = P[o];
+      r[u] = e[u] = p[u] = 0.0;
+      if (k + u < nz) {
+        const size_t o = c + sxy * (k + u);
+        r[u] = src[o];
+        e[u] = E[o];
+        p[u] = P[o];
+      }
     }
     asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      sk = (rhs_of(r[u]) + e[u] * sk) * p[u];
-      keep(k + u, sk);
+      if (k + u < nz) {
+        sk = (rhs_of(r[u]) + e[u] * sk) * p[u];
+        keep(k + u, sk);
+      }
     }
-  }
-  for (; k < nz; ++k) {
-    const size_t o = c + sxy * k;
-    sk = (rhs_of(src[o]) + E[o] * sk) * P[o];
-    keep(k, sk);
   }
   // the top level's upward value is its solution
   out[c + sxy * (nz - 1)] = sk;
   // downward sweep
-  k = nz - 2;
-  for (; k - (U - 1) >= 0; k -= U) {
+  for (int k = nz - 2; k >= 0; k -= U) {
     double r[U], q[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      r[u] = kept(k - u);
-      q[u] = Q[c + sxy * (k - u)];
+      r[u] = q[u] = 0.0;
+      if (k - u >= 0) {
+        r[u] = kept(k - u);
+        q[u] = Q[c + sxy * (k - u)];
+      }
     }
     asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      sk = r[u] + q[u] * sk;
-      out[c + sxy * (k - u)] = sk;
+      if (k - u >= 0) {
+        sk = r[u] + q[u] * sk;
+        out[c + sxy * (k - u)] = sk;
+      }
     }
-  }
-  for (; k >= 0; --k) {
-    sk = kept(k) + Q[c + sxy * k] * sk;
-    out[c + sxy * k] = sk;
   }
 }
 template <bool FIRST, int NLDS>
