@@ -1,5 +1,5 @@
-// C ABI of libpincflow_hip.so (include/pincflow_hip.h) and the host driver
-// of the time loop (src/Integration).  The driver only sequences kernels on
+// C ABI of libpincflow_hip.so (include/pincflow_hip.h) and the host side
+// of the time loop (src/Integration).  The host only sequences kernels on
 // the state's stream; fields never leave HBM during a step.  Host and device
 // synchronise once per step for the CFL time step and a few times per
 // BiCGSTAB solve (see poisson.hip).
